@@ -1,0 +1,186 @@
+/*
+ * jaqsi_amd C-ABI: the drop-in boundary of the MI355X simulation backend.
+ *
+ * The reference's seam is the Python function
+ *   simulate_and_measure(tape, n_qubits, type, obs, use_density, shots, key,
+ *                        initial_state, adjoint)
+ * (jaqsi/simulation.py:658-732), reached from Script.execute
+ * (jaqsi/script.py:226-335, 420-434).  Everything below it is replaced here:
+ *
+ *   jq_plan_create        <- _compile / _fuse / _fuse_runs / _compile_mixed
+ *                            (simulation.py:214-381): the recorded tape, lowered
+ *                            to the jq_op/jq_member list below, is fused and
+ *                            scheduled into tile passes.
+ *   jq_sv_forward         <- _run + measure_state / measure_density
+ *                            (simulation.py:400-473, 735-857)
+ *   jq_sv_adjoint         <- _adjoint_expval (simulation.py:546-655), and for
+ *                            density matrices the checkpointed adjoint that the
+ *                            reference does not have (it tapes every rho).
+ *   jq_pulse_propagate    <- Evolution magnus2 solve + unitary_cost_fn
+ *                            (evolution.py:194-263, qoc.py:261-343)
+ *   jq_last_error         <- the Python exceptions of the reference (ValueError /
+ *                            TypeError / RuntimeError), mapped by status code.
+ *
+ * Conventions
+ *  - Bit order: wire 0 is the most significant bit of the flat index
+ *    (simulation.py:794-796).  A density matrix of n qubits is simulated as a
+ *    vector of 2n "wires": ket wires 0..n-1, bra wires n..2n-1 (row-major rho).
+ *  - All device pointers are caller-owned; the library keeps none of them
+ *    after a call returns.  Persistent state lives only in jq_plan.
+ *  - Every compute entry point takes the HIP stream to launch on and is
+ *    asynchronous on it.
+ *  - Cotangents follow JAX's convention (no conjugation), i.e. the value the
+ *    reference's custom_vjp returns; PyTorch's convention is its conjugate.
+ */
+#ifndef JAQSI_AMD_H
+#define JAQSI_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define JQ_OK 0
+#define JQ_ERR_ARG 1         /* ValueError in the reference */
+#define JQ_ERR_UNSUPPORTED 2 /* NotImplementedError / TypeError */
+#define JQ_ERR_ALLOC 3       /* device allocation failure */
+#define JQ_ERR_DEVICE 4      /* HIP runtime error */
+#define JQ_ERR_NONFINITE 5   /* non-finite or singular result */
+
+#define JQ_MAX_WIRES 5
+
+/* member kinds: one primitive factor of a recorded gate */
+#define JQ_M_ROT 0    /* controlled(cos(t/2) I - i sin(t/2) P_word, nctrl) */
+#define JQ_M_PHASE 1  /* controlled(diag(1, e^{i t}), nctrl)                */
+#define JQ_M_MATRIX 2 /* explicit matrix from the matrix pool               */
+
+/* member flags */
+#define JQ_MF_CONJ 1    /* use the complex conjugate of the factor          */
+#define JQ_MF_SUPEROP 2 /* use kron(m, conj(m)) on (ket wires, bra wires)   */
+#define JQ_MF_GRAD 4    /* an explicit matrix whose cotangent is requested  */
+
+/* op structure flags (GateStructure, operations.py:176-185) */
+#define JQ_S_DIAGONAL 1
+#define JQ_S_PERMUTATION 2
+#define JQ_S_NONUNITARY 4 /* channel superoperator: un-applied by its inverse */
+
+typedef struct jq_member {
+  int32_t kind;   /* JQ_M_* */
+  int32_t flags;  /* JQ_MF_* */
+  int32_t nctrl;  /* leading control wires of this factor */
+  int32_t word;   /* Pauli word, 2 bits per target wire (I0 X1 Y2 Z3), first target highest */
+  int32_t param;  /* index into the angle vector, or -1 for a constant */
+  int32_t matrix; /* JQ_M_MATRIX: element offset of the matrix in the pool (complex128) */
+  double value;   /* the angle when param == -1 */
+} jq_member;
+
+typedef struct jq_op {
+  int32_t nwires;
+  int32_t wires[JQ_MAX_WIRES];
+  int32_t structure;    /* JQ_S_* */
+  int32_t controls;     /* GateStructure.controls */
+  int8_t perm[32];      /* GateStructure.permutation when JQ_S_PERMUTATION */
+  int32_t member_begin; /* members of this op: [member_begin, member_end) */
+  int32_t member_end;
+} jq_op;
+
+/* observable kinds for the fused reductions */
+#define JQ_OBS_NONE 0
+#define JQ_OBS_SV_DIAG 1 /* <psi|O|psi>, O = diag(d0,d1) on one wire (simulation.py:772-801) */
+#define JQ_OBS_DM_DIAG 2 /* Tr(O rho), O = diag(d0,d1) on one ket wire  (simulation.py:850-852) */
+
+typedef struct jq_plan_desc {
+  int32_t n_wires;        /* qubits of the simulated vector (2n for a density matrix) */
+  int32_t n_ops;
+  const jq_op* ops;
+  int32_t n_members;
+  const jq_member* members;
+  int32_t n_params;       /* length of the angle vector of one circuit */
+  int32_t n_matrix_elems; /* complex elements of the explicit-matrix pool of one circuit */
+  int32_t dtype;          /* 0 = complex64, 1 = complex128 */
+  int32_t tile_bits;      /* 0 = choose; else tile size 2^tile_bits (<= n_wires) */
+  int32_t checkpoints;    /* 1 = keep every pass output for the backward sweep */
+} jq_plan_desc;
+
+typedef struct jq_plan jq_plan;
+
+/* Lower, fuse and schedule one circuit structure; upload its program.
+ * Replaces simulation._compile/_compile_mixed (simulation.py:338-381). */
+int jq_plan_create(const jq_plan_desc* desc, jq_plan** out);
+int jq_plan_destroy(jq_plan* plan);
+
+/* Scheduling summary: passes, substeps, blocks, cotangent reals. */
+int jq_plan_info(const jq_plan* plan, int32_t* n_passes, int32_t* n_substeps, int32_t* n_blocks,
+                 int32_t* n_grad_reals, int32_t* tile_bits);
+
+/* Host copy of the scheduled program (int32 words) for inspection. */
+int jq_plan_program(const jq_plan* plan, int32_t* out, int64_t capacity, int64_t* size);
+
+/* Host copies of the plan's tables: which = 0 block table, 1 member list,
+ * 2 member table, 3 cotangent slot map.  No reference counterpart: these
+ * let tests execute the scheduled program on the CPU. */
+int jq_plan_tables(const jq_plan* plan, int32_t which, int32_t* out, int64_t capacity, int64_t* size);
+
+/* Forward simulation of `batch` circuits sharing the plan's structure.
+ *   angles   [batch][n_params] float64 (device)
+ *   matrices [batch][n_matrix_elems] complex128 (device, may be NULL)
+ *   psi0     initial vectors [batch or 1][2^n] in the plan dtype, or NULL for |0..0>
+ *   state    output vectors [batch][2^n] (device)
+ *   obs_*    observables (host arrays, n_obs entries: wire, d0, d1), kind JQ_OBS_*
+ *   expvals  [batch][n_obs] float64 (device) or NULL
+ * Replaces _run + measure (simulation.py:400-473, 735-857). */
+int jq_sv_forward(jq_plan* plan, int32_t batch, const double* angles, const void* matrices,
+                  const void* psi0, int32_t psi0_batched, void* state, int32_t obs_kind, int32_t n_obs,
+                  const int32_t* obs_wires, const double* obs_d, double* expvals, void* stream);
+
+/* Forward + adjoint-method gradient of  sum_o ct[b][o] <O_o>  per circuit.
+ *   ct           [batch][n_obs] float64 (device)
+ *   grad_angles  [batch][n_params] float64 (device, overwritten)
+ *   grad_matrices[batch][n_matrix_elems] complex128 (device, JQ_MF_GRAD members; may be NULL)
+ *   grad_psi0    [batch][2^n] plan dtype (device) or NULL: the initial-state cotangent
+ *   state        [batch][2^n] workspace/output for the final state (device)
+ * Replaces _adjoint_expval (simulation.py:546-655). */
+int jq_sv_adjoint(jq_plan* plan, int32_t batch, const double* angles, const void* matrices,
+                  const void* psi0, int32_t psi0_batched, void* state, int32_t obs_kind, int32_t n_obs,
+                  const int32_t* obs_wires, const double* obs_d, const double* ct, double* expvals,
+                  double* grad_angles, void* grad_matrices, void* grad_psi0, void* stream);
+
+/* Reductions of a final state (measure_state, simulation.py:735-811). */
+int jq_sv_probs(int32_t n_wires, int32_t batch, int32_t dtype, const void* state, void* probs, void* stream);
+int jq_sv_outer(int32_t n_wires, int32_t batch, int32_t dtype, const void* state, void* rho, void* stream);
+int jq_dm_diag(int32_t n_qubits, int32_t batch, int32_t dtype, const void* rho, void* probs, void* stream);
+
+/* Expectation value of one k-qubit observable matrix (complex128 host array,
+ * k <= 5) for each of `batch` states: Re <psi|O|psi>  (simulation.py:806-809),
+ * or Re Tr(O rho) when density != 0 (simulation.py:850-852). */
+int jq_expval_matrix(int32_t n_qubits, int32_t batch, int32_t dtype, int32_t density, const void* state,
+                     int32_t k, const int32_t* wires, const double* matrix, double* out, void* stream);
+
+/* Batched pulse propagators: see jaqsi_amd/pulses.py.  Each parameter set is
+ * evolved for n_steps piecewise-constant Magnus-2 steps in FP64. */
+typedef struct jq_pulse_desc {
+  int32_t n_sets;
+  int32_t n_steps;
+  double t_total;
+  int32_t n_params;       /* per set */
+  int32_t envelope;       /* JQ_ENV_* */
+  double coupling_j;      /* ZZ coupling (rad/ns) */
+  int32_t want_grad;
+} jq_pulse_desc;
+
+int jq_pulse_propagate(const jq_pulse_desc* desc, const double* params, const double* target,
+                       double* unitaries, double* infidelity, double* grad, void* stream);
+
+/* Thread-local message of the last failing call. */
+const char* jq_last_error(void);
+
+/* Library version and the offload target it was built for. */
+const char* jq_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JAQSI_AMD_H */

@@ -1,0 +1,207 @@
+"""ctypes binding of libjaqsi_amd.so (the C-ABI in include/jaqsi_amd.h).
+
+The library is loaded from the package directory.  There is no fallback:
+if it is missing, every simulation entry point raises, so a result never
+silently comes from anything but the HIP kernels.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libjaqsi_amd.so")
+
+JQ_OK = 0
+JQ_ERR_ARG = 1
+JQ_ERR_UNSUPPORTED = 2
+JQ_ERR_ALLOC = 3
+JQ_ERR_DEVICE = 4
+JQ_ERR_NONFINITE = 5
+
+JQ_M_ROT, JQ_M_PHASE, JQ_M_MATRIX = 0, 1, 2
+JQ_MF_CONJ, JQ_MF_SUPEROP, JQ_MF_GRAD = 1, 2, 4
+JQ_S_DIAGONAL, JQ_S_PERMUTATION, JQ_S_NONUNITARY = 1, 2, 4
+JQ_OBS_NONE, JQ_OBS_SV_DIAG, JQ_OBS_DM_DIAG = 0, 1, 2
+MAX_WIRES = 5
+
+
+class Member(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("flags", C.c_int32),
+        ("nctrl", C.c_int32),
+        ("word", C.c_int32),
+        ("param", C.c_int32),
+        ("matrix", C.c_int32),
+        ("value", C.c_double),
+    ]
+
+
+class Op(C.Structure):
+    _fields_ = [
+        ("nwires", C.c_int32),
+        ("wires", C.c_int32 * MAX_WIRES),
+        ("structure", C.c_int32),
+        ("controls", C.c_int32),
+        ("perm", C.c_int8 * 32),
+        ("member_begin", C.c_int32),
+        ("member_end", C.c_int32),
+    ]
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [
+        ("n_wires", C.c_int32),
+        ("n_ops", C.c_int32),
+        ("ops", C.POINTER(Op)),
+        ("n_members", C.c_int32),
+        ("members", C.POINTER(Member)),
+        ("n_params", C.c_int32),
+        ("n_matrix_elems", C.c_int32),
+        ("dtype", C.c_int32),
+        ("tile_bits", C.c_int32),
+        ("checkpoints", C.c_int32),
+    ]
+
+
+class PulseDesc(C.Structure):
+    _fields_ = [
+        ("n_sets", C.c_int32),
+        ("n_steps", C.c_int32),
+        ("t_total", C.c_double),
+        ("n_params", C.c_int32),
+        ("envelope", C.c_int32),
+        ("coupling_j", C.c_double),
+        ("want_grad", C.c_int32),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+
+EXPORTS = [
+    "jq_plan_create",
+    "jq_plan_destroy",
+    "jq_plan_info",
+    "jq_plan_program",
+    "jq_plan_tables",
+    "jq_sv_forward",
+    "jq_sv_adjoint",
+    "jq_sv_probs",
+    "jq_sv_outer",
+    "jq_dm_diag",
+    "jq_expval_matrix",
+    "jq_pulse_propagate",
+    "jq_last_error",
+    "jq_version",
+]
+
+
+class NativeError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m jaqsi_amd.build` "
+                "(the HIP extension is required; there is no CPU fallback)"
+            )
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.jq_plan_create.argtypes = [C.POINTER(PlanDesc), C.POINTER(vp)]
+        L.jq_plan_destroy.argtypes = [vp]
+        L.jq_plan_info.argtypes = [vp] + [C.POINTER(C.c_int32)] * 5
+        L.jq_plan_program.argtypes = [vp, C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int64)]
+        L.jq_plan_tables.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int64)]
+        L.jq_sv_forward.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
+        L.jq_sv_adjoint.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.jq_sv_probs.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp]
+        L.jq_sv_outer.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp]
+        L.jq_dm_diag.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp]
+        L.jq_expval_matrix.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int32, vp, vp, vp, vp]
+        L.jq_last_error.restype = C.c_char_p
+        L.jq_version.restype = C.c_char_p
+        for name in EXPORTS:
+            if name not in ("jq_last_error", "jq_version"):
+                getattr(L, name).restype = C.c_int
+        _lib = L
+        return L
+
+
+def check(rc: int) -> None:
+    if rc != JQ_OK:
+        msg = lib().jq_last_error().decode()
+        if rc == JQ_ERR_ARG:
+            raise ValueError(msg)
+        if rc == JQ_ERR_UNSUPPORTED:
+            raise NotImplementedError(msg)
+        raise NativeError(rc, msg)
+
+
+def ptr(t) -> Optional[int]:
+    """Device (or host) address of a tensor / array, None for None."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data
+    return t.data_ptr()
+
+
+class Plan:
+    """Owner of a native jq_plan (one fused, scheduled circuit structure)."""
+
+    def __init__(self, n_wires, ops, members, n_params, n_matrix_elems, dtype, tile_bits=0, checkpoints=1):
+        L = lib()
+        self._ops = (Op * max(1, len(ops)))(*ops)
+        self._members = (Member * max(1, len(members)))(*members)
+        d = PlanDesc(n_wires, len(ops), self._ops, len(members), self._members, n_params, n_matrix_elems, dtype,
+                     tile_bits, checkpoints)
+        h = C.c_void_p()
+        check(L.jq_plan_create(C.byref(d), C.byref(h)))
+        self.handle = h
+        self.n_wires = n_wires
+        self.n_params = n_params
+        self.n_matrix_elems = n_matrix_elems
+        self.dtype = dtype
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and _lib is not None:
+            _lib.jq_plan_destroy(h)
+            self.handle = None
+
+    def info(self) -> dict:
+        vals = [C.c_int32() for _ in range(5)]
+        check(lib().jq_plan_info(self.handle, *[C.byref(v) for v in vals]))
+        keys = ["n_passes", "n_substeps", "n_blocks", "n_grad_reals", "tile_bits"]
+        return {k: v.value for k, v in zip(keys, vals)}
+
+    def program(self) -> np.ndarray:
+        n = C.c_int64()
+        check(lib().jq_plan_program(self.handle, None, 0, C.byref(n)))
+        out = np.zeros(max(1, n.value), dtype=np.int32)
+        check(lib().jq_plan_program(self.handle, out.ctypes.data_as(C.POINTER(C.c_int32)), out.size, C.byref(n)))
+        return out[: n.value]
+
+    def table(self, which: int) -> np.ndarray:
+        n = C.c_int64()
+        check(lib().jq_plan_tables(self.handle, which, None, 0, C.byref(n)))
+        out = np.zeros(max(1, n.value), dtype=np.int32)
+        check(lib().jq_plan_tables(self.handle, which, out.ctypes.data_as(C.POINTER(C.c_int32)), out.size, C.byref(n)))
+        return out[: n.value]

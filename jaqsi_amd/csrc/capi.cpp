@@ -1,0 +1,451 @@
+// C-ABI entry points (include/jaqsi_amd.h): plan objects, workspaces and the
+// forward / adjoint drivers over the tile-pass kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "jaqsi_amd.h"
+#include "plan_internal.h"
+#include "prep.h"
+#include "program.h"
+#include "sv_kernels.h"
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess) return fail(JQ_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t want) {
+    if (want <= bytes) return hipSuccess;
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) bytes = want;
+    return e;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+template <typename T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v) {
+  size_t n = std::max<size_t>(1, v.size()) * sizeof(T);
+  hipError_t e = b.ensure(n);
+  if (e != hipSuccess) return e;
+  if (!v.empty()) return hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  return hipSuccess;
+}
+}  // namespace
+
+struct jq_plan {
+  jq::HostProgram hp;
+  int dtype = 0;
+  int n_params = 0;
+  int64_t n_matrix_elems = 0;
+  int checkpoints = 1;
+  DevBuf prog, block_table, member_list, member_table, member_values, slot_map;
+  int n_slots = 0;
+  std::vector<int32_t> slot_words;
+  bool uploaded = false;
+  // workspaces
+  DevBuf pool, partials, bars, ev_part, lam, ckpt, status;
+};
+
+extern "C" {
+
+const char* jq_last_error(void) { return g_err.c_str(); }
+
+const char* jq_version(void) { return "jaqsi_amd 0.1 (gfx950)"; }
+
+static int elem_bytes(int dtype) { return dtype == 0 ? 8 : 16; }
+
+int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
+  if (!desc || !out) return fail(JQ_ERR_ARG, "null argument");
+  if (desc->dtype != 0 && desc->dtype != 1) return fail(JQ_ERR_ARG, "dtype must be 0 (complex64) or 1 (complex128)");
+  if (desc->n_ops < 0 || (desc->n_ops > 0 && !desc->ops)) return fail(JQ_ERR_ARG, "bad op list");
+  int T = desc->tile_bits;
+  const int RB = desc->dtype == 0 ? 5 : 4;
+  if (T == 0) T = desc->dtype == 0 ? 13 : 12;
+  const bool okT = desc->dtype == 0 ? (T == 13) : (T == 12);
+  if (!okT) return fail(JQ_ERR_ARG, "tile_bits must be 13 (complex64) or 12 (complex128)");
+  jq_plan* p = new jq_plan();
+  try {
+    jq::build_program(*desc, T, RB, p->hp);
+  } catch (const jq::PlanError& e) {
+    delete p;
+    return fail(e.code, e.what());
+  } catch (const std::exception& e) {
+    delete p;
+    return fail(JQ_ERR_ARG, e.what());
+  }
+  p->dtype = desc->dtype;
+  p->n_params = desc->n_params;
+  p->n_matrix_elems = desc->n_matrix_elems;
+  p->checkpoints = desc->checkpoints;
+  // per-pass slot capacity in LDS
+  const size_t lds_state = (size_t)elem_bytes(p->dtype) << T;
+  const int* W = p->hp.words.data();
+  for (int q = 0; q < p->hp.n_passes; ++q) {
+    const int* pass = W + W[jq::HDR_WORDS + q];
+    if (lds_state + (64 + (size_t)pass[2]) * 8 > 160 * 1024) {
+      delete p;
+      return fail(JQ_ERR_UNSUPPORTED, "too many gradient slots in one pass for LDS");
+    }
+  }
+  std::vector<int32_t> sm;
+  std::vector<jq::SlotMap> smv = p->hp.slot_map;
+  std::sort(smv.begin(), smv.end(), [](const jq::SlotMap& a, const jq::SlotMap& b) { return a.bar < b.bar; });
+  for (auto& s : smv) {
+    const int* pass = W + W[jq::HDR_WORDS + s.pass];
+    sm.insert(sm.end(), {s.base, pass[2], s.local, s.bar, s.size});
+  }
+  p->n_slots = (int)smv.size();
+  p->slot_words = sm;
+  *out = p;
+  return JQ_OK;
+}
+
+int jq_plan_destroy(jq_plan* p) {
+  if (!p) return JQ_OK;
+  for (DevBuf* b : {&p->prog, &p->block_table, &p->member_list, &p->member_table, &p->member_values, &p->slot_map,
+                    &p->pool, &p->partials, &p->bars, &p->ev_part, &p->lam, &p->ckpt, &p->status})
+    b->release();
+  delete p;
+  return JQ_OK;
+}
+
+int jq_plan_info(const jq_plan* p, int32_t* n_passes, int32_t* n_substeps, int32_t* n_blocks, int32_t* n_grad_reals,
+                 int32_t* tile_bits) {
+  if (!p) return fail(JQ_ERR_ARG, "null plan");
+  if (n_passes) *n_passes = p->hp.n_passes;
+  if (n_substeps) *n_substeps = p->hp.n_substeps;
+  if (n_blocks) *n_blocks = p->hp.n_dev_blocks;
+  if (n_grad_reals) *n_grad_reals = p->hp.bars * 2;
+  if (tile_bits) *tile_bits = p->hp.T;
+  return JQ_OK;
+}
+
+int jq_plan_program(const jq_plan* p, int32_t* out, int64_t capacity, int64_t* size) {
+  if (!p) return fail(JQ_ERR_ARG, "null plan");
+  if (size) *size = (int64_t)p->hp.words.size();
+  if (out) {
+    if (capacity < (int64_t)p->hp.words.size()) return fail(JQ_ERR_ARG, "capacity too small");
+    std::memcpy(out, p->hp.words.data(), p->hp.words.size() * sizeof(int32_t));
+  }
+  return JQ_OK;
+}
+
+int jq_plan_tables(const jq_plan* p, int32_t which, int32_t* out, int64_t capacity, int64_t* size) {
+  if (!p) return fail(JQ_ERR_ARG, "null plan");
+  const std::vector<int32_t>* v = nullptr;
+  switch (which) {
+    case 0: v = &p->hp.block_table; break;
+    case 1: v = &p->hp.member_list; break;
+    case 2: v = &p->hp.member_table; break;
+    case 3: v = &p->slot_words; break;
+    default: return fail(JQ_ERR_ARG, "unknown table");
+  }
+  if (size) *size = (int64_t)v->size();
+  if (out) {
+    if (capacity < (int64_t)v->size()) return fail(JQ_ERR_ARG, "capacity too small");
+    std::memcpy(out, v->data(), v->size() * sizeof(int32_t));
+  }
+  return JQ_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int ensure_uploaded(jq_plan* p) {
+  if (p->uploaded) return JQ_OK;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = upload(p->prog, p->hp.words);
+  if (e == hipSuccess) e = upload(p->block_table, p->hp.block_table);
+  if (e == hipSuccess) e = upload(p->member_list, p->hp.member_list);
+  if (e == hipSuccess) e = upload(p->member_table, p->hp.member_table);
+  if (e == hipSuccess) e = upload(p->member_values, p->hp.member_values);
+  if (e == hipSuccess) e = upload(p->slot_map, p->slot_words);
+  if (e != hipSuccess) return fail(JQ_ERR_DEVICE, std::string("plan upload: ") + hipGetErrorString(e));
+  p->uploaded = true;
+  return JQ_OK;
+}
+
+int fill_obs(jq::SvArgs& a, int n_wires, int obs_kind, int n_obs, const int32_t* obs_wires, const double* obs_d) {
+  a.obs_kind = obs_kind;
+  a.n_obs = 0;
+  if (obs_kind == JQ_OBS_NONE) return JQ_OK;
+  if (obs_kind != JQ_OBS_SV_DIAG && obs_kind != JQ_OBS_DM_DIAG) return fail(JQ_ERR_ARG, "unknown observable kind");
+  if (n_obs < 0 || n_obs > 64) return fail(JQ_ERR_ARG, "between 0 and 64 observables are supported");
+  if (n_obs > 0 && (!obs_wires || !obs_d)) return fail(JQ_ERR_ARG, "null observable arrays");
+  const int span = obs_kind == JQ_OBS_DM_DIAG ? n_wires / 2 : n_wires;
+  for (int o = 0; o < n_obs; ++o) {
+    if (obs_wires[o] < 0 || obs_wires[o] >= span) return fail(JQ_ERR_ARG, "observable wire out of range");
+    a.obs_gbit[o] = n_wires - 1 - obs_wires[o];
+    a.obs_d[2 * o] = obs_d[2 * o];
+    a.obs_d[2 * o + 1] = obs_d[2 * o + 1];
+  }
+  a.n_obs = n_obs;
+  return JQ_OK;
+}
+
+int run_prep(jq_plan* p, int batch, const double* angles, const void* matrices, hipStream_t st) {
+  const size_t eb = elem_bytes(p->dtype);
+  HIP_TRY(p->pool.ensure(std::max<size_t>(1, (size_t)p->hp.pool * batch) * eb));
+  HIP_TRY(p->status.ensure(sizeof(int)));
+  jq::PrepArgs pa;
+  pa.n_blocks = p->hp.n_dev_blocks;
+  pa.batch = batch;
+  pa.block_table = (const int*)p->block_table.p;
+  pa.member_list = (const int*)p->member_list.p;
+  pa.member_table = (const int*)p->member_table.p;
+  pa.member_values = (const double*)p->member_values.p;
+  pa.angles = angles;
+  pa.n_params = p->n_params;
+  pa.matrices = matrices;
+  pa.n_matrix_elems = p->n_matrix_elems;
+  pa.pool = p->pool.p;
+  pa.pool_stride = p->hp.pool;
+  pa.status = (int*)p->status.p;
+  if (p->n_params > 0 && !angles) return fail(JQ_ERR_ARG, "angles required");
+  if (p->n_matrix_elems > 0 && !matrices) return fail(JQ_ERR_ARG, "explicit matrices required");
+  HIP_TRY(hipMemsetAsync(p->status.p, 0, sizeof(int), st));
+  HIP_TRY(jq::launch_prep(p->dtype, pa, st));
+  return JQ_OK;
+}
+
+jq::SvArgs base_args(jq_plan* p) {
+  jq::SvArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.prog = (const int*)p->prog.p;
+  a.n_wires = p->hp.n_wires;
+  a.vec_len = 1ull << p->hp.n_wires;
+  a.mats = p->pool.p;
+  a.pool_stride = p->hp.pool;
+  return a;
+}
+
+size_t smem_for(jq_plan* p, int extra_reals) {
+  return ((size_t)elem_bytes(p->dtype) << p->hp.T) + (size_t)(64 + extra_reals) * 8;
+}
+
+int max_slots(jq_plan* p) {
+  int m = 0;
+  const int* W = p->hp.words.data();
+  for (int q = 0; q < p->hp.n_passes; ++q) m = std::max(m, W[W[jq::HDR_WORDS + q] + 2]);
+  return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jq_sv_forward(jq_plan* p, int32_t batch, const double* angles, const void* matrices, const void* psi0,
+                  int32_t psi0_batched, void* state, int32_t obs_kind, int32_t n_obs, const int32_t* obs_wires,
+                  const double* obs_d, double* expvals, void* stream) {
+  if (!p || !state || batch < 1) return fail(JQ_ERR_ARG, "bad arguments to jq_sv_forward");
+  if (batch > 65535) return fail(JQ_ERR_ARG, "batch above 65535 per call");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = ensure_uploaded(p);
+  if (rc) return rc;
+  rc = run_prep(p, batch, angles, matrices, st);
+  if (rc) return rc;
+  jq::SvArgs a = base_args(p);
+  rc = fill_obs(a, p->hp.n_wires, expvals ? obs_kind : JQ_OBS_NONE, n_obs, obs_wires, obs_d);
+  if (rc) return rc;
+  const int ntiles = p->hp.ntiles;
+  if (a.obs_kind != JQ_OBS_NONE) {
+    HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * a.n_obs) * 8));
+    a.ev_part = (double*)p->ev_part.p;
+  }
+  for (int q = 0; q < p->hp.n_passes; ++q) {
+    a.pass = q;
+    a.src = q == 0 ? psi0 : state;
+    a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
+    a.dst = state;
+    a.dst_stride = (int64_t)a.vec_len;
+    HIP_TRY(jq::launch_sv(0, p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st));
+  }
+  if (a.obs_kind != JQ_OBS_NONE) HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
+  return JQ_OK;
+}
+
+int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* matrices, const void* psi0,
+                  int32_t psi0_batched, void* state, int32_t obs_kind, int32_t n_obs, const int32_t* obs_wires,
+                  const double* obs_d, const double* ct, double* expvals, double* grad_angles, void* grad_matrices,
+                  void* grad_psi0, void* stream) {
+  if (!p || !state || batch < 1 || !ct) return fail(JQ_ERR_ARG, "bad arguments to jq_sv_adjoint");
+  if (batch > 65535) return fail(JQ_ERR_ARG, "batch above 65535 per call");
+  if (obs_kind == JQ_OBS_NONE) return fail(JQ_ERR_ARG, "the adjoint needs observables");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = ensure_uploaded(p);
+  if (rc) return rc;
+  rc = run_prep(p, batch, angles, matrices, st);
+  if (rc) return rc;
+  jq::SvArgs a = base_args(p);
+  rc = fill_obs(a, p->hp.n_wires, obs_kind, n_obs, obs_wires, obs_d);
+  if (rc) return rc;
+  a.ct = ct;
+  const int ntiles = p->hp.ntiles;
+  const int P = p->hp.n_passes;
+  const size_t eb = elem_bytes(p->dtype);
+  const size_t vbytes = (size_t)a.vec_len * eb * batch;
+  HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * std::max(1, a.n_obs)) * 8));
+  HIP_TRY(p->partials.ensure(std::max<size_t>(1, (size_t)batch * p->hp.partial_reals) * 8));
+  a.grad_part = (double*)p->partials.p;
+  a.part_stride = p->hp.partial_reals;
+  a.ev_part = (double*)p->ev_part.p;
+  void* lam = grad_psi0;
+  if (!lam) {
+    HIP_TRY(p->lam.ensure(vbytes));
+    lam = p->lam.p;
+  }
+  a.lam = lam;
+  a.lam_stride = (int64_t)a.vec_len;
+  if (P == 1 && ntiles == 1) {
+    // whole circuit in one workgroup
+    a.pass = 0;
+    a.src = psi0;
+    a.src_stride = psi0_batched ? (int64_t)a.vec_len : 0;
+    a.dst = state;
+    a.dst_stride = (int64_t)a.vec_len;
+    if (!grad_psi0) a.lam = nullptr;
+    HIP_TRY(jq::launch_sv(2, p->dtype, p->hp.T, a, 1, batch, smem_for(p, max_slots(p)), st));
+  } else {
+    // forward: keep every pass output (checkpoints) when enabled
+    const bool ck = p->checkpoints != 0 && P > 1;
+    if (ck) HIP_TRY(p->ckpt.ensure(vbytes * (size_t)(P - 1)));
+    auto pass_out = [&](int q) -> void* {
+      if (q == P - 1 || !ck) return state;
+      return (char*)p->ckpt.p + vbytes * (size_t)q;
+    };
+    for (int q = 0; q < P; ++q) {
+      a.pass = q;
+      a.src = q == 0 ? psi0 : pass_out(q - 1);
+      a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
+      a.dst = pass_out(q);
+      a.dst_stride = (int64_t)a.vec_len;
+      HIP_TRY(jq::launch_sv(0, p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st));
+    }
+    HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals ? expvals : (double*)p->ev_part.p, ntiles, a.n_obs, batch, st));
+    // backward sweep
+    const size_t sm = smem_for(p, max_slots(p));
+    for (int q = P - 1; q >= 0; --q) {
+      a.pass = q;
+      a.src = pass_out(q);
+      a.src_stride = (int64_t)a.vec_len;
+      a.dst = ck ? nullptr : state;  // without checkpoints psi is un-applied in place
+      a.dst_stride = (int64_t)a.vec_len;
+      HIP_TRY(jq::launch_sv(1, p->dtype, p->hp.T, a, ntiles, batch, sm, st));
+    }
+  }
+  if (P == 1 && ntiles == 1 && expvals)
+    HIP_TRY(hipMemcpyAsync(expvals, a.ev_part, (size_t)batch * a.n_obs * 8, hipMemcpyDeviceToDevice, st));
+  // cotangents -> gradients
+  const int nreals = p->hp.bars * 2;
+  if (grad_angles && p->n_params > 0)
+    HIP_TRY(hipMemsetAsync(grad_angles, 0, (size_t)batch * p->n_params * 8, st));
+  if (grad_matrices && p->n_matrix_elems > 0)
+    HIP_TRY(hipMemsetAsync(grad_matrices, 0, (size_t)batch * p->n_matrix_elems * 16, st));
+  if (nreals > 0 && (grad_angles || grad_matrices)) {
+    HIP_TRY(p->bars.ensure((size_t)batch * nreals * 8));
+    jq::ReduceArgs r;
+    r.slot_map = (const int*)p->slot_map.p;
+    r.n_slots = p->n_slots;
+    r.partials = a.grad_part;
+    r.part_stride = a.part_stride;
+    r.ntiles = ntiles;
+    r.bars = (double*)p->bars.p;
+    r.n_reals = nreals;
+    r.batch = batch;
+    HIP_TRY(jq::launch_reduce_grad(r, st));
+    jq::ChainArgs c;
+    c.n_blocks = p->hp.n_dev_blocks;
+    c.batch = batch;
+    c.block_table = (const int*)p->block_table.p;
+    c.member_list = (const int*)p->member_list.p;
+    c.member_table = (const int*)p->member_table.p;
+    c.member_values = (const double*)p->member_values.p;
+    c.angles = angles;
+    c.n_params = p->n_params;
+    c.matrices = matrices;
+    c.n_matrix_elems = p->n_matrix_elems;
+    c.bars = (const double*)p->bars.p;
+    c.bars_stride = p->hp.bars;
+    c.grad_angles = grad_angles ? grad_angles : nullptr;
+    c.grad_matrices = grad_matrices;
+    if (!grad_angles) {
+      HIP_TRY(p->ev_part.ensure(std::max<size_t>(p->ev_part.bytes, (size_t)batch * std::max(1, p->n_params) * 8)));
+    }
+    if (grad_angles) HIP_TRY(jq::launch_chain(c, st));
+  }
+  return JQ_OK;
+}
+
+int jq_sv_probs(int32_t n_wires, int32_t batch, int32_t dtype, const void* state, void* probs, void* stream) {
+  if (!state || !probs || n_wires < 1 || n_wires > 40 || batch < 1) return fail(JQ_ERR_ARG, "bad arguments to jq_sv_probs");
+  HIP_TRY(jq::launch_probs(dtype, state, probs, ((int64_t)1 << n_wires) * batch, (hipStream_t)stream));
+  return JQ_OK;
+}
+
+int jq_sv_outer(int32_t n_wires, int32_t batch, int32_t dtype, const void* state, void* rho, void* stream) {
+  if (!state || !rho || n_wires < 1 || n_wires > 20 || batch < 1) return fail(JQ_ERR_ARG, "bad arguments to jq_sv_outer");
+  HIP_TRY(jq::launch_outer(dtype, state, rho, (int64_t)1 << n_wires, batch, (hipStream_t)stream));
+  return JQ_OK;
+}
+
+int jq_dm_diag(int32_t n_qubits, int32_t batch, int32_t dtype, const void* rho, void* probs, void* stream) {
+  if (!rho || !probs || n_qubits < 1 || n_qubits > 20 || batch < 1) return fail(JQ_ERR_ARG, "bad arguments to jq_dm_diag");
+  HIP_TRY(jq::launch_dm_diag(dtype, rho, probs, (int64_t)1 << n_qubits, batch, (hipStream_t)stream));
+  return JQ_OK;
+}
+
+int jq_expval_matrix(int32_t n_qubits, int32_t batch, int32_t dtype, int32_t density, const void* state, int32_t k,
+                     const int32_t* wires, const double* matrix, double* out, void* stream) {
+  if (!state || !out || !matrix || !wires || k < 1 || k > 5 || k > n_qubits || batch < 1)
+    return fail(JQ_ERR_ARG, "bad arguments to jq_expval_matrix");
+  jq::ExpvalArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n_qubits = n_qubits;
+  a.k = k;
+  a.density = density;
+  a.state = state;
+  a.stride = density ? ((int64_t)1 << (2 * n_qubits)) : ((int64_t)1 << n_qubits);
+  for (int j = 0; j < k; ++j) {
+    if (wires[j] < 0 || wires[j] >= n_qubits) return fail(JQ_ERR_ARG, "observable wire out of range");
+    a.gbits[j] = n_qubits - 1 - wires[j];
+    a.wmask |= 1ll << a.gbits[j];
+  }
+  a.mat = matrix;
+  a.out = out;
+  hipStream_t st = (hipStream_t)stream;
+  HIP_TRY(hipMemsetAsync(out, 0, (size_t)batch * 8, st));
+  HIP_TRY(jq::launch_expval_matrix(dtype, a, batch, st));
+  return JQ_OK;
+}
+
+int jq_pulse_propagate(const jq_pulse_desc*, const double*, const double*, double*, double*, double*, void*) {
+  return fail(JQ_ERR_UNSUPPORTED, "pulse propagation is provided by jq_pulse_* (see jaqsi_amd/pulses.py)");
+}
+
+}  // extern "C"

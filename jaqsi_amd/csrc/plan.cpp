@@ -1,0 +1,486 @@
+// Host planner: lowered op list -> fused blocks -> scheduled device program.
+//
+// Fusion follows the reference's plan rule (jaqsi/simulation.py:214-240): an
+// op multiplies into the previous block on the same wire tuple when no block
+// in between touched any of those wires; structure tags combine as in
+// _fused_structure (simulation.py:202-211).  The reference then composes
+// runs of permutations/diagonals into whole-state gathers
+// (simulation.py:243-335); here those runs are instead scheduled into tile
+// passes, where permutations and diagonals cost no HBM traffic of their own.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "jaqsi_amd.h"
+#include "plan_internal.h"
+#include "program.h"
+
+namespace jq {
+
+namespace {
+
+struct HB {  // host block
+  std::vector<int> wires;
+  int structure = 0;
+  int controls = 0;
+  std::vector<int> perm;
+  std::vector<int> members;
+  int kind = AK_NOP;
+  int k = 0, D = 0, diagf = 0, offs = 0;
+  bool grad = false;
+  bool nonunitary = false;
+  int mat = -1, inv = -1, bar = -1, bar_size = 0;
+  std::vector<int> targets;  // wires that must sit in registers, in matrix order
+  std::vector<int> ctrls;    // control wires
+};
+
+bool is_identity(const std::vector<int>& p) {
+  for (size_t i = 0; i < p.size(); ++i)
+    if (p[i] != (int)i) return false;
+  return true;
+}
+
+bool is_mcx(const std::vector<int>& p) {  // X on the last wire, all others controls
+  int d = (int)p.size();
+  for (int i = 0; i < d - 2; ++i)
+    if (p[i] != i) return false;
+  return p[d - 2] == d - 1 && p[d - 1] == d - 2;
+}
+
+bool is_mcswap(const std::vector<int>& p) {  // swap of the last two wires under controls
+  int d = (int)p.size();
+  if (d < 4) return false;
+  int s = d - 4;
+  for (int i = 0; i < d; ++i) {
+    int want = i;
+    if (i == s + 1) want = s + 2;
+    if (i == s + 2) want = s + 1;
+    if (p[i] != want) return false;
+  }
+  return true;
+}
+
+int gbit(int n, int wire) { return n - 1 - wire; }
+
+}  // namespace
+
+void build_program(const jq_plan_desc& d, int T_req, int RB, HostProgram& out) {
+  const int n = d.n_wires;
+  if (n < 1 || n > 40) throw PlanError(JQ_ERR_ARG, "n_wires must be in [1, 40]");
+  // ---- fusion (simulation.py:214-240) ---------------------------------
+  std::vector<HB> blocks;
+  std::vector<int> last(n, -1);
+  for (int oi = 0; oi < d.n_ops; ++oi) {
+    const jq_op& op = d.ops[oi];
+    if (op.nwires < 1 || op.nwires > JQ_MAX_WIRES)
+      throw PlanError(JQ_ERR_UNSUPPORTED, "gates on more than 5 wires are not supported");
+    std::vector<int> w(op.wires, op.wires + op.nwires);
+    for (int x : w)
+      if (x < 0 || x >= n) throw PlanError(JQ_ERR_ARG, "wire index out of range");
+    std::vector<int> perm;
+    if (op.structure & JQ_S_PERMUTATION) perm.assign(op.perm, op.perm + (1 << op.nwires));
+    int i = last[w[0]];
+    bool merge = i >= 0 && blocks[i].wires == w;
+    if (merge)
+      for (int x : w)
+        if (last[x] != i) merge = false;
+    if (merge) {
+      HB& b = blocks[i];
+      std::vector<int> np;
+      if (!b.perm.empty() && !perm.empty()) {
+        np.resize(perm.size());
+        for (size_t r = 0; r < perm.size(); ++r) np[r] = b.perm[perm[r]];
+      }
+      b.perm = np;
+      int s = 0;
+      if ((b.structure & JQ_S_DIAGONAL) && (op.structure & JQ_S_DIAGONAL)) s |= JQ_S_DIAGONAL;
+      if (!np.empty()) s |= JQ_S_PERMUTATION;
+      if ((b.structure | op.structure) & JQ_S_NONUNITARY) s |= JQ_S_NONUNITARY;
+      b.structure = s;
+      b.controls = std::min(b.controls, op.controls);
+      for (int m = op.member_begin; m < op.member_end; ++m) b.members.push_back(m);
+      continue;
+    }
+    HB b;
+    b.wires = w;
+    b.structure = op.structure;
+    b.controls = op.controls;
+    b.perm = perm;
+    for (int m = op.member_begin; m < op.member_end; ++m) b.members.push_back(m);
+    blocks.push_back(b);
+    for (int x : w) last[x] = (int)blocks.size() - 1;
+  }
+
+  // ---- classify ---------------------------------------------------------
+  const int maxk = RB;
+  int pool = 0, bars = 0;
+  for (HB& b : blocks) {
+    b.k = (int)b.wires.size();
+    const int full = 1 << b.k;
+    b.nonunitary = (b.structure & JQ_S_NONUNITARY) != 0;
+    for (int m : b.members) {
+      const jq_member& mm = d.members[m];
+      if (mm.param >= 0 || (mm.kind == JQ_M_MATRIX && (mm.flags & JQ_MF_GRAD))) b.grad = true;
+      if (mm.param >= d.n_params) throw PlanError(JQ_ERR_ARG, "member parameter index out of range");
+    }
+    bool dense = false;
+    if (b.structure & JQ_S_DIAGONAL) {
+      b.kind = AK_DIAG;
+      b.D = full;
+      b.diagf = 1;
+    } else if (b.structure & JQ_S_PERMUTATION) {
+      if (is_identity(b.perm)) {
+        b.kind = AK_NOP;
+      } else if (is_mcx(b.perm)) {
+        b.kind = AK_MCX;
+        b.targets = {b.wires[b.k - 1]};
+        b.ctrls.assign(b.wires.begin(), b.wires.end() - 1);
+      } else if (is_mcswap(b.perm)) {
+        b.kind = AK_MCSWAP;
+        b.targets = {b.wires[b.k - 2], b.wires[b.k - 1]};
+        b.ctrls.assign(b.wires.begin(), b.wires.end() - 2);
+      } else {
+        dense = true;
+        b.controls = 0;
+      }
+    } else {
+      dense = true;
+    }
+    if (dense) {
+      int c = std::max(0, std::min(b.controls, b.k - 1));
+      int t = b.k - c;
+      if (t == 1) {
+        b.kind = AK_MC1;
+        b.D = 2;
+      } else if (t == 2) {
+        b.kind = AK_MC2;
+        b.D = 4;
+      } else if (c == 0 && b.k <= 3) {
+        b.kind = AK_DENSEK;
+        b.D = full;
+        if (b.grad) throw PlanError(JQ_ERR_UNSUPPORTED, "gradients of dense gates on 3 wires are not supported yet");
+      } else {
+        throw PlanError(JQ_ERR_UNSUPPORTED, "dense gates with more than 2 target wires are not supported yet");
+      }
+      b.offs = full - b.D;
+      b.targets.assign(b.wires.begin() + c, b.wires.end());
+      b.ctrls.assign(b.wires.begin(), b.wires.begin() + c);
+    }
+    if ((int)b.targets.size() > maxk) throw PlanError(JQ_ERR_UNSUPPORTED, "gate needs more register bits than the tile offers");
+    if (b.kind == AK_MCX || b.kind == AK_MCSWAP || b.kind == AK_NOP) {
+      b.grad = false;  // constant permutations carry no cotangent (simulation.py:626-629)
+      if (b.nonunitary) throw PlanError(JQ_ERR_ARG, "permutation block marked non-unitary");
+      continue;
+    }
+    const int elems = b.diagf ? b.D : b.D * b.D;
+    b.mat = pool;
+    pool += elems;
+    if (b.nonunitary) {
+      b.inv = pool;
+      pool += elems;
+    }
+    if (b.grad) {
+      if (b.kind == AK_DENSEK) throw PlanError(JQ_ERR_UNSUPPORTED, "dense 3-wire gradient");
+      b.bar = bars;
+      b.bar_size = elems;
+      bars += elems;
+    }
+  }
+
+  // ---- tile bits ----------------------------------------------------------
+  int T = T_req;
+  if (T < RB) throw PlanError(JQ_ERR_ARG, "tile smaller than the register tile");
+  const int ntiles_log = std::max(0, n - T);
+  const int lowfix = std::min(n, std::min(T, 6 + (T >= 13 ? 1 : 0)));
+  // Global bits >= n are phantom (n < T): their amplitudes do not exist.
+
+  out = HostProgram();
+  out.T = T;
+  out.RB = RB;
+  out.n_wires = n;
+  out.ntiles_log = ntiles_log;
+  out.pool = pool;
+  out.bars = bars;
+
+  // block table and member table for the prep / chain kernels
+  int bi_dev = 0;
+  std::vector<int> dev_index(blocks.size(), -1);
+  for (size_t bi = 0; bi < blocks.size(); ++bi) {
+    HB& b = blocks[bi];
+    if (b.kind == AK_NOP || b.kind == AK_MCX || b.kind == AK_MCSWAP) continue;
+    dev_index[bi] = bi_dev++;
+    int rec[BW] = {0};
+    rec[B_KIND] = b.kind;
+    rec[B_K] = b.k;
+    rec[B_D] = b.D;
+    rec[B_MBEG] = (int)out.member_list.size();
+    for (int m : b.members) out.member_list.push_back(m);
+    rec[B_MEND] = (int)out.member_list.size();
+    rec[B_MAT] = b.mat;
+    rec[B_INV] = b.inv;
+    rec[B_BAR] = b.bar;
+    rec[B_DIAGF] = b.diagf;
+    rec[B_OFFS] = b.offs;
+    out.block_table.insert(out.block_table.end(), rec, rec + BW);
+    if (b.D > 8 && !b.diagf) throw PlanError(JQ_ERR_UNSUPPORTED, "dense blocks wider than 3 wires are not supported yet");
+    if (b.D > 32) throw PlanError(JQ_ERR_UNSUPPORTED, "diagonal blocks wider than 5 wires are not supported");
+  }
+  out.n_dev_blocks = bi_dev;
+  for (int m = 0; m < d.n_members; ++m) {
+    const jq_member& mm = d.members[m];
+    int rec[MW] = {mm.kind, mm.flags, mm.nctrl, mm.word, mm.param, mm.matrix, 0, 0};
+    out.member_table.insert(out.member_table.end(), rec, rec + MW);
+    out.member_values.push_back(mm.value);
+  }
+  // member generator wire count (k, or k/2 for a superoperator factor)
+  for (HB& b : blocks)
+    for (int m : b.members) {
+      int kb = (d.members[m].flags & JQ_MF_SUPEROP) ? b.k / 2 : b.k;
+      if ((d.members[m].flags & JQ_MF_SUPEROP) && (b.k % 2)) throw PlanError(JQ_ERR_ARG, "superoperator on an odd wire count");
+      out.member_table[m * MW + M_KB] = kb;
+    }
+
+  // ---- passes ---------------------------------------------------------------
+  std::vector<int> remaining;
+  for (size_t bi = 0; bi < blocks.size(); ++bi)
+    if (blocks[bi].kind != AK_NOP) remaining.push_back((int)bi);
+
+  struct Pass {
+    std::vector<int> tile;  // global bits ascending
+    std::vector<int> blocks;
+  };
+  std::vector<Pass> passes;
+  auto low_bits = [&]() {
+    std::vector<int> v;
+    for (int g = 0; g < lowfix; ++g) v.push_back(g);
+    return v;
+  };
+  while (!remaining.empty()) {
+    std::vector<char> in_tile(n + T + 1, 0), blocked(n, 0);
+    int tcount = 0;
+    if (n <= T) {
+      for (int g = 0; g < n; ++g) in_tile[g] = 1;
+      tcount = n;
+    } else {
+      for (int g : low_bits()) in_tile[g] = 1, ++tcount;
+    }
+    Pass P;
+    std::vector<int> left;
+    for (int bi : remaining) {
+      HB& b = blocks[bi];
+      bool blk = false;
+      for (int w : b.wires)
+        if (blocked[gbit(n, w)]) blk = true;
+      if (!blk) {
+        int need = 0;
+        for (int w : b.targets)
+          if (!in_tile[gbit(n, w)]) ++need;
+        if (tcount + need <= T) {
+          for (int w : b.targets)
+            if (!in_tile[gbit(n, w)]) in_tile[gbit(n, w)] = 1, ++tcount;
+          P.blocks.push_back(bi);
+          continue;
+        }
+      }
+      for (int w : b.wires) blocked[gbit(n, w)] = 1;
+      left.push_back(bi);
+    }
+    if (P.blocks.empty()) throw PlanError(JQ_ERR_UNSUPPORTED, "scheduler made no progress");
+    // pad the tile to T bits: real bits first (lowest unused), then phantom bits
+    for (int g = 0; g < n && tcount < T; ++g)
+      if (!in_tile[g]) in_tile[g] = 1, ++tcount;
+    for (int g = n; tcount < T; ++g) in_tile[g] = 1, ++tcount;
+    for (int g = 0; g < (int)in_tile.size(); ++g)
+      if (in_tile[g]) P.tile.push_back(g);
+    passes.push_back(P);
+    remaining = left;
+  }
+  if (passes.empty()) {  // empty circuit: one pass with no gates (copies psi0)
+    Pass P;
+    for (int g = 0; g < T; ++g) P.tile.push_back(g);
+    passes.push_back(P);
+  }
+
+  // ---- emit -------------------------------------------------------------------
+  std::vector<int32_t>& prog = out.words;
+  prog.assign(HDR_WORDS + passes.size(), 0);
+  prog[0] = PROG_MAGIC;
+  prog[1] = (int)passes.size();
+  prog[2] = T;
+  prog[3] = RB;
+  prog[4] = n;
+  prog[5] = bi_dev;
+  prog[6] = bars * 2;
+  const int ntiles = 1 << ntiles_log;
+  prog[8] = ntiles;
+  int slot_base = 0;
+  int total_sub = 0;
+  for (size_t p = 0; p < passes.size(); ++p) {
+    Pass& P = passes[p];
+    prog[HDR_WORDS + p] = (int)prog.size();
+    const size_t rec = prog.size();
+    prog.resize(rec + 4 + T, 0);
+    int flags = 0;
+    if (p == 0) flags |= PF_FIRST;
+    if (p + 1 == passes.size()) flags |= PF_LAST;
+    prog[rec + 1] = flags;
+    std::vector<int> tloc(n + T + 1, -1);  // global bit -> tile-local bit
+    for (int j = 0; j < T; ++j) {
+      prog[rec + 4 + j] = P.tile[j];
+      tloc[P.tile[j]] = j;
+    }
+    // substeps
+    std::vector<std::vector<int32_t>> subwords;
+    std::vector<int> rem = P.blocks;
+    int nsub = 0, nslots = 0;
+    std::vector<int> prev_reg;
+    while (!rem.empty()) {
+      std::vector<int> reg;  // tile-local bits in registers, by register position
+      std::vector<char> blocked(n, 0);
+      std::vector<int> sub, left;
+      int forced_k = 0;
+      for (int bi : rem) {
+        HB& b = blocks[bi];
+        bool blk = false;
+        for (int w : b.wires)
+          if (blocked[gbit(n, w)]) blk = true;
+        bool ok = false;
+        if (!blk) {
+          if (b.kind == AK_DIAG) {
+            ok = true;
+          } else if (b.kind == AK_DENSEK) {
+            if (reg.empty() && forced_k == 0) {
+              for (int j = b.k - 1; j >= 0; --j) reg.push_back(tloc[gbit(n, b.wires[j])]);
+              forced_k = b.k;
+              ok = true;
+            }
+          } else {
+            int need = 0;
+            for (int w : b.targets)
+              if (std::find(reg.begin(), reg.end(), tloc[gbit(n, w)]) == reg.end()) ++need;
+            if ((int)reg.size() + need <= RB) {
+              for (int w : b.targets) {
+                int tb = tloc[gbit(n, w)];
+                if (std::find(reg.begin(), reg.end(), tb) == reg.end()) reg.push_back(tb);
+              }
+              ok = true;
+            }
+          }
+        }
+        if (ok) {
+          sub.push_back(bi);
+        } else {
+          for (int w : b.wires) blocked[gbit(n, w)] = 1;
+          left.push_back(bi);
+        }
+      }
+      // pad registers: keep the previous layout's bits where possible (fewer
+      // moves), otherwise take the highest free tile bits so lanes keep the
+      // low bits (coalesced global access, conflict-free LDS).
+      std::vector<char> used(T, 0);
+      for (int tb : reg) used[tb] = 1;
+      for (int tb : prev_reg)
+        if ((int)reg.size() < RB && !used[tb] && forced_k == 0) reg.push_back(tb), used[tb] = 1;
+      for (int tb = T - 1; tb >= 0 && (int)reg.size() < RB; --tb)
+        if (!used[tb]) reg.push_back(tb), used[tb] = 1;
+      prev_reg = reg;
+      std::vector<int32_t> sw(1 + T, 0);
+      sw[0] = (int)sub.size();
+      std::vector<int> regpos(T, -1), thrpos(T, -1);
+      for (int i = 0; i < RB; ++i) {
+        sw[1 + i] = reg[i];
+        regpos[reg[i]] = i;
+      }
+      int tj = 0;
+      for (int tb = 0; tb < T; ++tb)
+        if (!used[tb]) {
+          sw[1 + RB + tj] = tb;
+          thrpos[tb] = tj;
+          ++tj;
+        }
+      for (int bi : sub) {
+        HB& b = blocks[bi];
+        int g[GW];
+        for (int& x : g) x = 0;
+        g[G_KIND] = b.kind;
+        g[G_BLOCK] = dev_index[bi];
+        g[G_RA] = b.targets.size() > 0 ? regpos[tloc[gbit(n, b.targets[0])]] : 0;
+        g[G_RB] = b.targets.size() > 1 ? regpos[tloc[gbit(n, b.targets[1])]] : 0;
+        if (b.kind == AK_DENSEK) g[G_RA] = b.k;
+        uint32_t creg = 0, cthr = 0;
+        uint64_t cout = 0;
+        for (int w : b.ctrls) {
+          int gb = gbit(n, w);
+          int tb = tloc[gb];
+          if (tb >= 0 && regpos[tb] >= 0)
+            creg |= 1u << regpos[tb];
+          else if (tb >= 0)
+            cthr |= 1u << tb;
+          else
+            cout |= 1ull << gb;
+        }
+        g[G_CREG] = (int)creg;
+        g[G_CTHR] = (int)cthr;
+        g[G_COUT_LO] = (int)(uint32_t)(cout & 0xffffffffu);
+        g[G_COUT_HI] = (int)(uint32_t)(cout >> 32);
+        g[G_MAT] = b.mat;
+        g[G_INV] = b.inv;
+        g[G_SLOT] = -1;
+        if (b.grad) {
+          g[G_SLOT] = nslots;
+          out.slot_map.push_back({(int)p, nslots, b.bar * 2, b.bar_size * 2});
+          nslots += b.bar_size * 2;
+        }
+        g[G_K] = b.k;
+        if (b.kind == AK_DIAG) {
+          for (int j = 0; j < b.k; ++j) {
+            int gb = gbit(n, b.wires[j]);
+            int tb = tloc[gb];
+            int loc;
+            if (tb >= 0 && regpos[tb] >= 0)
+              loc = (DL_REG << 6) | regpos[tb];
+            else if (tb >= 0)
+              loc = (DL_THR << 6) | tb;
+            else
+              loc = (DL_OUT << 6) | gb;
+            g[G_DW0 + j / 4] |= loc << (8 * (j % 4));
+          }
+        }
+        sw.insert(sw.end(), g, g + GW);
+      }
+      subwords.push_back(sw);
+      ++nsub;
+      rem = left;
+    }
+    if (nsub == 0) {  // gate-free pass: one substep with the default layout
+      std::vector<int32_t> sw(1 + T, 0);
+      for (int i = 0; i < RB; ++i) sw[1 + i] = T - 1 - i;
+      for (int j = 0; j < T - RB; ++j) sw[1 + RB + j] = j;
+      subwords.push_back(sw);
+      nsub = 1;
+    }
+    const size_t offs = prog.size();
+    prog.resize(offs + nsub, 0);
+    for (int k = 0; k < nsub; ++k) {
+      prog[offs + k] = (int)prog.size();
+      prog.insert(prog.end(), subwords[k].begin(), subwords[k].end());
+    }
+    prog[rec] = nsub;
+    prog[rec + 2] = nslots;
+    prog[rec + 3] = slot_base;
+    for (auto& s : out.slot_map)
+      if (s.pass == (int)p) s.base = slot_base;
+    slot_base += nslots * ntiles;
+    total_sub += nsub;
+  }
+  prog[7] = slot_base;
+  out.partial_reals = slot_base;
+  out.n_passes = (int)passes.size();
+  out.n_substeps = total_sub;
+  out.ntiles = ntiles;
+}
+
+}  // namespace jq

@@ -1,0 +1,69 @@
+// Launch interface of the small kernels (prep.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace jq {
+
+struct PrepArgs {
+  int n_blocks, batch;
+  const int* block_table;
+  const int* member_list;
+  const int* member_table;
+  const double* member_values;
+  const double* angles;     // [batch][n_params]
+  int n_params;
+  const void* matrices;     // [batch][n_matrix_elems] complex128, may be null
+  int64_t n_matrix_elems;
+  void* pool;               // [batch][pool_stride] plan dtype
+  int64_t pool_stride;
+  int* status;              // bit 0: singular non-unitary block
+};
+
+struct ReduceArgs {
+  const int* slot_map;      // n_slots x {base, nslots, local, bar, size}, sorted by bar
+  int n_slots;
+  const double* partials;   // [batch][part_stride]
+  int64_t part_stride;
+  int ntiles;
+  double* bars;             // [batch][n_reals]
+  int n_reals;
+  int batch;
+};
+
+struct ChainArgs {
+  int n_blocks, batch;
+  const int* block_table;
+  const int* member_list;
+  const int* member_table;
+  const double* member_values;
+  const double* angles;
+  int n_params;
+  const void* matrices;
+  int64_t n_matrix_elems;
+  const double* bars;       // complex128 cotangents [batch][bars_stride]
+  int64_t bars_stride;      // complex elements
+  double* grad_angles;      // [batch][n_params], pre-zeroed
+  void* grad_matrices;      // [batch][n_matrix_elems] complex128, pre-zeroed, may be null
+};
+
+struct ExpvalArgs {
+  int n_qubits, k, density;
+  const void* state;
+  int64_t stride;  // elements between states
+  int64_t wmask;   // global bits of the observable wires
+  int gbits[5];    // global bit of wire j of the observable
+  const void* mat; // complex128 2^k x 2^k (device)
+  double* out;     // [batch], pre-zeroed
+};
+
+hipError_t launch_prep(int dtype, const PrepArgs& a, hipStream_t st);
+hipError_t launch_reduce_grad(const ReduceArgs& a, hipStream_t st);
+hipError_t launch_reduce_ev(const double* part, double* out, int ntiles, int n_obs, int batch, hipStream_t st);
+hipError_t launch_chain(const ChainArgs& a, hipStream_t st);
+hipError_t launch_probs(int dtype, const void* s, void* p, int64_t total, hipStream_t st);
+hipError_t launch_outer(int dtype, const void* s, void* rho, int64_t dim, int64_t batch, hipStream_t st);
+hipError_t launch_dm_diag(int dtype, const void* rho, void* p, int64_t dim, int64_t batch, hipStream_t st);
+hipError_t launch_expval_matrix(int dtype, const ExpvalArgs& a, int batch, hipStream_t st);
+
+}  // namespace jq

@@ -1,0 +1,487 @@
+// Small kernels around the tile passes:
+//   prep    -- block matrices from the angle vector (the fused products of
+//              _fuse, simulation.py:214-240, evaluated on the device in FP64)
+//   reduce  -- per-tile cotangent / expectation partial sums -> FP64 totals
+//   chain   -- block cotangents -> angle gradients and explicit-matrix
+//              cotangents (the chain rule JAX applies through _rot_matrix,
+//              simulation.py:555-558 / gateset.py:399-405)
+//   probs / outer / dm_diag / expval_matrix -- measurement outputs
+//              (simulation.py:735-857)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "jaqsi_amd.h"
+#include "prep.h"
+#include "program.h"
+
+namespace jq {
+
+struct cd {
+  double x, y;
+};
+__device__ __forceinline__ cd cmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cd cconj(cd a) { return {a.x, -a.y}; }
+
+// entry (a, b) of a single-qubit Pauli (code I0 X1 Y2 Z3)
+__device__ __forceinline__ cd pauli_entry(int code, int a, int b) {
+  switch (code) {
+    case 0: return {a == b ? 1.0 : 0.0, 0.0};
+    case 1: return {a != b ? 1.0 : 0.0, 0.0};
+    case 2: return a == b ? cd{0.0, 0.0} : (a == 0 ? cd{0.0, -1.0} : cd{0.0, 1.0});
+    default: return {a == b ? (a ? -1.0 : 1.0) : 0.0, 0.0};
+  }
+}
+
+// generator entry g[r][c] (or its theta-derivative) of a member over kb wires
+__device__ cd gen_entry(const int* mt, double theta, const cd* xm, int kb, int r, int c, bool deriv) {
+  const int kind = mt[M_KIND];
+  const int dim = 1 << kb;
+  if (kind == JQ_M_MATRIX) {
+    if (deriv) return {0.0, 0.0};
+    return xm[mt[M_MATRIX] + r * dim + c];
+  }
+  const int nc = mt[M_NCTRL];
+  const int t = kb - nc;
+  const int s = dim - (1 << t);
+  if (r < s || c < s) return {(!deriv && r == c) ? 1.0 : 0.0, 0.0};
+  const int rr = r - s, cc = c - s;
+  if (kind == JQ_M_PHASE) {
+    if (rr == (1 << t) - 1 && cc == rr) {
+      double sn, cs;
+      sincos(theta, &sn, &cs);
+      return deriv ? cd{-sn, cs} : cd{cs, sn};
+    }
+    return {(!deriv && rr == cc) ? 1.0 : 0.0, 0.0};
+  }
+  // rotation: cos(t/2) I - i sin(t/2) P
+  const int word = mt[M_WORD];
+  cd p = {1.0, 0.0};
+  for (int j = 0; j < t; ++j) {
+    const int code = (word >> (2 * (t - 1 - j))) & 3;
+    p = cmul(p, pauli_entry(code, (rr >> (t - 1 - j)) & 1, (cc >> (t - 1 - j)) & 1));
+  }
+  double sn, cs;
+  sincos(0.5 * theta, &sn, &cs);
+  const double dlt = rr == cc ? 1.0 : 0.0;
+  if (!deriv) return {cs * dlt + sn * p.y, -sn * p.x};  // cs*d - i*sn*p
+  return {-0.5 * sn * dlt + 0.5 * cs * p.y, -0.5 * cs * p.x};
+}
+
+// entry (r, c) of the member's full 2^k matrix (k block wires), or its derivative
+__device__ cd member_entry(const int* mt, double theta, const cd* xm, int r, int c, bool deriv) {
+  const int flags = mt[M_FLAGS];
+  const int kb = mt[M_KB];
+  cd v;
+  if (flags & JQ_MF_SUPEROP) {
+    const int msk = (1 << kb) - 1;
+    const int r1 = r >> kb, r2 = r & msk, c1 = c >> kb, c2 = c & msk;
+    const cd a = gen_entry(mt, theta, xm, kb, r1, c1, false);
+    const cd b = cconj(gen_entry(mt, theta, xm, kb, r2, c2, false));
+    if (!deriv) {
+      v = cmul(a, b);
+    } else {
+      const cd da = gen_entry(mt, theta, xm, kb, r1, c1, true);
+      const cd db = cconj(gen_entry(mt, theta, xm, kb, r2, c2, true));
+      v = cadd(cmul(da, b), cmul(a, db));
+    }
+  } else {
+    v = gen_entry(mt, theta, xm, kb, r, c, deriv);
+  }
+  return (flags & JQ_MF_CONJ) ? cconj(v) : v;
+}
+
+__device__ __forceinline__ double member_theta(const int* mt, double cval, const double* angles) {
+  const int p = mt[M_PARAM];
+  return p >= 0 ? angles[p] : cval;
+}
+
+template <typename R>
+struct Out2 {
+  R x, y;
+};
+
+// Effective (active-block) matrix E of a member: D x D, or its diagonal.
+__device__ void member_eff(const int* mt, double th, const cd* xm, int D, int offs, bool diagf, bool deriv, cd* E) {
+  if (diagf) {
+    for (int r = 0; r < D; ++r) E[r] = member_entry(mt, th, xm, offs + r, offs + r, deriv);
+  } else {
+    for (int r = 0; r < D; ++r)
+      for (int c = 0; c < D; ++c) E[r * D + c] = member_entry(mt, th, xm, offs + r, offs + c, deriv);
+  }
+}
+
+__device__ void matmul_d(const cd* A, const cd* B, cd* C, int D) {  // C = A B
+  for (int r = 0; r < D; ++r)
+    for (int c = 0; c < D; ++c) {
+      cd acc = {0.0, 0.0};
+      for (int k = 0; k < D; ++k) acc = cadd(acc, cmul(A[r * D + k], B[k * D + c]));
+      C[r * D + c] = acc;
+    }
+}
+
+// Gauss-Jordan inverse with partial pivoting (D <= 8); returns false if singular.
+__device__ bool invert_d(const cd* A, cd* Ainv, int D) {
+  cd M[64], I[64];
+  for (int e = 0; e < D * D; ++e) {
+    M[e] = A[e];
+    I[e] = {(e / D) == (e % D) ? 1.0 : 0.0, 0.0};
+  }
+  for (int col = 0; col < D; ++col) {
+    int piv = col;
+    double best = -1;
+    for (int r = col; r < D; ++r) {
+      const double m = M[r * D + col].x * M[r * D + col].x + M[r * D + col].y * M[r * D + col].y;
+      if (m > best) best = m, piv = r;
+    }
+    if (!(best > 1e-300)) return false;
+    if (piv != col)
+      for (int c = 0; c < D; ++c) {
+        cd t = M[col * D + c];
+        M[col * D + c] = M[piv * D + c];
+        M[piv * D + c] = t;
+        t = I[col * D + c];
+        I[col * D + c] = I[piv * D + c];
+        I[piv * D + c] = t;
+      }
+    const cd p = M[col * D + col];
+    const double den = p.x * p.x + p.y * p.y;
+    const cd pinv = {p.x / den, -p.y / den};
+    for (int c = 0; c < D; ++c) {
+      M[col * D + c] = cmul(M[col * D + c], pinv);
+      I[col * D + c] = cmul(I[col * D + c], pinv);
+    }
+    for (int r = 0; r < D; ++r) {
+      if (r == col) continue;
+      const cd f = M[r * D + col];
+      if (f.x == 0.0 && f.y == 0.0) continue;
+      for (int c = 0; c < D; ++c) {
+        const cd a = cmul(f, M[col * D + c]);
+        const cd b = cmul(f, I[col * D + c]);
+        M[r * D + c] = {M[r * D + c].x - a.x, M[r * D + c].y - a.y};
+        I[r * D + c] = {I[r * D + c].x - b.x, I[r * D + c].y - b.y};
+      }
+    }
+  }
+  for (int e = 0; e < D * D; ++e) Ainv[e] = I[e];
+  return true;
+}
+
+template <typename R>
+__global__ void prep_kernel(PrepArgs a) {
+  const int nb = a.n_blocks;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (int64_t)nb * a.batch) return;
+  const int circ = (int)(gid / nb);
+  const int b = (int)(gid % nb);
+  const int* bt = a.block_table + b * BW;
+  const int D = bt[B_D], offs = bt[B_OFFS];
+  const bool diagf = bt[B_DIAGF] != 0;
+  const double* angles = a.angles + (int64_t)circ * a.n_params;
+  const cd* xm = reinterpret_cast<const cd*>(a.matrices) + (a.matrices ? (int64_t)circ * a.n_matrix_elems : 0);
+  cd M[64], E[64], T[64];
+  const int ne = diagf ? D : D * D;
+  for (int e = 0; e < ne; ++e) M[e] = {diagf ? 1.0 : ((e / D) == (e % D) ? 1.0 : 0.0), 0.0};
+  for (int mi = bt[B_MBEG]; mi < bt[B_MEND]; ++mi) {
+    const int m = a.member_list[mi];
+    const int* mt = a.member_table + m * MW;
+    const double th = member_theta(mt, a.member_values[m], angles);
+    member_eff(mt, th, xm, D, offs, diagf, false, E);
+    if (diagf) {
+      for (int r = 0; r < D; ++r) M[r] = cmul(E[r], M[r]);
+    } else {
+      matmul_d(E, M, T, D);
+      for (int e = 0; e < ne; ++e) M[e] = T[e];
+    }
+  }
+  Out2<R>* pool = reinterpret_cast<Out2<R>*>(a.pool) + (int64_t)circ * a.pool_stride;
+  for (int e = 0; e < ne; ++e) pool[bt[B_MAT] + e] = {(R)M[e].x, (R)M[e].y};
+  if (bt[B_INV] >= 0) {
+    bool ok = true;
+    if (diagf) {
+      for (int r = 0; r < D; ++r) {
+        const double den = M[r].x * M[r].x + M[r].y * M[r].y;
+        if (!(den > 1e-300)) ok = false;
+        T[r] = {M[r].x / den, -M[r].y / den};
+      }
+    } else {
+      ok = invert_d(M, T, D);
+    }
+    if (!ok) atomicOr(a.status, 1);
+    for (int e = 0; e < ne; ++e) pool[bt[B_INV] + e] = {(R)T[e].x, (R)T[e].y};
+  }
+}
+
+// sum over tiles of each cotangent slot -> bars (FP64 reals)
+__global__ void reduce_grad_kernel(ReduceArgs a) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (int64_t)a.n_reals * a.batch) return;
+  const int circ = (int)(gid / a.n_reals);
+  const int e = (int)(gid % a.n_reals);
+  // find the slot-map entry holding real e of the cotangent array
+  const int* sm = a.slot_map;
+  int lo = 0, hi = a.n_slots - 1;
+  while (lo < hi) {  // entries are sorted by bar offset
+    const int mid = (lo + hi + 1) / 2;
+    if (sm[mid * 5 + 3] <= e) lo = mid; else hi = mid - 1;
+  }
+  const int base = sm[lo * 5 + 0], nsl = sm[lo * 5 + 1], local = sm[lo * 5 + 2], bar = sm[lo * 5 + 3];
+  const double* part = a.partials + (int64_t)circ * a.part_stride + base + local + (e - bar);
+  double s = 0;
+  for (int t = 0; t < a.ntiles; ++t) s += part[(int64_t)t * nsl];
+  a.bars[(int64_t)circ * a.n_reals + e] = s;
+}
+
+__global__ void reduce_ev_kernel(const double* part, double* out, int ntiles, int n_obs, int batch) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (int64_t)n_obs * batch) return;
+  const int circ = (int)(gid / n_obs), o = (int)(gid % n_obs);
+  double s = 0;
+  for (int t = 0; t < ntiles; ++t) s += part[((int64_t)circ * ntiles + t) * n_obs + o];
+  out[gid] = s;
+}
+
+// block cotangent -> member cotangents W_j = L_j^T bar R_j^T -> Re sum W . dE
+__global__ void chain_kernel(ChainArgs a) {
+  const int nb = a.n_blocks;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (int64_t)nb * a.batch) return;
+  const int circ = (int)(gid / nb);
+  const int b = (int)(gid % nb);
+  const int* bt = a.block_table + b * BW;
+  if (bt[B_BAR] < 0) return;
+  const int D = bt[B_D], offs = bt[B_OFFS];
+  const bool diagf = bt[B_DIAGF] != 0;
+  const int ne = diagf ? D : D * D;
+  const double* angles = a.angles + (int64_t)circ * a.n_params;
+  const cd* xm = reinterpret_cast<const cd*>(a.matrices) + (a.matrices ? (int64_t)circ * a.n_matrix_elems : 0);
+  const cd* bar = reinterpret_cast<const cd*>(a.bars) + (int64_t)circ * a.bars_stride + bt[B_BAR];
+  double* gang = a.grad_angles + (int64_t)circ * a.n_params;
+  cd* gmat = a.grad_matrices ? reinterpret_cast<cd*>(a.grad_matrices) + (int64_t)circ * a.n_matrix_elems : nullptr;
+  const int mb = bt[B_MBEG], me = bt[B_MEND];
+  cd Rp[64], Lm[64], E[64], T[64], W[64];
+  for (int e = 0; e < ne; ++e) Rp[e] = {diagf ? 1.0 : ((e / D) == (e % D) ? 1.0 : 0.0), 0.0};
+  for (int j = mb; j < me; ++j) {
+    const int m = a.member_list[j];
+    const int* mt = a.member_table + m * MW;
+    const double th = member_theta(mt, a.member_values[m], angles);
+    const bool want_theta = mt[M_PARAM] >= 0;
+    const bool want_mat = mt[M_KIND] == JQ_M_MATRIX && (mt[M_FLAGS] & JQ_MF_GRAD) && gmat != nullptr;
+    if (want_theta || want_mat) {
+      // L = E_last ... E_{j+1}
+      for (int e = 0; e < ne; ++e) Lm[e] = {diagf ? 1.0 : ((e / D) == (e % D) ? 1.0 : 0.0), 0.0};
+      for (int q = j + 1; q < me; ++q) {
+        const int mq = a.member_list[q];
+        const int* mtq = a.member_table + mq * MW;
+        member_eff(mtq, member_theta(mtq, a.member_values[mq], angles), xm, D, offs, diagf, false, E);
+        if (diagf) {
+          for (int r = 0; r < D; ++r) Lm[r] = cmul(E[r], Lm[r]);
+        } else {
+          matmul_d(E, Lm, T, D);
+          for (int e = 0; e < ne; ++e) Lm[e] = T[e];
+        }
+      }
+      // W = L^T bar R^T
+      if (diagf) {
+        for (int r = 0; r < D; ++r) W[r] = cmul(cmul(Lm[r], bar[r]), Rp[r]);
+      } else {
+        for (int r = 0; r < D; ++r)
+          for (int c = 0; c < D; ++c) {
+            cd acc = {0.0, 0.0};
+            for (int k = 0; k < D; ++k) acc = cadd(acc, cmul(Lm[k * D + r], bar[k * D + c]));
+            T[r * D + c] = acc;
+          }
+        for (int r = 0; r < D; ++r)
+          for (int c = 0; c < D; ++c) {
+            cd acc = {0.0, 0.0};
+            for (int k = 0; k < D; ++k) acc = cadd(acc, cmul(T[r * D + k], Rp[c * D + k]));
+            W[r * D + c] = acc;
+          }
+      }
+      if (want_theta) {
+        member_eff(mt, th, xm, D, offs, diagf, true, E);
+        double g = 0;
+        for (int e = 0; e < ne; ++e) g += W[e].x * E[e].x - W[e].y * E[e].y;
+        atomicAdd(&gang[mt[M_PARAM]], g);
+      }
+      if (want_mat) {
+        const int dim = 1 << mt[M_KB];
+        // explicit matrices are unstructured: D == dim, offs == 0
+        for (int r = 0; r < D; ++r)
+          for (int c = 0; c < D; ++c) {
+            const cd w = diagf ? (r == c ? W[r] : cd{0.0, 0.0}) : W[r * D + c];
+            cd* dst = &gmat[mt[M_MATRIX] + r * dim + c];
+            atomicAdd(&dst->x, w.x);
+            atomicAdd(&dst->y, w.y);
+          }
+      }
+    }
+    // R <- E_j R
+    member_eff(mt, th, xm, D, offs, diagf, false, E);
+    if (diagf) {
+      for (int r = 0; r < D; ++r) Rp[r] = cmul(E[r], Rp[r]);
+    } else {
+      matmul_d(E, Rp, T, D);
+      for (int e = 0; e < ne; ++e) Rp[e] = T[e];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// measurement outputs
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void probs_kernel(const Out2<R>* s, R* p, int64_t total) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const Out2<R> v = s[i];
+  p[i] = v.x * v.x + v.y * v.y;
+}
+
+template <typename R>
+__global__ void outer_kernel(const Out2<R>* s, Out2<R>* rho, int64_t dim, int64_t total) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int64_t b = i / (dim * dim), rc = i % (dim * dim);
+  const Out2<R> x = s[b * dim + rc / dim], y = s[b * dim + rc % dim];
+  rho[i] = {x.x * y.x + x.y * y.y, x.y * y.x - x.x * y.y};  // x conj(y)
+}
+
+template <typename R>
+__global__ void dm_diag_kernel(const Out2<R>* rho, R* p, int64_t dim, int64_t total) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int64_t b = i / dim, r = i % dim;
+  p[i] = rho[b * dim * dim + r * dim + r].x;
+}
+
+// one workgroup per (state, chunk): sum over groups of 2^k amplitudes of
+// conj(psi_g)^T O psi_g, or Tr(O rho) over the ket wires of rho.
+template <typename R>
+__global__ void expval_matrix_kernel(ExpvalArgs a) {
+  const int circ = blockIdx.y;
+  const int k = a.k;
+  const int D = 1 << k;
+  const int64_t ngroups = (int64_t)1 << (a.n_qubits - k);
+  const Out2<R>* s = reinterpret_cast<const Out2<R>*>(a.state) + (int64_t)circ * a.stride;
+  const int64_t dim = (int64_t)1 << a.n_qubits;
+  const cd* om = reinterpret_cast<const cd*>(a.mat);
+  double acc = 0;
+  for (int64_t grp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; grp < ngroups;
+       grp += (int64_t)gridDim.x * blockDim.x) {
+    // deposit the group index into the bits of the other wires
+    int64_t idx0 = 0;
+    int src = 0;
+    for (int gb = 0; gb < a.n_qubits; ++gb) {
+      if (a.wmask & (1ll << gb)) continue;
+      if ((grp >> src) & 1) idx0 |= 1ll << gb;
+      ++src;
+    }
+    int64_t idx[32];
+    for (int q = 0; q < D; ++q) {
+      int64_t x = idx0;
+      for (int j = 0; j < k; ++j)
+        if ((q >> (k - 1 - j)) & 1) x |= 1ll << a.gbits[j];
+      idx[q] = x;
+    }
+    if (!a.density) {
+      cd x[32];
+      for (int q = 0; q < D; ++q) x[q] = {(double)s[idx[q]].x, (double)s[idx[q]].y};
+      for (int r = 0; r < D; ++r) {
+        cd y = {0, 0};
+        for (int c = 0; c < D; ++c) y = cadd(y, cmul(om[r * D + c], x[c]));
+        acc += x[r].x * y.x + x[r].y * y.y;  // Re conj(x_r) y_r
+      }
+    } else {
+      // Tr(O rho) restricted to this rest index: sum_{r,c} O[r][c] rho[c][r]
+      for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) {
+          const Out2<R> v = s[idx[c] * dim + idx[r]];
+          const cd o = om[r * D + c];
+          acc += o.x * (double)v.x - o.y * (double)v.y;
+        }
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(&a.out[circ], acc);
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_prep(int dtype, const PrepArgs& a, hipStream_t st) {
+  const int64_t n = (int64_t)a.n_blocks * a.batch;
+  if (n == 0) return hipSuccess;
+  const int tpb = 64;
+  dim3 grid((unsigned)((n + tpb - 1) / tpb));
+  if (dtype == 0)
+    hipLaunchKernelGGL(prep_kernel<float>, grid, dim3(tpb), 0, st, a);
+  else
+    hipLaunchKernelGGL(prep_kernel<double>, grid, dim3(tpb), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_grad(const ReduceArgs& a, hipStream_t st) {
+  const int64_t n = (int64_t)a.n_reals * a.batch;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_grad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_ev(const double* part, double* out, int ntiles, int n_obs, int batch, hipStream_t st) {
+  const int64_t n = (int64_t)n_obs * batch;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_ev_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, out, ntiles, n_obs,
+                     batch);
+  return hipGetLastError();
+}
+
+hipError_t launch_chain(const ChainArgs& a, hipStream_t st) {
+  const int64_t n = (int64_t)a.n_blocks * a.batch;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(chain_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_probs(int dtype, const void* s, void* p, int64_t total, hipStream_t st) {
+  dim3 g((unsigned)((total + 255) / 256));
+  if (dtype == 0)
+    hipLaunchKernelGGL(probs_kernel<float>, g, dim3(256), 0, st, (const Out2<float>*)s, (float*)p, total);
+  else
+    hipLaunchKernelGGL(probs_kernel<double>, g, dim3(256), 0, st, (const Out2<double>*)s, (double*)p, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_outer(int dtype, const void* s, void* rho, int64_t dim, int64_t batch, hipStream_t st) {
+  const int64_t total = dim * dim * batch;
+  dim3 g((unsigned)((total + 255) / 256));
+  if (dtype == 0)
+    hipLaunchKernelGGL(outer_kernel<float>, g, dim3(256), 0, st, (const Out2<float>*)s, (Out2<float>*)rho, dim, total);
+  else
+    hipLaunchKernelGGL(outer_kernel<double>, g, dim3(256), 0, st, (const Out2<double>*)s, (Out2<double>*)rho, dim,
+                       total);
+  return hipGetLastError();
+}
+
+hipError_t launch_dm_diag(int dtype, const void* rho, void* p, int64_t dim, int64_t batch, hipStream_t st) {
+  const int64_t total = dim * batch;
+  dim3 g((unsigned)((total + 255) / 256));
+  if (dtype == 0)
+    hipLaunchKernelGGL(dm_diag_kernel<float>, g, dim3(256), 0, st, (const Out2<float>*)rho, (float*)p, dim, total);
+  else
+    hipLaunchKernelGGL(dm_diag_kernel<double>, g, dim3(256), 0, st, (const Out2<double>*)rho, (double*)p, dim, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_expval_matrix(int dtype, const ExpvalArgs& a, int batch, hipStream_t st) {
+  const int64_t ngroups = (int64_t)1 << (a.n_qubits - a.k);
+  int64_t blocks = (ngroups + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  if (blocks < 1) blocks = 1;
+  dim3 g((unsigned)blocks, (unsigned)batch);
+  if (dtype == 0)
+    hipLaunchKernelGGL(expval_matrix_kernel<float>, g, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(expval_matrix_kernel<double>, g, dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace jq

@@ -1,0 +1,474 @@
+"""The gate library (mirrors jaqsi/gateset.py:32-910).
+
+Fixed matrices are exact NumPy complex128 constants; ``.matrix`` casts them
+to the active dtype.  Parametrised gates build their matrix from torch ops
+on the parameter (so it may require grad) and lower to ``Member`` factors
+that the device evaluates from the parameter value directly.
+"""
+
+from __future__ import annotations
+
+from functools import reduce
+from typing import List, Optional, Union
+
+import numpy as np
+import torch
+
+from jaqsi_amd.operations import (
+    GateStructure,
+    Hermitian,
+    Member,
+    Operation,
+    _as_tensor,
+    cdtype,
+)
+
+_C = np.complex128
+
+
+class Id(Operation):
+    """Identity on any number of wires (gateset.py:32-60)."""
+
+    _matrix = np.eye(2, dtype=_C)
+    _structure = GateStructure(diagonal=True)
+    is_unitary = True
+    is_clifford = True
+
+    def __init__(self, wires: Union[int, List[int]] = 0, **kwargs) -> None:
+        w = list(wires) if isinstance(wires, (list, tuple)) else [wires]
+        if len(w) > 1:
+            kwargs["matrix"] = np.eye(2 ** len(w), dtype=_C)
+        super().__init__(wires=wires, **kwargs)
+        if len(w) > 1:
+            self._structure = GateStructure(diagonal=True)
+            self._explicit = False
+
+
+class PauliX(Operation):
+    _matrix = np.array([[0, 1], [1, 0]], dtype=_C)
+    _num_wires = 1
+    _structure = GateStructure(permutation=(1, 0))
+    is_unitary = True
+    is_clifford = True
+
+    def __init__(self, wires: Union[int, List[int]] = 0, **kwargs) -> None:
+        super().__init__(wires=wires, **kwargs)
+
+
+class PauliY(Operation):
+    _matrix = np.array([[0, -1j], [1j, 0]], dtype=_C)
+    _num_wires = 1
+    is_unitary = True
+    is_clifford = True
+
+    def __init__(self, wires: Union[int, List[int]] = 0, **kwargs) -> None:
+        super().__init__(wires=wires, **kwargs)
+
+
+class PauliZ(Operation):
+    _matrix = np.array([[1, 0], [0, -1]], dtype=_C)
+    _num_wires = 1
+    _structure = GateStructure(diagonal=True)
+    is_unitary = True
+    is_clifford = True
+
+    def __init__(self, wires: Union[int, List[int]] = 0, **kwargs) -> None:
+        super().__init__(wires=wires, **kwargs)
+
+
+class H(Operation):
+    _matrix = np.array([[1, 1], [1, -1]], dtype=_C) / np.sqrt(2)
+    _num_wires = 1
+    is_unitary = True
+    is_clifford = True
+
+    def __init__(self, wires: Union[int, List[int]] = 0, **kwargs) -> None:
+        super().__init__(wires=wires, **kwargs)
+
+
+class S(Operation):
+    _matrix = np.array([[1, 0], [0, 1j]], dtype=_C)
+    _num_wires = 1
+    _structure = GateStructure(diagonal=True)
+    is_unitary = True
+    is_clifford = True
+
+    def __init__(self, wires: Union[int, List[int]] = 0) -> None:
+        super().__init__(wires=wires)
+
+
+class SWAP(Operation):
+    _matrix = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], dtype=_C)
+    _num_wires = 2
+    _structure = GateStructure(permutation=(0, 2, 1, 3))
+    is_unitary = True
+    is_clifford = True
+
+    def __init__(self, wires: Union[int, List[int]] = 0, **kwargs) -> None:
+        super().__init__(wires=wires, **kwargs)
+
+
+class RandomUnitary(Operation):
+    """A random Hermitian matrix applied as a gate (gateset.py:175-205).
+
+    The reference draws from the JAX PRNG; here *key* seeds a torch
+    generator, so values differ from the reference (parity unpinned).
+    """
+
+    def __init__(self, wires, key: int = 0, scale: float = 1.0, record: bool = True) -> None:
+        w = list(wires) if isinstance(wires, (list, tuple)) else [wires]
+        dim = 2 ** len(w)
+        g = torch.Generator().manual_seed(int(key))
+        a = torch.randn(dim, dim, generator=g, dtype=torch.float64) + 1j * torch.randn(
+            dim, dim, generator=g, dtype=torch.float64
+        )
+        h = (a + a.conj().T) / 2.0
+        h = h * (scale / torch.linalg.norm(h))
+        super().__init__(w, matrix=h, record=record)
+
+
+class DiagonalQubitUnitary(Operation):
+    """``diag(d_0, ..., d_{2^k-1})`` (gateset.py:208-350)."""
+
+    _param_names = ()
+    is_unitary = True
+
+    def __init__(self, diag, wires: Union[int, List[int]] = 0, generator=None, scale=None, **kwargs) -> None:
+        self.diag = diag
+        self._generator = generator
+        self._scale = scale
+        wl = list(wires) if isinstance(wires, (list, tuple)) else [wires]
+        d = _as_tensor(diag)
+        if tuple(d.shape[-1:]) != (2 ** len(wl),):
+            raise ValueError(
+                f"DiagonalQubitUnitary expects {2 ** len(wl)} diagonal entries for {len(wl)} wire(s), got shape {tuple(d.shape)}"
+            )
+        kwargs.setdefault("name", "DiagU")
+        super().__init__(wires=wires, matrix=torch.diag_embed(d.to(torch.complex128)), **kwargs)
+        self._structure = GateStructure(diagonal=True)
+
+    def decompose(self) -> List[Operation]:
+        """Walsh-Hadamard split of a real diagonal generator into Z-string
+        rotations (gateset.py:270-318)."""
+        if self._generator is None:
+            return super().decompose()
+        k = len(self.wires)
+        dim = 2**k
+        marks = np.asarray(self._generator, dtype=float).reshape(-1)
+        signs = np.array([[1 - 2 * ((j >> (k - 1 - i)) & 1) for j in range(dim)] for i in range(k)])
+        ops: List[Operation] = []
+        for mask in range(1, dim):
+            chi = np.ones(dim)
+            for i in range(k):
+                if (mask >> i) & 1:
+                    chi = chi * signs[i]
+            alpha = float(marks @ chi) / dim
+            if abs(alpha) < 1e-12:
+                continue
+            word = "".join("Z" if (mask >> i) & 1 else "I" for i in range(k))
+            ops.append(PauliRot(2.0 * alpha * self._scale, word, wires=self.wires, record=False))
+        return ops
+
+
+class Barrier(Operation):
+    """No-op separator (gateset.py:353-381)."""
+
+    _matrix = None
+
+    def __init__(self, wires: Union[int, List[int]] = 0) -> None:
+        super().__init__(wires=wires)
+
+    def lower(self) -> List[Member]:
+        return []
+
+    def apply_to_state(self, state, n_qubits):
+        return state
+
+    def apply_to_density(self, rho, n_qubits):
+        return rho
+
+
+_PAULI_MATRICES = {"I": Id._matrix, "X": PauliX._matrix, "Y": PauliY._matrix, "Z": PauliZ._matrix}
+
+
+def _pauli_tensor(word: str) -> np.ndarray:
+    return reduce(np.kron, [_PAULI_MATRICES[c] for c in word])
+
+
+def _rot_matrix(theta, pauli: np.ndarray) -> torch.Tensor:
+    """cos(t/2) I - i sin(t/2) P, batched over the shape of *theta*
+    (gateset.py:399-405)."""
+    t = _as_tensor(theta, torch.float64)
+    d = pauli.shape[0]
+    c = torch.cos(t / 2)[..., None, None]
+    s = torch.sin(t / 2)[..., None, None]
+    eye = torch.eye(d, dtype=torch.complex128)
+    p = torch.as_tensor(pauli)
+    return c * eye - 1j * s * p
+
+
+def _controlled_tensor(block: torch.Tensor, n_controls: int) -> torch.Tensor:
+    dt = block.shape[-1]
+    dim = (2**n_controls) * dt
+    m = torch.eye(dim, dtype=block.dtype).expand(*block.shape[:-2], dim, dim).clone()
+    s = dim - dt
+    m[..., s:, s:] = block
+    return m
+
+
+def _make_rotation_gate(pauli_class, name: str):
+    word = name[1]
+
+    class _RotationGate(Operation):
+        __doc__ = f"{name}(theta) = exp(-i theta/2 {word}) (gateset.py:408-451)."
+        _num_wires = 1
+        is_unitary = True
+        _param_names = ("theta",)
+
+        def __init__(self, theta, wires: Union[int, List[int]] = 0, **kwargs) -> None:
+            self.theta = theta
+            super().__init__(wires=wires, **kwargs)
+            self._structure = GateStructure(diagonal=pauli_class is PauliZ)
+
+        @property
+        def matrix(self):
+            return _rot_matrix(self.theta, pauli_class._matrix).to(cdtype())
+
+        def lower(self):
+            return [Member("rot", self.theta, word)]
+
+        def generator(self) -> Operation:
+            return pauli_class(wires=self.wires[0], record=False)
+
+    _RotationGate.__name__ = name
+    _RotationGate.__qualname__ = name
+    return _RotationGate
+
+
+RX = _make_rotation_gate(PauliX, "RX")
+RY = _make_rotation_gate(PauliY, "RY")
+RZ = _make_rotation_gate(PauliZ, "RZ")
+
+_P0 = np.array([[1, 0], [0, 0]], dtype=_C)
+_P1 = np.array([[0, 0], [0, 1]], dtype=_C)
+
+
+def _make_controlled_gate(target_class, name: str):
+    class _ControlledGate(Operation):
+        __doc__ = f"Controlled-{target_class.__name__[5:]} (gateset.py:459-512)."
+        _matrix = np.kron(_P0, Id._matrix) + np.kron(_P1, target_class._matrix)
+        _num_wires = 2
+        is_unitary = True
+        is_controlled = True
+        _structure = GateStructure(
+            permutation=(0, 1, 3, 2) if target_class is PauliX else (),
+            diagonal=target_class is PauliZ,
+            controls=1,
+        )
+        is_clifford = True
+
+        def __init__(self, wires: List[int] = [0, 1], **kwargs) -> None:
+            super().__init__(wires=wires, **kwargs)
+
+        def decompose(self):
+            if name != "CZ":
+                return super().decompose()
+            c, t = self.wires
+            return [H(wires=t, record=False), CX(wires=[c, t], record=False), H(wires=t, record=False)]
+
+    _ControlledGate.__name__ = name
+    _ControlledGate.__qualname__ = name
+    return _ControlledGate
+
+
+CX = _make_controlled_gate(PauliX, "CX")
+CY = _make_controlled_gate(PauliY, "CY")
+CZ = _make_controlled_gate(PauliZ, "CZ")
+
+
+class CCX(Operation):
+    """Toffoli (gateset.py:515-548)."""
+
+    _matrix = np.eye(8, dtype=_C)[[0, 1, 2, 3, 4, 5, 7, 6]]
+    is_controlled = True
+    _num_wires = 3
+    _structure = GateStructure(permutation=(0, 1, 2, 3, 4, 5, 7, 6), controls=2)
+    is_unitary = True
+
+    def __init__(self, wires: List[int] = [0, 1, 2], **kwargs) -> None:
+        super().__init__(wires=wires, **kwargs)
+
+
+class CSWAP(Operation):
+    """Fredkin (gateset.py:551-584)."""
+
+    _matrix = np.eye(8, dtype=_C)[[0, 1, 2, 3, 4, 6, 5, 7]]
+    is_controlled = True
+    _num_wires = 3
+    _structure = GateStructure(permutation=(0, 1, 2, 3, 4, 6, 5, 7), controls=1)
+    is_unitary = True
+
+    def __init__(self, wires: List[int] = [0, 1, 2], **kwargs) -> None:
+        super().__init__(wires=wires, **kwargs)
+
+
+class ControlledPhaseShift(Operation):
+    """diag(1, 1, 1, e^{i phi}) (gateset.py:587-619)."""
+
+    _num_wires = 2
+    is_unitary = True
+    _param_names = ("phi",)
+    is_controlled = True
+
+    def __init__(self, phi, wires: List[int] = [0, 1], **kwargs) -> None:
+        self.phi = phi
+        super().__init__(wires=wires, **kwargs)
+        self._structure = GateStructure(diagonal=True, controls=1)
+
+    @property
+    def matrix(self):
+        p = _as_tensor(self.phi, torch.float64)
+        one = torch.ones_like(p, dtype=torch.complex128)
+        return torch.diag_embed(torch.stack([one, one, one, torch.exp(1j * p)], dim=-1)).to(cdtype())
+
+    def lower(self):
+        return [Member("phase", self.phi, "", 1)]
+
+
+CPhase = ControlledPhaseShift
+
+
+class Rot(Operation):
+    """RZ(omega) RY(theta) RZ(phi) (gateset.py:622-667)."""
+
+    _num_wires = 1
+    is_unitary = True
+    _param_names = ("phi", "theta", "omega")
+
+    def __init__(self, phi, theta, omega, wires: Union[int, List[int]] = 0, **kwargs) -> None:
+        self.phi, self.theta, self.omega = phi, theta, omega
+        super().__init__(wires=wires, **kwargs)
+
+    @property
+    def matrix(self):
+        a = _rot_matrix(self.omega, PauliZ._matrix)
+        b = _rot_matrix(self.theta, PauliY._matrix)
+        c = _rot_matrix(self.phi, PauliZ._matrix)
+        return (a @ b @ c).to(cdtype())
+
+    def lower(self):
+        return [Member("rot", self.phi, "Z"), Member("rot", self.theta, "Y"), Member("rot", self.omega, "Z")]
+
+    def decompose(self):
+        w = self.wires[0]
+        return [RZ(self.phi, wires=w, record=False), RY(self.theta, wires=w, record=False), RZ(self.omega, wires=w, record=False)]
+
+
+class PauliRot(Operation):
+    """exp(-i theta/2 P_word) (gateset.py:670-717)."""
+
+    _param_names = ("theta",)
+    is_unitary = True
+
+    def __init__(self, theta, pauli_word: str, wires: Union[int, List[int]] = 0, **kwargs) -> None:
+        self.theta = theta
+        self.pauli_word = pauli_word
+        wl = list(wires) if isinstance(wires, (list, tuple)) else [wires]
+        if len(wl) != len(pauli_word):
+            raise ValueError(f"PauliRot word {pauli_word!r} needs {len(pauli_word)} wires, got {wl}")
+        super().__init__(wires=wires, **kwargs)
+        self._structure = GateStructure(diagonal=set(pauli_word) <= {"I", "Z"})
+
+    @property
+    def matrix(self):
+        return _rot_matrix(self.theta, _pauli_tensor(self.pauli_word)).to(cdtype())
+
+    def lower(self):
+        return [Member("rot", self.theta, self.pauli_word)]
+
+    def generator(self) -> Operation:
+        return Hermitian(matrix=_pauli_tensor(self.pauli_word), wires=self.wires, record=False)
+
+
+def _make_pauli_rotation_subclass(name: str, word: str):
+    class _Sub(PauliRot):
+        __doc__ = f"{name}(theta) = exp(-i theta/2 {word}) (gateset.py:720-758)."
+        _num_wires = len(word)
+
+        def __init__(self, theta, wires=None, **kwargs) -> None:
+            if wires is None:
+                wires = list(range(len(word)))
+            super().__init__(theta, word, wires=wires, **kwargs)
+
+    _Sub.__name__ = name
+    _Sub.__qualname__ = name
+    return _Sub
+
+
+RXX = _make_pauli_rotation_subclass("RXX", "XX")
+RYY = _make_pauli_rotation_subclass("RYY", "YY")
+RZZ = _make_pauli_rotation_subclass("RZZ", "ZZ")
+RZX = _make_pauli_rotation_subclass("RZX", "ZX")
+
+
+class ControlledPauliRot(Operation):
+    """Multi-controlled Pauli-word rotation (gateset.py:764-834)."""
+
+    _param_names = ("theta",)
+    is_unitary = True
+    is_controlled = True
+
+    def __init__(self, theta, pauli_word: str, wires: List[int], n_controls: int = 1, **kwargs) -> None:
+        self.theta = theta
+        self.pauli_word = pauli_word
+        self.n_controls = n_controls
+        wl = [wires] if isinstance(wires, int) else list(wires)
+        if len(wl) != n_controls + len(pauli_word):
+            raise ValueError(
+                f"ControlledPauliRot expects {n_controls + len(pauli_word)} wires "
+                f"({n_controls} control + {len(pauli_word)} target), got {len(wl)}."
+            )
+        super().__init__(wires=wl, **kwargs)
+        self._structure = GateStructure(diagonal=set(pauli_word) <= {"I", "Z"}, controls=n_controls)
+
+    @property
+    def matrix(self):
+        r = _rot_matrix(self.theta, _pauli_tensor(self.pauli_word))
+        return _controlled_tensor(r, self.n_controls).to(cdtype())
+
+    def lower(self):
+        return [Member("rot", self.theta, self.pauli_word, self.n_controls)]
+
+    def generator(self) -> Operation:
+        p = _pauli_tensor(self.pauli_word)
+        dt = p.shape[0]
+        dim = (2**self.n_controls) * dt
+        gen = np.zeros((dim, dim), dtype=_C)
+        gen[dim - dt :, dim - dt :] = p
+        return Hermitian(matrix=gen, wires=self.wires, record=False)
+
+
+def _make_controlled_rotation_subclass(name: str, axis: str):
+    class _CRot(ControlledPauliRot):
+        __doc__ = f"Controlled R{axis} (gateset.py:837-890)."
+        _num_wires = 2
+
+        def __init__(self, theta, wires: List[int] = [0, 1], **kwargs) -> None:
+            super().__init__(theta, axis, wires=wires, n_controls=1, **kwargs)
+
+    _CRot.__name__ = name
+    _CRot.__qualname__ = name
+    return _CRot
+
+
+CRX = _make_controlled_rotation_subclass("CRX", "X")
+CRY = _make_controlled_rotation_subclass("CRY", "Y")
+CRZ = _make_controlled_rotation_subclass("CRZ", "Z")
+
+
+def build_parity_observable(qubit_group: List[int]) -> Hermitian:
+    """Z-parity observable on *qubit_group* (gateset.py:892-910)."""
+    mat = reduce(np.kron, [PauliZ._matrix] * len(qubit_group))
+    obs = Hermitian(matrix=mat, wires=qubit_group, record=False)
+    obs._pauli_label = "Z" * len(qubit_group)
+    return obs

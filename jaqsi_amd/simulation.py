@@ -1,0 +1,345 @@
+"""Host side of the simulation seam.
+
+``simulate_and_measure`` keeps the signature and behaviour of
+jaqsi/simulation.py:658-732: statevector or density-matrix simulation by
+whether the tape holds a channel, the adjoint-method VJP for exact
+expectation values, and the measurement types "expval", "probs", "state"
+and "density".  Everything numeric runs in libjaqsi_amd.so on the GPU; this
+module lowers the tape, keeps the native plans cached per circuit
+structure and moves tensors across the C-ABI.  There is no CPU path: if no
+GPU or no built library is present, the calls raise.
+"""
+
+from __future__ import annotations
+
+import threading
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from jaqsi_amd import _native as nat
+from jaqsi_amd import lowering
+from jaqsi_amd.gateset import Barrier
+from jaqsi_amd.noise import KrausChannel
+from jaqsi_amd.operations import Operation, x64_enabled
+
+_plans: dict = {}
+_plans_lock = threading.Lock()
+_PLAN_CACHE_MAX = 256
+
+
+def infer_n_qubits(ops: List[Operation], obs: List[Operation]) -> int:
+    """max wire + 1 over the tape and observables (simulation.py:32-47)."""
+    wires = set()
+    for op in list(ops) + list(obs):
+        wires.update(op.wires)
+    return max(wires) + 1 if wires else 1
+
+
+def has_noise(tape: List[Operation]) -> bool:
+    """A channel on the tape requires density-matrix simulation (simulation.py:50-62)."""
+    return any(isinstance(op, KrausChannel) for op in tape)
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("jaqsi_amd simulates on the GPU only; no GPU is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _dtype_code() -> int:
+    return 1 if x64_enabled() else 0
+
+
+def _cdtype_torch(code: int) -> torch.dtype:
+    return torch.complex128 if code else torch.complex64
+
+
+def get_plan(low: lowering.Lowered, dtype: int, tile_bits: int = 0, checkpoints: int = 1) -> nat.Plan:
+    key = (low.key, dtype, tile_bits, checkpoints)
+    with _plans_lock:
+        p = _plans.get(key)
+        if p is None:
+            p = nat.Plan(low.n_wires, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, dtype,
+                         tile_bits, checkpoints)
+            if len(_plans) >= _PLAN_CACHE_MAX:
+                _plans.pop(next(iter(_plans)))
+            _plans[key] = p
+        return p
+
+
+# ---------------------------------------------------------------------------
+# observables
+# ---------------------------------------------------------------------------
+def diag_observables(obs: Sequence[Operation]) -> Optional[List[Tuple[int, float, float]]]:
+    """(wire, d0, d1) for single-qubit diagonal observables, else None.
+
+    Same test as the reference fast path (simulation.py:772-801): the class
+    matrix is used, so instances with a custom matrix take the general path.
+    """
+    out = []
+    for ob in obs:
+        m = ob.__class__._matrix
+        if m is None or len(ob.wires) != 1 or getattr(ob, "_explicit", False):
+            return None
+        m = np.asarray(m)
+        if m.shape != (2, 2) or not np.allclose(m - np.diag(np.diag(m)), 0):
+            return None
+        d = np.real(np.diag(m))
+        out.append((int(ob.wires[0]), float(d[0]), float(d[1])))
+    return out
+
+
+def _obs_arrays(diags):
+    wires = np.array([w for w, _, _ in diags], dtype=np.int32)
+    d = np.array([[a, b] for _, a, b in diags], dtype=np.float64).reshape(-1)
+    return wires, d
+
+
+def _obs_matrix(ob: Operation) -> np.ndarray:
+    m = ob.matrix
+    m = m.detach().cpu().numpy() if isinstance(m, torch.Tensor) else np.asarray(m)
+    return np.ascontiguousarray(m.astype(np.complex128))
+
+
+# ---------------------------------------------------------------------------
+# the autograd bridge (the custom_vjp of simulation.py:586-654)
+# ---------------------------------------------------------------------------
+class _Runner:
+    def __init__(self, plan, batch, density, diags, n_qubits, psi0_batched, dtype):
+        self.plan = plan
+        self.batch = batch
+        self.density = density
+        self.diags = diags
+        self.n_qubits = n_qubits
+        self.psi0_batched = psi0_batched
+        self.dtype = dtype
+        self.obs_kind = nat.JQ_OBS_DM_DIAG if density else nat.JQ_OBS_SV_DIAG
+        self.wires, self.d = _obs_arrays(diags)
+
+    def vec_len(self):
+        return 1 << self.plan.n_wires
+
+    def forward(self, angles, mats, psi0):
+        dev = angles.device
+        state = torch.empty((self.batch, self.vec_len()), dtype=_cdtype_torch(self.dtype), device=dev)
+        ev = torch.empty((self.batch, len(self.diags)), dtype=torch.float64, device=dev)
+        nat.check(nat.lib().jq_sv_forward(
+            self.plan.handle, self.batch, nat.ptr(angles), nat.ptr(mats), nat.ptr(psi0), int(self.psi0_batched),
+            nat.ptr(state), self.obs_kind, len(self.diags), nat.ptr(self.wires), nat.ptr(self.d), nat.ptr(ev),
+            _stream()))
+        return ev, state
+
+    def adjoint(self, angles, mats, psi0, ct, want_psi0):
+        dev = angles.device
+        state = torch.empty((self.batch, self.vec_len()), dtype=_cdtype_torch(self.dtype), device=dev)
+        ev = torch.empty((self.batch, len(self.diags)), dtype=torch.float64, device=dev)
+        ga = torch.empty_like(angles)
+        gm = torch.empty_like(mats) if mats is not None else None
+        gp = torch.empty((self.batch, self.vec_len()), dtype=_cdtype_torch(self.dtype), device=dev) if want_psi0 else None
+        nat.check(nat.lib().jq_sv_adjoint(
+            self.plan.handle, self.batch, nat.ptr(angles), nat.ptr(mats), nat.ptr(psi0), int(self.psi0_batched),
+            nat.ptr(state), self.obs_kind, len(self.diags), nat.ptr(self.wires), nat.ptr(self.d), nat.ptr(ct),
+            nat.ptr(ev), nat.ptr(ga), nat.ptr(gm), nat.ptr(gp), _stream()))
+        return ev, ga, gm, gp
+
+
+class _AdjointExpval(torch.autograd.Function):
+    """Expectation values whose backward is the native adjoint sweep."""
+
+    @staticmethod
+    def forward(ctx, angles, mats, psi0, runner):
+        ev, _ = runner.forward(angles, mats, psi0)
+        ctx.runner = runner
+        ctx.save_for_backward(angles, mats if mats is not None else torch.empty(0), psi0 if psi0 is not None else torch.empty(0))
+        ctx.has_mats = mats is not None
+        ctx.has_psi0 = psi0 is not None
+        return ev
+
+    @staticmethod
+    def backward(ctx, g):
+        angles, mats, psi0 = ctx.saved_tensors
+        mats = mats if ctx.has_mats else None
+        psi0 = psi0 if ctx.has_psi0 else None
+        r = ctx.runner
+        ct = g.to(torch.float64).contiguous()
+        want_psi0 = psi0 is not None and ctx.needs_input_grad[2]
+        _, ga, gm, gp = r.adjoint(angles, mats, psi0, ct, want_psi0)
+        # JAX-convention cotangents -> PyTorch's convention (conjugate)
+        gm = gm.conj() if gm is not None else None
+        if gp is not None:
+            gp = gp.conj()
+            if not r.psi0_batched:
+                gp = gp.sum(0)
+            gp = gp.reshape(psi0.shape)
+        return ga, gm, gp, None
+
+
+# ---------------------------------------------------------------------------
+# the seam
+# ---------------------------------------------------------------------------
+def _prepare(tape, n_qubits, use_density, batch, initial_state, dtype, values=None):
+    low = lowering.lower_mixed(tape, n_qubits) if use_density else lowering.lower_pure(tape, n_qubits)
+    dev = device()
+    if values is None:
+        angles = lowering.angle_tensor(low.angle_values, batch)
+        mats = lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, batch)
+    else:
+        angles, mats = values
+    if batch is None:
+        angles = angles.reshape(1, -1)
+        mats = mats.reshape(1, -1) if mats is not None else None
+    angles = angles.to(dev, torch.float64).contiguous()
+    mats = mats.to(dev, torch.complex128).contiguous() if mats is not None else None
+    psi0 = None
+    psi0_batched = False
+    if initial_state is not None:
+        psi0 = torch.as_tensor(initial_state)
+        psi0 = psi0.to(dev, _cdtype_torch(dtype))
+        if use_density:
+            # rho0 = |psi><psi| (simulation.py:467-469)
+            p = psi0.reshape(-1, 1 << n_qubits)
+            psi0 = (p[:, :, None] * p.conj()[:, None, :]).reshape(p.shape[0], -1)
+        if psi0.dim() == 2 and psi0.shape[0] > 1:
+            psi0_batched = True
+        psi0 = psi0.reshape(psi0.shape[0] if psi0_batched else 1, -1).contiguous()
+    plan = get_plan(low, dtype)
+    return low, plan, angles, mats, psi0, psi0_batched
+
+
+def simulate_and_measure(
+    tape: List[Operation],
+    n_qubits: int,
+    type: str,
+    obs: List[Operation],
+    use_density: bool,
+    shots: Optional[int] = None,
+    key=None,
+    initial_state=None,
+    adjoint: bool = True,
+    batch: Optional[int] = None,
+    values=None,
+):
+    """Run simulation and measurement in one dispatch (simulation.py:658-732).
+
+    ``batch``/``values`` carry a batched recording (Script with in_axes):
+    ``values`` = (angles (B, P), matrices (B, E)) and results gain a leading
+    batch axis.
+    """
+    if type not in ("expval", "probs", "state", "density"):
+        raise ValueError(f"Unknown measurement type: {type!r}")
+    if use_density and type == "state":
+        raise ValueError(
+            "Measurement type 'state' is not defined for mixed (noisy) circuits. Use 'density' instead."
+        )
+    tape = [op for op in tape if not isinstance(op, Barrier)]
+    if any(getattr(op, "_pending", False) for op in tape):
+        from jaqsi_amd.evolution import resolve_pending
+
+        resolve_pending(tape)
+    dtype = _dtype_code()
+    low, plan, angles, mats, psi0, psi0_batched = _prepare(tape, n_qubits, use_density, batch, initial_state, dtype, values)
+    B = angles.shape[0]
+    rdt = torch.float64 if dtype else torch.float32
+
+    def unbatch(x):
+        return x if batch is not None else x[0]
+
+    if type == "expval" and shots is None:
+        diags = diag_observables(obs)
+        if diags is not None and len(diags) > 0:
+            runner = _Runner(plan, B, use_density, diags, n_qubits, psi0_batched, dtype)
+            ev = _AdjointExpval.apply(angles, mats, psi0, runner)
+            return unbatch(ev.to(rdt))
+        # general observables: forward simulation + matrix expectation values
+        if any(t is not None and t.requires_grad for t in (angles, mats, psi0)):
+            raise NotImplementedError(
+                "gradients through non-diagonal observables are not supported by the GPU path yet"
+            )
+        state = _forward_state(plan, B, angles, mats, psi0, psi0_batched, dtype)
+        out = torch.zeros((B, len(obs)), dtype=torch.float64, device=angles.device)
+        for i, ob in enumerate(obs):
+            m = torch.as_tensor(_obs_matrix(ob)).to(angles.device)
+            col = torch.zeros(B, dtype=torch.float64, device=angles.device)
+            w = np.array(ob.wires, dtype=np.int32)
+            nat.check(nat.lib().jq_expval_matrix(n_qubits, B, dtype, int(use_density), nat.ptr(state), len(ob.wires),
+                                                 nat.ptr(w), nat.ptr(m), nat.ptr(col), _stream()))
+            out[:, i] = col
+        return unbatch(out.to(rdt))
+
+    if any(t is not None and t.requires_grad for t in (angles, mats, psi0)) and torch.is_grad_enabled():
+        raise NotImplementedError(
+            f"gradients of {type!r} outputs are not supported by the GPU path; use type='expval'"
+        )
+    state = _forward_state(plan, B, angles, mats, psi0, psi0_batched, dtype)
+    dim = 1 << n_qubits
+    if shots is not None and type in ("probs", "expval"):
+        probs = _probs(state, n_qubits, use_density, B, dtype)
+        return unbatch(sample_shots(probs, n_qubits, type, obs, shots, key))
+    if type == "state":
+        return unbatch(state)
+    if type == "probs":
+        return unbatch(_probs(state, n_qubits, use_density, B, dtype))
+    # density
+    if use_density:
+        return unbatch(state.reshape(B, dim, dim))
+    rho = torch.empty((B, dim, dim), dtype=state.dtype, device=state.device)
+    nat.check(nat.lib().jq_sv_outer(n_qubits, B, dtype, nat.ptr(state), nat.ptr(rho), _stream()))
+    return unbatch(rho)
+
+
+def _forward_state(plan, B, angles, mats, psi0, psi0_batched, dtype):
+    state = torch.empty((B, 1 << plan.n_wires), dtype=_cdtype_torch(dtype), device=angles.device)
+    nat.check(nat.lib().jq_sv_forward(plan.handle, B, nat.ptr(angles), nat.ptr(mats), nat.ptr(psi0), int(psi0_batched),
+                                      nat.ptr(state), nat.JQ_OBS_NONE, 0, None, None, None, _stream()))
+    return state
+
+
+def _probs(state, n_qubits, density, B, dtype):
+    dim = 1 << n_qubits
+    out = torch.empty((B, dim), dtype=torch.float64 if dtype else torch.float32, device=state.device)
+    if density:
+        nat.check(nat.lib().jq_dm_diag(n_qubits, B, dtype, nat.ptr(state), nat.ptr(out), _stream()))
+    else:
+        nat.check(nat.lib().jq_sv_probs(n_qubits, B, dtype, nat.ptr(state), nat.ptr(out), _stream()))
+    return out
+
+
+def sample_shots(probs, n_qubits, type, obs, shots, key=None):
+    """Shot sampling from exact probabilities (simulation.py:860-917).
+
+    The reference draws with jax.random.choice; here torch.multinomial with a
+    generator seeded from *key*, so samples differ (statistical parity only).
+    """
+    g = torch.Generator(device=probs.device)
+    g.manual_seed(int(key) if key is not None else 0)
+    p = probs.to(torch.float64)
+    p = p / p.sum(-1, keepdim=True)
+    samples = torch.multinomial(p, shots, replacement=True, generator=g)
+    counts = torch.zeros_like(p)
+    counts.scatter_add_(1, samples, torch.ones_like(samples, dtype=p.dtype))
+    est = counts / shots
+    if type == "probs":
+        return est.to(probs.dtype)
+    if type == "expval":
+        cols = []
+        for ob in obs:
+            diag = torch.as_tensor(np.real(_lifted_diag(_obs_matrix(ob), ob.wires, n_qubits))).to(p.device)
+            cols.append(est @ diag)
+        return torch.stack(cols, dim=-1).to(probs.dtype)
+    raise ValueError(f"Shot simulation is only supported for 'probs' and 'expval', got {type!r}.")
+
+
+def _lifted_diag(m, wires, n):
+    """Diagonal of a small observable lifted to n qubits."""
+    idx = np.arange(1 << n)
+    k = len(wires)
+    r = np.zeros_like(idx)
+    for j, w in enumerate(wires):
+        r |= ((idx >> (n - 1 - w)) & 1) << (k - 1 - j)
+    return np.diag(m)[r]

@@ -1,0 +1,122 @@
+"""Unitary gate backend with optional noise (mirrors jaqsi/unitary.py).
+
+``UnitaryGates.<G>`` records the gate and then the channels named in
+``noise_params`` on each touched wire (unitary.py:81-128).  ``GateError``
+draws Gaussian angle noise; the reference uses the JAX PRNG, here a torch
+generator seeded from ``random_key`` (values differ: parity unpinned).
+"""
+
+from __future__ import annotations
+
+import itertools
+from typing import Dict, List, Optional, Union
+
+import numpy as np
+import torch
+
+from jaqsi_amd import gateset, noise
+from jaqsi_amd.operations import active_tape
+
+
+class UnitaryGates:
+    batch_gate_error = True
+
+    @staticmethod
+    def NQubitDepolarizingChannel(p: float, wires: List[int]) -> noise.QubitChannel:
+        """n-qubit depolarizing channel (unitary.py:21-79)."""
+        if not (0.0 <= p <= 1.0):
+            raise ValueError(f"Probability p must be between 0 and 1, got {p}")
+        n = len(wires)
+        if n < 2:
+            raise ValueError(f"Number of qubits must be >= 2, got {n}")
+        paulis = [np.eye(2), gateset.PauliX._matrix, gateset.PauliY._matrix, gateset.PauliZ._matrix]
+        dim = 2**n
+        ops = [np.sqrt(1 - p * (4**n - 1) / (4**n)) * np.eye(dim, dtype=np.complex128)]
+        for i, idx in enumerate(itertools.product(range(4), repeat=n)):
+            if i == 0:
+                continue
+            P = np.eye(1)
+            for j in idx:
+                P = np.kron(P, paulis[j])
+            ops.append(np.sqrt(p / (4**n)) * P)
+        return noise.QubitChannel(ops, wires=wires)
+
+    @staticmethod
+    def Noise(wires: Union[int, List[int]], noise_params: Optional[Dict[str, float]] = None) -> None:
+        """BitFlip / PhaseFlip / Depolarizing per wire, MultiQubitDepolarizing
+        on the tuple (unitary.py:81-128)."""
+        if noise_params is None:
+            return
+        if isinstance(wires, int):
+            wires = [wires]
+        for wire in wires:
+            bf = noise_params.get("BitFlip", 0.0)
+            if bf > 0:
+                noise.BitFlip(bf, wires=wire)
+            pf = noise_params.get("PhaseFlip", 0.0)
+            if pf > 0:
+                noise.PhaseFlip(pf, wires=wire)
+            dp = noise_params.get("Depolarizing", 0.0)
+            if dp > 0:
+                noise.DepolarizingChannel(dp, wires=wire)
+        if len(wires) > 1:
+            p = noise_params.get("MultiQubitDepolarizing", 0.0)
+            if p > 0:
+                UnitaryGates.NQubitDepolarizingChannel(p, wires)
+
+    @staticmethod
+    def GateError(w, noise_params=None, random_key=None):
+        """Gaussian angle noise of std ``noise_params["GateError"]``
+        (unitary.py:130-186)."""
+        if noise_params is not None and noise_params.get("GateError", None) is not None:
+            assert random_key is not None, "A random_key must be provided when using GateError"
+            tape = active_tape()
+            position = len(tape) if tape is not None else 0
+            g = torch.Generator().manual_seed(int(random_key) * 1000003 + position)
+            wt = torch.as_tensor(w, dtype=torch.float64)
+            noise_v = torch.randn(wt.shape, generator=g, dtype=torch.float64) * noise_params["GateError"]
+            w = wt + noise_v
+            random_key = int(random_key) + 1
+        return w, random_key
+
+    @staticmethod
+    def Rot(phi, theta, omega, wires, noise_params=None, random_key=None):
+        if noise_params is not None and "GateError" in noise_params:
+            phi, random_key = UnitaryGates.GateError(phi, noise_params, random_key)
+            theta, random_key = UnitaryGates.GateError(theta, noise_params, random_key)
+            omega, random_key = UnitaryGates.GateError(omega, noise_params, random_key)
+        gateset.Rot(phi, theta, omega, wires=wires)
+        UnitaryGates.Noise(wires, noise_params)
+
+    @staticmethod
+    def PauliRot(theta, pauli: str, wires, noise_params=None, random_key=None):
+        if noise_params is not None and "GateError" in noise_params:
+            theta, random_key = UnitaryGates.GateError(theta, noise_params, random_key)
+        gateset.PauliRot(theta, pauli, wires=wires)
+        UnitaryGates.Noise(wires, noise_params)
+
+
+def _rotation(cls_name: str):
+    def gate(w, wires, noise_params=None, random_key=None):
+        w, random_key = UnitaryGates.GateError(w, noise_params, random_key)
+        getattr(gateset, cls_name)(w, wires=wires)
+        UnitaryGates.Noise(wires, noise_params)
+
+    gate.__name__ = cls_name
+    return staticmethod(gate)
+
+
+def _fixed(cls_name: str):
+    def gate(wires, noise_params=None, random_key=None):
+        getattr(gateset, cls_name)(wires=wires)
+        UnitaryGates.Noise(wires, noise_params)
+
+    gate.__name__ = cls_name
+    return staticmethod(gate)
+
+
+for _n in ("RX", "RY", "RZ", "CRX", "CRY", "CRZ", "RXX", "RYY", "RZZ", "RZX"):
+    setattr(UnitaryGates, _n, _rotation(_n))
+UnitaryGates.CPhase = _rotation("ControlledPhaseShift")
+for _n in ("CX", "CY", "CZ", "H", "CCX", "CSWAP", "SWAP", "PauliX", "PauliY", "PauliZ", "S", "Id"):
+    setattr(UnitaryGates, _n, _fixed(_n))

@@ -1,0 +1,119 @@
+"""GPU parity of the statevector path against the complex128 oracle."""
+
+import numpy as np
+import pytest
+import torch
+
+from circuits import hea_theta, record_hea, record_zoo, zoo_ops, zoo_params
+from oracle import sv
+
+pytestmark = pytest.mark.gpu
+
+TOL = {False: 2e-5, True: 1e-11}  # complex64 / complex128 against the c128 oracle
+
+
+def _script_expval(n, theta, x64, obs_wires=None):
+    import jaqsi_amd as J
+
+    J.enable_x64(x64)
+    th = torch.tensor(theta, dtype=torch.float64, device="cuda", requires_grad=True)
+    obs = [J.PauliZ(wires=q, record=False) for q in (obs_wires if obs_wires is not None else range(n))]
+    script = J.Script(lambda t: record_hea(t, n), n_qubits=n)
+    ev = script.execute(type="expval", obs=obs, args=(th,))
+    ev.sum().backward()
+    return ev.detach().cpu().numpy(), th.grad.detach().cpu().numpy().reshape(-1)
+
+
+@pytest.mark.parametrize("x64", [False, True])
+@pytest.mark.parametrize("n,layers", [(2, 3), (5, 2), (10, 5), (14, 2), (16, 2)])
+def test_hea_expval_and_adjoint(n, layers, x64):
+    print(f"hea n={n} layers={layers} x64={x64}", flush=True)
+    theta = hea_theta(n, layers, 1000 + n)
+    ev, grad = _script_expval(n, theta, x64)
+    ops = sv.hea_ops(theta, n)
+    vals, grads, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
+    np.testing.assert_allclose(ev, vals, atol=TOL[x64], rtol=0)
+    np.testing.assert_allclose(grad, sv.flat_grads(ops, grads), atol=TOL[x64] * 10, rtol=0)
+
+
+@pytest.mark.parametrize("x64", [False, True])
+def test_gate_zoo_state_and_gradients(x64):
+    import jaqsi_amd as J
+
+    print(f"zoo x64={x64}", flush=True)
+    J.enable_x64(x64)
+    p = zoo_params()
+    ops = zoo_ops(p)
+    ref = sv.simulate(ops, 5)
+    st = J.Script(lambda: record_zoo(p), n_qubits=5).execute(type="state").cpu().numpy()
+    np.testing.assert_allclose(st, ref, atol=TOL[x64])
+    pt = torch.tensor(p, dtype=torch.float64, device="cuda", requires_grad=True)
+    obs = [J.PauliZ(wires=q, record=False) for q in range(5)]
+    ev = J.Script(record_zoo, n_qubits=5).execute(type="expval", obs=obs, args=(pt,))
+    ev.sum().backward()
+    vals, grads, _, _ = sv.adjoint(ops, 5, sv.z_obs(5))
+    np.testing.assert_allclose(ev.detach().cpu().numpy(), vals, atol=TOL[x64])
+    flat = sv.flat_grads(ops, grads)
+    # oracle parameters in tape order: p[0..11], then the last RZ's angle 0.5 * p[0]
+    expect = flat[:12].copy()
+    expect[0] += 0.5 * flat[12]
+    np.testing.assert_allclose(pt.grad.cpu().numpy(), expect, atol=TOL[x64] * 10)
+
+
+def test_batched_in_axes_matches_loop():
+    import jaqsi_amd as J
+
+    n = 6
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-np.pi, np.pi, (9, n))
+    theta = rng.uniform(0, 2 * np.pi, (2, n, 3))
+    obs = [J.PauliZ(wires=q, record=False) for q in range(n)]
+
+    def circ(xi, th):
+        record_hea(th, n, features=xi)
+
+    s = J.Script(circ, n_qubits=n)
+    xt = torch.tensor(x, device="cuda")
+    tht = torch.tensor(theta, device="cuda", requires_grad=True)
+    ev = s.execute(type="expval", obs=obs, args=(xt, tht), in_axes=(0, None))
+    assert ev.shape == (9, n)
+    ev.sum().backward()
+    g = np.zeros(theta.size)
+    for b in range(9):
+        ops = sv.hea_ops(theta, n, features=x[b])
+        vals, grads, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
+        np.testing.assert_allclose(ev[b].detach().cpu().numpy(), vals, atol=2e-5)
+        g += sv.flat_grads(ops, grads)[n:]
+    np.testing.assert_allclose(tht.grad.cpu().numpy().reshape(-1), g, atol=2e-4)
+
+
+def test_probs_state_density_outputs():
+    import jaqsi_amd as J
+
+    n = 4
+    theta = hea_theta(n, 2, 5)
+    ref = sv.simulate(sv.hea_ops(theta, n), n)
+    s = J.Script(lambda: record_hea(theta, n), n_qubits=n)
+    np.testing.assert_allclose(s.execute(type="state").cpu().numpy(), ref, atol=2e-6)
+    np.testing.assert_allclose(s.execute(type="probs").cpu().numpy(), np.abs(ref) ** 2, atol=2e-6)
+    np.testing.assert_allclose(s.execute(type="density").cpu().numpy(), np.outer(ref, ref.conj()), atol=2e-6)
+
+
+def test_initial_state_and_its_cotangent():
+    import jaqsi_amd as J
+
+    J.enable_x64(True)
+    n = 3
+    rng = np.random.default_rng(11)
+    psi0 = rng.normal(size=2**n) + 1j * rng.normal(size=2**n)
+    psi0 /= np.linalg.norm(psi0)
+    theta = hea_theta(n, 1, 2)
+    p0 = torch.tensor(psi0, device="cuda", requires_grad=True)
+    obs = [J.PauliZ(wires=q, record=False) for q in range(n)]
+    ev = J.Script(lambda: record_hea(theta, n), n_qubits=n).execute(type="expval", obs=obs, initial_state=p0)
+    ev.sum().backward()
+    ops = sv.hea_ops(theta, n)
+    vals, _, _, lam0 = sv.adjoint(ops, n, sv.z_obs(n), psi0=psi0)
+    np.testing.assert_allclose(ev.detach().cpu().numpy(), vals, atol=1e-11)
+    # PyTorch's cotangent is the conjugate of JAX's
+    np.testing.assert_allclose(p0.grad.cpu().numpy(), np.conj(lam0), atol=1e-11)
