@@ -1,0 +1,214 @@
+"""Benchmark of the adjoint-gradient hot path on one MI355X.
+
+Workload (default): 24-qubit statevector, complex64, hardware-efficient
+ansatz with 8 layers (RX, RY, RZ on every wire, then the CX ring
+q -> q+1 mod n), 576 angles uniform[0, 2pi) from numpy default_rng(1002);
+one step = one forward pass plus the full adjoint gradient of
+sum_q <Z_q> w.r.t. all 576 angles, with the angles already resident in HBM.
+
+Prints ONE JSON line with the metric, the roofline of the dominant kernel
+(HIP-event timed inside the timed region) and the CPU baseline (the
+C++/OpenMP restatement of the reference algorithm timed on this host).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "adjoint-grad circuits/s at 24q & batch 4096×14q; HBM GB/s vs peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def record_hea(theta, n, features=None):
+    from jaqsi_amd import Gates
+
+    if features is not None:
+        for q in range(n):
+            Gates.RX(float(features[q]), wires=q)
+    for layer in range(theta.shape[0]):
+        for q in range(n):
+            Gates.RX(float(theta[layer, q, 0]), wires=q)
+            Gates.RY(float(theta[layer, q, 1]), wires=q)
+            Gates.RZ(float(theta[layer, q, 2]), wires=q)
+        for q in range(n):
+            Gates.CX(wires=[q, (q + 1) % n])
+
+
+def build_plan(n, layers, features=False, tile_bits=0):
+    from jaqsi_amd import lowering
+    from jaqsi_amd import _native as nat
+    from jaqsi_amd.tape import recording
+
+    theta = np.zeros((layers, n, 3))
+    with recording() as tape:
+        record_hea(theta, n, np.zeros(n) if features else None)
+    low = lowering.lower_pure(tape, n)
+    plan = nat.Plan(n, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0, tile_bits, 1)
+    mats = lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, None)
+    return plan, mats
+
+
+def bench_c2(steps, warmup, n=24, layers=8):
+    import torch
+
+    from jaqsi_amd import _native as nat
+
+    L = nat.lib()
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(1002)
+    theta = rng.uniform(0, 2 * np.pi, (layers, n, 3))
+    plan, mats = build_plan(n, layers)
+    mats = mats.reshape(1, -1).to(dev) if mats is not None else None
+    info = plan.info()
+    angles = torch.tensor(theta.reshape(1, -1), dtype=torch.float64, device=dev)
+    ct = torch.ones((1, n), dtype=torch.float64, device=dev)
+    state = torch.empty((1, 1 << n), dtype=torch.complex64, device=dev)
+    ev = torch.empty((1, n), dtype=torch.float64, device=dev)
+    ga = torch.empty_like(angles)
+    wires = np.arange(n, dtype=np.int32)
+    d = np.tile(np.array([1.0, -1.0]), n)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def step():
+        nat.check(L.jq_sv_adjoint(plan.handle, 1, nat.ptr(angles), nat.ptr(mats), None, 0, nat.ptr(state), nat.JQ_OBS_SV_DIAG,
+                                  n, nat.ptr(wires), nat.ptr(d), nat.ptr(ct), nat.ptr(ev), nat.ptr(ga), None, None,
+                                  stream))
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    nat.check(L.jq_timing_enable(1))
+    for w in range(3):
+        nat.check(L.jq_timing_read(w, None, None))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    nat.check(L.jq_timing_enable(0))
+    import ctypes as C
+
+    kt = {}
+    for w, name in ((0, "sv_forward_kernel"), (1, "sv_backward_kernel")):
+        tot, cnt = C.c_double(), C.c_int32()
+        nat.check(L.jq_timing_read(w, C.byref(tot), C.byref(cnt)))
+        kt[name] = (tot.value, cnt.value)
+    return {
+        "theta": theta,
+        "seconds": t1 - t0,
+        "info": info,
+        "kernels": kt,
+        "expvals": ev.cpu().numpy()[0],
+        "grads": ga.cpu().numpy()[0],
+        "n": n,
+    }
+
+
+def roofline_c2(res):
+    n = res["n"]
+    sweep = 2 * 8 * (1 << n)  # read + write of one complex64 vector
+    bytes_per = {"sv_forward_kernel": sweep, "sv_backward_kernel": sweep * 3 // 2}
+    dom = max(res["kernels"], key=lambda k: res["kernels"][k][0])
+    tot, cnt = res["kernels"][dom]
+    avg_s = tot / 1e3 / max(cnt, 1)
+    achieved = bytes_per[dom] / avg_s / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_c2.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get(dom)
+    others = {k: {"avg_us": round(v[0] / max(v[1], 1) * 1e3, 2), "launches": v[1]} for k, v in res["kernels"].items()}
+    return {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": bytes_per[dom],
+        "kernels": others,
+    }
+
+
+def cpu_baseline_c2(theta):
+    from oracle import cpu
+
+    cores = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    cpu.hea_grad(theta, None, cores)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(1.0 / dt, 5),
+        "unit": "gradient evaluations/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": "one forward + adjoint gradient of the full 24-qubit, 8-layer workload (complex64), "
+        "C++/OpenMP restatement of the reference plan (same-wire fusion, composed CX-ring gather)",
+        "seconds": round(dt, 2),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            "bench.py measures one GPU: every run in this build uses a single MI355X; "
+            "the multi-GPU design (batch / qubit sharding over RCCL) is described in DESIGN.md",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    res = bench_c2(args.steps, args.warmup)
+    ms = res["seconds"] / args.steps * 1e3
+    value = args.steps / res["seconds"]
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "gradient evaluations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": "statevector 24 qubits, hardware-efficient ansatz 8 layers (RX,RY,RZ per wire + CX ring), "
+            "576 angles uniform[0,2pi) seed 1002, forward + full adjoint gradient of sum_q <Z_q>",
+            "n_qubits": 24,
+            "layers": 8,
+            "params": 576,
+            "passes": res["info"]["n_passes"],
+            "tile_bits": res["info"]["tile_bits"],
+        },
+        "roofline": roofline_c2(res),
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_c2(res["theta"])
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -173,6 +173,12 @@ typedef struct jq_pulse_desc {
 int jq_pulse_propagate(const jq_pulse_desc* desc, const double* params, const double* target,
                        double* unitaries, double* infidelity, double* grad, void* stream);
 
+/* Kernel timing for measurement: when enabled, every tile-pass launch is
+ * bracketed by HIP events on its stream; jq_timing_read sums and clears
+ * them (which = 0 forward, 1 backward, 2 fused).  No reference counterpart. */
+int jq_timing_enable(int32_t on);
+int jq_timing_read(int32_t which, double* total_ms, int32_t* count);
+
 /* Thread-local message of the last failing call. */
 const char* jq_last_error(void);
 

@@ -98,6 +98,8 @@ EXPORTS = [
     "jq_dm_diag",
     "jq_expval_matrix",
     "jq_pulse_propagate",
+    "jq_timing_enable",
+    "jq_timing_read",
     "jq_last_error",
     "jq_version",
 ]
@@ -135,6 +137,8 @@ def lib():
         L.jq_sv_outer.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp]
         L.jq_dm_diag.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp]
         L.jq_expval_matrix.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int32, vp, vp, vp, vp]
+        L.jq_timing_enable.argtypes = [C.c_int32]
+        L.jq_timing_read.argtypes = [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
         L.jq_last_error.restype = C.c_char_p
         L.jq_version.restype = C.c_char_p
         for name in EXPORTS:

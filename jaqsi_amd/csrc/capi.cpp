@@ -17,6 +17,26 @@
 namespace {
 thread_local std::string g_err;
 
+// Optional per-kernel timing: HIP events around every tile-pass launch.
+struct KTimer {
+  bool on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
+};
+KTimer g_timer;
+
+hipError_t timed_launch(int which, int dtype, int T, const jq::SvArgs& a, int ntiles, int batch, size_t smem,
+                        hipStream_t st) {
+  if (!g_timer.on) return jq::launch_sv(which, dtype, T, a, ntiles, batch, smem, st);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, st);
+  hipError_t e = jq::launch_sv(which, dtype, T, a, ntiles, batch, smem, st);
+  hipEventRecord(e1, st);
+  g_timer.ev[which].push_back({e0, e1});
+  return e;
+}
+
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -63,12 +83,12 @@ struct jq_plan {
   int n_params = 0;
   int64_t n_matrix_elems = 0;
   int checkpoints = 1;
-  DevBuf prog, block_table, member_list, member_table, member_values, slot_map;
+  DevBuf prog, prog_b, block_table, member_list, member_table, member_values, slot_map;
   int n_slots = 0;
   std::vector<int32_t> slot_words;
   bool uploaded = false;
   // workspaces
-  DevBuf pool, partials, bars, ev_part, lam, ckpt, status;
+  DevBuf pool, partials, bars, ev_part, lam, ckpt, status, obs;
 };
 
 extern "C" {
@@ -84,13 +104,16 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   if (desc->dtype != 0 && desc->dtype != 1) return fail(JQ_ERR_ARG, "dtype must be 0 (complex64) or 1 (complex128)");
   if (desc->n_ops < 0 || (desc->n_ops > 0 && !desc->ops)) return fail(JQ_ERR_ARG, "bad op list");
   int T = desc->tile_bits;
+  // forward: 32 (complex64) / 16 (complex128) amplitudes per thread; the
+  // backward holds psi and lambda, so it uses half as many per thread.
   const int RB = desc->dtype == 0 ? 5 : 4;
+  const int RBb = RB - 1;
   if (T == 0) T = desc->dtype == 0 ? 13 : 12;
   const bool okT = desc->dtype == 0 ? (T == 13) : (T == 12);
   if (!okT) return fail(JQ_ERR_ARG, "tile_bits must be 13 (complex64) or 12 (complex128)");
   jq_plan* p = new jq_plan();
   try {
-    jq::build_program(*desc, T, RB, p->hp);
+    jq::build_program(*desc, T, RB, RBb, p->hp);
   } catch (const jq::PlanError& e) {
     delete p;
     return fail(e.code, e.what());
@@ -107,7 +130,7 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   const int* W = p->hp.words.data();
   for (int q = 0; q < p->hp.n_passes; ++q) {
     const int* pass = W + W[jq::HDR_WORDS + q];
-    if (lds_state + (64 + (size_t)pass[2]) * 8 > 160 * 1024) {
+    if (lds_state + 64 * 4 + 128 * 8 + (64 + (size_t)pass[2]) * 8 > 160 * 1024) {
       delete p;
       return fail(JQ_ERR_UNSUPPORTED, "too many gradient slots in one pass for LDS");
     }
@@ -127,8 +150,8 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
 
 int jq_plan_destroy(jq_plan* p) {
   if (!p) return JQ_OK;
-  for (DevBuf* b : {&p->prog, &p->block_table, &p->member_list, &p->member_table, &p->member_values, &p->slot_map,
-                    &p->pool, &p->partials, &p->bars, &p->ev_part, &p->lam, &p->ckpt, &p->status})
+  for (DevBuf* b : {&p->prog, &p->prog_b, &p->block_table, &p->member_list, &p->member_table, &p->member_values, &p->slot_map,
+                    &p->pool, &p->partials, &p->bars, &p->ev_part, &p->lam, &p->ckpt, &p->status, &p->obs})
     b->release();
   delete p;
   return JQ_OK;
@@ -163,6 +186,7 @@ int jq_plan_tables(const jq_plan* p, int32_t which, int32_t* out, int64_t capaci
     case 1: v = &p->hp.member_list; break;
     case 2: v = &p->hp.member_table; break;
     case 3: v = &p->slot_words; break;
+    case 4: v = &p->hp.words_b; break;
     default: return fail(JQ_ERR_ARG, "unknown table");
   }
   if (size) *size = (int64_t)v->size();
@@ -181,6 +205,7 @@ int ensure_uploaded(jq_plan* p) {
   if (p->uploaded) return JQ_OK;
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = upload(p->prog, p->hp.words);
+  if (e == hipSuccess) e = upload(p->prog_b, p->hp.words_b);
   if (e == hipSuccess) e = upload(p->block_table, p->hp.block_table);
   if (e == hipSuccess) e = upload(p->member_list, p->hp.member_list);
   if (e == hipSuccess) e = upload(p->member_table, p->hp.member_table);
@@ -191,20 +216,27 @@ int ensure_uploaded(jq_plan* p) {
   return JQ_OK;
 }
 
-int fill_obs(jq::SvArgs& a, int n_wires, int obs_kind, int n_obs, const int32_t* obs_wires, const double* obs_d) {
+int fill_obs(jq_plan* p, jq::SvArgs& a, int n_wires, int obs_kind, int n_obs, const int32_t* obs_wires,
+             const double* obs_d, hipStream_t st) {
   a.obs_kind = obs_kind;
   a.n_obs = 0;
+  a.obs = nullptr;
   if (obs_kind == JQ_OBS_NONE) return JQ_OK;
   if (obs_kind != JQ_OBS_SV_DIAG && obs_kind != JQ_OBS_DM_DIAG) return fail(JQ_ERR_ARG, "unknown observable kind");
   if (n_obs < 0 || n_obs > 64) return fail(JQ_ERR_ARG, "between 0 and 64 observables are supported");
   if (n_obs > 0 && (!obs_wires || !obs_d)) return fail(JQ_ERR_ARG, "null observable arrays");
   const int span = obs_kind == JQ_OBS_DM_DIAG ? n_wires / 2 : n_wires;
+  jq::ObsTable t;
+  std::memset(&t, 0, sizeof(t));
   for (int o = 0; o < n_obs; ++o) {
     if (obs_wires[o] < 0 || obs_wires[o] >= span) return fail(JQ_ERR_ARG, "observable wire out of range");
-    a.obs_gbit[o] = n_wires - 1 - obs_wires[o];
-    a.obs_d[2 * o] = obs_d[2 * o];
-    a.obs_d[2 * o + 1] = obs_d[2 * o + 1];
+    t.gbit[o] = n_wires - 1 - obs_wires[o];
+    t.d[2 * o] = obs_d[2 * o];
+    t.d[2 * o + 1] = obs_d[2 * o + 1];
   }
+  HIP_TRY(p->obs.ensure(sizeof(jq::ObsTable)));
+  HIP_TRY(jq::launch_obs_upload(t, p->obs.p, st));
+  a.obs = p->obs.p;
   a.n_obs = n_obs;
   return JQ_OK;
 }
@@ -246,7 +278,8 @@ jq::SvArgs base_args(jq_plan* p) {
 }
 
 size_t smem_for(jq_plan* p, int extra_reals) {
-  return ((size_t)elem_bytes(p->dtype) << p->hp.T) + (size_t)(64 + extra_reals) * 8;
+  // state tile | observable table (64 ints + 128 doubles) | 64 expval + slot reals
+  return ((size_t)elem_bytes(p->dtype) << p->hp.T) + 64 * 4 + 128 * 8 + (size_t)(64 + extra_reals) * 8;
 }
 
 int max_slots(jq_plan* p) {
@@ -271,7 +304,7 @@ int jq_sv_forward(jq_plan* p, int32_t batch, const double* angles, const void* m
   rc = run_prep(p, batch, angles, matrices, st);
   if (rc) return rc;
   jq::SvArgs a = base_args(p);
-  rc = fill_obs(a, p->hp.n_wires, expvals ? obs_kind : JQ_OBS_NONE, n_obs, obs_wires, obs_d);
+  rc = fill_obs(p, a, p->hp.n_wires, expvals ? obs_kind : JQ_OBS_NONE, n_obs, obs_wires, obs_d, st);
   if (rc) return rc;
   const int ntiles = p->hp.ntiles;
   if (a.obs_kind != JQ_OBS_NONE) {
@@ -284,7 +317,7 @@ int jq_sv_forward(jq_plan* p, int32_t batch, const double* angles, const void* m
     a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
     a.dst = state;
     a.dst_stride = (int64_t)a.vec_len;
-    HIP_TRY(jq::launch_sv(0, p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st));
+    HIP_TRY(timed_launch(0, p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st));
   }
   if (a.obs_kind != JQ_OBS_NONE) HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
   return JQ_OK;
@@ -303,7 +336,7 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
   rc = run_prep(p, batch, angles, matrices, st);
   if (rc) return rc;
   jq::SvArgs a = base_args(p);
-  rc = fill_obs(a, p->hp.n_wires, obs_kind, n_obs, obs_wires, obs_d);
+  rc = fill_obs(p, a, p->hp.n_wires, obs_kind, n_obs, obs_wires, obs_d, st);
   if (rc) return rc;
   a.ct = ct;
   const int ntiles = p->hp.ntiles;
@@ -322,16 +355,7 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
   }
   a.lam = lam;
   a.lam_stride = (int64_t)a.vec_len;
-  if (P == 1 && ntiles == 1) {
-    // whole circuit in one workgroup
-    a.pass = 0;
-    a.src = psi0;
-    a.src_stride = psi0_batched ? (int64_t)a.vec_len : 0;
-    a.dst = state;
-    a.dst_stride = (int64_t)a.vec_len;
-    if (!grad_psi0) a.lam = nullptr;
-    HIP_TRY(jq::launch_sv(2, p->dtype, p->hp.T, a, 1, batch, smem_for(p, max_slots(p)), st));
-  } else {
+  {
     // forward: keep every pass output (checkpoints) when enabled
     const bool ck = p->checkpoints != 0 && P > 1;
     if (ck) HIP_TRY(p->ckpt.ensure(vbytes * (size_t)(P - 1)));
@@ -345,22 +369,21 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
       a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
       a.dst = pass_out(q);
       a.dst_stride = (int64_t)a.vec_len;
-      HIP_TRY(jq::launch_sv(0, p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st));
+      HIP_TRY(timed_launch(0, p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st));
     }
     HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals ? expvals : (double*)p->ev_part.p, ntiles, a.n_obs, batch, st));
     // backward sweep
     const size_t sm = smem_for(p, max_slots(p));
+    a.prog = (const int*)p->prog_b.p;
     for (int q = P - 1; q >= 0; --q) {
       a.pass = q;
       a.src = pass_out(q);
       a.src_stride = (int64_t)a.vec_len;
       a.dst = ck ? nullptr : state;  // without checkpoints psi is un-applied in place
       a.dst_stride = (int64_t)a.vec_len;
-      HIP_TRY(jq::launch_sv(1, p->dtype, p->hp.T, a, ntiles, batch, sm, st));
+      HIP_TRY(timed_launch(1, p->dtype, p->hp.T, a, ntiles, batch, sm, st));
     }
   }
-  if (P == 1 && ntiles == 1 && expvals)
-    HIP_TRY(hipMemcpyAsync(expvals, a.ev_part, (size_t)batch * a.n_obs * 8, hipMemcpyDeviceToDevice, st));
   // cotangents -> gradients
   const int nreals = p->hp.bars * 2;
   if (grad_angles && p->n_params > 0)
@@ -441,6 +464,30 @@ int jq_expval_matrix(int32_t n_qubits, int32_t batch, int32_t dtype, int32_t den
   hipStream_t st = (hipStream_t)stream;
   HIP_TRY(hipMemsetAsync(out, 0, (size_t)batch * 8, st));
   HIP_TRY(jq::launch_expval_matrix(dtype, a, batch, st));
+  return JQ_OK;
+}
+
+int jq_timing_enable(int32_t on) {
+  g_timer.on = on != 0;
+  return JQ_OK;
+}
+
+int jq_timing_read(int32_t which, double* total_ms, int32_t* count) {
+  if (which < 0 || which > 2) return fail(JQ_ERR_ARG, "which must be 0 (forward), 1 (backward) or 2 (fused)");
+  double tot = 0;
+  int n = 0;
+  for (auto& pr : g_timer.ev[which]) {
+    HIP_TRY(hipEventSynchronize(pr.second));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second));
+    tot += ms;
+    ++n;
+    hipEventDestroy(pr.first);
+    hipEventDestroy(pr.second);
+  }
+  g_timer.ev[which].clear();
+  if (total_ms) *total_ms = tot;
+  if (count) *count = n;
   return JQ_OK;
 }
 

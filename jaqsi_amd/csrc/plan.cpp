@@ -67,7 +67,7 @@ int gbit(int n, int wire) { return n - 1 - wire; }
 
 }  // namespace
 
-void build_program(const jq_plan_desc& d, int T_req, int RB, HostProgram& out) {
+void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgram& out) {
   const int n = d.n_wires;
   if (n < 1 || n > 40) throw PlanError(JQ_ERR_ARG, "n_wires must be in [1, 40]");
   // ---- fusion (simulation.py:214-240) ---------------------------------
@@ -115,7 +115,7 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, HostProgram& out) {
   }
 
   // ---- classify ---------------------------------------------------------
-  const int maxk = RB;
+  const int maxk = std::min(RB, RBb);
   int pool = 0, bars = 0;
   for (HB& b : blocks) {
     b.k = (int)b.wires.size();
@@ -304,8 +304,30 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, HostProgram& out) {
     passes.push_back(P);
   }
 
-  // ---- emit -------------------------------------------------------------------
-  std::vector<int32_t>& prog = out.words;
+  // ---- cotangent slots (shared by the forward and backward programs) -------
+  const int ntiles = 1 << ntiles_log;
+  std::vector<int> slot_of(blocks.size(), -1);
+  std::vector<int> pass_nslots(passes.size(), 0), pass_slot_base(passes.size(), 0);
+  {
+    int slot_base = 0;
+    for (size_t p = 0; p < passes.size(); ++p) {
+      int nslots = 0;
+      for (int bi : passes[p].blocks) {
+        HB& b = blocks[bi];
+        if (!b.grad) continue;
+        slot_of[bi] = nslots;
+        out.slot_map.push_back({(int)p, nslots, b.bar * 2, b.bar_size * 2, slot_base});
+        nslots += b.bar_size * 2;
+      }
+      pass_nslots[p] = nslots;
+      pass_slot_base[p] = slot_base;
+      slot_base += nslots * ntiles;
+    }
+    out.partial_reals = slot_base;
+  }
+
+  // ---- emit (once per register-tile size) -----------------------------------
+  auto emit = [&](const int RB, std::vector<int32_t>& prog, int& total_sub) {
   prog.assign(HDR_WORDS + passes.size(), 0);
   prog[0] = PROG_MAGIC;
   prog[1] = (int)passes.size();
@@ -314,10 +336,8 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, HostProgram& out) {
   prog[4] = n;
   prog[5] = bi_dev;
   prog[6] = bars * 2;
-  const int ntiles = 1 << ntiles_log;
   prog[8] = ntiles;
-  int slot_base = 0;
-  int total_sub = 0;
+  total_sub = 0;
   for (size_t p = 0; p < passes.size(); ++p) {
     Pass& P = passes[p];
     prog[HDR_WORDS + p] = (int)prog.size();
@@ -335,7 +355,7 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, HostProgram& out) {
     // substeps
     std::vector<std::vector<int32_t>> subwords;
     std::vector<int> rem = P.blocks;
-    int nsub = 0, nslots = 0;
+    int nsub = 0;
     std::vector<int> prev_reg;
     while (!rem.empty()) {
       std::vector<int> reg;  // tile-local bits in registers, by register position
@@ -377,15 +397,50 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, HostProgram& out) {
           left.push_back(bi);
         }
       }
-      // pad registers: keep the previous layout's bits where possible (fewer
-      // moves), otherwise take the highest free tile bits so lanes keep the
-      // low bits (coalesced global access, conflict-free LDS).
+      // Register layout of this substep.  Passes load and store under the
+      // canonical layout (register bit 0 <-> tile bit 0, the other register
+      // bits <-> the top tile bits, lanes <-> the low tile bits); a substep
+      // keeps the previous layout, else the canonical one, whenever its
+      // targets fit, so the LDS re-layout between substeps is skipped.
+      std::vector<int> canon(RB);
+      canon[0] = 0;
+      for (int b2 = 1; b2 < RB; ++b2) canon[b2] = T - RB + b2;
+      auto covers = [&](const std::vector<int>& lay) {
+        for (int tb : reg)
+          if (std::find(lay.begin(), lay.end(), tb) == lay.end()) return false;
+        return true;
+      };
       std::vector<char> used(T, 0);
+      if (forced_k == 0) {
+        std::vector<int> lay;
+        if (prev_reg.empty()) prev_reg = canon;
+        if (covers(prev_reg)) {
+          lay = prev_reg;
+        } else if (covers(canon)) {
+          lay = canon;
+        } else {
+          lay = canon;
+          std::vector<char> keep(RB, 0);
+          for (int b2 = 0; b2 < RB; ++b2)
+            if (std::find(reg.begin(), reg.end(), canon[b2]) != reg.end()) keep[b2] = 1;
+          int pos = RB - 1;
+          for (int tb : reg) {
+            if (std::find(canon.begin(), canon.end(), tb) != canon.end()) continue;
+            while (pos >= 0 && keep[pos]) --pos;
+            lay[pos] = tb;
+            keep[pos] = 1;
+          }
+        }
+        reg = lay;
+      } else {
+        for (int tb : reg) used[tb] = 1;
+        for (int tb : canon)
+          if ((int)reg.size() < RB && !used[tb]) reg.push_back(tb), used[tb] = 1;
+        for (int tb = T - 1; tb >= 0 && (int)reg.size() < RB; --tb)
+          if (!used[tb]) reg.push_back(tb), used[tb] = 1;
+      }
+      std::fill(used.begin(), used.end(), 0);
       for (int tb : reg) used[tb] = 1;
-      for (int tb : prev_reg)
-        if ((int)reg.size() < RB && !used[tb] && forced_k == 0) reg.push_back(tb), used[tb] = 1;
-      for (int tb = T - 1; tb >= 0 && (int)reg.size() < RB; --tb)
-        if (!used[tb]) reg.push_back(tb), used[tb] = 1;
       prev_reg = reg;
       std::vector<int32_t> sw(1 + T, 0);
       sw[0] = (int)sub.size();
@@ -428,12 +483,7 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, HostProgram& out) {
         g[G_COUT_HI] = (int)(uint32_t)(cout >> 32);
         g[G_MAT] = b.mat;
         g[G_INV] = b.inv;
-        g[G_SLOT] = -1;
-        if (b.grad) {
-          g[G_SLOT] = nslots;
-          out.slot_map.push_back({(int)p, nslots, b.bar * 2, b.bar_size * 2});
-          nslots += b.bar_size * 2;
-        }
+        g[G_SLOT] = slot_of[bi];
         g[G_K] = b.k;
         if (b.kind == AK_DIAG) {
           for (int j = 0; j < b.k; ++j) {
@@ -469,17 +519,17 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, HostProgram& out) {
       prog.insert(prog.end(), subwords[k].begin(), subwords[k].end());
     }
     prog[rec] = nsub;
-    prog[rec + 2] = nslots;
-    prog[rec + 3] = slot_base;
-    for (auto& s : out.slot_map)
-      if (s.pass == (int)p) s.base = slot_base;
-    slot_base += nslots * ntiles;
+    prog[rec + 2] = pass_nslots[p];
+    prog[rec + 3] = pass_slot_base[p];
     total_sub += nsub;
   }
-  prog[7] = slot_base;
-  out.partial_reals = slot_base;
+  prog[7] = out.partial_reals;
+  };
+  int sub_f = 0, sub_b = 0;
+  emit(RB, out.words, sub_f);
+  emit(RBb, out.words_b, sub_b);
   out.n_passes = (int)passes.size();
-  out.n_substeps = total_sub;
+  out.n_substeps = sub_f;
   out.ntiles = ntiles;
 }
 

@@ -22,7 +22,8 @@ struct SlotMap {
 };
 
 struct HostProgram {
-  std::vector<int32_t> words;         // scheduled program (program.h)
+  std::vector<int32_t> words;         // scheduled program for the forward sweep (program.h)
+  std::vector<int32_t> words_b;       // the same passes, laid out for the backward sweep
   std::vector<int32_t> block_table;   // BW words per device block
   std::vector<int32_t> member_list;   // member indices, grouped by block
   std::vector<int32_t> member_table;  // MW words per member
@@ -35,6 +36,6 @@ struct HostProgram {
   int n_dev_blocks = 0, n_passes = 0, n_substeps = 0;
 };
 
-void build_program(const jq_plan_desc& d, int T, int RB, HostProgram& out);
+void build_program(const jq_plan_desc& d, int T, int RB, int RBb, HostProgram& out);
 
 }  // namespace jq

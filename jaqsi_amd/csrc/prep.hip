@@ -12,6 +12,7 @@
 
 #include "jaqsi_amd.h"
 #include "prep.h"
+#include "sv_kernels.h"
 #include "program.h"
 
 namespace jq {
@@ -404,6 +405,17 @@ __global__ void expval_matrix_kernel(ExpvalArgs a) {
   }
   for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
   if ((threadIdx.x & 63) == 0) atomicAdd(&a.out[circ], acc);
+}
+
+__global__ void obs_upload_kernel(ObsTable t, ObsTable* dst) {
+  const int i = threadIdx.x;
+  if (i < 64) dst->gbit[i] = t.gbit[i];
+  if (i < 128) dst->d[i] = t.d[i];
+}
+
+hipError_t launch_obs_upload(const ObsTable& t, void* dst, hipStream_t st) {
+  hipLaunchKernelGGL(obs_upload_kernel, dim3(1), dim3(128), 0, st, t, (ObsTable*)dst);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

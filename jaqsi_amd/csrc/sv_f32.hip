@@ -1,9 +1,11 @@
-// complex64 instantiation of the tile-pass kernels (T = 13: 256 threads x 32 amplitudes).
+// complex64 instantiation of the tile-pass kernels (T = 13):
+// forward 32 amplitudes x 256 threads, backward 16 amplitudes x 512 threads.
 #include "sv_impl.h"
 
 namespace jq {
 hipError_t launch_sv_f32(int which, int T, const SvArgs& a, int ntiles, int batch, size_t smem, hipStream_t st) {
-  if (T == 13) return launch_cfg<float, 5, 256>(which, a, ntiles, batch, smem, st);
-  return hipErrorInvalidValue;
+  if (T != 13) return hipErrorInvalidValue;
+  if (which == 0) return launch_forward<float, 5, 256>(a, ntiles, batch, smem, st);
+  return launch_backward<float, 4, 512>(a, ntiles, batch, smem, st);
 }
 }  // namespace jq

@@ -1,9 +1,11 @@
-// complex128 instantiation of the tile-pass kernels (T = 12: 256 threads x 16 amplitudes).
+// complex128 instantiation of the tile-pass kernels (T = 12):
+// forward 16 amplitudes x 256 threads, backward 8 amplitudes x 512 threads.
 #include "sv_impl.h"
 
 namespace jq {
 hipError_t launch_sv_f64(int which, int T, const SvArgs& a, int ntiles, int batch, size_t smem, hipStream_t st) {
-  if (T == 12) return launch_cfg<double, 4, 256>(which, a, ntiles, batch, smem, st);
-  return hipErrorInvalidValue;
+  if (T != 12) return hipErrorInvalidValue;
+  if (which == 0) return launch_forward<double, 4, 256>(a, ntiles, batch, smem, st);
+  return launch_backward<double, 3, 512>(a, ntiles, batch, smem, st);
 }
 }  // namespace jq

@@ -38,11 +38,58 @@ __device__ __forceinline__ void cmul_acc(R& ar, R& ai, R br, R bi, R cr, R ci) {
   ai = fma(bi, cr, ai);
 }
 
+// Sum over the 64 lanes of a wave, returned wave-uniform.  Within each row
+// of 16 lanes the DPP network (quad_perm xor 1, xor 2, half-row mirror, row
+// mirror) leaves every lane with its row sum; the four row sums are then
+// combined with readlane.  All lanes of the wave must be active.
+__device__ __forceinline__ float dpp_row_sum(float v) {
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));  // quad_perm(1,0,3,2)
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));  // quad_perm(2,3,0,1)
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false)); // row_half_mirror
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, false)); // row_mirror
+  return v;
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_step_d(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return x + __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double dpp_row_sum(double v) {
+  // move both 32-bit halves through the same network, add as doubles
+  v = dpp_step_d<0xB1>(v);
+  v = dpp_step_d<0x4E>(v);
+  v = dpp_step_d<0x141>(v);
+  v = dpp_step_d<0x140>(v);
+  return v;
+}
+
 template <typename R>
 __device__ __forceinline__ R wave_sum(R v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v = dpp_row_sum(v);
+  if constexpr (sizeof(R) == 4) {
+    const float a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return (a + b) + (c + d);
+  } else {
+    auto rl = [](double x, int lane) {
+      const long long bb = __double_as_longlong(x);
+      const int lo = __builtin_amdgcn_readlane((int)(bb & 0xffffffffll), lane);
+      const int hi = __builtin_amdgcn_readlane((int)(bb >> 32), lane);
+      return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    };
+    return (rl(v, 0) + rl(v, 16)) + (rl(v, 32) + rl(v, 48));
+  }
+}
+
+// Add a wave-uniform value into an LDS double from one lane.
+__device__ __forceinline__ void lds_add(double* p, double v) {
+  if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // ---------------------------------------------------------------------------
@@ -66,14 +113,13 @@ template <int RB, int LNT>
 __device__ __forceinline__ void decode_layout(const int* sub, const int* tbits, Layout& L) {
   uint32_t toff = 0;
   uint64_t gthr = 0;
-  const int tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < LNT; ++j) {
     const int tb = sub[1 + RB + j];
-    if ((tid >> j) & 1) {
-      toff |= 1u << tb;
-      gthr |= 1ull << tbits[tb];
-    }
+    const uint32_t bit = (tid >> j) & 1u;
+    toff |= bit << tb;
+    gthr |= (uint64_t)bit << tbits[tb];
   }
   L.toff = toff;
   L.gthr = gthr;
@@ -94,6 +140,26 @@ __device__ __forceinline__ uint32_t reg_toff(const Layout& L, int i) {
   return x;
 }
 
+// Tile-local offset of register i alone (wave-uniform).
+template <int RB>
+__device__ __forceinline__ uint32_t reg_roff(const Layout& L, int i) {
+  uint32_t x = 0;
+#pragma unroll
+  for (int b = 0; b < RB; ++b)
+    if ((i >> b) & 1) x |= 1u << L.rpos[b];
+  return x;
+}
+
+// Global offset of register i alone (wave-uniform).
+template <int RB>
+__device__ __forceinline__ uint64_t reg_goff(const Layout& L, int i) {
+  uint64_t g = 0;
+#pragma unroll
+  for (int b = 0; b < RB; ++b)
+    if ((i >> b) & 1) g |= L.gpos[b];
+  return g;
+}
+
 template <int RB>
 __device__ __forceinline__ uint64_t reg_gidx(const Layout& L, uint64_t base, int i) {
   uint64_t g = base | L.gthr;
@@ -106,113 +172,251 @@ __device__ __forceinline__ uint64_t reg_gidx(const Layout& L, uint64_t base, int
 // ---------------------------------------------------------------------------
 // register-level gate application
 // ---------------------------------------------------------------------------
-// a 2x2 (or 4x4, 8x8) complex matrix held in registers
+// A small complex matrix held in (scalar) registers.
 template <typename R, int D>
 struct Mat {
   R r[D * D], i[D * D];
 };
 
-// Load a D x D matrix as M (mode 0), M^dagger (mode 1) or M^T (mode 2).
+template <typename R>
+__device__ __forceinline__ void load2(const C2<R>* p, Mat<R, 2>& m) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    m.r[e] = p[e].x;
+    m.i[e] = p[e].y;
+  }
+}
+
+// 4x4 matrix; `swap` exchanges the roles of the two target bits, so the
+// kernels only need the register-bit pairs A > B.
+template <typename R>
+__device__ __forceinline__ void load4(const C2<R>* p, Mat<R, 4>& m, bool swap) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int sr = swap ? (((r & 1) << 1) | (r >> 1)) : r;
+      const int sc = swap ? (((c & 1) << 1) | (c >> 1)) : c;
+      m.r[r * 4 + c] = p[sr * 4 + sc].x;
+      m.i[r * 4 + c] = p[sr * 4 + sc].y;
+    }
+}
+
+// conjugate transpose (the inverse of a unitary)
 template <typename R, int D>
-__device__ __forceinline__ void load_mat(const C2<R>* p, Mat<R, D>& m, int mode) {
-  if (mode == 0) {
+__device__ __forceinline__ void adjoint_of(const Mat<R, D>& a, Mat<R, D>& o) {
 #pragma unroll
-    for (int e = 0; e < D * D; ++e) {
-      m.r[e] = p[e].x;
-      m.i[e] = p[e].y;
+  for (int r = 0; r < D; ++r)
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      o.r[r * D + c] = a.r[c * D + r];
+      o.i[r * D + c] = -a.i[c * D + r];
     }
-  } else {
-    const R sg = mode == 1 ? R(-1) : R(1);
+}
+
+// y = M x for a D-vector held in registers at indices idx[]
+template <typename R, int D>
+__device__ __forceinline__ void mv(const Mat<R, D>& m, const R* xr, const R* xi, R* yr, R* yi) {
 #pragma unroll
-    for (int r = 0; r < D; ++r)
+  for (int r = 0; r < D; ++r) {
+    R ar = xr[0] * m.r[r * D];
+    R ai = xi[0] * m.r[r * D];
+    ar = fma(-m.i[r * D], xi[0], ar);
+    ai = fma(m.i[r * D], xr[0], ai);
 #pragma unroll
-      for (int c = 0; c < D; ++c) {
-        m.r[r * D + c] = p[c * D + r].x;
-        m.i[r * D + c] = sg * p[c * D + r].y;
-      }
+    for (int c = 1; c < D; ++c) cmul_acc(ar, ai, m.r[r * D + c], m.i[r * D + c], xr[c], xi[c]);
+    yr[r] = ar;
+    yi[r] = ai;
   }
 }
 
-// Bits of the register index addressed by a gate, as template parameters.
-// MC1: target A.  MC2: first target A (matrix MSB), second target B.
-template <typename R, int NR, int A>
-__device__ __forceinline__ void apply_mc1(R* re, R* im, const Mat<R, 2>& m, uint32_t creg, bool ok) {
+// y = M^T x
+template <typename R, int D>
+__device__ __forceinline__ void mvt(const Mat<R, D>& m, const R* xr, const R* xi, R* yr, R* yi) {
 #pragma unroll
-  for (int i = 0; i < NR; ++i) {
-    if (i & (1 << A)) continue;
-    const int j = i | (1 << A);
-    const bool act = ok && ((uint32_t)(i & creg) == creg);
-    const R x0r = re[i], x0i = im[i], x1r = re[j], x1i = im[j];
-    R y0r = 0, y0i = 0, y1r = 0, y1i = 0;
-    cmul_acc(y0r, y0i, m.r[0], m.i[0], x0r, x0i);
-    cmul_acc(y0r, y0i, m.r[1], m.i[1], x1r, x1i);
-    cmul_acc(y1r, y1i, m.r[2], m.i[2], x0r, x0i);
-    cmul_acc(y1r, y1i, m.r[3], m.i[3], x1r, x1i);
-    re[i] = act ? y0r : x0r;
-    im[i] = act ? y0i : x0i;
-    re[j] = act ? y1r : x1r;
-    im[j] = act ? y1i : x1i;
+  for (int c = 0; c < D; ++c) {
+    R ar = xr[0] * m.r[c];
+    R ai = xi[0] * m.r[c];
+    ar = fma(-m.i[c], xi[0], ar);
+    ai = fma(m.i[c], xr[0], ai);
+#pragma unroll
+    for (int r = 1; r < D; ++r) cmul_acc(ar, ai, m.r[r * D + c], m.i[r * D + c], xr[r], xi[r]);
+    yr[c] = ar;
+    yi[c] = ai;
+  }
+}
+
+// Register index group of a gate: MC1 target A; MC2 targets A > B with the
+// matrix index (bit A, bit B).  creg is a wave-uniform mask over register
+// bits: groups whose register controls are not all one are skipped by a
+// scalar branch.  Controls on thread bits / outside the tile are tested by
+// the caller.
+template <int NR, int A>
+struct G1 {
+  static constexpr int n = NR / 2;
+  __device__ static constexpr int base(int k) { return ((k >> A) << (A + 1)) | (k & ((1 << A) - 1)); }
+  __device__ static constexpr int at(int k, int q) { return base(k) | (q << A); }
+};
+template <int NR, int A, int B>
+struct G2 {
+  static constexpr int n = NR / 4;
+  __device__ static constexpr int base(int k) {
+    const int lo = ((k >> B) << (B + 1)) | (k & ((1 << B) - 1));
+    return ((lo >> A) << (A + 1)) | (lo & ((1 << A) - 1));
+  }
+  __device__ static constexpr int at(int k, int q) { return base(k) | ((q >> 1) << A) | ((q & 1) << B); }
+};
+
+template <typename R, int NR, int D, typename G, bool CTRL>
+__device__ __forceinline__ void fw_dense(R* re, R* im, const Mat<R, D>& m, uint32_t creg) {
+#pragma unroll
+  for (int k = 0; k < G::n; ++k) {
+    if (CTRL && ((uint32_t)G::base(k) & creg) != creg) continue;
+    R xr[D], xi[D], yr[D], yi[D];
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      xr[q] = re[G::at(k, q)];
+      xi[q] = im[G::at(k, q)];
+    }
+    mv<R, D>(m, xr, xi, yr, yi);
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      re[G::at(k, q)] = yr[q];
+      im[G::at(k, q)] = yi[q];
+    }
+  }
+}
+
+// Backward step of one dense block on one register group set:
+//   psi <- Ui psi ; acc[o][c] += lambda[o] psi[c] ; lambda <- U^T lambda
+template <typename R, int NR, int D, typename G, bool CTRL>
+__device__ __forceinline__ void bw_dense(R* pr, R* pi, R* lr, R* li, const Mat<R, D>& u, const Mat<R, D>& ui,
+                                         uint32_t creg, R* acc) {
+#pragma unroll
+  for (int k = 0; k < G::n; ++k) {
+    if (CTRL && ((uint32_t)G::base(k) & creg) != creg) continue;
+    R xr[D], xi[D], yr[D], yi[D], lr0[D], li0[D], l1r[D], l1i[D];
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      xr[q] = pr[G::at(k, q)];
+      xi[q] = pi[G::at(k, q)];
+      lr0[q] = lr[G::at(k, q)];
+      li0[q] = li[G::at(k, q)];
+    }
+    mv<R, D>(ui, xr, xi, yr, yi);
+#pragma unroll
+    for (int o = 0; o < D; ++o)
+#pragma unroll
+      for (int c = 0; c < D; ++c) cmul_acc(acc[2 * (o * D + c)], acc[2 * (o * D + c) + 1], lr0[o], li0[o], yr[c], yi[c]);
+    mvt<R, D>(u, lr0, li0, l1r, l1i);
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      pr[G::at(k, q)] = yr[q];
+      pi[G::at(k, q)] = yi[q];
+      lr[G::at(k, q)] = l1r[q];
+      li[G::at(k, q)] = l1i[q];
+    }
+  }
+}
+
+// The same step in three light sweeps (fewer live registers for 4x4 blocks):
+template <typename R, int NR, int D, typename G>
+__device__ __forceinline__ void sweep_apply(R* re, R* im, const Mat<R, D>& m, uint32_t creg, bool transpose) {
+#pragma unroll
+  for (int k = 0; k < G::n; ++k) {
+    if (((uint32_t)G::base(k) & creg) != creg) continue;
+    R xr[D], xi[D], yr[D], yi[D];
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      xr[q] = re[G::at(k, q)];
+      xi[q] = im[G::at(k, q)];
+    }
+    if (transpose)
+      mvt<R, D>(m, xr, xi, yr, yi);
+    else
+      mv<R, D>(m, xr, xi, yr, yi);
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      re[G::at(k, q)] = yr[q];
+      im[G::at(k, q)] = yi[q];
+    }
+  }
+}
+
+// acc[(o - o0) * D + c] += lambda[o] psi[c] for rows o0 .. o0 + D/2 - 1
+template <typename R, int NR, int D, typename G>
+__device__ __forceinline__ void sweep_grad(const R* pr, const R* pi, const R* lr, const R* li, uint32_t creg, int o0,
+                                           R* acc) {
+#pragma unroll
+  for (int k = 0; k < G::n; ++k) {
+    if (((uint32_t)G::base(k) & creg) != creg) continue;
+#pragma unroll
+    for (int oo = 0; oo < D / 2; ++oo) {
+      const int o = o0 + oo;
+#pragma unroll
+      for (int c = 0; c < D; ++c)
+        cmul_acc(acc[2 * (oo * D + c)], acc[2 * (oo * D + c) + 1], lr[G::at(k, o)], li[G::at(k, o)],
+                 pr[G::at(k, c)], pi[G::at(k, c)]);
+    }
+  }
+}
+
+template <typename R, int NR, int A>
+__device__ __forceinline__ void perm_x(R* re, R* im, uint32_t creg) {
+#pragma unroll
+  for (int k = 0; k < NR / 2; ++k) {
+    const int i = G1<NR, A>::base(k), j = i | (1 << A);
+    if (((uint32_t)i & creg) != creg) continue;
+    const R ar = re[i], ai = im[i];
+    re[i] = re[j];
+    im[i] = im[j];
+    re[j] = ar;
+    im[j] = ai;
   }
 }
 
 template <typename R, int NR, int A, int B>
-__device__ __forceinline__ void apply_mc2(R* re, R* im, const Mat<R, 4>& m, uint32_t creg, bool ok) {
+__device__ __forceinline__ void perm_swap(R* re, R* im, uint32_t creg) {
 #pragma unroll
-  for (int i = 0; i < NR; ++i) {
-    if (i & ((1 << A) | (1 << B))) continue;
-    const bool act = ok && ((uint32_t)(i & creg) == creg);
-    int idx[4] = {i, i | (1 << B), i | (1 << A), i | (1 << A) | (1 << B)};
-    R xr[4], xi[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      xr[q] = re[idx[q]];
-      xi[q] = im[idx[q]];
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      R ar = 0, ai = 0;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) cmul_acc(ar, ai, m.r[r * 4 + c], m.i[r * 4 + c], xr[c], xi[c]);
-      re[idx[r]] = act ? ar : xr[r];
-      im[idx[r]] = act ? ai : xi[r];
-    }
-  }
-}
-
-template <typename R, int NR, int A>
-__device__ __forceinline__ void apply_mcx(R* re, R* im, uint32_t creg, bool ok) {
-#pragma unroll
-  for (int i = 0; i < NR; ++i) {
-    if (i & (1 << A)) continue;
-    const int j = i | (1 << A);
-    const bool act = ok && ((uint32_t)(i & creg) == creg);
-    const R ar = re[i], ai = im[i], br = re[j], bi = im[j];
-    re[i] = act ? br : ar;
-    im[i] = act ? bi : ai;
-    re[j] = act ? ar : br;
-    im[j] = act ? ai : bi;
-  }
-}
-
-template <typename R, int NR, int A, int B>
-__device__ __forceinline__ void apply_mcswap(R* re, R* im, uint32_t creg, bool ok) {
-#pragma unroll
-  for (int i = 0; i < NR; ++i) {
-    if (i & ((1 << A) | (1 << B))) continue;
+  for (int k = 0; k < NR / 4; ++k) {
+    const int i = G2<NR, A, B>::base(k);
+    if (((uint32_t)i & creg) != creg) continue;
     const int p = i | (1 << A), q = i | (1 << B);
-    const bool act = ok && ((uint32_t)(i & creg) == creg);
-    const R ar = re[p], ai = im[p], br = re[q], bi = im[q];
-    re[p] = act ? br : ar;
-    im[p] = act ? bi : ai;
-    re[q] = act ? ar : br;
-    im[q] = act ? ai : bi;
+    const R ar = re[p], ai = im[p];
+    re[p] = re[q];
+    im[p] = im[q];
+    re[q] = ar;
+    im[q] = ai;
+  }
+}
+
+// diagonal on one wire held by register bit A: factor d[bit]
+template <typename R, int NR, int A>
+__device__ __forceinline__ void diag1(R* re, R* im, R d0r, R d0i, R d1r, R d1i) {
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    const bool hi = (i >> A) & 1;
+    const R fr = hi ? d1r : d0r, fi = hi ? d1i : d0i;
+    const R ar = re[i], ai = im[i];
+    re[i] = fma(ar, fr, -ai * fi);
+    im[i] = fma(ar, fi, ai * fr);
+  }
+}
+
+template <typename R, int NR, int A>
+__device__ __forceinline__ void diag1_grad(const R* pr, const R* pi, const R* lr, const R* li, R* acc) {
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    const int o = (i >> A) & 1;
+    cmul_acc(acc[2 * o], acc[2 * o + 1], lr[i], li[i], pr[i], pi[i]);
   }
 }
 
 // Dense 3-wire block on register bits 2..0 (wire j <-> register bit 2-j),
 // evaluated through LDS so the 8x8 matrix never occupies registers.
+// mode 0: M, 1: M^dagger, 2: M^T.
 template <typename R, int RB>
-__device__ __forceinline__ void apply_dense3(R* re, R* im, const C2<R>* mp, int mode, const Layout& L, C2<R>* lds) {
+__device__ __forceinline__ void dense3(R* re, R* im, const C2<R>* mp, int mode, const Layout& L, C2<R>* lds) {
   constexpr int NR = 1 << RB;
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
@@ -239,10 +443,37 @@ __device__ __forceinline__ void apply_dense3(R* re, R* im, const C2<R>* mp, int 
   __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// dispatch on register bit positions
+// ---------------------------------------------------------------------------
+#define JQ_CASE1(...)                                  \
+  switch (ra) {                                        \
+    case 0: { constexpr int A = 0; __VA_ARGS__; } break; \
+    case 1: { constexpr int A = 1; __VA_ARGS__; } break; \
+    case 2: { constexpr int A = 2; __VA_ARGS__; } break; \
+    case 3: { constexpr int A = 3; __VA_ARGS__; } break; \
+    default:                                           \
+      if constexpr (RB > 4) { constexpr int A = RB > 4 ? 4 : 0; __VA_ARGS__; } \
+      break;                                           \
+  }
+
+#define JQ_PAIR_U(A_, B_, ...) \
+  case (A_ * 8 + B_): if constexpr (A_ < RB) { constexpr int A = A_, B = B_; __VA_ARGS__; } break;
+
+// unordered pairs, A > B
+#define JQ_CASE2(...)                                                                     \
+  switch (hi * 8 + lo) {                                                                  \
+    JQ_PAIR_U(1, 0, __VA_ARGS__) JQ_PAIR_U(2, 0, __VA_ARGS__) JQ_PAIR_U(2, 1, __VA_ARGS__) \
+    JQ_PAIR_U(3, 0, __VA_ARGS__) JQ_PAIR_U(3, 1, __VA_ARGS__) JQ_PAIR_U(3, 2, __VA_ARGS__) \
+    JQ_PAIR_U(4, 0, __VA_ARGS__) JQ_PAIR_U(4, 1, __VA_ARGS__) JQ_PAIR_U(4, 2, __VA_ARGS__) \
+    JQ_PAIR_U(4, 3, __VA_ARGS__)                                                          \
+    default: break;                                                                       \
+  }
+
 // Diagonal index of register i: uniform part + register bits.
 struct DiagLoc {
-  uint32_t uni;     // contribution of thread / outside bits
-  int rbit[5];      // register bit of wire j, -1 if not a register bit
+  uint32_t uni;  // contribution of thread / outside bits
+  int rbit[5];   // register bit of wire j, -1 if not a register bit
   int k;
 };
 
@@ -275,16 +506,38 @@ __device__ __forceinline__ uint32_t diag_index(const DiagLoc& dl, int i) {
   return x;
 }
 
-// mode 0: d, 1: conj(d) (U^dagger), 2: d (U^T)
-template <typename R, int NR>
-__device__ __forceinline__ void apply_diag(R* re, R* im, const C2<R>* d, const DiagLoc& dl, int mode) {
-  if (dl.k <= 2) {
+// multiply by the diagonal d (conj when cj)
+template <typename R, int RB>
+__device__ __forceinline__ void apply_diag(R* re, R* im, const C2<R>* d, const DiagLoc& dl, bool cj) {
+  constexpr int NR = 1 << RB;
+  const R sg = cj ? R(-1) : R(1);
+  if (dl.k == 1 && dl.rbit[0] >= 0) {
+    const int ra = dl.rbit[0];
+    const C2<R> v0 = d[0], v1 = d[1];
+    JQ_CASE1(diag1<R, NR, A>(re, im, v0.x, sg * v0.y, v1.x, sg * v1.y))
+    return;
+  }
+  bool any_reg = false;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) any_reg = any_reg || (j < dl.k && dl.rbit[j] >= 0);
+  if (!any_reg) {  // every wire on a thread or outside bit: one factor per thread
+    const C2<R> v = d[dl.uni];
+    const R fr = v.x, fi = sg * v.y;
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const R ar = re[i], ai = im[i];
+      re[i] = fma(ar, fr, -ai * fi);
+      im[i] = fma(ar, fi, ai * fr);
+    }
+    return;
+  }
+  if (dl.k <= 2) {  // up to four entries, selected per register
     R dr[4], di[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const C2<R> v = d[q < (1 << dl.k) ? q : 0];
       dr[q] = v.x;
-      di[q] = mode == 1 ? -v.y : v.y;
+      di[q] = sg * v.y;
     }
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
@@ -297,77 +550,25 @@ __device__ __forceinline__ void apply_diag(R* re, R* im, const C2<R>* d, const D
       fr = x == 3 ? dr[3] : fr;
       fi = x == 3 ? di[3] : fi;
       const R ar = re[i], ai = im[i];
-      re[i] = ar * fr - ai * fi;
-      im[i] = ar * fi + ai * fr;
+      re[i] = fma(ar, fr, -ai * fi);
+      im[i] = fma(ar, fi, ai * fr);
     }
-  } else {
-#pragma unroll
-    for (int i = 0; i < NR; ++i) {
-      const uint32_t x = diag_index(dl, i);
-      const C2<R> v = d[x];
-      const R fr = v.x, fi = mode == 1 ? -v.y : v.y;
-      const R ar = re[i], ai = im[i];
-      re[i] = ar * fr - ai * fi;
-      im[i] = ar * fi + ai * fr;
-    }
+    return;
   }
-}
-
-// ---------------------------------------------------------------------------
-// cotangent accumulation: bar[o][i] += lambda[o] psi[i] (no conjugation)
-// ---------------------------------------------------------------------------
-template <typename R, int NR, int A>
-__device__ __forceinline__ void grad_mc1(const R* pr, const R* pi, const R* lr, const R* li, uint32_t creg, bool ok,
-                                          R* acc) {
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
-    if (i & (1 << A)) continue;
-    const int j = i | (1 << A);
-    const bool act = ok && ((uint32_t)(i & creg) == creg);
-    const int id[2] = {i, j};
-#pragma unroll
-    for (int o = 0; o < 2; ++o)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        R ar = 0, ai = 0;
-        cmul_acc(ar, ai, lr[id[o]], li[id[o]], pr[id[c]], pi[id[c]]);
-        acc[2 * (o * 2 + c)] += act ? ar : R(0);
-        acc[2 * (o * 2 + c) + 1] += act ? ai : R(0);
-      }
-  }
-}
-
-template <typename R>
-__device__ __forceinline__ void flush_vals(const R* acc, int nv, double* slots, int base);
-
-template <typename R, int NR, int A, int B>
-__device__ __forceinline__ void grad_mc2(const R* pr, const R* pi, const R* lr, const R* li, uint32_t creg, bool ok,
-                                          double* slots, int slot) {
-#pragma unroll
-  for (int o = 0; o < 4; ++o) {
-    R acc[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) acc[e] = 0;
-#pragma unroll
-    for (int i = 0; i < NR; ++i) {
-      if (i & ((1 << A) | (1 << B))) continue;
-      const bool act = ok && ((uint32_t)(i & creg) == creg);
-      const int id[4] = {i, i | (1 << B), i | (1 << A), i | (1 << A) | (1 << B)};
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        R ar = 0, ai = 0;
-        cmul_acc(ar, ai, lr[id[o]], li[id[o]], pr[id[c]], pi[id[c]]);
-        acc[2 * c] += act ? ar : R(0);
-        acc[2 * c + 1] += act ? ai : R(0);
-      }
-    }
-    flush_vals<R>(acc, 8, slots, slot + 8 * o);
+    const uint32_t x = diag_index(dl, i);
+    const C2<R> v = d[x];
+    const R fr = v.x, fi = sg * v.y;
+    const R ar = re[i], ai = im[i];
+    re[i] = fma(ar, fr, -ai * fi);
+    im[i] = fma(ar, fi, ai * fr);
   }
 }
 
 template <typename R, int NR>
-__device__ __forceinline__ void grad_diag(const R* pr, const R* pi, const R* lr, const R* li, const DiagLoc& dl,
-                                           R* acc) {
+__device__ __forceinline__ void diag_grad(const R* pr, const R* pi, const R* lr, const R* li, const DiagLoc& dl,
+                                          R* acc) {
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
     const uint32_t x = diag_index(dl, i);
@@ -382,42 +583,15 @@ __device__ __forceinline__ void grad_diag(const R* pr, const R* pi, const R* lr,
 }
 
 // Reduce per-thread accumulators over the workgroup into LDS slots (double).
+// (static loop over the register array; nv <= 8 entries are live)
 template <typename R>
 __device__ __forceinline__ void flush_vals(const R* acc, int nv, double* slots, int base) {
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     if (e >= nv) break;
-    const R v = wave_sum<R>(acc[e]);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&slots[base + e], (double)v);
+    lds_add(&slots[base + e], (double)wave_sum<R>(acc[e]));
   }
 }
-
-// ---------------------------------------------------------------------------
-// dispatch on register bit positions
-// ---------------------------------------------------------------------------
-#define JQ_CASE1(F, ...)                      \
-  switch (ra) {                               \
-    case 0: F<R, NR, 0>(__VA_ARGS__); break;  \
-    case 1: F<R, NR, 1>(__VA_ARGS__); break;  \
-    case 2: F<R, NR, 2>(__VA_ARGS__); break;  \
-    case 3: F<R, NR, 3>(__VA_ARGS__); break;  \
-    default: if constexpr (RB > 4) F<R, NR, (RB > 4 ? 4 : 0)>(__VA_ARGS__); break; \
-  }
-
-#define JQ_PAIR(F, A_, B_, ...) \
-  case (A_ * 8 + B_): if constexpr (A_ < RB && B_ < RB) F<R, NR, A_, B_>(__VA_ARGS__); break;
-
-#define JQ_CASE2(F, ...)                                                                          \
-  switch (ra * 8 + rb) {                                                                          \
-    JQ_PAIR(F, 0, 1, __VA_ARGS__) JQ_PAIR(F, 0, 2, __VA_ARGS__) JQ_PAIR(F, 0, 3, __VA_ARGS__)     \
-    JQ_PAIR(F, 0, 4, __VA_ARGS__) JQ_PAIR(F, 1, 0, __VA_ARGS__) JQ_PAIR(F, 1, 2, __VA_ARGS__)     \
-    JQ_PAIR(F, 1, 3, __VA_ARGS__) JQ_PAIR(F, 1, 4, __VA_ARGS__) JQ_PAIR(F, 2, 0, __VA_ARGS__)     \
-    JQ_PAIR(F, 2, 1, __VA_ARGS__) JQ_PAIR(F, 2, 3, __VA_ARGS__) JQ_PAIR(F, 2, 4, __VA_ARGS__)     \
-    JQ_PAIR(F, 3, 0, __VA_ARGS__) JQ_PAIR(F, 3, 1, __VA_ARGS__) JQ_PAIR(F, 3, 2, __VA_ARGS__)     \
-    JQ_PAIR(F, 3, 4, __VA_ARGS__) JQ_PAIR(F, 4, 0, __VA_ARGS__) JQ_PAIR(F, 4, 1, __VA_ARGS__)     \
-    JQ_PAIR(F, 4, 2, __VA_ARGS__) JQ_PAIR(F, 4, 3, __VA_ARGS__)                                   \
-    default: break;                                                                               \
-  }
 
 struct GateCtx {
   uint32_t creg;
@@ -433,47 +607,54 @@ __device__ __forceinline__ GateCtx gate_ctx(const int* g, const Layout& L, uint6
   return c;
 }
 
-// Apply gate record g to one register state.  mode 0: U, 1: U^dagger (or the
-// stored inverse), 2: U^T.
+// Forward: apply gate record g to the register state.
 template <typename R, int RB>
-__device__ __forceinline__ void apply_gate(R* re, R* im, const int* g, const C2<R>* pool, const Layout& L, uint64_t base, int mode,
-                           C2<R>* lds) {
+__device__ __forceinline__ void fwd_gate(R* re, R* im, const int* g, const C2<R>* pool, const Layout& L,
+                                         uint64_t base, C2<R>* lds) {
   constexpr int NR = 1 << RB;
   const int kind = g[G_KIND];
-  const int ra = g[G_RA], rb = g[G_RB];
+  const C2<R>* mp = pool + g[G_MAT];
+  if (kind == AK_DENSEK) {  // no controls; contains barriers, so all threads enter
+    if constexpr (RB >= 3) dense3<R, RB>(re, im, mp, 0, L, lds);
+    return;
+  }
   const GateCtx cx = gate_ctx(g, L, base);
-  const bool use_inv = (mode == 1) && g[G_INV] >= 0;
-  const C2<R>* mp = pool + (use_inv ? g[G_INV] : g[G_MAT]);
-  const int mm = use_inv ? 0 : mode;
+  if (!cx.ok) return;
+  const int ra = g[G_RA], rb = g[G_RB];
+  const int hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
   switch (kind) {
     case AK_MC1: {
       Mat<R, 2> m;
-      load_mat<R, 2>(mp, m, mm);
-      JQ_CASE1(apply_mc1, re, im, m, cx.creg, cx.ok)
+      load2<R>(mp, m);
+      if (cx.creg == 0) {
+        JQ_CASE1(fw_dense<R, NR, 2, G1<NR, A>, false>(re, im, m, 0))
+      } else {
+        JQ_CASE1(fw_dense<R, NR, 2, G1<NR, A>, true>(re, im, m, cx.creg))
+      }
       break;
     }
     case AK_MC2: {
       Mat<R, 4> m;
-      load_mat<R, 4>(mp, m, mm);
-      JQ_CASE2(apply_mc2, re, im, m, cx.creg, cx.ok)
+      load4<R>(mp, m, ra < rb);
+      if (cx.creg == 0) {
+        JQ_CASE2(fw_dense<R, NR, 4, G2<NR, A, B>, false>(re, im, m, 0))
+      } else {
+        JQ_CASE2(fw_dense<R, NR, 4, G2<NR, A, B>, true>(re, im, m, cx.creg))
+      }
       break;
     }
     case AK_MCX: {
-      JQ_CASE1(apply_mcx, re, im, cx.creg, cx.ok)
+      JQ_CASE1(perm_x<R, NR, A>(re, im, cx.creg))
       break;
     }
     case AK_MCSWAP: {
-      JQ_CASE2(apply_mcswap, re, im, cx.creg, cx.ok)
+      JQ_CASE2(perm_swap<R, NR, A, B>(re, im, cx.creg))
       break;
     }
     case AK_DIAG: {
       DiagLoc dl;
       decode_diag(g, L, base, dl);
-      apply_diag<R, NR>(re, im, mp, dl, use_inv ? 0 : mode);
-      break;
-    }
-    case AK_DENSEK: {
-      if constexpr (RB >= 3) apply_dense3<R, RB>(re, im, mp, mm, L, lds);
+      apply_diag<R, RB>(re, im, mp, dl, false);
       break;
     }
     default:
@@ -481,69 +662,269 @@ __device__ __forceinline__ void apply_gate(R* re, R* im, const int* g, const C2<
   }
 }
 
+// Backward: psi <- U^{-1} psi, bar += lambda (x) psi, lambda <- U^T lambda.
 template <typename R, int RB>
-__device__ __forceinline__ void accumulate_grad(const R* pr, const R* pi, const R* lr, const R* li, const int* g, const Layout& L,
-                                uint64_t base, double* slots) {
+__device__ __forceinline__ void bwd_gate(R* pr, R* pi, R* lr, R* li, const int* g, const C2<R>* pool,
+                                         const Layout& L, uint64_t base, C2<R>* lds, double* slots) {
   constexpr int NR = 1 << RB;
   const int kind = g[G_KIND];
-  const int ra = g[G_RA], rb = g[G_RB];
+  const C2<R>* mp = pool + g[G_MAT];
+  const bool inv = g[G_INV] >= 0;
+  const C2<R>* ip = pool + (inv ? g[G_INV] : g[G_MAT]);
   const int slot = g[G_SLOT];
+  if (kind == AK_DENSEK) {
+    if constexpr (RB >= 3) {
+      dense3<R, RB>(pr, pi, inv ? ip : mp, inv ? 0 : 1, L, lds);
+      dense3<R, RB>(lr, li, mp, 2, L, lds);
+    }
+    return;
+  }
   const GateCtx cx = gate_ctx(g, L, base);
-  if (kind == AK_MC1) {
-    R acc[8];
+  const int ra = g[G_RA], rb = g[G_RB];
+  const int hi = ra > rb ? ra : rb, lo = ra > rb ? rb : ra;
+  switch (kind) {
+    case AK_MC1: {
+      Mat<R, 2> u, ui;
+      load2<R>(mp, u);
+      if (inv)
+        load2<R>(ip, ui);
+      else
+        adjoint_of<R, 2>(u, ui);
+      R acc[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) acc[e] = 0;
-    JQ_CASE1(grad_mc1, pr, pi, lr, li, cx.creg, cx.ok, acc)
-    flush_vals<R>(acc, 8, slots, slot);
-  } else if (kind == AK_MC2) {
-    JQ_CASE2(grad_mc2, pr, pi, lr, li, cx.creg, cx.ok, slots, slot)
-  } else if (kind == AK_DIAG) {
-    DiagLoc dl;
-    decode_diag(g, L, base, dl);
-    R acc[8];
+      for (int e = 0; e < 8; ++e) acc[e] = 0;
+      if (cx.ok) {
+        if (cx.creg == 0) {
+          JQ_CASE1(bw_dense<R, NR, 2, G1<NR, A>, false>(pr, pi, lr, li, u, ui, 0, acc))
+        } else {
+          JQ_CASE1(bw_dense<R, NR, 2, G1<NR, A>, true>(pr, pi, lr, li, u, ui, cx.creg, acc))
+        }
+      }
+      if (slot >= 0) flush_vals<R>(acc, 8, slots, slot);
+      break;
+    }
+    case AK_MC2: {
+      const bool sw = ra < rb;
+      {
+        Mat<R, 4> ui;
+        if (inv) {
+          load4<R>(ip, ui, sw);
+        } else {
+          Mat<R, 4> u;
+          load4<R>(mp, u, sw);
+          adjoint_of<R, 4>(u, ui);
+        }
+        if (cx.ok) {
+          JQ_CASE2(sweep_apply<R, NR, 4, G2<NR, A, B>>(pr, pi, ui, cx.creg, false))
+        }
+      }
+      if (slot >= 0) {
+        // cotangent rows in the block's own target order: kernel row s(o)
 #pragma unroll
-    for (int e = 0; e < 8; ++e) acc[e] = 0;
-    grad_diag<R, NR>(pr, pi, lr, li, dl, acc);
-    flush_vals<R>(acc, 2 << dl.k, slots, slot);
+        for (int half = 0; half < 2; ++half) {
+          R acc[16];
+#pragma unroll
+          for (int e = 0; e < 16; ++e) acc[e] = 0;
+          if (cx.ok) {
+            JQ_CASE2(sweep_grad<R, NR, 4, G2<NR, A, B>>(pr, pi, lr, li, cx.creg, 2 * half, acc))
+          }
+          // kernel rows 2h, 2h+1; kernel column c -> block (s(row), s(c)) when swapped
+#pragma unroll
+          for (int oo = 0; oo < 2; ++oo)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              const int ko = 2 * half + oo;
+              const int bo = sw ? (((ko & 1) << 1) | (ko >> 1)) : ko;
+              const int bc = sw ? (((c & 1) << 1) | (c >> 1)) : c;
+              lds_add(&slots[slot + 2 * (bo * 4 + bc)], (double)wave_sum<R>(acc[2 * (oo * 4 + c)]));
+              lds_add(&slots[slot + 2 * (bo * 4 + bc) + 1], (double)wave_sum<R>(acc[2 * (oo * 4 + c) + 1]));
+            }
+        }
+      }
+      {
+        Mat<R, 4> u;
+        load4<R>(mp, u, sw);
+        if (cx.ok) {
+          JQ_CASE2(sweep_apply<R, NR, 4, G2<NR, A, B>>(lr, li, u, cx.creg, true))
+        }
+      }
+      break;
+    }
+    case AK_MCX: {
+      if (cx.ok) {
+        JQ_CASE1(perm_x<R, NR, A>(pr, pi, cx.creg); perm_x<R, NR, A>(lr, li, cx.creg))
+      }
+      break;
+    }
+    case AK_MCSWAP: {
+      if (cx.ok) {
+        JQ_CASE2(perm_swap<R, NR, A, B>(pr, pi, cx.creg); perm_swap<R, NR, A, B>(lr, li, cx.creg))
+      }
+      break;
+    }
+    case AK_DIAG: {
+      DiagLoc dl;
+      decode_diag(g, L, base, dl);
+      if (inv)
+        apply_diag<R, RB>(pr, pi, ip, dl, false);
+      else
+        apply_diag<R, RB>(pr, pi, mp, dl, true);
+      if (slot >= 0) {
+        R acc[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[e] = 0;
+        if (dl.k == 1 && dl.rbit[0] >= 0) {
+          const int ra1 = dl.rbit[0];
+          const int ra = ra1;
+          JQ_CASE1(diag1_grad<R, NR, A>(pr, pi, lr, li, acc))
+        } else {
+          diag_grad<R, NR>(pr, pi, lr, li, dl, acc);
+        }
+        flush_vals<R>(acc, 2 << dl.k, slots, slot);
+      }
+      apply_diag<R, RB>(lr, li, mp, dl, false);
+      break;
+    }
+    default:
+      break;
   }
 }
 
 // ---------------------------------------------------------------------------
 // global <-> register, register <-> LDS
 // ---------------------------------------------------------------------------
+// Canonical layout: register bit 0 <-> tile bit 0, thread bit j <-> tile bit
+// 1+j, register bits 1.. <-> the top tile bits.  Tile bits 0..6 are the
+// lowest global bits, so a wave reads 64 consecutive amplitude pairs (16-byte
+// vector accesses, 1 KiB contiguous for complex64) per register pair.
+template <int RB, int LNT>
+__device__ __forceinline__ void canonical_layout(const int* tbits, Layout& L) {
+  constexpr int T = RB + LNT;
+  uint64_t gthr = 0;
+  const uint32_t tid = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < LNT; ++j) gthr |= (uint64_t)((tid >> j) & 1u) << tbits[1 + j];
+  L.toff = tid << 1;
+  L.gthr = gthr;
+  L.rpos[0] = 0;
+  L.gpos[0] = 1ull << tbits[0];
+#pragma unroll
+  for (int b = 1; b < RB; ++b) {
+    L.rpos[b] = T - RB + b;
+    L.gpos[b] = 1ull << tbits[T - RB + b];
+  }
+}
+
+template <int RB>
+__device__ __forceinline__ bool same_layout(const int* sub, const Layout& L) {
+  bool same = true;
+#pragma unroll
+  for (int b = 0; b < RB; ++b) same = same && ((uint32_t)sub[1 + b] == L.rpos[b]);
+  return same;
+}
+
+// Load/store under the canonical layout: registers (2k, 2k+1) hold adjacent
+// amplitudes, moved as one 16-byte (complex64) or two 16-byte (complex128)
+// vector accesses.
 template <typename R, int RB>
-__device__ __forceinline__ void load_global(R* re, R* im, const C2<R>* src, uint64_t vec_len, const Layout& L,
-                                            uint64_t base, bool zero_init) {
+__device__ __forceinline__ void zero_state(R* re, R* im, const Layout& L, uint64_t base) {
   constexpr int NR = 1 << RB;
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
-    const uint64_t g = reg_gidx<RB>(L, base, i);
-    if (zero_init) {
-      re[i] = g == 0 ? R(1) : R(0);
-      im[i] = 0;
-    } else if (g < vec_len) {
-      const C2<R> v = src[g];
-      re[i] = v.x;
-      im[i] = v.y;
-    } else {
-      re[i] = 0;
-      im[i] = 0;
+    re[i] = (reg_gidx<RB>(L, base, i) == 0) ? R(1) : R(0);
+    im[i] = 0;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void ld_pair(const C2<R>* p, R& r0, R& i0, R& r1, R& i1) {
+  if constexpr (sizeof(R) == 4) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    r0 = v.x;
+    i0 = v.y;
+    r1 = v.z;
+    i1 = v.w;
+  } else {
+    const double2 v0 = *reinterpret_cast<const double2*>(p);
+    const double2 v1 = *reinterpret_cast<const double2*>(p + 1);
+    r0 = v0.x;
+    i0 = v0.y;
+    r1 = v1.x;
+    i1 = v1.y;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void st_pair(C2<R>* p, R r0, R i0, R r1, R i1) {
+  if constexpr (sizeof(R) == 4) {
+    *reinterpret_cast<float4*>(p) = make_float4(r0, i0, r1, i1);
+  } else {
+    *reinterpret_cast<double2*>(p) = make_double2(r0, i0);
+    *reinterpret_cast<double2*>(p + 1) = make_double2(r1, i1);
+  }
+}
+
+// Load/store under the canonical layout.  When the vector is smaller than
+// a tile (phantom tile bits) the out-of-range pairs are masked; otherwise
+// every access is unconditional so the loads issue back to back.
+template <typename R, int RB>
+__device__ __forceinline__ void load_canon(R* re, R* im, const C2<R>* src, uint64_t vec_len, const Layout& L,
+                                           uint64_t base, bool full) {
+  constexpr int NR = 1 << RB;
+  if (full && vec_len <= (1ull << 27)) {
+    // uniform pointer (scalar registers) + one 32-bit per-thread byte offset
+    const uint32_t tb = (uint32_t)L.gthr * (uint32_t)sizeof(C2<R>);
+#pragma unroll
+    for (int i = 0; i < NR; i += 2) {
+      const char* up = reinterpret_cast<const char*>(src + (base | reg_goff<RB>(L, i)));
+      ld_pair<R>(reinterpret_cast<const C2<R>*>(up + tb), re[i], im[i], re[i + 1], im[i + 1]);
+    }
+  } else if (full) {
+#pragma unroll
+    for (int i = 0; i < NR; i += 2) ld_pair<R>(src + reg_gidx<RB>(L, base, i), re[i], im[i], re[i + 1], im[i + 1]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < NR; i += 2) {
+      const uint64_t g = reg_gidx<RB>(L, base, i);
+      const bool ok = g < vec_len;
+      ld_pair<R>(src + (ok ? g : 0), re[i], im[i], re[i + 1], im[i + 1]);
+      re[i] = ok ? re[i] : R(0);
+      im[i] = ok ? im[i] : R(0);
+      re[i + 1] = ok ? re[i + 1] : R(0);
+      im[i + 1] = ok ? im[i + 1] : R(0);
     }
   }
 }
 
+// An empty asm barrier: the per-thread address base becomes opaque, so the
+// compiler recomputes store addresses instead of keeping the load addresses
+// of all registers live across the gate loop.
+__device__ __forceinline__ uint64_t opaque(uint64_t x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 template <typename R, int RB>
-__device__ __forceinline__ void store_global(const R* re, const R* im, C2<R>* dst, uint64_t vec_len, const Layout& L,
-                                             uint64_t base) {
+__device__ __forceinline__ void store_canon(const R* re, const R* im, C2<R>* dst, uint64_t vec_len, const Layout& L0,
+                                            uint64_t base, bool full) {
   constexpr int NR = 1 << RB;
+  Layout L = L0;
+  L.gthr = opaque(L0.gthr);
+  if (full && vec_len <= (1ull << 27)) {
+    const uint32_t tb = (uint32_t)L.gthr * (uint32_t)sizeof(C2<R>);
 #pragma unroll
-  for (int i = 0; i < NR; ++i) {
-    const uint64_t g = reg_gidx<RB>(L, base, i);
-    if (g < vec_len) {
-      C2<R> v;
-      v.x = re[i];
-      v.y = im[i];
-      dst[g] = v;
+    for (int i = 0; i < NR; i += 2) {
+      char* up = reinterpret_cast<char*>(dst + (base | reg_goff<RB>(L, i)));
+      st_pair<R>(reinterpret_cast<C2<R>*>(up + tb), re[i], im[i], re[i + 1], im[i + 1]);
+    }
+  } else if (full) {
+#pragma unroll
+    for (int i = 0; i < NR; i += 2) st_pair<R>(dst + reg_gidx<RB>(L, base, i), re[i], im[i], re[i + 1], im[i + 1]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < NR; i += 2) {
+      const uint64_t g = reg_gidx<RB>(L, base, i);
+      if (g < vec_len) st_pair<R>(dst + g, re[i], im[i], re[i + 1], im[i + 1]);
     }
   }
 }
@@ -551,17 +932,20 @@ __device__ __forceinline__ void store_global(const R* re, const R* im, C2<R>* ds
 template <typename R, int RB>
 __device__ __forceinline__ void relayout(R* re, R* im, C2<R>* lds, const Layout& from, const Layout& to) {
   constexpr int NR = 1 << RB;
+  // The swizzle is GF(2)-linear and thread / register bits are disjoint, so
+  // swz(toff | roff) = swz(toff) ^ swz(roff): one XOR per access.
+  const uint32_t sf = lds_swz(from.toff), st = lds_swz(to.toff);
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
     C2<R> v;
     v.x = re[i];
     v.y = im[i];
-    lds[lds_swz(reg_toff<RB>(from, i))] = v;
+    lds[sf ^ lds_swz(reg_roff<RB>(from, i))] = v;
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
-    const C2<R> v = lds[lds_swz(reg_toff<RB>(to, i))];
+    const C2<R> v = lds[st ^ lds_swz(reg_roff<RB>(to, i))];
     re[i] = v.x;
     im[i] = v.y;
   }
@@ -586,111 +970,247 @@ __device__ __forceinline__ uint64_t tile_base(int tile, const int* tbits, int T,
 // ---------------------------------------------------------------------------
 // observables
 // ---------------------------------------------------------------------------
+// Observable table, staged in LDS for the per-amplitude reductions.
+typedef ObsTable ObsLds;
+
+__device__ __forceinline__ void load_obs(const SvArgs& a, ObsLds* t) {
+  const ObsTable* src = reinterpret_cast<const ObsTable*>(a.obs);
+  if (src != nullptr) {
+    for (int o = threadIdx.x; o < 64; o += blockDim.x) t->gbit[o] = src->gbit[o];
+    for (int o = threadIdx.x; o < 128; o += blockDim.x) t->d[o] = src->d[o];
+  }
+  __syncthreads();
+}
+
 // weight of amplitude g for the lambda seed: sum_o ct_o d_o[bit]
-__device__ __forceinline__ double obs_weight(uint64_t g, const SvArgs& a, const double* ct) {
+__device__ __forceinline__ double obs_weight(uint64_t g, int n_obs, const ObsLds* t, const double* ct) {
   double w = 0;
-  for (int o = 0; o < a.n_obs; ++o) {
-    const int bit = (int)((g >> a.obs_gbit[o]) & 1ull);
-    w += ct[o] * a.obs_d[2 * o + bit];
+  for (int o = 0; o < n_obs; ++o) {
+    const int bit = (int)((g >> t->gbit[o]) & 1ull);
+    w += ct[o] * t->d[2 * o + bit];
   }
   return w;
 }
 
+// Register bit holding global bit gb, or -1 (wave-uniform).
+template <int RB>
+__device__ __forceinline__ int reg_bit_of(const Layout& L, int gb) {
+  int r = -1;
+#pragma unroll
+  for (int b = 0; b < RB; ++b)
+    if (L.gpos[b] == (1ull << gb)) r = b;
+  return r;
+}
+
+// Expectation-value partial sums of this tile (simulation.py:772-801 for
+// statevectors, the diagonal trace of simulation.py:850-852 for rho).  For
+// statevectors the probability mass is split once by register bit, so each
+// observable costs O(1) per thread.
 template <typename R, int RB>
-__device__ __forceinline__ void expval_partials(const R* re, const R* im, const SvArgs& a, const Layout& L, uint64_t base,
-                                double* ev_lds) {
+__device__ __forceinline__ void expval_partials(const R* re, const R* im, const SvArgs& a, const ObsLds* t,
+                                                const Layout& L, uint64_t base, double* ev_lds) {
   constexpr int NR = 1 << RB;
+  if (a.obs_kind == JQ_OBS_SV_DIAG) {
+    R ptot = 0, pb[RB];
+#pragma unroll
+    for (int b = 0; b < RB; ++b) pb[b] = 0;
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const R p = fma(re[i], re[i], im[i] * im[i]);
+      ptot += p;
+#pragma unroll
+      for (int b = 0; b < RB; ++b)
+        if ((i >> b) & 1) pb[b] += p;
+    }
+    const uint64_t tg = base | L.gthr;
+    for (int o = 0; o < a.n_obs; ++o) {
+      const int gb = t->gbit[o];
+      const R d0 = (R)t->d[2 * o], d1 = (R)t->d[2 * o + 1];
+      const int rb = reg_bit_of<RB>(L, gb);
+      R v;
+      if (rb >= 0) {
+        R p1 = 0;
+#pragma unroll
+        for (int b = 0; b < RB; ++b) p1 = b == rb ? pb[b] : p1;
+        v = d0 * (ptot - p1) + d1 * p1;
+      } else {
+        v = (((tg >> gb) & 1ull) ? d1 : d0) * ptot;
+      }
+      lds_add(&ev_lds[o], (double)wave_sum<R>(v));
+    }
+    return;
+  }
   const int nq = a.n_wires / 2;
   const uint64_t lowmask = (1ull << nq) - 1;
   for (int o = 0; o < a.n_obs; ++o) {
-    const int gb = a.obs_gbit[o];
-    const double d0 = a.obs_d[2 * o], d1 = a.obs_d[2 * o + 1];
+    const int gb = t->gbit[o];
+    const double d0 = t->d[2 * o], d1 = t->d[2 * o + 1];
     R acc = 0;
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
       const uint64_t g = reg_gidx<RB>(L, base, i);
       const R d = ((g >> gb) & 1ull) ? (R)d1 : (R)d0;
-      R v;
-      if (a.obs_kind == JQ_OBS_SV_DIAG)
-        v = re[i] * re[i] + im[i] * im[i];
-      else
-        v = ((g >> nq) == (g & lowmask)) ? re[i] : R(0);
+      const R v = ((g >> nq) == (g & lowmask)) ? re[i] : R(0);
       acc = fma(v, d, acc);
     }
-    const R s = wave_sum<R>(acc);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&ev_lds[o], (double)s);
+    lds_add(&ev_lds[o], (double)wave_sum<R>(acc));
   }
 }
 
-// ---------------------------------------------------------------------------
-// kernels
-// ---------------------------------------------------------------------------
 template <typename R, int RB, int NT>
-__device__ __forceinline__ void forward_tile(const SvArgs& a, int pass_idx, int circ, int tile, R* re, R* im, Layout& L,
-                             uint64_t& base, C2<R>* lds, bool do_load) {
+__device__ __forceinline__ void forward_tile(const SvArgs& a, int pass_idx, int circ, int tile, R* re, R* im,
+                                             Layout& L, uint64_t& base, C2<R>* lds, bool do_load) {
   const int* P = a.prog;
   const int T = P[2];
   const int* pass = P + P[HDR_WORDS + pass_idx];
   const int nsub = pass[0];
-  const int flags = pass[1];
   const int* tbits = pass + 4;
   constexpr int LNT = __builtin_ctz(NT);
   base = tile_base(tile, tbits, T, a.n_wires);
   const C2<R>* pool = reinterpret_cast<const C2<R>*>(a.mats) + (int64_t)circ * a.pool_stride;
   const int* soff = pass + 4 + T;
-  decode_layout<RB, LNT>(P + soff[0], tbits, L);
-  (void)flags;
+  canonical_layout<RB, LNT>(tbits, L);
   if (do_load) {
-    const bool zero = a.src == nullptr;
-    const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src);
-    load_global<R, RB>(re, im, zero ? nullptr : src + (int64_t)circ * a.src_stride, a.vec_len, L, base, zero);
+    if (a.src == nullptr)
+      zero_state<R, RB>(re, im, L, base);
+    else
+      load_canon<R, RB>(re, im, reinterpret_cast<const C2<R>*>(a.src) + (int64_t)circ * a.src_stride, a.vec_len, L,
+                        base, a.n_wires >= T);
   }
   for (int s = 0; s < nsub; ++s) {
     const int* sub = P + soff[s];
     const int ng = sub[0];
-    if (s > 0) {
+    if (!same_layout<RB>(sub, L)) {
       Layout nl;
       decode_layout<RB, LNT>(sub, tbits, nl);
       relayout<R, RB>(re, im, lds, L, nl);
       L = nl;
     }
     const int* g = sub + 1 + T;
-    for (int q = 0; q < ng; ++q, g += GW) apply_gate<R, RB>(re, im, g, pool, L, base, 0, lds);
+    for (int q = 0; q < ng; ++q, g += GW) fwd_gate<R, RB>(re, im, g, pool, L, base, lds);
+  }
+}
+
+// Bring the registers back to the canonical layout.
+template <typename R, int RB, int NT>
+__device__ __forceinline__ void to_canonical(R* re, R* im, const int* tbits, Layout& L, C2<R>* lds) {
+  constexpr int LNT = __builtin_ctz(NT);
+  Layout cl;
+  canonical_layout<RB, LNT>(tbits, cl);
+  bool same = true;
+#pragma unroll
+  for (int b = 0; b < RB; ++b) same = same && (cl.rpos[b] == L.rpos[b]);
+  if (!same) {
+    relayout<R, RB>(re, im, lds, L, cl);
+    L = cl;
   }
 }
 
 template <typename R, int RB, int NT>
-__global__ __launch_bounds__(NT) void sv_forward_kernel(SvArgs a) {
+__global__ __launch_bounds__(NT, 2) void sv_forward_kernel(SvArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int T = a.prog[2];
   C2<R>* lds = reinterpret_cast<C2<R>*>(smem);
-  double* ev = reinterpret_cast<double*>(smem + (sizeof(C2<R>) << T));
+  ObsLds* obs = reinterpret_cast<ObsLds*>(smem + (sizeof(C2<R>) << T));
+  double* ev = reinterpret_cast<double*>(obs + 1);
   const int circ = blockIdx.y;
   const int tile = blockIdx.x;
   const int* pass = a.prog + a.prog[HDR_WORDS + a.pass];
   const bool last = (pass[1] & PF_LAST) != 0;
   const bool want_ev = last && a.obs_kind != JQ_OBS_NONE && a.ev_part != nullptr;
-  if (want_ev)
+  if (want_ev) {
     for (int o = threadIdx.x; o < a.n_obs; o += NT) ev[o] = 0;
+    load_obs(a, obs);
+  }
   R re[1 << RB], im[1 << RB];
   Layout L;
   uint64_t base;
   forward_tile<R, RB, NT>(a, a.pass, circ, tile, re, im, L, base, lds, true);
-  store_global<R, RB>(re, im, reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride, a.vec_len, L, base);
+  to_canonical<R, RB, NT>(re, im, pass + 4, L, lds);
+  store_canon<R, RB>(re, im, reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride, a.vec_len, L, base,
+                     a.n_wires >= T);
   if (want_ev) {
     __syncthreads();
-    expval_partials<R, RB>(re, im, a, L, base, ev);
+    expval_partials<R, RB>(re, im, a, obs, L, base, ev);
     __syncthreads();
     for (int o = threadIdx.x; o < a.n_obs; o += NT)
       a.ev_part[((int64_t)circ * gridDim.x + tile) * a.n_obs + o] = ev[o];
   }
 }
 
-// Backward sweep over one pass (reverse substeps, reverse gates).
-// psi: loaded from a.src (the pass output), lambda from a.lam or seeded.
+// Lambda seed: the JAX cotangent of the measurement (simulation.py:617),
+// 2 conj(psi) sum_o ct_o d_o for statevectors, sum_o ct_o d_o on the
+// diagonal of rho for density matrices.
+template <typename R, int RB>
+__device__ __forceinline__ void seed_lambda(const R* pr, const R* pi, R* lr, R* li, const SvArgs& a, const ObsLds* t,
+                                            int circ, const Layout& L, uint64_t base) {
+  constexpr int NR = 1 << RB;
+  const double* ct = a.ct + (int64_t)circ * a.n_obs;
+  if (a.obs_kind == JQ_OBS_SV_DIAG) {
+    // w(i) = c + sum_b A_b bit_b(i): observables on thread / outside bits
+    // fold into c, those on register bit b into A_b.
+    const uint64_t tg = base | L.gthr;
+    double c = 0, A[RB];
+#pragma unroll
+    for (int b = 0; b < RB; ++b) A[b] = 0;
+    for (int o = 0; o < a.n_obs; ++o) {
+      const int gb = t->gbit[o];
+      const double d0 = t->d[2 * o], d1 = t->d[2 * o + 1], w = ct[o];
+      const int rb = reg_bit_of<RB>(L, gb);
+      if (rb >= 0) {
+        c += w * d0;
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+          if (b == rb) A[b] += w * (d1 - d0);
+      } else {
+        c += w * (((tg >> gb) & 1ull) ? d1 : d0);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      double w = c;
+#pragma unroll
+      for (int b = 0; b < RB; ++b)
+        if ((i >> b) & 1) w += A[b];
+      const R w2 = (R)(2.0 * w);
+      lr[i] = w2 * pr[i];
+      li[i] = -w2 * pi[i];
+    }
+    return;
+  }
+  const int nq = a.n_wires / 2;
+  const uint64_t lowmask = (1ull << nq) - 1;
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    const uint64_t g = reg_gidx<RB>(L, base, i);
+    const double w = obs_weight(g, a.n_obs, t, ct);
+    lr[i] = ((g >> nq) == (g & lowmask) && g < a.vec_len) ? (R)w : R(0);
+    li[i] = 0;
+  }
+}
+
 template <typename R, int RB, int NT>
-__device__ __forceinline__ void backward_tile(const SvArgs& a, int pass_idx, int circ, int tile, R* pr, R* pi, R* lr, R* li,
-                              Layout& L, uint64_t& base, C2<R>* lds, double* slots, bool loaded) {
+__device__ __forceinline__ void to_canonical2(R* pr, R* pi, R* lr, R* li, const int* tbits, Layout& L, C2<R>* lds) {
+  constexpr int LNT = __builtin_ctz(NT);
+  Layout cl;
+  canonical_layout<RB, LNT>(tbits, cl);
+  bool same = true;
+#pragma unroll
+  for (int b = 0; b < RB; ++b) same = same && (cl.rpos[b] == L.rpos[b]);
+  if (!same) {
+    relayout<R, RB>(pr, pi, lds, L, cl);
+    relayout<R, RB>(lr, li, lds, L, cl);
+    L = cl;
+  }
+}
+
+// Backward sweep over one pass (reverse substeps, reverse gates).  When
+// `loaded` is false, psi (the pass output) and lambda are read under the
+// canonical layout; otherwise they arrive in registers under layout L.
+template <typename R, int RB, int NT>
+__device__ __forceinline__ void backward_tile(const SvArgs& a, int pass_idx, int circ, int tile, R* pr, R* pi, R* lr,
+                                              R* li, Layout& L, uint64_t& base, C2<R>* lds, double* slots,
+                                              const ObsLds* obs, bool loaded) {
   const int* P = a.prog;
   const int T = P[2];
   const int* pass = P + P[HDR_WORDS + pass_idx];
@@ -698,39 +1218,23 @@ __device__ __forceinline__ void backward_tile(const SvArgs& a, int pass_idx, int
   const int flags = pass[1];
   const int* tbits = pass + 4;
   constexpr int LNT = __builtin_ctz(NT);
-  constexpr int NR = 1 << RB;
   base = tile_base(tile, tbits, T, a.n_wires);
   const C2<R>* pool = reinterpret_cast<const C2<R>*>(a.mats) + (int64_t)circ * a.pool_stride;
   const int* soff = pass + 4 + T;
-  decode_layout<RB, LNT>(P + soff[nsub - 1], tbits, L);
   if (!loaded) {
+    canonical_layout<RB, LNT>(tbits, L);
     const C2<R>* ps = reinterpret_cast<const C2<R>*>(a.src) + (int64_t)circ * a.src_stride;
-    load_global<R, RB>(pr, pi, ps, a.vec_len, L, base, false);
+    load_canon<R, RB>(pr, pi, ps, a.vec_len, L, base, a.n_wires >= T);
     if (flags & PF_LAST) {
-      // lambda seed: JAX cotangent of the measurement (simulation.py:617)
-      const double* ct = a.ct + (int64_t)circ * a.n_obs;
-      const int nq = a.n_wires / 2;
-      const uint64_t lowmask = (1ull << nq) - 1;
-#pragma unroll
-      for (int i = 0; i < NR; ++i) {
-        const uint64_t g = reg_gidx<RB>(L, base, i);
-        const double w = obs_weight(g, a, ct);
-        if (a.obs_kind == JQ_OBS_SV_DIAG) {
-          lr[i] = (R)(2.0 * w) * pr[i];
-          li[i] = -(R)(2.0 * w) * pi[i];
-        } else {
-          lr[i] = ((g >> nq) == (g & lowmask) && g < a.vec_len) ? (R)w : R(0);
-          li[i] = 0;
-        }
-      }
+      seed_lambda<R, RB>(pr, pi, lr, li, a, obs, circ, L, base);
     } else {
       const C2<R>* ls = reinterpret_cast<const C2<R>*>(a.lam) + (int64_t)circ * a.lam_stride;
-      load_global<R, RB>(lr, li, ls, a.vec_len, L, base, false);
+      load_canon<R, RB>(lr, li, ls, a.vec_len, L, base, a.n_wires >= T);
     }
   }
   for (int s = nsub - 1; s >= 0; --s) {
     const int* sub = P + soff[s];
-    if (s < nsub - 1) {
+    if (!same_layout<RB>(sub, L)) {
       Layout nl;
       decode_layout<RB, LNT>(sub, tbits, nl);
       relayout<R, RB>(pr, pi, lds, L, nl);
@@ -741,11 +1245,10 @@ __device__ __forceinline__ void backward_tile(const SvArgs& a, int pass_idx, int
     const int* g0 = sub + 1 + T;
     for (int q = ng - 1; q >= 0; --q) {
       const int* g = g0 + q * GW;
-      apply_gate<R, RB>(pr, pi, g, pool, L, base, 1, lds);
-      if (g[G_SLOT] >= 0 && slots != nullptr) accumulate_grad<R, RB>(pr, pi, lr, li, g, L, base, slots);
-      apply_gate<R, RB>(lr, li, g, pool, L, base, 2, lds);
+      bwd_gate<R, RB>(pr, pi, lr, li, g, pool, L, base, lds, slots);
     }
   }
+  to_canonical2<R, RB, NT>(pr, pi, lr, li, tbits, L, lds);
 }
 
 template <typename R, int RB, int NT>
@@ -753,97 +1256,44 @@ __global__ __launch_bounds__(NT) void sv_backward_kernel(SvArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int T = a.prog[2];
   C2<R>* lds = reinterpret_cast<C2<R>*>(smem);
-  double* slots = reinterpret_cast<double*>(smem + (sizeof(C2<R>) << T));
+  ObsLds* obs = reinterpret_cast<ObsLds*>(smem + (sizeof(C2<R>) << T));
+  double* slots = reinterpret_cast<double*>(obs + 1);
   const int circ = blockIdx.y;
   const int tile = blockIdx.x;
   const int* pass = a.prog + a.prog[HDR_WORDS + a.pass];
   const int nslots = pass[2];
   const int slot_base = pass[3];
   for (int e = threadIdx.x; e < nslots; e += NT) slots[e] = 0;
-  __syncthreads();
+  load_obs(a, obs);
   R pr[1 << RB], pi[1 << RB], lr[1 << RB], li[1 << RB];
   Layout L;
   uint64_t base;
-  backward_tile<R, RB, NT>(a, a.pass, circ, tile, pr, pi, lr, li, L, base, lds, slots, false);
+  backward_tile<R, RB, NT>(a, a.pass, circ, tile, pr, pi, lr, li, L, base, lds, slots, obs, false);
   C2<R>* lam = reinterpret_cast<C2<R>*>(a.lam) + (int64_t)circ * a.lam_stride;
-  store_global<R, RB>(lr, li, lam, a.vec_len, L, base);
+  store_canon<R, RB>(lr, li, lam, a.vec_len, L, base, a.n_wires >= T);
   if (a.dst != nullptr)
-    store_global<R, RB>(pr, pi, reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride, a.vec_len, L, base);
+    store_canon<R, RB>(pr, pi, reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride, a.vec_len, L, base, a.n_wires >= T);
   __syncthreads();
   double* gp = a.grad_part + (int64_t)circ * a.part_stride + slot_base + (int64_t)tile * nslots;
   for (int e = threadIdx.x; e < nslots; e += NT) gp[e] = slots[e];
-}
-
-// Whole circuit in one workgroup (the vector fits one tile): forward, the
-// expectation values, the lambda seed and the backward sweep without the
-// vector ever leaving registers.
-template <typename R, int RB, int NT>
-__global__ __launch_bounds__(NT) void sv_fused_kernel(SvArgs a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int T = a.prog[2];
-  C2<R>* lds = reinterpret_cast<C2<R>*>(smem);
-  double* ev = reinterpret_cast<double*>(smem + (sizeof(C2<R>) << T));
-  const int* pass = a.prog + a.prog[HDR_WORDS + 0];
-  const int nslots = pass[2];
-  double* slots = ev + 64;
-  const int circ = blockIdx.y;
-  for (int e = threadIdx.x; e < nslots; e += NT) slots[e] = 0;
-  for (int o = threadIdx.x; o < 64; o += NT) ev[o] = 0;
-  __syncthreads();
-  constexpr int NR = 1 << RB;
-  R pr[NR], pi[NR], lr[NR], li[NR];
-  Layout L;
-  uint64_t base;
-  forward_tile<R, RB, NT>(a, 0, circ, 0, pr, pi, L, base, lds, true);
-  if (a.dst != nullptr)
-    store_global<R, RB>(pr, pi, reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride, a.vec_len, L, base);
-  if (a.ev_part != nullptr) {
-    expval_partials<R, RB>(pr, pi, a, L, base, ev);
-  }
-  // seed lambda under the last forward layout
-  {
-    const double* ct = a.ct + (int64_t)circ * a.n_obs;
-    const int nq = a.n_wires / 2;
-    const uint64_t lowmask = (1ull << nq) - 1;
-#pragma unroll
-    for (int i = 0; i < NR; ++i) {
-      const uint64_t g = reg_gidx<RB>(L, base, i);
-      const double w = obs_weight(g, a, ct);
-      if (a.obs_kind == JQ_OBS_SV_DIAG) {
-        lr[i] = (R)(2.0 * w) * pr[i];
-        li[i] = -(R)(2.0 * w) * pi[i];
-      } else {
-        lr[i] = ((g >> nq) == (g & lowmask) && g < a.vec_len) ? (R)w : R(0);
-        li[i] = 0;
-      }
-    }
-  }
-  backward_tile<R, RB, NT>(a, 0, circ, 0, pr, pi, lr, li, L, base, lds, slots, true);
-  if (a.lam != nullptr)
-    store_global<R, RB>(lr, li, reinterpret_cast<C2<R>*>(a.lam) + (int64_t)circ * a.lam_stride, a.vec_len, L, base);
-  __syncthreads();
-  double* gp = a.grad_part + (int64_t)circ * a.part_stride + pass[3];
-  for (int e = threadIdx.x; e < nslots; e += NT) gp[e] = slots[e];
-  if (a.ev_part != nullptr)
-    for (int o = threadIdx.x; o < a.n_obs; o += NT) a.ev_part[(int64_t)circ * a.n_obs + o] = ev[o];
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <typename R, int RB, int NT>
-hipError_t launch_cfg(int which, const SvArgs& a, int ntiles, int batch, size_t smem, hipStream_t st) {
-  dim3 grid(ntiles, batch);
-  if (which == 0) {
-    (void)hipFuncSetAttribute((const void*)sv_forward_kernel<R, RB, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    hipLaunchKernelGGL((sv_forward_kernel<R, RB, NT>), grid, dim3(NT), smem, st, a);
-  } else if (which == 1) {
-    (void)hipFuncSetAttribute((const void*)sv_backward_kernel<R, RB, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    hipLaunchKernelGGL((sv_backward_kernel<R, RB, NT>), grid, dim3(NT), smem, st, a);
-  } else {
-    (void)hipFuncSetAttribute((const void*)sv_fused_kernel<R, RB, NT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    hipLaunchKernelGGL((sv_fused_kernel<R, RB, NT>), grid, dim3(NT), smem, st, a);
-  }
+hipError_t launch_forward(const SvArgs& a, int ntiles, int batch, size_t smem, hipStream_t st) {
+  (void)hipFuncSetAttribute((const void*)sv_forward_kernel<R, RB, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)smem);
+  hipLaunchKernelGGL((sv_forward_kernel<R, RB, NT>), dim3(ntiles, batch), dim3(NT), smem, st, a);
+  return hipGetLastError();
+}
+
+template <typename R, int RB, int NT>
+hipError_t launch_backward(const SvArgs& a, int ntiles, int batch, size_t smem, hipStream_t st) {
+  (void)hipFuncSetAttribute((const void*)sv_backward_kernel<R, RB, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)smem);
+  hipLaunchKernelGGL((sv_backward_kernel<R, RB, NT>), dim3(ntiles, batch), dim3(NT), smem, st, a);
   return hipGetLastError();
 }
 

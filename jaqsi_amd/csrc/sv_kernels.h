@@ -20,13 +20,22 @@ struct SvArgs {
   int64_t lam_stride;
   int obs_kind;
   int n_obs;
-  int obs_gbit[64];      // global bit of each observable wire
-  double obs_d[128];     // [n_obs][2] diagonal of each observable
+  const void* obs;       // device ObsTable (observable wires and diagonals)
   const double* ct;      // [batch][n_obs] cotangents (device)
   double* ev_part;       // [batch][ntiles][n_obs]
   double* grad_part;     // [batch][part_stride]
   int64_t part_stride;
 };
+
+// Observable table: global bit and diagonal (d0, d1) of each observable wire.
+struct ObsTable {
+  int gbit[64];
+  double d[128];
+};
+
+// Copy a host-built table to device memory on the stream (a tiny kernel, so
+// the call stays asynchronous and capturable).
+hipError_t launch_obs_upload(const ObsTable& t, void* dst, hipStream_t st);
 
 // which: 0 forward pass, 1 backward pass, 2 fused single-tile forward+backward
 hipError_t launch_sv_f32(int which, int T, const SvArgs& a, int ntiles, int batch, size_t smem, hipStream_t st);

@@ -66,6 +66,7 @@ def member_full(mt, theta, xm, deriv=False):
 class Program:
     def __init__(self, plan):
         self.w = plan.program()
+        self.wb = plan.table(4)
         self.bt = plan.table(0).reshape(-1, BW)
         self.ml = plan.table(1)
         self.mt = plan.table(2).reshape(-1, MW)
@@ -100,8 +101,9 @@ class Program:
         return M
 
     # ------------------------------------------------------------------
-    def passes(self):
-        w, T = self.w, self.T
+    def passes(self, backward=False):
+        w, T = (self.wb if backward else self.w), self.T
+        RB = int(w[3])
         for p in range(self.P):
             rec = int(w[HDR + p])
             nsub, flags, nslots, sbase = (int(x) for x in w[rec : rec + 4])
@@ -110,8 +112,8 @@ class Program:
             subs = []
             for o in soff:
                 ng = int(w[o])
-                reg = [int(x) for x in w[o + 1 : o + 1 + self.RB]]
-                thr = [int(x) for x in w[o + 1 + self.RB : o + 1 + T]]
+                reg = [int(x) for x in w[o + 1 : o + 1 + RB]]
+                thr = [int(x) for x in w[o + 1 + RB : o + 1 + T]]
                 gates = [w[o + 1 + T + q * GW : o + 1 + T + (q + 1) * GW].astype(np.int64) for q in range(ng)]
                 subs.append((reg, thr, gates))
             yield p, flags, nslots, sbase, tbits, subs
@@ -146,7 +148,7 @@ class Program:
         creg, cthr = int(g[4]), int(g[5])
         cout = (int(g[7]) << 32) | (int(g[6]) & 0xFFFFFFFF)
         cmask = cthr
-        for b in range(self.RB):
+        for b in range(len(reg)):
             if (creg >> b) & 1:
                 cmask |= 1 << reg[b]
         ok = ((base & cout) == cout) and True
@@ -266,7 +268,7 @@ class Program:
         final = psi
         lam = lam_seed_fn(final)
         partial = {}
-        for p, flags, nslots, sbase, tbits, subs in reversed(list(self.passes())):
+        for p, flags, nslots, sbase, tbits, subs in reversed(list(self.passes(backward=True))):
             src = ck[p]
             newlam = lam.copy()
             for t, base in self.tiles(tbits):
@@ -288,7 +290,7 @@ class Program:
         for base_, nsl, local, bar, size in self.sm:
             pass
         # partial[(p, slot)] accumulated over tiles; map to bars by slot map
-        for p, flags, nslots, sbase, tbits, subs in self.passes():
+        for p, flags, nslots, sbase, tbits, subs in self.passes(backward=True):
             for reg, thr, gates in subs:
                 for g in gates:
                     if g[10] < 0:
@@ -307,7 +309,7 @@ class Program:
         creg, cthr = int(g[4]), int(g[5])
         cout = (int(g[7]) << 32) | (int(g[6]) & 0xFFFFFFFF)
         cmask = cthr
-        for b in range(self.RB):
+        for b in range(len(reg)):
             if (creg >> b) & 1:
                 cmask |= 1 << reg[b]
         act = ((t & cmask) == cmask) & ((base & cout) == cout)
