@@ -1,0 +1,88 @@
+"""Host-side API behaviour (no GPU): tape recording, validation, lowering."""
+
+import numpy as np
+import pytest
+import torch
+
+
+def test_recording_and_wire_validation():
+    import jaqsi_amd as J
+
+    with J.recording() as tape:
+        J.Gates.RX(0.1, wires=0)
+        J.Gates.CX(wires=[0, 1])
+        J.Gates.Barrier(wires=[0, 1])
+    assert [op.name for op in tape] == ["RX", "CX", "Barrier"]
+    with pytest.raises(ValueError):
+        J.CX(wires=[0, 0])
+    with pytest.raises(ValueError):
+        J.RX(0.1, wires=[0, 1])
+    with pytest.raises(TypeError):
+        J.Gates.RX(0.1, wires=0, pulse="yes")
+
+
+def test_noise_params_attach_channels():
+    import jaqsi_amd as J
+    from jaqsi_amd.noise import BitFlip, DepolarizingChannel, QubitChannel
+
+    with J.recording() as tape:
+        J.Gates.RY(0.2, wires=1, noise_params={"BitFlip": 0.1, "Depolarizing": 0.01})
+        J.Gates.CX(wires=[0, 1], noise_params={"MultiQubitDepolarizing": 0.02})
+    kinds = [type(op) for op in tape]
+    assert kinds[1] is BitFlip and kinds[2] is DepolarizingChannel and kinds[-1] is QubitChannel
+
+
+def test_kraus_channels_are_trace_preserving():
+    from jaqsi_amd import noise
+
+    for ch in (noise.BitFlip(0.1), noise.PhaseFlip(0.2), noise.DepolarizingChannel(0.3), noise.AmplitudeDamping(0.4),
+               noise.PhaseDamping(0.5), noise.ThermalRelaxationError(0.1, 1.0, 0.8, 0.2),
+               noise.ThermalRelaxationError(0.1, 1.0, 1.5, 0.2)):
+        ks = ch.kraus_matrices()
+        assert np.allclose(sum(k.conj().T @ k for k in ks), np.eye(2)), type(ch).__name__
+    with pytest.raises(ValueError):
+        noise.BitFlip(1.5)
+
+
+def test_gate_matrices_match_oracle():
+    import jaqsi_amd as J
+    from oracle import sv
+
+    t = 0.37
+    pairs = [
+        (J.RX(t, record=False), sv.Op("RX", (0,), (t,))),
+        (J.CRY(t, wires=[0, 1], record=False), sv.Op("CRY", (0, 1), (t,))),
+        (J.Rot(0.1, 0.2, 0.3, record=False), sv.Op("Rot", (0,), (0.1, 0.2, 0.3))),
+        (J.PauliRot(t, "XZ", wires=[0, 1], record=False), sv.Op("PauliRot", (0, 1), (t,), "XZ")),
+        (J.ControlledPhaseShift(t, record=False), sv.Op("CPhase", (0, 1), (t,))),
+        (J.CCX(record=False), sv.Op("CCX", (0, 1, 2))),
+    ]
+    for op, ref in pairs:
+        assert np.allclose(op.matrix.numpy(), sv.op_matrix(ref)), op.name
+
+
+def test_lowering_structure_key_ignores_values():
+    import jaqsi_amd as J
+    from jaqsi_amd import lowering
+
+    def tape(v):
+        with J.recording() as t:
+            J.RX(v, wires=0)
+            J.CX(wires=[0, 1])
+        return t
+
+    a, b = lowering.lower_pure(tape(0.1), 2), lowering.lower_pure(tape(0.9), 2)
+    assert a.key == b.key
+    assert lowering.angle_tensor(a.angle_values, None).tolist() == [0.1]
+
+
+def test_operation_algebra_on_host():
+    import jaqsi_amd as J
+
+    x = J.PauliX(0, record=False)
+    z = J.PauliZ(1, record=False)
+    prod = x @ z
+    assert prod.wires == [0, 1]
+    assert np.allclose(prod.matrix.numpy(), np.kron([[0, 1], [1, 0]], [[1, 0], [0, -1]]))
+    d = J.RX(0.3, record=False).dagger()
+    assert np.allclose((d.matrix @ J.RX(0.3, record=False).matrix).numpy(), np.eye(2), atol=1e-6)

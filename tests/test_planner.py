@@ -1,0 +1,80 @@
+"""The native planner's programs, executed on the CPU by tests/plan_interp.py,
+reproduce the oracle (statevector forward and adjoint gradients)."""
+
+import numpy as np
+import pytest
+
+from circuits import record_zoo, zoo_ops, zoo_params
+from oracle import sv
+
+
+def _plan(tape, n, dtype=0):
+    from jaqsi_amd import _native as nat, lowering
+
+    low = lowering.lower_pure(tape, n)
+    plan = nat.Plan(n, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, dtype)
+    angles = np.array([float(a) for a in low.angle_values])
+    xm = lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, None)
+    return plan, angles, (xm.numpy() if xm is not None else None)
+
+
+def _record(fn):
+    from jaqsi_amd.tape import recording
+
+    with recording() as tape:
+        fn()
+    return tape
+
+
+@pytest.mark.parametrize("dtype", [0, 1])
+@pytest.mark.parametrize("n,layers", [(3, 2), (9, 2), (14, 1), (16, 1)])
+def test_hea_program_matches_oracle(n, layers, dtype):
+    from circuits import record_hea
+    from plan_interp import Program
+
+    theta = np.random.default_rng(n).uniform(0, 2 * np.pi, (layers, n, 3))
+    plan, angles, xm = _plan(_record(lambda: record_hea(theta, n)), n, dtype)
+    prog = Program(plan)
+    ops = sv.hea_ops(theta, n)
+    np.testing.assert_allclose(prog.forward(angles, xm), sv.simulate(ops, n), atol=1e-12)
+    zs = sv.z_signs(n).sum(0)
+    _, _, grads, _ = prog.adjoint(angles, lambda f: 2 * zs * np.conj(f), xm)
+    _, og, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
+    np.testing.assert_allclose(grads, sv.flat_grads(ops, og), atol=1e-12)
+
+
+@pytest.mark.parametrize("dtype", [0, 1])
+def test_gate_zoo_program_matches_oracle(dtype):
+    from plan_interp import Program
+
+    p = zoo_params()
+    plan, angles, xm = _plan(_record(lambda: record_zoo(p)), 5, dtype)
+    prog = Program(plan)
+    ops = zoo_ops(p)
+    np.testing.assert_allclose(prog.forward(angles, xm), sv.simulate(ops, 5), atol=1e-12)
+    zs = sv.z_signs(5).sum(0)
+    _, _, grads, _ = prog.adjoint(angles, lambda f: 2 * zs * np.conj(f), xm)
+    _, og, _, _ = sv.adjoint(ops, 5, sv.z_obs(5))
+    np.testing.assert_allclose(grads, sv.flat_grads(ops, og), atol=1e-12)
+
+
+def test_c2_schedule_shape():
+    """The 24-qubit, 8-layer workload schedules into 3 passes per layer."""
+    from circuits import record_hea
+
+    theta = np.zeros((8, 24, 3))
+    plan, _, _ = _plan(_record(lambda: record_hea(theta, 24)), 24)
+    info = plan.info()
+    assert info["n_passes"] == 24 and info["n_blocks"] == 192 and info["tile_bits"] == 13
+
+
+def test_planner_errors_map_to_exceptions():
+    import jaqsi_amd as J
+    from jaqsi_amd import _native as nat, lowering
+
+    tape = _record(lambda: J.PauliRot(0.3, "XYZX", wires=[0, 1, 2, 3]))
+    low = lowering.lower_pure(tape, 4)
+    with pytest.raises(NotImplementedError):
+        nat.Plan(4, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0)
+    with pytest.raises(ValueError):
+        nat.Plan(2, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0)
