@@ -165,6 +165,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument(
+        "--copy-probe",
+        type=int,
+        default=0,
+        metavar="MIB",
+        help="only run a plain device copy of MIB MiB three times (calibrates the HBM byte counters)",
+    )
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -178,6 +185,14 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe << 20}), flush=True)
+        return
     res = bench_c2(args.steps, args.warmup)
     ms = res["seconds"] / args.steps * 1e3
     value = args.steps / res["seconds"]

@@ -24,14 +24,17 @@ struct KTimer {
 };
 KTimer g_timer;
 
-hipError_t timed_launch(int which, int dtype, int T, const jq::SvArgs& a, int ntiles, int batch, size_t smem,
-                        hipStream_t st) {
-  if (!g_timer.on) return jq::launch_sv(which, dtype, T, a, ntiles, batch, smem, st);
+// which: 0 forward pass, 1 backward pass; `full` selects the kernel variant
+// that also carries 4x4 register blocks and the dense 3-wire path.
+hipError_t timed_launch(int which, bool full, int dtype, int T, const jq::SvArgs& a, int ntiles, int batch,
+                        size_t smem, hipStream_t st) {
+  const int variant = which + (full ? 2 : 0);
+  if (!g_timer.on) return jq::launch_sv(variant, dtype, T, a, ntiles, batch, smem, st);
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   hipEventRecord(e0, st);
-  hipError_t e = jq::launch_sv(which, dtype, T, a, ntiles, batch, smem, st);
+  hipError_t e = jq::launch_sv(variant, dtype, T, a, ntiles, batch, smem, st);
   hipEventRecord(e1, st);
   g_timer.ev[which].push_back({e0, e1});
   return e;
@@ -86,6 +89,7 @@ struct jq_plan {
   DevBuf prog, prog_b, block_table, member_list, member_table, member_values, slot_map;
   int n_slots = 0;
   std::vector<int32_t> slot_words;
+  std::vector<char> pass_full;  // pass holds a 4x4 register block or a dense 3-wire block
   bool uploaded = false;
   // workspaces
   DevBuf pool, partials, bars, ev_part, lam, ckpt, status, obs;
@@ -133,6 +137,17 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
     if (lds_state + 64 * 4 + 128 * 8 + (64 + (size_t)pass[2]) * 8 > 160 * 1024) {
       delete p;
       return fail(JQ_ERR_UNSUPPORTED, "too many gradient slots in one pass for LDS");
+    }
+  }
+  p->pass_full.assign(p->hp.n_passes, 0);
+  for (int q = 0; q < p->hp.n_passes; ++q) {
+    const int* pass = W + W[jq::HDR_WORDS + q];
+    const int* soff = pass + 4 + T;
+    for (int s = 0; s < pass[0]; ++s) {
+      const int* sub = W + soff[s];
+      const int* g = sub + 1 + T;
+      for (int k = 0; k < sub[0]; ++k, g += jq::GW)
+        if (g[jq::G_KIND] == jq::AK_MC2 || g[jq::G_KIND] == jq::AK_DENSEK) p->pass_full[q] = 1;
     }
   }
   std::vector<int32_t> sm;
@@ -317,7 +332,7 @@ int jq_sv_forward(jq_plan* p, int32_t batch, const double* angles, const void* m
     a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
     a.dst = state;
     a.dst_stride = (int64_t)a.vec_len;
-    HIP_TRY(timed_launch(0, p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st));
+    HIP_TRY(timed_launch(0, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st));
   }
   if (a.obs_kind != JQ_OBS_NONE) HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
   return JQ_OK;
@@ -369,7 +384,7 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
       a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
       a.dst = pass_out(q);
       a.dst_stride = (int64_t)a.vec_len;
-      HIP_TRY(timed_launch(0, p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st));
+      HIP_TRY(timed_launch(0, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st));
     }
     HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals ? expvals : (double*)p->ev_part.p, ntiles, a.n_obs, batch, st));
     // backward sweep
@@ -381,7 +396,7 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
       a.src_stride = (int64_t)a.vec_len;
       a.dst = ck ? nullptr : state;  // without checkpoints psi is un-applied in place
       a.dst_stride = (int64_t)a.vec_len;
-      HIP_TRY(timed_launch(1, p->dtype, p->hp.T, a, ntiles, batch, sm, st));
+      HIP_TRY(timed_launch(1, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, sm, st));
     }
   }
   // cotangents -> gradients

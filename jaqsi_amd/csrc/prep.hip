@@ -214,11 +214,27 @@ __global__ void prep_kernel(PrepArgs a) {
 }
 
 // sum over tiles of each cotangent slot -> bars (FP64 reals)
+// One wave per output real: the per-tile partials of a real are contiguous
+// ([real][tile] within each pass area), so the wave reads them coalesced and
+// sums in a fixed order (bit-reproducible from run to run).
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double sum_row(const double* row, int n) {
+  const int lane = threadIdx.x & 63;
+  double s = 0;
+  for (int t = lane; t < n; t += 64) s += row[t];
+  return wave_sum_d(s);
+}
+
 __global__ void reduce_grad_kernel(ReduceArgs a) {
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (int64_t)a.n_reals * a.batch) return;
-  const int circ = (int)(gid / a.n_reals);
-  const int e = (int)(gid % a.n_reals);
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (wid >= (int64_t)a.n_reals * a.batch) return;
+  const int circ = (int)(wid / a.n_reals);
+  const int e = (int)(wid % a.n_reals);
   // find the slot-map entry holding real e of the cotangent array
   const int* sm = a.slot_map;
   int lo = 0, hi = a.n_slots - 1;
@@ -226,20 +242,18 @@ __global__ void reduce_grad_kernel(ReduceArgs a) {
     const int mid = (lo + hi + 1) / 2;
     if (sm[mid * 5 + 3] <= e) lo = mid; else hi = mid - 1;
   }
-  const int base = sm[lo * 5 + 0], nsl = sm[lo * 5 + 1], local = sm[lo * 5 + 2], bar = sm[lo * 5 + 3];
-  const double* part = a.partials + (int64_t)circ * a.part_stride + base + local + (e - bar);
-  double s = 0;
-  for (int t = 0; t < a.ntiles; ++t) s += part[(int64_t)t * nsl];
-  a.bars[(int64_t)circ * a.n_reals + e] = s;
+  const int base = sm[lo * 5 + 0], local = sm[lo * 5 + 2], bar = sm[lo * 5 + 3];
+  const double* row = a.partials + (int64_t)circ * a.part_stride + base + (int64_t)(local + (e - bar)) * a.ntiles;
+  const double s = sum_row(row, a.ntiles);
+  if ((threadIdx.x & 63) == 0) a.bars[(int64_t)circ * a.n_reals + e] = s;
 }
 
+// partials: [batch][n_obs][ntiles]
 __global__ void reduce_ev_kernel(const double* part, double* out, int ntiles, int n_obs, int batch) {
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (int64_t)n_obs * batch) return;
-  const int circ = (int)(gid / n_obs), o = (int)(gid % n_obs);
-  double s = 0;
-  for (int t = 0; t < ntiles; ++t) s += part[((int64_t)circ * ntiles + t) * n_obs + o];
-  out[gid] = s;
+  const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (wid >= (int64_t)n_obs * batch) return;
+  const double s = sum_row(part + wid * ntiles, ntiles);
+  if ((threadIdx.x & 63) == 0) out[wid] = s;
 }
 
 // block cotangent -> member cotangents W_j = L_j^T bar R_j^T -> Re sum W . dE
@@ -434,14 +448,14 @@ hipError_t launch_prep(int dtype, const PrepArgs& a, hipStream_t st) {
 hipError_t launch_reduce_grad(const ReduceArgs& a, hipStream_t st) {
   const int64_t n = (int64_t)a.n_reals * a.batch;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(reduce_grad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(reduce_grad_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 hipError_t launch_reduce_ev(const double* part, double* out, int ntiles, int n_obs, int batch, hipStream_t st) {
   const int64_t n = (int64_t)n_obs * batch;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(reduce_ev_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, out, ntiles, n_obs,
+  hipLaunchKernelGGL(reduce_ev_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, part, out, ntiles, n_obs,
                      batch);
   return hipGetLastError();
 }

@@ -5,7 +5,12 @@
 namespace jq {
 hipError_t launch_sv_f32(int which, int T, const SvArgs& a, int ntiles, int batch, size_t smem, hipStream_t st) {
   if (T != 13) return hipErrorInvalidValue;
-  if (which == 0) return launch_forward<float, 5, 256>(a, ntiles, batch, smem, st);
-  return launch_backward<float, 4, 512>(a, ntiles, batch, smem, st);
+  switch (which) {
+    case 0: return launch_forward<float, 5, 256, false>(a, ntiles, batch, smem, st);
+    case 1: return launch_backward<float, 4, 512, false>(a, ntiles, batch, smem, st);
+    case 2: return launch_forward<float, 5, 256, true>(a, ntiles, batch, smem, st);
+    case 3: return launch_backward<float, 4, 512, true>(a, ntiles, batch, smem, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 }  // namespace jq

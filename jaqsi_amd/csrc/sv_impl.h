@@ -28,6 +28,19 @@ struct C2 {
   R x, y;
 };
 
+// Program words, block matrices and observable tables are read-only for the
+// whole launch: reading them through the constant address space lets every
+// wave-uniform access compile to a scalar load (s_load into SGPRs) instead of
+// a vector load whose result occupies VGPRs.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define JQ_CONST __attribute__((address_space(4)))
+#else
+#define JQ_CONST
+#endif
+using cint = const JQ_CONST int;
+template <typename R>
+using cmat = const JQ_CONST C2<R>;
+
 __device__ __forceinline__ uint32_t lds_swz(uint32_t x) { return x ^ ((x >> 5) & 31u); }
 
 template <typename R>
@@ -110,7 +123,7 @@ struct Layout {
 };
 
 template <int RB, int LNT>
-__device__ __forceinline__ void decode_layout(const int* sub, const int* tbits, Layout& L) {
+__device__ __forceinline__ void decode_layout(cint* sub, cint* tbits, Layout& L) {
   uint32_t toff = 0;
   uint64_t gthr = 0;
   const uint32_t tid = threadIdx.x;
@@ -179,7 +192,7 @@ struct Mat {
 };
 
 template <typename R>
-__device__ __forceinline__ void load2(const C2<R>* p, Mat<R, 2>& m) {
+__device__ __forceinline__ void load2(cmat<R>* p, Mat<R, 2>& m) {
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     m.r[e] = p[e].x;
@@ -190,7 +203,7 @@ __device__ __forceinline__ void load2(const C2<R>* p, Mat<R, 2>& m) {
 // 4x4 matrix; `swap` exchanges the roles of the two target bits, so the
 // kernels only need the register-bit pairs A > B.
 template <typename R>
-__device__ __forceinline__ void load4(const C2<R>* p, Mat<R, 4>& m, bool swap) {
+__device__ __forceinline__ void load4(cmat<R>* p, Mat<R, 4>& m, bool swap) {
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
@@ -416,7 +429,7 @@ __device__ __forceinline__ void diag1_grad(const R* pr, const R* pi, const R* lr
 // evaluated through LDS so the 8x8 matrix never occupies registers.
 // mode 0: M, 1: M^dagger, 2: M^T.
 template <typename R, int RB>
-__device__ __forceinline__ void dense3(R* re, R* im, const C2<R>* mp, int mode, const Layout& L, C2<R>* lds) {
+__device__ __forceinline__ void dense3(R* re, R* im, cmat<R>* mp, int mode, const Layout& L, C2<R>* lds) {
   constexpr int NR = 1 << RB;
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
@@ -477,7 +490,7 @@ struct DiagLoc {
   int k;
 };
 
-__device__ __forceinline__ void decode_diag(const int* g, const Layout& L, uint64_t base, DiagLoc& dl) {
+__device__ __forceinline__ void decode_diag(cint* g, const Layout& L, uint64_t base, DiagLoc& dl) {
   const int k = g[G_K];
   dl.k = k;
   dl.uni = 0;
@@ -508,7 +521,7 @@ __device__ __forceinline__ uint32_t diag_index(const DiagLoc& dl, int i) {
 
 // multiply by the diagonal d (conj when cj)
 template <typename R, int RB>
-__device__ __forceinline__ void apply_diag(R* re, R* im, const C2<R>* d, const DiagLoc& dl, bool cj) {
+__device__ __forceinline__ void apply_diag(R* re, R* im, cmat<R>* d, const DiagLoc& dl, bool cj) {
   constexpr int NR = 1 << RB;
   const R sg = cj ? R(-1) : R(1);
   if (dl.k == 1 && dl.rbit[0] >= 0) {
@@ -598,7 +611,7 @@ struct GateCtx {
   bool ok;  // thread / outside control bits satisfied
 };
 
-__device__ __forceinline__ GateCtx gate_ctx(const int* g, const Layout& L, uint64_t base) {
+__device__ __forceinline__ GateCtx gate_ctx(cint* g, const Layout& L, uint64_t base) {
   GateCtx c;
   c.creg = (uint32_t)g[G_CREG];
   const uint32_t cthr = (uint32_t)g[G_CTHR];
@@ -607,16 +620,20 @@ __device__ __forceinline__ GateCtx gate_ctx(const int* g, const Layout& L, uint6
   return c;
 }
 
-// Forward: apply gate record g to the register state.
-template <typename R, int RB>
-__device__ __forceinline__ void fwd_gate(R* re, R* im, const int* g, const C2<R>* pool, const Layout& L,
+// Forward: apply gate record g to the register state.  FULL = false drops
+// the 4x4 register blocks and the LDS dense-3 path: passes without them run
+// a variant with far fewer live registers.
+template <typename R, int RB, bool FULL>
+__device__ __forceinline__ void fwd_gate(R* re, R* im, cint* g, cmat<R>* pool, const Layout& L,
                                          uint64_t base, C2<R>* lds) {
   constexpr int NR = 1 << RB;
   const int kind = g[G_KIND];
-  const C2<R>* mp = pool + g[G_MAT];
-  if (kind == AK_DENSEK) {  // no controls; contains barriers, so all threads enter
-    if constexpr (RB >= 3) dense3<R, RB>(re, im, mp, 0, L, lds);
-    return;
+  cmat<R>* mp = pool + g[G_MAT];
+  if constexpr (FULL) {
+    if (kind == AK_DENSEK) {  // no controls; contains barriers, so all threads enter
+      if constexpr (RB >= 3) dense3<R, RB>(re, im, mp, 0, L, lds);
+      return;
+    }
   }
   const GateCtx cx = gate_ctx(g, L, base);
   if (!cx.ok) return;
@@ -634,12 +651,14 @@ __device__ __forceinline__ void fwd_gate(R* re, R* im, const int* g, const C2<R>
       break;
     }
     case AK_MC2: {
-      Mat<R, 4> m;
-      load4<R>(mp, m, ra < rb);
-      if (cx.creg == 0) {
-        JQ_CASE2(fw_dense<R, NR, 4, G2<NR, A, B>, false>(re, im, m, 0))
-      } else {
-        JQ_CASE2(fw_dense<R, NR, 4, G2<NR, A, B>, true>(re, im, m, cx.creg))
+      if constexpr (FULL) {
+        Mat<R, 4> m;
+        load4<R>(mp, m, ra < rb);
+        if (cx.creg == 0) {
+          JQ_CASE2(fw_dense<R, NR, 4, G2<NR, A, B>, false>(re, im, m, 0))
+        } else {
+          JQ_CASE2(fw_dense<R, NR, 4, G2<NR, A, B>, true>(re, im, m, cx.creg))
+        }
       }
       break;
     }
@@ -663,21 +682,23 @@ __device__ __forceinline__ void fwd_gate(R* re, R* im, const int* g, const C2<R>
 }
 
 // Backward: psi <- U^{-1} psi, bar += lambda (x) psi, lambda <- U^T lambda.
-template <typename R, int RB>
-__device__ __forceinline__ void bwd_gate(R* pr, R* pi, R* lr, R* li, const int* g, const C2<R>* pool,
+template <typename R, int RB, bool FULL>
+__device__ __forceinline__ void bwd_gate(R* pr, R* pi, R* lr, R* li, cint* g, cmat<R>* pool,
                                          const Layout& L, uint64_t base, C2<R>* lds, double* slots) {
   constexpr int NR = 1 << RB;
   const int kind = g[G_KIND];
-  const C2<R>* mp = pool + g[G_MAT];
+  cmat<R>* mp = pool + g[G_MAT];
   const bool inv = g[G_INV] >= 0;
-  const C2<R>* ip = pool + (inv ? g[G_INV] : g[G_MAT]);
+  cmat<R>* ip = pool + (inv ? g[G_INV] : g[G_MAT]);
   const int slot = g[G_SLOT];
-  if (kind == AK_DENSEK) {
-    if constexpr (RB >= 3) {
-      dense3<R, RB>(pr, pi, inv ? ip : mp, inv ? 0 : 1, L, lds);
-      dense3<R, RB>(lr, li, mp, 2, L, lds);
+  if constexpr (FULL) {
+    if (kind == AK_DENSEK) {
+      if constexpr (RB >= 3) {
+        dense3<R, RB>(pr, pi, inv ? ip : mp, inv ? 0 : 1, L, lds);
+        dense3<R, RB>(lr, li, mp, 2, L, lds);
+      }
+      return;
     }
-    return;
   }
   const GateCtx cx = gate_ctx(g, L, base);
   const int ra = g[G_RA], rb = g[G_RB];
@@ -704,6 +725,7 @@ __device__ __forceinline__ void bwd_gate(R* pr, R* pi, R* lr, R* li, const int* 
       break;
     }
     case AK_MC2: {
+      if constexpr (!FULL) break;
       const bool sw = ra < rb;
       {
         Mat<R, 4> ui;
@@ -798,7 +820,7 @@ __device__ __forceinline__ void bwd_gate(R* pr, R* pi, R* lr, R* li, const int* 
 // lowest global bits, so a wave reads 64 consecutive amplitude pairs (16-byte
 // vector accesses, 1 KiB contiguous for complex64) per register pair.
 template <int RB, int LNT>
-__device__ __forceinline__ void canonical_layout(const int* tbits, Layout& L) {
+__device__ __forceinline__ void canonical_layout(cint* tbits, Layout& L) {
   constexpr int T = RB + LNT;
   uint64_t gthr = 0;
   const uint32_t tid = threadIdx.x;
@@ -816,7 +838,7 @@ __device__ __forceinline__ void canonical_layout(const int* tbits, Layout& L) {
 }
 
 template <int RB>
-__device__ __forceinline__ bool same_layout(const int* sub, const Layout& L) {
+__device__ __forceinline__ bool same_layout(cint* sub, const Layout& L) {
   bool same = true;
 #pragma unroll
   for (int b = 0; b < RB; ++b) same = same && ((uint32_t)sub[1 + b] == L.rpos[b]);
@@ -952,7 +974,7 @@ __device__ __forceinline__ void relayout(R* re, R* im, C2<R>* lds, const Layout&
   __syncthreads();
 }
 
-__device__ __forceinline__ uint64_t tile_base(int tile, const int* tbits, int T, int n) {
+__device__ __forceinline__ uint64_t tile_base(int tile, cint* tbits, int T, int n) {
   // deposit the tile number into the real global bits that are not tile bits
   uint64_t mask = 0;
   for (int j = 0; j < T; ++j)
@@ -1057,18 +1079,18 @@ __device__ __forceinline__ void expval_partials(const R* re, const R* im, const 
   }
 }
 
-template <typename R, int RB, int NT>
+template <typename R, int RB, int NT, bool FULL>
 __device__ __forceinline__ void forward_tile(const SvArgs& a, int pass_idx, int circ, int tile, R* re, R* im,
                                              Layout& L, uint64_t& base, C2<R>* lds, bool do_load) {
-  const int* P = a.prog;
+  cint* P = (cint*)a.prog;
   const int T = P[2];
-  const int* pass = P + P[HDR_WORDS + pass_idx];
+  cint* pass = P + P[HDR_WORDS + pass_idx];
   const int nsub = pass[0];
-  const int* tbits = pass + 4;
+  cint* tbits = pass + 4;
   constexpr int LNT = __builtin_ctz(NT);
   base = tile_base(tile, tbits, T, a.n_wires);
-  const C2<R>* pool = reinterpret_cast<const C2<R>*>(a.mats) + (int64_t)circ * a.pool_stride;
-  const int* soff = pass + 4 + T;
+  cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride;
+  cint* soff = pass + 4 + T;
   canonical_layout<RB, LNT>(tbits, L);
   if (do_load) {
     if (a.src == nullptr)
@@ -1078,7 +1100,7 @@ __device__ __forceinline__ void forward_tile(const SvArgs& a, int pass_idx, int 
                         base, a.n_wires >= T);
   }
   for (int s = 0; s < nsub; ++s) {
-    const int* sub = P + soff[s];
+    cint* sub = P + soff[s];
     const int ng = sub[0];
     if (!same_layout<RB>(sub, L)) {
       Layout nl;
@@ -1086,14 +1108,14 @@ __device__ __forceinline__ void forward_tile(const SvArgs& a, int pass_idx, int 
       relayout<R, RB>(re, im, lds, L, nl);
       L = nl;
     }
-    const int* g = sub + 1 + T;
-    for (int q = 0; q < ng; ++q, g += GW) fwd_gate<R, RB>(re, im, g, pool, L, base, lds);
+    cint* g = sub + 1 + T;
+    for (int q = 0; q < ng; ++q, g += GW) fwd_gate<R, RB, FULL>(re, im, g, pool, L, base, lds);
   }
 }
 
 // Bring the registers back to the canonical layout.
 template <typename R, int RB, int NT>
-__device__ __forceinline__ void to_canonical(R* re, R* im, const int* tbits, Layout& L, C2<R>* lds) {
+__device__ __forceinline__ void to_canonical(R* re, R* im, cint* tbits, Layout& L, C2<R>* lds) {
   constexpr int LNT = __builtin_ctz(NT);
   Layout cl;
   canonical_layout<RB, LNT>(tbits, cl);
@@ -1106,16 +1128,16 @@ __device__ __forceinline__ void to_canonical(R* re, R* im, const int* tbits, Lay
   }
 }
 
-template <typename R, int RB, int NT>
+template <typename R, int RB, int NT, bool FULL>
 __global__ __launch_bounds__(NT, 2) void sv_forward_kernel(SvArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int T = a.prog[2];
+  const int T = ((cint*)a.prog)[2];
   C2<R>* lds = reinterpret_cast<C2<R>*>(smem);
   ObsLds* obs = reinterpret_cast<ObsLds*>(smem + (sizeof(C2<R>) << T));
   double* ev = reinterpret_cast<double*>(obs + 1);
   const int circ = blockIdx.y;
   const int tile = blockIdx.x;
-  const int* pass = a.prog + a.prog[HDR_WORDS + a.pass];
+  cint* pass = (cint*)a.prog + ((cint*)a.prog)[HDR_WORDS + a.pass];
   const bool last = (pass[1] & PF_LAST) != 0;
   const bool want_ev = last && a.obs_kind != JQ_OBS_NONE && a.ev_part != nullptr;
   if (want_ev) {
@@ -1125,7 +1147,7 @@ __global__ __launch_bounds__(NT, 2) void sv_forward_kernel(SvArgs a) {
   R re[1 << RB], im[1 << RB];
   Layout L;
   uint64_t base;
-  forward_tile<R, RB, NT>(a, a.pass, circ, tile, re, im, L, base, lds, true);
+  forward_tile<R, RB, NT, FULL>(a, a.pass, circ, tile, re, im, L, base, lds, true);
   to_canonical<R, RB, NT>(re, im, pass + 4, L, lds);
   store_canon<R, RB>(re, im, reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride, a.vec_len, L, base,
                      a.n_wires >= T);
@@ -1134,7 +1156,7 @@ __global__ __launch_bounds__(NT, 2) void sv_forward_kernel(SvArgs a) {
     expval_partials<R, RB>(re, im, a, obs, L, base, ev);
     __syncthreads();
     for (int o = threadIdx.x; o < a.n_obs; o += NT)
-      a.ev_part[((int64_t)circ * gridDim.x + tile) * a.n_obs + o] = ev[o];
+      a.ev_part[((int64_t)circ * a.n_obs + o) * gridDim.x + tile] = ev[o];
   }
 }
 
@@ -1190,7 +1212,7 @@ __device__ __forceinline__ void seed_lambda(const R* pr, const R* pi, R* lr, R* 
 }
 
 template <typename R, int RB, int NT>
-__device__ __forceinline__ void to_canonical2(R* pr, R* pi, R* lr, R* li, const int* tbits, Layout& L, C2<R>* lds) {
+__device__ __forceinline__ void to_canonical2(R* pr, R* pi, R* lr, R* li, cint* tbits, Layout& L, C2<R>* lds) {
   constexpr int LNT = __builtin_ctz(NT);
   Layout cl;
   canonical_layout<RB, LNT>(tbits, cl);
@@ -1207,20 +1229,20 @@ __device__ __forceinline__ void to_canonical2(R* pr, R* pi, R* lr, R* li, const 
 // Backward sweep over one pass (reverse substeps, reverse gates).  When
 // `loaded` is false, psi (the pass output) and lambda are read under the
 // canonical layout; otherwise they arrive in registers under layout L.
-template <typename R, int RB, int NT>
+template <typename R, int RB, int NT, bool FULL>
 __device__ __forceinline__ void backward_tile(const SvArgs& a, int pass_idx, int circ, int tile, R* pr, R* pi, R* lr,
                                               R* li, Layout& L, uint64_t& base, C2<R>* lds, double* slots,
                                               const ObsLds* obs, bool loaded) {
-  const int* P = a.prog;
+  cint* P = (cint*)a.prog;
   const int T = P[2];
-  const int* pass = P + P[HDR_WORDS + pass_idx];
+  cint* pass = P + P[HDR_WORDS + pass_idx];
   const int nsub = pass[0];
   const int flags = pass[1];
-  const int* tbits = pass + 4;
+  cint* tbits = pass + 4;
   constexpr int LNT = __builtin_ctz(NT);
   base = tile_base(tile, tbits, T, a.n_wires);
-  const C2<R>* pool = reinterpret_cast<const C2<R>*>(a.mats) + (int64_t)circ * a.pool_stride;
-  const int* soff = pass + 4 + T;
+  cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride;
+  cint* soff = pass + 4 + T;
   if (!loaded) {
     canonical_layout<RB, LNT>(tbits, L);
     const C2<R>* ps = reinterpret_cast<const C2<R>*>(a.src) + (int64_t)circ * a.src_stride;
@@ -1233,7 +1255,7 @@ __device__ __forceinline__ void backward_tile(const SvArgs& a, int pass_idx, int
     }
   }
   for (int s = nsub - 1; s >= 0; --s) {
-    const int* sub = P + soff[s];
+    cint* sub = P + soff[s];
     if (!same_layout<RB>(sub, L)) {
       Layout nl;
       decode_layout<RB, LNT>(sub, tbits, nl);
@@ -1242,25 +1264,25 @@ __device__ __forceinline__ void backward_tile(const SvArgs& a, int pass_idx, int
       L = nl;
     }
     const int ng = sub[0];
-    const int* g0 = sub + 1 + T;
+    cint* g0 = sub + 1 + T;
     for (int q = ng - 1; q >= 0; --q) {
-      const int* g = g0 + q * GW;
-      bwd_gate<R, RB>(pr, pi, lr, li, g, pool, L, base, lds, slots);
+      cint* g = g0 + q * GW;
+      bwd_gate<R, RB, FULL>(pr, pi, lr, li, g, pool, L, base, lds, slots);
     }
   }
   to_canonical2<R, RB, NT>(pr, pi, lr, li, tbits, L, lds);
 }
 
-template <typename R, int RB, int NT>
+template <typename R, int RB, int NT, bool FULL>
 __global__ __launch_bounds__(NT) void sv_backward_kernel(SvArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int T = a.prog[2];
+  const int T = ((cint*)a.prog)[2];
   C2<R>* lds = reinterpret_cast<C2<R>*>(smem);
   ObsLds* obs = reinterpret_cast<ObsLds*>(smem + (sizeof(C2<R>) << T));
   double* slots = reinterpret_cast<double*>(obs + 1);
   const int circ = blockIdx.y;
   const int tile = blockIdx.x;
-  const int* pass = a.prog + a.prog[HDR_WORDS + a.pass];
+  cint* pass = (cint*)a.prog + ((cint*)a.prog)[HDR_WORDS + a.pass];
   const int nslots = pass[2];
   const int slot_base = pass[3];
   for (int e = threadIdx.x; e < nslots; e += NT) slots[e] = 0;
@@ -1268,32 +1290,33 @@ __global__ __launch_bounds__(NT) void sv_backward_kernel(SvArgs a) {
   R pr[1 << RB], pi[1 << RB], lr[1 << RB], li[1 << RB];
   Layout L;
   uint64_t base;
-  backward_tile<R, RB, NT>(a, a.pass, circ, tile, pr, pi, lr, li, L, base, lds, slots, obs, false);
+  backward_tile<R, RB, NT, FULL>(a, a.pass, circ, tile, pr, pi, lr, li, L, base, lds, slots, obs, false);
   C2<R>* lam = reinterpret_cast<C2<R>*>(a.lam) + (int64_t)circ * a.lam_stride;
   store_canon<R, RB>(lr, li, lam, a.vec_len, L, base, a.n_wires >= T);
   if (a.dst != nullptr)
     store_canon<R, RB>(pr, pi, reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride, a.vec_len, L, base, a.n_wires >= T);
   __syncthreads();
-  double* gp = a.grad_part + (int64_t)circ * a.part_stride + slot_base + (int64_t)tile * nslots;
-  for (int e = threadIdx.x; e < nslots; e += NT) gp[e] = slots[e];
+  // partials are real-major ([real][tile]) so the reduction reads them contiguously
+  double* gp = a.grad_part + (int64_t)circ * a.part_stride + slot_base + tile;
+  for (int e = threadIdx.x; e < nslots; e += NT) gp[(int64_t)e * gridDim.x] = slots[e];
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <typename R, int RB, int NT>
+template <typename R, int RB, int NT, bool FULL>
 hipError_t launch_forward(const SvArgs& a, int ntiles, int batch, size_t smem, hipStream_t st) {
-  (void)hipFuncSetAttribute((const void*)sv_forward_kernel<R, RB, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)smem);
-  hipLaunchKernelGGL((sv_forward_kernel<R, RB, NT>), dim3(ntiles, batch), dim3(NT), smem, st, a);
+  (void)hipFuncSetAttribute((const void*)sv_forward_kernel<R, RB, NT, FULL>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  hipLaunchKernelGGL((sv_forward_kernel<R, RB, NT, FULL>), dim3(ntiles, batch), dim3(NT), smem, st, a);
   return hipGetLastError();
 }
 
-template <typename R, int RB, int NT>
+template <typename R, int RB, int NT, bool FULL>
 hipError_t launch_backward(const SvArgs& a, int ntiles, int batch, size_t smem, hipStream_t st) {
-  (void)hipFuncSetAttribute((const void*)sv_backward_kernel<R, RB, NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)smem);
-  hipLaunchKernelGGL((sv_backward_kernel<R, RB, NT>), dim3(ntiles, batch), dim3(NT), smem, st, a);
+  (void)hipFuncSetAttribute((const void*)sv_backward_kernel<R, RB, NT, FULL>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  hipLaunchKernelGGL((sv_backward_kernel<R, RB, NT, FULL>), dim3(ntiles, batch), dim3(NT), smem, st, a);
   return hipGetLastError();
 }
 
