@@ -141,6 +141,116 @@ def roofline_c2(res):
     }
 
 
+def bench_c3(steps, warmup, B=4096, n=14, seed=1003):
+    """Ragged batch: B circuits on n qubits, depth d_b ~ U{2..12}, an RX(x)
+    encoding layer with x ~ U[-pi, pi), angles ~ U[0, 2pi); one step = the
+    14 Z expvals and the adjoint gradient of sum_q <Z_q> of every circuit."""
+    import ctypes as C
+
+    import torch
+
+    from jaqsi_amd import _native as nat
+    from jaqsi_amd.ragged import RaggedBatch, record
+
+    L = nat.lib()
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(seed)
+    depths = rng.integers(2, 13, size=B)
+    feats = rng.uniform(-np.pi, np.pi, (B, n))
+    thetas = [rng.uniform(0, 2 * np.pi, (int(d), n, 3)) for d in depths]
+    uniq = sorted(set(int(d) for d in depths))
+    members = {d: [b for b in range(B) if depths[b] == d] for d in uniq}
+    tapes = [record(record_hea, np.zeros((d, n, 3)), n, np.zeros(n)) for d in uniq]
+    rb = RaggedBatch(tapes, [len(members[d]) for d in uniq], n, [(q, 1.0, -1.0) for q in range(n)], dtype=0)
+    angles = [
+        torch.tensor(np.stack([np.concatenate([feats[b], thetas[b].reshape(-1)]) for b in members[d]]), device=dev)
+        for d in uniq
+    ]
+    cts = [torch.ones((len(members[d]), n), dtype=torch.float64, device=dev) for d in uniq]
+    for _ in range(warmup):
+        rb.adjoint(angles, cts)
+    torch.cuda.synchronize()
+    nat.check(L.jq_timing_enable(1))
+    for w in range(3):
+        nat.check(L.jq_timing_read(w, None, None))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        evs, grads = rb.adjoint(angles, cts)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    nat.check(L.jq_timing_enable(0))
+    kt = {}
+    for w, name in ((0, "sv_forward_kernel"), (1, "sv_backward_kernel")):
+        tot, cnt = C.c_double(), C.c_int32()
+        nat.check(L.jq_timing_read(w, C.byref(tot), C.byref(cnt)))
+        kt[name] = (tot.value, cnt.value)
+    circuit_passes = sum(len(members[d]) * p.info()["n_passes"] for d, p in zip(uniq, rb.plans))
+    return {
+        "seconds": t1 - t0,
+        "kernels": kt,
+        "circuit_passes": circuit_passes,
+        "groups": len(uniq),
+        "depths": depths,
+        "feats": feats,
+        "thetas": thetas,
+        "n": n,
+        "B": B,
+        "check": (members, uniq, evs, grads),
+    }
+
+
+def roofline_c3(res, steps):
+    n = res["n"]
+    sweep = 2 * 8 * (1 << n)
+    per = {"sv_forward_kernel": sweep, "sv_backward_kernel": sweep * 3 // 2}
+    out = {}
+    for k, (tot, cnt) in res["kernels"].items():
+        s_ = tot / 1e3
+        bytes_ = per[k] * res["circuit_passes"] * steps
+        out[k] = {"ms_total": round(tot, 3), "launches": cnt, "GB/s": round(bytes_ / max(s_, 1e-12) / 1e9, 1)}
+    dom = max(res["kernels"], key=lambda k: res["kernels"][k][0])
+    ach = out[dom]["GB/s"]
+    return {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": ach,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(ach / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "algorithmic_bytes_per_launch_note": "sum over active circuits of the per-circuit pass bytes (2*8*2^14 forward, 3*8*2^14 backward)",
+        "kernels": out,
+    }
+
+
+def cpu_baseline_c3(res, budget_s=15.0):
+    from oracle import cpu
+
+    from concurrent.futures import ThreadPoolExecutor
+
+    cores = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    done = 0
+    # one circuit per core at a time (the C call releases the GIL); chunks of
+    # `cores` circuits until the time budget is used
+    with ThreadPoolExecutor(cores) as ex:
+        while time.perf_counter() - t0 < budget_s and done < res["B"]:
+            hi = min(res["B"], done + cores)
+            list(ex.map(lambda b: cpu.hea_grad(res["thetas"][b], res["feats"][b], 1), range(done, hi)))
+            done = hi
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(done / dt, 2),
+        "unit": "circuits/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"the first {done} circuits of the batch, {cores} at a time (one per core; forward + adjoint "
+        "each), C++ restatement of the reference plan",
+        "seconds": round(dt, 2),
+    }
+
+
 def cpu_baseline_c2(theta):
     from oracle import cpu
 
@@ -165,6 +275,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 4096x14-qubit ragged-batch leg")
+    ap.add_argument("--c3-steps", type=int, default=5)
     ap.add_argument(
         "--copy-probe",
         type=int,
@@ -222,6 +334,27 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_c2(res["theta"])
+    if not args.no_c3:
+        r3 = bench_c3(args.c3_steps, 1)
+        c3 = {
+            "metric": "ragged-batch adjoint-grad circuits/s (4096 x 14 qubits, depth U{2..12})",
+            "value": round(r3["B"] * args.c3_steps / r3["seconds"], 1),
+            "unit": "circuits/s",
+            "steps": args.c3_steps,
+            "ms_per_step": round(r3["seconds"] / args.c3_steps * 1e3, 3),
+            "dtype": "c64",
+            "config": {
+                "workload": "4096 circuits, 14 qubits, RX(x) encoding + d_b in U{2..12} ansatz layers "
+                "(RX,RY,RZ per wire + CX ring), x ~ U[-pi,pi), angles ~ U[0,2pi), seed 1003; "
+                "14 Z expvals + adjoint gradient of sum_q <Z_q> per circuit",
+                "groups": r3["groups"],
+                "circuit_passes": r3["circuit_passes"],
+            },
+            "roofline": roofline_c3(r3, args.c3_steps),
+        }
+        if not args.no_cpu:
+            c3["cpu_baseline"] = cpu_baseline_c3(r3)
+        line["c3"] = c3
     print(json.dumps(line), flush=True)
 
 

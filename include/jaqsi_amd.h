@@ -147,6 +147,29 @@ int jq_sv_adjoint(jq_plan* plan, int32_t batch, const double* angles, const void
                   const int32_t* obs_wires, const double* obs_d, const double* ct, double* expvals,
                   double* grad_angles, void* grad_matrices, void* grad_psi0, void* stream);
 
+/* One group of a ragged batch: `batch` circuits sharing `plan`'s structure. */
+typedef struct jq_group {
+  jq_plan* plan;
+  int32_t batch;
+  int32_t psi0_batched;
+  const double* angles;  /* [batch][n_params] float64 (device) */
+  const void* matrices;  /* [batch][n_matrix_elems] complex128 (device) or NULL */
+  const void* psi0;      /* [batch or 1][2^n] or NULL */
+  const double* ct;      /* [batch][n_obs] float64 (device) */
+  double* expvals;       /* [batch][n_obs] float64 (device) or NULL */
+  double* grad_angles;   /* [batch][n_params] float64 (device) or NULL */
+  void* state;           /* [batch][2^n] final states (device) */
+} jq_group;
+
+/* Forward + adjoint gradient of a ragged batch: several circuit structures
+ * (e.g. uneven depths) of the same width and dtype, observables shared.
+ * Every pass step is ONE launch over all circuits that still have a pass at
+ * that step.  The reference runs one vmapped program per structure group
+ * (script.py:406-434, the vmap in _vectorize script.py:43-127); this call
+ * replaces that loop of execute() calls. */
+int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_kind, int32_t n_obs,
+                         const int32_t* obs_wires, const double* obs_d, void* stream);
+
 /* Reductions of a final state (measure_state, simulation.py:735-811). */
 int jq_sv_probs(int32_t n_wires, int32_t batch, int32_t dtype, const void* state, void* probs, void* stream);
 int jq_sv_outer(int32_t n_wires, int32_t batch, int32_t dtype, const void* state, void* rho, void* stream);
@@ -158,16 +181,29 @@ int jq_dm_diag(int32_t n_qubits, int32_t batch, int32_t dtype, const void* rho, 
 int jq_expval_matrix(int32_t n_qubits, int32_t batch, int32_t dtype, int32_t density, const void* state,
                      int32_t k, const int32_t* wires, const double* matrix, double* out, void* stream);
 
-/* Batched pulse propagators: see jaqsi_amd/pulses.py.  Each parameter set is
- * evolved for n_steps piecewise-constant Magnus-2 steps in FP64. */
+/* Batched two-qubit pulse propagators in FP64 and the gradient of the
+ * process infidelity.  Per parameter set the Hamiltonian is
+ *   H(t) = w0/2 Z0 + w1/2 Z1 + J Z0 Z1
+ *          + E(t) (cos phi X0 + sin phi Y0) + Q(t) (-sin phi X0 + cos phi Y0)
+ * with E the lifted Gaussian of amplitude A and width sigma centred at T/2
+ * and Q = -beta dE/dt (DRAG).  Magnus-2 midpoint steps, n_steps of them.
+ *   params     [n_sets][6] float64 (device): A, sigma, beta, phi, w0, w1
+ *   target     4x4 complex128 V, row-major, re/im interleaved (HOST memory)
+ *   unitaries  [n_sets][4][4] complex128 (device) or NULL
+ *   infidelity [n_sets] float64 (device): 1 - |Tr(V^dag U)|^2 / 16
+ *   grad       [n_sets][4] float64 (device) or NULL: d infidelity / d(A, sigma, beta, phi)
+ * Replaces the Magnus-2 solve of Evolution._evolve_parametrized
+ * (evolution.py:194-263, 457-689) with the DRAG envelopes of
+ * pulses.py:324-390, and unitary_cost_fn's process loss (qoc.py:261-343). */
+#define JQ_ENV_DRAG_GAUSSIAN 1
+
 typedef struct jq_pulse_desc {
   int32_t n_sets;
   int32_t n_steps;
-  double t_total;
-  int32_t n_params;       /* per set */
-  int32_t envelope;       /* JQ_ENV_* */
-  double coupling_j;      /* ZZ coupling (rad/ns) */
-  int32_t want_grad;
+  double t_total;    /* ns */
+  double coupling_j; /* ZZ coupling (rad/ns) */
+  int32_t envelope;  /* JQ_ENV_DRAG_GAUSSIAN */
+  int32_t reserved;
 } jq_pulse_desc;
 
 int jq_pulse_propagate(const jq_pulse_desc* desc, const double* params, const double* target,

@@ -70,16 +70,35 @@ class PlanDesc(C.Structure):
     ]
 
 
+class Group(C.Structure):
+    """jq_group: one structure group of a ragged batch."""
+
+    _fields_ = [
+        ("plan", C.c_void_p),
+        ("batch", C.c_int32),
+        ("psi0_batched", C.c_int32),
+        ("angles", C.c_void_p),
+        ("matrices", C.c_void_p),
+        ("psi0", C.c_void_p),
+        ("ct", C.c_void_p),
+        ("expvals", C.c_void_p),
+        ("grad_angles", C.c_void_p),
+        ("state", C.c_void_p),
+    ]
+
+
 class PulseDesc(C.Structure):
     _fields_ = [
         ("n_sets", C.c_int32),
         ("n_steps", C.c_int32),
         ("t_total", C.c_double),
-        ("n_params", C.c_int32),
-        ("envelope", C.c_int32),
         ("coupling_j", C.c_double),
-        ("want_grad", C.c_int32),
+        ("envelope", C.c_int32),
+        ("reserved", C.c_int32),
     ]
+
+
+JQ_ENV_DRAG_GAUSSIAN = 1
 
 
 _lib = None
@@ -93,6 +112,7 @@ EXPORTS = [
     "jq_plan_tables",
     "jq_sv_forward",
     "jq_sv_adjoint",
+    "jq_sv_adjoint_groups",
     "jq_sv_probs",
     "jq_sv_outer",
     "jq_dm_diag",
@@ -133,6 +153,8 @@ def lib():
         L.jq_plan_tables.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int64)]
         L.jq_sv_forward.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
         L.jq_sv_adjoint.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.jq_sv_adjoint_groups.argtypes = [C.c_int32, C.POINTER(Group), C.c_int32, C.c_int32, vp, vp, vp]
+        L.jq_pulse_propagate.argtypes = [C.POINTER(PulseDesc), vp, vp, vp, vp, vp, vp]
         L.jq_sv_probs.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp]
         L.jq_sv_outer.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp]
         L.jq_dm_diag.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp]

@@ -11,6 +11,7 @@
 #include "jaqsi_amd.h"
 #include "plan_internal.h"
 #include "prep.h"
+#include "pulse.h"
 #include "program.h"
 #include "sv_kernels.h"
 
@@ -92,7 +93,7 @@ struct jq_plan {
   std::vector<char> pass_full;  // pass holds a 4x4 register block or a dense 3-wire block
   bool uploaded = false;
   // workspaces
-  DevBuf pool, partials, bars, ev_part, lam, ckpt, status, obs;
+  DevBuf pool, partials, bars, ev_part, lam, ckpt, status, obs, fstate, groups;
 };
 
 extern "C" {
@@ -166,7 +167,7 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
 int jq_plan_destroy(jq_plan* p) {
   if (!p) return JQ_OK;
   for (DevBuf* b : {&p->prog, &p->prog_b, &p->block_table, &p->member_list, &p->member_table, &p->member_values, &p->slot_map,
-                    &p->pool, &p->partials, &p->bars, &p->ev_part, &p->lam, &p->ckpt, &p->status, &p->obs})
+                    &p->pool, &p->partials, &p->bars, &p->ev_part, &p->lam, &p->ckpt, &p->status, &p->obs, &p->fstate, &p->groups})
     b->release();
   delete p;
   return JQ_OK;
@@ -304,6 +305,47 @@ int max_slots(jq_plan* p) {
   return m;
 }
 
+
+// per-tile cotangent partials -> block cotangents -> angle / matrix gradients
+int finish_grads(jq_plan* p, int batch, const double* angles, const void* matrices, double* grad_part,
+                 int64_t part_stride, double* grad_angles, void* grad_matrices, hipStream_t st) {
+  const int ntiles = p->hp.ntiles;
+  const int nreals = p->hp.bars * 2;
+  if (grad_angles && p->n_params > 0)
+    HIP_TRY(hipMemsetAsync(grad_angles, 0, (size_t)batch * p->n_params * 8, st));
+  if (grad_matrices && p->n_matrix_elems > 0)
+    HIP_TRY(hipMemsetAsync(grad_matrices, 0, (size_t)batch * p->n_matrix_elems * 16, st));
+  if (nreals > 0 && (grad_angles || grad_matrices)) {
+    HIP_TRY(p->bars.ensure((size_t)batch * nreals * 8));
+    jq::ReduceArgs r;
+    r.slot_map = (const int*)p->slot_map.p;
+    r.n_slots = p->n_slots;
+    r.partials = grad_part;
+    r.part_stride = part_stride;
+    r.ntiles = ntiles;
+    r.bars = (double*)p->bars.p;
+    r.n_reals = nreals;
+    r.batch = batch;
+    HIP_TRY(jq::launch_reduce_grad(r, st));
+    jq::ChainArgs c;
+    c.n_blocks = p->hp.n_dev_blocks;
+    c.batch = batch;
+    c.block_table = (const int*)p->block_table.p;
+    c.member_list = (const int*)p->member_list.p;
+    c.member_table = (const int*)p->member_table.p;
+    c.member_values = (const double*)p->member_values.p;
+    c.angles = angles;
+    c.n_params = p->n_params;
+    c.matrices = matrices;
+    c.n_matrix_elems = p->n_matrix_elems;
+    c.bars = (const double*)p->bars.p;
+    c.bars_stride = p->hp.bars;
+    c.grad_angles = grad_angles;
+    c.grad_matrices = grad_matrices;
+    if (grad_angles || grad_matrices) HIP_TRY(jq::launch_chain(c, st));
+  }
+  return JQ_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -386,7 +428,7 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
       a.dst_stride = (int64_t)a.vec_len;
       HIP_TRY(timed_launch(0, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st));
     }
-    HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals ? expvals : (double*)p->ev_part.p, ntiles, a.n_obs, batch, st));
+    if (expvals) HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
     // backward sweep
     const size_t sm = smem_for(p, max_slots(p));
     a.prog = (const int*)p->prog_b.p;
@@ -399,43 +441,120 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
       HIP_TRY(timed_launch(1, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, sm, st));
     }
   }
-  // cotangents -> gradients
-  const int nreals = p->hp.bars * 2;
-  if (grad_angles && p->n_params > 0)
-    HIP_TRY(hipMemsetAsync(grad_angles, 0, (size_t)batch * p->n_params * 8, st));
-  if (grad_matrices && p->n_matrix_elems > 0)
-    HIP_TRY(hipMemsetAsync(grad_matrices, 0, (size_t)batch * p->n_matrix_elems * 16, st));
-  if (nreals > 0 && (grad_angles || grad_matrices)) {
-    HIP_TRY(p->bars.ensure((size_t)batch * nreals * 8));
-    jq::ReduceArgs r;
-    r.slot_map = (const int*)p->slot_map.p;
-    r.n_slots = p->n_slots;
-    r.partials = a.grad_part;
-    r.part_stride = a.part_stride;
-    r.ntiles = ntiles;
-    r.bars = (double*)p->bars.p;
-    r.n_reals = nreals;
-    r.batch = batch;
-    HIP_TRY(jq::launch_reduce_grad(r, st));
-    jq::ChainArgs c;
-    c.n_blocks = p->hp.n_dev_blocks;
-    c.batch = batch;
-    c.block_table = (const int*)p->block_table.p;
-    c.member_list = (const int*)p->member_list.p;
-    c.member_table = (const int*)p->member_table.p;
-    c.member_values = (const double*)p->member_values.p;
-    c.angles = angles;
-    c.n_params = p->n_params;
-    c.matrices = matrices;
-    c.n_matrix_elems = p->n_matrix_elems;
-    c.bars = (const double*)p->bars.p;
-    c.bars_stride = p->hp.bars;
-    c.grad_angles = grad_angles ? grad_angles : nullptr;
-    c.grad_matrices = grad_matrices;
-    if (!grad_angles) {
-      HIP_TRY(p->ev_part.ensure(std::max<size_t>(p->ev_part.bytes, (size_t)batch * std::max(1, p->n_params) * 8)));
+  return finish_grads(p, batch, angles, matrices, a.grad_part, a.part_stride, grad_angles, grad_matrices, st);
+}
+
+int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_kind, int32_t n_obs,
+                         const int32_t* obs_wires, const double* obs_d, void* stream) {
+  if (n_groups < 1 || !groups) return fail(JQ_ERR_ARG, "no groups");
+  if (obs_kind == JQ_OBS_NONE) return fail(JQ_ERR_ARG, "the adjoint needs observables");
+  hipStream_t st = (hipStream_t)stream;
+  jq_plan* p0 = groups[0].plan;
+  if (!p0) return fail(JQ_ERR_ARG, "null plan");
+  std::vector<int> order(n_groups);
+  int64_t total = 0;
+  for (int g = 0; g < n_groups; ++g) {
+    const jq_group& G = groups[g];
+    if (!G.plan || G.batch < 1 || !G.ct || !G.state) return fail(JQ_ERR_ARG, "bad group");
+    if (G.plan->hp.n_wires != p0->hp.n_wires || G.plan->dtype != p0->dtype || G.plan->hp.T != p0->hp.T)
+      return fail(JQ_ERR_ARG, "groups must share width, dtype and tile size");
+    for (int h = 0; h < g; ++h)
+      if (groups[h].plan == G.plan)
+        return fail(JQ_ERR_ARG, "two groups share a plan: merge circuits of one structure into one group");
+    order[g] = g;
+    total += G.batch;
+  }
+  if (total > 65535) return fail(JQ_ERR_ARG, "more than 65535 circuits per call");
+  // groups with more passes first: the circuits still active at a step are a prefix
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return groups[x].plan->hp.n_passes > groups[y].plan->hp.n_passes; });
+  jq::SvArgs a = base_args(p0);
+  int rc = fill_obs(p0, a, p0->hp.n_wires, obs_kind, n_obs, obs_wires, obs_d, st);
+  if (rc) return rc;
+  const size_t eb = elem_bytes(p0->dtype);
+  const int ntiles = p0->hp.ntiles;
+  std::vector<jq::SvGroup> table(n_groups);
+  std::vector<int64_t> row_begin(n_groups + 1, 0);
+  int maxP = 0;
+  size_t smem_f = smem_for(p0, 0), smem_b = 0;
+  for (int k = 0; k < n_groups; ++k) {
+    const jq_group& G = groups[order[k]];
+    jq_plan* p = G.plan;
+    rc = ensure_uploaded(p);
+    if (rc) return rc;
+    rc = run_prep(p, G.batch, G.angles, G.matrices, st);
+    if (rc) return rc;
+    const int P = p->hp.n_passes;
+    const size_t vbytes = (size_t)a.vec_len * eb * G.batch;
+    HIP_TRY(p->ckpt.ensure(std::max<size_t>(1, vbytes * (size_t)std::max(0, P - 1))));
+    HIP_TRY(p->lam.ensure(vbytes));
+    HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)G.batch * ntiles * std::max(1, n_obs)) * 8));
+    HIP_TRY(p->partials.ensure(std::max<size_t>(1, (size_t)G.batch * p->hp.partial_reals) * 8));
+    jq::SvGroup& t = table[k];
+    std::memset(&t, 0, sizeof(t));
+    t.prog_f = (const int*)p->prog.p;
+    t.prog_b = (const int*)p->prog_b.p;
+    t.mats = p->pool.p;
+    t.pool_stride = p->hp.pool;
+    t.psi0 = G.psi0;
+    t.psi0_stride = G.psi0_batched ? (int64_t)a.vec_len : 0;
+    t.ckpt = p->ckpt.p;
+    t.state = G.state;
+    t.lam = p->lam.p;
+    t.ev_part = (double*)p->ev_part.p;
+    t.grad_part = (double*)p->partials.p;
+    t.part_stride = p->hp.partial_reals;
+    t.ct = G.ct;
+    t.n_passes = P;
+    t.circ_begin = (int)row_begin[k];
+    t.batch = G.batch;
+    row_begin[k + 1] = row_begin[k] + G.batch;
+    maxP = std::max(maxP, P);
+    smem_b = std::max(smem_b, smem_for(p, max_slots(p)));
+  }
+  HIP_TRY(p0->groups.ensure(sizeof(jq::SvGroup) * n_groups));
+  HIP_TRY(jq::launch_bytes_upload(table.data(), sizeof(jq::SvGroup) * n_groups, p0->groups.p, st));
+  a.groups = (const jq::SvGroup*)p0->groups.p;
+  a.n_groups = n_groups;
+  // rows active at step s: groups k with n_passes > s (a prefix of the order)
+  auto active_rows = [&](int s, bool& full, int dir) -> int {
+    int rows = 0;
+    full = false;
+    for (int k = 0; k < n_groups; ++k) {
+      jq_plan* p = groups[order[k]].plan;
+      const int P = p->hp.n_passes;
+      if (P <= s) break;
+      rows = (int)row_begin[k + 1];
+      const int q = dir == 0 ? s : P - 1 - s;
+      full = full || p->pass_full[q];
     }
-    if (grad_angles) HIP_TRY(jq::launch_chain(c, st));
+    return rows;
+  };
+  for (int s = 0; s < maxP; ++s) {
+    bool full;
+    const int rows = active_rows(s, full, 0);
+    a.step = s;
+    a.dir = 0;
+    HIP_TRY(timed_launch(0, full, p0->dtype, p0->hp.T, a, ntiles, rows, smem_f, st));
+  }
+  for (int k = 0; k < n_groups; ++k) {
+    const jq_group& G = groups[order[k]];
+    jq_plan* p = G.plan;
+    if (G.expvals) HIP_TRY(jq::launch_reduce_ev((double*)p->ev_part.p, G.expvals, ntiles, n_obs, G.batch, st));
+  }
+  for (int s = 0; s < maxP; ++s) {
+    bool full;
+    const int rows = active_rows(s, full, 1);
+    a.step = s;
+    a.dir = 1;
+    HIP_TRY(timed_launch(1, full, p0->dtype, p0->hp.T, a, ntiles, rows, smem_b, st));
+  }
+  for (int k = 0; k < n_groups; ++k) {
+    const jq_group& G = groups[order[k]];
+    jq_plan* p = G.plan;
+    rc = finish_grads(p, G.batch, G.angles, G.matrices, (double*)p->partials.p, p->hp.partial_reals, G.grad_angles,
+                      nullptr, st);
+    if (rc) return rc;
   }
   return JQ_OK;
 }
@@ -506,8 +625,25 @@ int jq_timing_read(int32_t which, double* total_ms, int32_t* count) {
   return JQ_OK;
 }
 
-int jq_pulse_propagate(const jq_pulse_desc*, const double*, const double*, double*, double*, double*, void*) {
-  return fail(JQ_ERR_UNSUPPORTED, "pulse propagation is provided by jq_pulse_* (see jaqsi_amd/pulses.py)");
+int jq_pulse_propagate(const jq_pulse_desc* desc, const double* params, const double* target, double* unitaries,
+                       double* infidelity, double* grad, void* stream) {
+  if (!desc || !params || !target || !infidelity) return fail(JQ_ERR_ARG, "null argument to jq_pulse_propagate");
+  if (desc->envelope != JQ_ENV_DRAG_GAUSSIAN) return fail(JQ_ERR_UNSUPPORTED, "unknown pulse envelope");
+  if (desc->n_sets < 1 || desc->n_sets > (1 << 30) || desc->n_steps < 1 || !(desc->t_total > 0))
+    return fail(JQ_ERR_ARG, "bad pulse batch sizes");
+  jq::PulseArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n_sets = desc->n_sets;
+  a.n_steps = desc->n_steps;
+  a.T = desc->t_total;
+  a.J = desc->coupling_j;
+  a.params = params;
+  std::memcpy(a.V, target, sizeof(a.V));
+  a.U = unitaries;
+  a.infid = infidelity;
+  a.grad = grad;
+  HIP_TRY(jq::launch_pulse(a, (hipStream_t)stream));
+  return JQ_OK;
 }
 
 }  // extern "C"

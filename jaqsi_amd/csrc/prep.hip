@@ -271,7 +271,7 @@ __global__ void chain_kernel(ChainArgs a) {
   const double* angles = a.angles + (int64_t)circ * a.n_params;
   const cd* xm = reinterpret_cast<const cd*>(a.matrices) + (a.matrices ? (int64_t)circ * a.n_matrix_elems : 0);
   const cd* bar = reinterpret_cast<const cd*>(a.bars) + (int64_t)circ * a.bars_stride + bt[B_BAR];
-  double* gang = a.grad_angles + (int64_t)circ * a.n_params;
+  double* gang = a.grad_angles ? a.grad_angles + (int64_t)circ * a.n_params : nullptr;
   cd* gmat = a.grad_matrices ? reinterpret_cast<cd*>(a.grad_matrices) + (int64_t)circ * a.n_matrix_elems : nullptr;
   const int mb = bt[B_MBEG], me = bt[B_MEND];
   cd Rp[64], Lm[64], E[64], T[64], W[64];
@@ -280,7 +280,7 @@ __global__ void chain_kernel(ChainArgs a) {
     const int m = a.member_list[j];
     const int* mt = a.member_table + m * MW;
     const double th = member_theta(mt, a.member_values[m], angles);
-    const bool want_theta = mt[M_PARAM] >= 0;
+    const bool want_theta = mt[M_PARAM] >= 0 && gang != nullptr;
     const bool want_mat = mt[M_KIND] == JQ_M_MATRIX && (mt[M_FLAGS] & JQ_MF_GRAD) && gmat != nullptr;
     if (want_theta || want_mat) {
       // L = E_last ... E_{j+1}
@@ -430,6 +430,33 @@ __global__ void obs_upload_kernel(ObsTable t, ObsTable* dst) {
 hipError_t launch_obs_upload(const ObsTable& t, void* dst, hipStream_t st) {
   hipLaunchKernelGGL(obs_upload_kernel, dim3(1), dim3(128), 0, st, t, (ObsTable*)dst);
   return hipGetLastError();
+}
+
+struct ByteChunk {
+  uint32_t w[512];  // 2 KiB per launch, passed by value
+};
+
+__global__ void bytes_upload_kernel(ByteChunk c, uint32_t nwords, uint32_t* dst) {
+  for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) dst[i] = c.w[i];
+}
+
+hipError_t launch_bytes_upload(const void* src, size_t n, void* dst, hipStream_t st) {
+  if (n % 4 != 0) return hipErrorInvalidValue;
+  const uint32_t* s = (const uint32_t*)src;
+  uint32_t* d = (uint32_t*)dst;
+  size_t words = n / 4;
+  while (words > 0) {
+    ByteChunk c;
+    const uint32_t k = (uint32_t)(words < 512 ? words : 512);
+    for (uint32_t i = 0; i < k; ++i) c.w[i] = s[i];
+    hipLaunchKernelGGL(bytes_upload_kernel, dim3(1), dim3(256), 0, st, c, k, d);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    s += k;
+    d += k;
+    words -= k;
+  }
+  return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------

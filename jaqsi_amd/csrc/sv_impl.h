@@ -28,14 +28,20 @@ struct C2 {
   R x, y;
 };
 
+// Tile bits of the pass kernels: 2^T amplitudes per tile (64 KiB).
+template <typename R>
+constexpr int kTileBits = sizeof(R) == 4 ? 13 : 12;
+
 // Program words, block matrices and observable tables are read-only for the
 // whole launch: reading them through the constant address space lets every
 // wave-uniform access compile to a scalar load (s_load into SGPRs) instead of
 // a vector load whose result occupies VGPRs.
 #if defined(__HIP_DEVICE_COMPILE__)
 #define JQ_CONST __attribute__((address_space(4)))
+#define JQ_LDS __attribute__((address_space(3)))
 #else
 #define JQ_CONST
+#define JQ_LDS
 #endif
 using cint = const JQ_CONST int;
 template <typename R>
@@ -118,9 +124,13 @@ struct Tile {
 struct Layout {
   uint32_t toff;   // tile-local offset of this thread
   uint64_t gthr;   // global offset of this thread's tile-local bits
-  uint32_t rpos[8];  // tile-local bit of each register bit
-  uint64_t gpos[8];  // global weight of each register bit
+  uint32_t rpos[8];  // tile-local bit of each register bit (wave-uniform)
+  cint* tbits;       // global bit of each tile bit (the pass record)
 };
+
+// Global weight of register bit b (wave-uniform; read from the pass record
+// instead of being kept live, which keeps the layouts small in SGPRs).
+__device__ __forceinline__ uint64_t gpos(const Layout& L, int b) { return 1ull << L.tbits[L.rpos[b]]; }
 
 template <int RB, int LNT>
 __device__ __forceinline__ void decode_layout(cint* sub, cint* tbits, Layout& L) {
@@ -136,12 +146,9 @@ __device__ __forceinline__ void decode_layout(cint* sub, cint* tbits, Layout& L)
   }
   L.toff = toff;
   L.gthr = gthr;
+  L.tbits = tbits;
 #pragma unroll
-  for (int b = 0; b < RB; ++b) {
-    const int tb = sub[1 + b];
-    L.rpos[b] = tb;
-    L.gpos[b] = 1ull << tbits[tb];
-  }
+  for (int b = 0; b < RB; ++b) L.rpos[b] = sub[1 + b];
 }
 
 template <int RB>
@@ -169,7 +176,7 @@ __device__ __forceinline__ uint64_t reg_goff(const Layout& L, int i) {
   uint64_t g = 0;
 #pragma unroll
   for (int b = 0; b < RB; ++b)
-    if ((i >> b) & 1) g |= L.gpos[b];
+    if ((i >> b) & 1) g |= gpos(L, b);
   return g;
 }
 
@@ -178,7 +185,7 @@ __device__ __forceinline__ uint64_t reg_gidx(const Layout& L, uint64_t base, int
   uint64_t g = base | L.gthr;
 #pragma unroll
   for (int b = 0; b < RB; ++b)
-    if ((i >> b) & 1) g |= L.gpos[b];
+    if ((i >> b) & 1) g |= gpos(L, b);
   return g;
 }
 
@@ -828,13 +835,10 @@ __device__ __forceinline__ void canonical_layout(cint* tbits, Layout& L) {
   for (int j = 0; j < LNT; ++j) gthr |= (uint64_t)((tid >> j) & 1u) << tbits[1 + j];
   L.toff = tid << 1;
   L.gthr = gthr;
+  L.tbits = tbits;
   L.rpos[0] = 0;
-  L.gpos[0] = 1ull << tbits[0];
 #pragma unroll
-  for (int b = 1; b < RB; ++b) {
-    L.rpos[b] = T - RB + b;
-    L.gpos[b] = 1ull << tbits[T - RB + b];
-  }
+  for (int b = 1; b < RB; ++b) L.rpos[b] = T - RB + b;
 }
 
 template <int RB>
@@ -951,25 +955,51 @@ __device__ __forceinline__ void store_canon(const R* re, const R* im, C2<R>* dst
   }
 }
 
+// Register <-> LDS exchange that moves the tile from layout `from` to
+// layout `to`.  LDS holds the tile as two planes (real parts, then imaginary
+// parts, 2^T entries each), so re[i] and im[i] are stored from any two
+// registers with one address and the DS offset field -- no register pairs
+// are assembled.  The swizzle is GF(2)-linear and thread / register bits are
+// disjoint, so the address of register i is
+//   swz(toff) ^ XOR_{b in i} swz(1 << rpos[b])     (entries)
+// whose second term is wave-uniform: it is formed in SGPRs (one s_xor per
+// register) and each access costs a single v_xor.
 template <typename R, int RB>
 __device__ __forceinline__ void relayout(R* re, R* im, C2<R>* lds, const Layout& from, const Layout& to) {
   constexpr int NR = 1 << RB;
-  // The swizzle is GF(2)-linear and thread / register bits are disjoint, so
-  // swz(toff | roff) = swz(toff) ^ swz(roff): one XOR per access.
-  const uint32_t sf = lds_swz(from.toff), st = lds_swz(to.toff);
+  constexpr uint32_t ES = sizeof(R);
+  uint32_t wf[RB], wt[RB];
+#pragma unroll
+  for (int b = 0; b < RB; ++b) {
+    wf[b] = __builtin_amdgcn_readfirstlane(lds_swz(1u << from.rpos[b]) * ES);
+    wt[b] = __builtin_amdgcn_readfirstlane(lds_swz(1u << to.rpos[b]) * ES);
+  }
+  // 32-bit LDS addresses.  The tile starts the dynamic LDS block, which is
+  // at LDS offset 0 (the kernels declare no static LDS), so the thread term
+  // can carry the base and the register term is XORed in.
+  constexpr uint32_t IM = ES << kTileBits<R>;  // byte distance of the imaginary plane
+  const uint32_t lb = (uint32_t)(uintptr_t)(JQ_LDS char*)(char*)lds;
+  const uint32_t sf = lb + lds_swz(from.toff) * ES, st = lb + lds_swz(to.toff) * ES;
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
-    C2<R> v;
-    v.x = re[i];
-    v.y = im[i];
-    lds[sf ^ lds_swz(reg_roff<RB>(from, i))] = v;
+    uint32_t o = 0;
+#pragma unroll
+    for (int b = 0; b < RB; ++b)
+      if ((i >> b) & 1) o ^= wf[b];
+    const uint32_t a = sf ^ o;
+    *reinterpret_cast<JQ_LDS R*>((uintptr_t)a) = re[i];
+    *reinterpret_cast<JQ_LDS R*>((uintptr_t)(a + IM)) = im[i];
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
-    const C2<R> v = lds[st ^ lds_swz(reg_roff<RB>(to, i))];
-    re[i] = v.x;
-    im[i] = v.y;
+    uint32_t o = 0;
+#pragma unroll
+    for (int b = 0; b < RB; ++b)
+      if ((i >> b) & 1) o ^= wt[b];
+    const uint32_t a = st ^ o;
+    re[i] = *reinterpret_cast<const JQ_LDS R*>((uintptr_t)a);
+    im[i] = *reinterpret_cast<const JQ_LDS R*>((uintptr_t)(a + IM));
   }
   __syncthreads();
 }
@@ -1020,7 +1050,7 @@ __device__ __forceinline__ int reg_bit_of(const Layout& L, int gb) {
   int r = -1;
 #pragma unroll
   for (int b = 0; b < RB; ++b)
-    if (L.gpos[b] == (1ull << gb)) r = b;
+    if ((int)L.tbits[L.rpos[b]] == gb) r = b;
   return r;
 }
 
@@ -1079,6 +1109,52 @@ __device__ __forceinline__ void expval_partials(const R* re, const R* im, const 
   }
 }
 
+// Per-circuit arguments of grid row blockIdx.y: the launch arguments
+// themselves, or (multi-plan launch) the row's group record.
+template <typename R>
+__device__ __forceinline__ SvArgs resolve_args(const SvArgs& a0, int& circ) {
+  if (a0.groups == nullptr) {
+    circ = blockIdx.y;
+    return a0;
+  }
+  typedef const JQ_CONST SvGroup cgroup;
+  cgroup* G = (cgroup*)a0.groups;
+  const int row = blockIdx.y;
+  int g = 0;
+  for (int k = 1; k < a0.n_groups; ++k)
+    if (G[k].circ_begin <= row) g = k;
+  SvArgs a = a0;
+  circ = row - G[g].circ_begin;
+  const int P = G[g].n_passes;
+  const int q = a0.dir == 0 ? a0.step : P - 1 - a0.step;
+  const int64_t vec = (int64_t)a0.vec_len;
+  const int64_t slice = vec * G[g].batch;  // elements of one checkpoint pass slice
+  C2<R>* ck = (C2<R>*)G[g].ckpt;
+  C2<R>* fin = (C2<R>*)G[g].state;
+  auto pass_out = [&](int p) -> C2<R>* { return p == P - 1 ? fin : ck + slice * p; };
+  a.pass = q;
+  a.prog = a0.dir == 0 ? G[g].prog_f : G[g].prog_b;
+  a.mats = G[g].mats;
+  a.pool_stride = G[g].pool_stride;
+  if (a0.dir == 0) {
+    a.src = q == 0 ? G[g].psi0 : (const void*)pass_out(q - 1);
+    a.src_stride = q == 0 ? G[g].psi0_stride : vec;
+    a.dst = pass_out(q);
+  } else {
+    a.src = pass_out(q);
+    a.src_stride = vec;
+    a.dst = nullptr;
+  }
+  a.dst_stride = vec;
+  a.lam = G[g].lam;
+  a.lam_stride = vec;
+  a.ev_part = G[g].ev_part;
+  a.grad_part = G[g].grad_part;
+  a.part_stride = G[g].part_stride;
+  a.ct = G[g].ct;
+  return a;
+}
+
 template <typename R, int RB, int NT, bool FULL>
 __device__ __forceinline__ void forward_tile(const SvArgs& a, int pass_idx, int circ, int tile, R* re, R* im,
                                              Layout& L, uint64_t& base, C2<R>* lds, bool do_load) {
@@ -1129,13 +1205,14 @@ __device__ __forceinline__ void to_canonical(R* re, R* im, cint* tbits, Layout& 
 }
 
 template <typename R, int RB, int NT, bool FULL>
-__global__ __launch_bounds__(NT, 2) void sv_forward_kernel(SvArgs a) {
+__global__ __launch_bounds__(NT, 2) void sv_forward_kernel(SvArgs a0) {
   extern __shared__ __align__(16) unsigned char smem[];
+  int circ;
+  const SvArgs a = resolve_args<R>(a0, circ);
   const int T = ((cint*)a.prog)[2];
   C2<R>* lds = reinterpret_cast<C2<R>*>(smem);
   ObsLds* obs = reinterpret_cast<ObsLds*>(smem + (sizeof(C2<R>) << T));
   double* ev = reinterpret_cast<double*>(obs + 1);
-  const int circ = blockIdx.y;
   const int tile = blockIdx.x;
   cint* pass = (cint*)a.prog + ((cint*)a.prog)[HDR_WORDS + a.pass];
   const bool last = (pass[1] & PF_LAST) != 0;
@@ -1274,13 +1351,14 @@ __device__ __forceinline__ void backward_tile(const SvArgs& a, int pass_idx, int
 }
 
 template <typename R, int RB, int NT, bool FULL>
-__global__ __launch_bounds__(NT) void sv_backward_kernel(SvArgs a) {
+__global__ __launch_bounds__(NT) void sv_backward_kernel(SvArgs a0) {
   extern __shared__ __align__(16) unsigned char smem[];
+  int circ;
+  const SvArgs a = resolve_args<R>(a0, circ);
   const int T = ((cint*)a.prog)[2];
   C2<R>* lds = reinterpret_cast<C2<R>*>(smem);
   ObsLds* obs = reinterpret_cast<ObsLds*>(smem + (sizeof(C2<R>) << T));
   double* slots = reinterpret_cast<double*>(obs + 1);
-  const int circ = blockIdx.y;
   const int tile = blockIdx.x;
   cint* pass = (cint*)a.prog + ((cint*)a.prog)[HDR_WORDS + a.pass];
   const int nslots = pass[2];

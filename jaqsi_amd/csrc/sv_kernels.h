@@ -5,6 +5,31 @@
 
 namespace jq {
 
+// One plan's circuits inside a multi-plan launch (circuits of several
+// structures -- e.g. uneven depths -- share every pass launch).  Pass q of a
+// group reads its input from the previous pass output and writes its own:
+// outputs of passes 0 .. n_passes-2 live in `ckpt` ([pass][circuit][amp]),
+// the last one in `state` ([circuit][amp]).
+struct SvGroup {
+  const int* prog_f;     // forward program (device)
+  const int* prog_b;     // backward program (device)
+  const void* mats;      // block matrices [batch][pool_stride]
+  int64_t pool_stride;
+  const void* psi0;      // initial states or nullptr
+  int64_t psi0_stride;
+  void* ckpt;
+  void* state;
+  void* lam;             // [circuit][amp]
+  double* ev_part;
+  double* grad_part;
+  int64_t part_stride;
+  const double* ct;      // [circuit][n_obs]
+  int n_passes;
+  int circ_begin;        // first grid row of the group
+  int batch;
+  int pad;
+};
+
 struct SvArgs {
   const int* prog;     // scheduled program (device)
   int pass;            // pass index of this launch
@@ -25,6 +50,15 @@ struct SvArgs {
   double* ev_part;       // [batch][ntiles][n_obs]
   double* grad_part;     // [batch][part_stride]
   int64_t part_stride;
+  // multi-plan launches: when `groups` is set, grid row y belongs to the
+  // group whose circ_begin is the largest <= y; the launch runs step `step`
+  // of direction `dir` (0 forward: pass = step; 1 backward: pass =
+  // n_passes - 1 - step) and every per-circuit pointer above comes from the
+  // group record.
+  const SvGroup* groups;
+  int n_groups;
+  int step;
+  int dir;
 };
 
 // Observable table: global bit and diagonal (d0, d1) of each observable wire.
@@ -36,6 +70,10 @@ struct ObsTable {
 // Copy a host-built table to device memory on the stream (a tiny kernel, so
 // the call stays asynchronous and capturable).
 hipError_t launch_obs_upload(const ObsTable& t, void* dst, hipStream_t st);
+
+// Copy n host bytes to device memory on the stream through kernel arguments
+// (asynchronous and capturable; for small tables).
+hipError_t launch_bytes_upload(const void* src, size_t n, void* dst, hipStream_t st);
 
 // which: 0 forward pass, 1 backward pass, 2 fused single-tile forward+backward
 hipError_t launch_sv_f32(int which, int T, const SvArgs& a, int ntiles, int batch, size_t smem, hipStream_t st);
