@@ -251,6 +251,99 @@ def cpu_baseline_c3(res, budget_s=15.0):
     }
 
 
+def record_noisy_hea(theta, n, p_dep, gamma):
+    """After every gate, on each touched wire in order: DepolarizingChannel
+    then AmplitudeDamping."""
+    from jaqsi_amd import Gates, noise
+
+    def after(wires):
+        for w in wires:
+            noise.DepolarizingChannel(p_dep, wires=w)
+            noise.AmplitudeDamping(gamma, wires=w)
+
+    for layer in range(theta.shape[0]):
+        for q in range(n):
+            Gates.RX(float(theta[layer, q, 0]), wires=q)
+            after([q])
+            Gates.RY(float(theta[layer, q, 1]), wires=q)
+            after([q])
+            Gates.RZ(float(theta[layer, q, 2]), wires=q)
+            after([q])
+        for q in range(n):
+            Gates.CX(wires=[q, (q + 1) % n])
+            after([q, (q + 1) % n])
+
+
+def bench_c4(steps, warmup, n=12, layers=6, seed=1004):
+    """Noisy density matrix: forward + checkpointed adjoint of Tr(Z0 rho)."""
+    import ctypes as C
+
+    import torch
+
+    from jaqsi_amd import _native as nat
+    from jaqsi_amd import simulation as S
+    from jaqsi_amd.tape import recording
+
+    L = nat.lib()
+    rng = np.random.default_rng(seed)
+    theta = rng.uniform(0, 2 * np.pi, (layers, n, 3))
+    with recording() as tape:
+        record_noisy_hea(theta, n, 0.01, 0.005)
+    low, plan, angles, mats, psi0, psi0_b = S._prepare(tape, n, True, None, None, 0)
+    runner = S._Runner(plan, 1, True, [(0, 1.0, -1.0)], n, False, 0)
+    ct = torch.ones((1, 1), dtype=torch.float64, device=angles.device)
+    for _ in range(warmup):
+        runner.adjoint(angles, mats, None, ct, False)
+    torch.cuda.synchronize()
+    nat.check(L.jq_timing_enable(1))
+    for w in range(3):
+        nat.check(L.jq_timing_read(w, None, None))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ev, ga, _, _ = runner.adjoint(angles, mats, None, ct, False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    nat.check(L.jq_timing_enable(0))
+    kt = {}
+    for w, name in ((0, "sv_forward_kernel"), (1, "sv_backward_kernel")):
+        tot, cnt = C.c_double(), C.c_int32()
+        nat.check(L.jq_timing_read(w, C.byref(tot), C.byref(cnt)))
+        kt[name] = (tot.value, cnt.value)
+    info = plan.info()
+    sweep = 2 * 8 * (1 << (2 * n))
+    per = {"sv_forward_kernel": sweep, "sv_backward_kernel": sweep * 3 // 2}
+    kern = {k: {"avg_us": round(v[0] / max(v[1], 1) * 1e3, 2), "launches": v[1],
+                "GB/s": round(per[k] * v[1] / max(v[0] / 1e3, 1e-12) / 1e9, 1)} for k, v in kt.items()}
+    dom = max(kt, key=lambda k: kt[k][0])
+    return {
+        "metric": "noisy density-matrix adjoint-grad evaluations/s (12 qubits, 6 layers, 720 Kraus channels)",
+        "value": round(steps / (t1 - t0), 3),
+        "unit": "gradient evaluations/s",
+        "steps": steps,
+        "ms_per_step": round((t1 - t0) / steps * 1e3, 3),
+        "dtype": "c64",
+        "config": {
+            "workload": "rho 4096x4096 complex64 as a 24-wire vector; HEA 6 layers (216 angles U[0,2pi), seed 1004), "
+            "DepolarizingChannel(0.01) + AmplitudeDamping(0.005) after every gate on each touched wire; "
+            "Tr(Z0 rho) and its gradient by the checkpointed adjoint",
+            "passes": info["n_passes"],
+            "blocks": info["n_blocks"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": kern[dom]["GB/s"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(kern[dom]["GB/s"] / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernels": kern,
+        },
+        "expval": float(ev.cpu()[0, 0]),
+    }
+
+
 def cpu_baseline_c2(theta):
     from oracle import cpu
 
@@ -277,6 +370,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true", help="skip the 4096x14-qubit ragged-batch leg")
     ap.add_argument("--c3-steps", type=int, default=5)
+    ap.add_argument("--no-c4", action="store_true", help="skip the 12-qubit noisy density-matrix leg")
     ap.add_argument(
         "--copy-probe",
         type=int,
@@ -355,6 +449,8 @@ def main():
         if not args.no_cpu:
             c3["cpu_baseline"] = cpu_baseline_c3(r3)
         line["c3"] = c3
+    if not args.no_c4:
+        line["c4"] = bench_c4(3, 1)
     print(json.dumps(line), flush=True)
 
 

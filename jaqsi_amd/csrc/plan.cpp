@@ -184,6 +184,8 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
     }
     if (b.grad) {
       if (b.kind == AK_DENSEK) throw PlanError(JQ_ERR_UNSUPPORTED, "dense 3-wire gradient");
+      if (b.kind == AK_DIAG && b.k > 2)
+        throw PlanError(JQ_ERR_UNSUPPORTED, "gradients of diagonal gates on more than 2 wires are not supported yet");
       b.bar = bars;
       b.bar_size = elems;
       bars += elems;

@@ -602,15 +602,72 @@ __device__ __forceinline__ void diag_grad(const R* pr, const R* pi, const R* lr,
   }
 }
 
-// Reduce per-thread accumulators over the workgroup into LDS slots (double).
-// (static loop over the register array; nv <= 8 entries are live)
+// DPP move of a 32-bit lane value (every control used here reads a lane of
+// the same row, so all lanes receive a value).
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f(double x) { return dpp_step_d<CTRL>(x); }
+
+// v_permlane32_swap: a' = [a.lanes 0-31, b.lanes 0-31], b' = [a.lanes 32-63, b.lanes 32-63]
+__device__ __forceinline__ float swap_add32(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+// v_permlane16_swap: odd rows of a <-> even rows of b
+__device__ __forceinline__ float swap_add16(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ double swap_add32(double a, double b) {
+  const uint64_t ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)ua, (uint32_t)ub, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(ua >> 32), (uint32_t)(ub >> 32), false, false);
+  return __longlong_as_double((int64_t)(((uint64_t)hi[0] << 32) | lo[0])) +
+         __longlong_as_double((int64_t)(((uint64_t)hi[1] << 32) | lo[1]));
+}
+__device__ __forceinline__ double swap_add16(double a, double b) {
+  const uint64_t ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)ua, (uint32_t)ub, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(ua >> 32), (uint32_t)(ub >> 32), false, false);
+  return __longlong_as_double((int64_t)(((uint64_t)hi[0] << 32) | lo[0])) +
+         __longlong_as_double((int64_t)(((uint64_t)hi[1] << 32) | lo[1]));
+}
+
+// Transposed wave reduction of 8 per-lane values v[0..7]: halving the set
+// at each exchange (lane bit 5 by permlane32 swap, bit 4 by permlane16 swap,
+// bit 3 by a row_ror:8 DPP add and a select), then the sum over 8 lanes
+// (quad_perm xor 1, xor 2, row_half_mirror).  Afterwards every lane L holds
+// the wave total of v[L >> 3]; lanes 8k (k < nv) add it to slots[dst(k)].
+// 18 VALU per 8 values instead of one full reduction per value.
+template <typename R, typename Dst>
+__device__ __forceinline__ void flush8(const R* v, int nv, double* slots, Dst dst) {
+  R w[4], u[2];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) w[k] = swap_add32(v[k], v[k + 4]);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) u[k] = swap_add16(w[k], w[k + 2]);
+  const int lane = threadIdx.x & 63;
+  const R t0 = u[0] + dpp_f<0x128>(u[0]);
+  const R t1 = u[1] + dpp_f<0x128>(u[1]);
+  R r = (lane & 8) ? t1 : t0;
+  r += dpp_f<0xB1>(r);
+  r += dpp_f<0x4E>(r);
+  r += dpp_f<0x141>(r);
+  const int k = lane >> 3;
+  if ((lane & 7) == 0 && k < nv) lds_add(&slots[dst(k)], (double)r);
+}
+
+// Reduce per-thread accumulators acc[0 .. nv) (nv <= 8) over the workgroup
+// into the LDS slots base .. base + nv.
 template <typename R>
 __device__ __forceinline__ void flush_vals(const R* acc, int nv, double* slots, int base) {
+  R v[8];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    if (e >= nv) break;
-    lds_add(&slots[base + e], (double)wave_sum<R>(acc[e]));
-  }
+  for (int e = 0; e < 8; ++e) v[e] = e < nv ? acc[e] : R(0);
+  flush8<R>(v, nv, slots, [base](int k) { return base + k; });
 }
 
 struct GateCtx {
@@ -759,15 +816,16 @@ __device__ __forceinline__ void bwd_gate(R* pr, R* pi, R* lr, R* li, cint* g, cm
           }
           // kernel rows 2h, 2h+1; kernel column c -> block (s(row), s(c)) when swapped
 #pragma unroll
-          for (int oo = 0; oo < 2; ++oo)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-              const int ko = 2 * half + oo;
-              const int bo = sw ? (((ko & 1) << 1) | (ko >> 1)) : ko;
+          for (int oo = 0; oo < 2; ++oo) {
+            const int ko = 2 * half + oo;
+            const int bo = sw ? (((ko & 1) << 1) | (ko >> 1)) : ko;
+            // acc[2 (oo 4 + c) + reim] -> slot + 2 (bo 4 + s(c)) + reim
+            flush8<R>(acc + 8 * oo, 8, slots, [=](int k) {
+              const int c = k >> 1;
               const int bc = sw ? (((c & 1) << 1) | (c >> 1)) : c;
-              lds_add(&slots[slot + 2 * (bo * 4 + bc)], (double)wave_sum<R>(acc[2 * (oo * 4 + c)]));
-              lds_add(&slots[slot + 2 * (bo * 4 + bc) + 1], (double)wave_sum<R>(acc[2 * (oo * 4 + c) + 1]));
-            }
+              return slot + 2 * (bo * 4 + bc) + (k & 1);
+            });
+          }
         }
       }
       {
