@@ -344,6 +344,106 @@ def bench_c4(steps, warmup, n=12, layers=6, seed=1004):
     }
 
 
+FP64_PEAK_TFLOPS = 78.6
+
+
+def c5_inputs(n_sets, seed):
+    """Parameter sets (A, sigma, beta, phi), frequencies (w0, w1) and the
+    target RX(pi/2) x I, drawn with numpy default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(0.1, 1.0, n_sets)
+    sigma = rng.uniform(5.0, 20.0, n_sets)
+    beta = rng.uniform(-1.0, 1.0, n_sets)
+    phi = rng.uniform(0.0, 2 * np.pi, n_sets)
+    omega = rng.uniform(4.5, 5.5, (n_sets, 2)) * 2 * np.pi
+    c, s_ = np.cos(np.pi / 4), np.sin(np.pi / 4)
+    rx = np.array([[c, -1j * s_], [-1j * s_, c]])
+    return np.stack([A, sigma, beta, phi], axis=1), omega, np.kron(rx, np.eye(2))
+
+
+def bench_c5(steps, warmup, n_sets=8192, n_steps=2000, seed=1005):
+    """Batched pulse propagators: 8192 DRAG-Gaussian parameter sets, 2 coupled
+    qubits, 2000 Magnus-2 steps over 50 ns in FP64; infidelity against
+    RX(pi/2) x I and its gradient w.r.t. (A, sigma, beta, phi)."""
+    import torch
+
+    from jaqsi_amd import _native as nat
+    params, omega, V = c5_inputs(n_sets, seed)
+    dev = torch.device("cuda", 0)
+    P = torch.tensor(np.concatenate([params, omega], axis=1), dtype=torch.float64, device=dev).contiguous()
+    Vh = np.ascontiguousarray(np.stack([V.real, V.imag], axis=-1).reshape(-1))
+    U = torch.empty((n_sets, 4, 4), dtype=torch.complex128, device=dev)
+    inf = torch.empty(n_sets, dtype=torch.float64, device=dev)
+    g = torch.empty((n_sets, 4), dtype=torch.float64, device=dev)
+    d = nat.PulseDesc(n_sets, n_steps, 50.0, 2 * np.pi * 0.005, nat.JQ_ENV_DRAG_GAUSSIAN, 0)
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        nat.check(nat.lib().jq_pulse_propagate(d, nat.ptr(P), nat.ptr(Vh), nat.ptr(U), nat.ptr(inf), nat.ptr(g),
+                                               stream.cuda_stream))
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kms = e0.elapsed_time(e1) / steps
+    flops = 1536.0 * n_steps * n_sets  # 4x4 complex product floor: 512 forward + 1024 backward per step
+    ach = flops / (kms / 1e3) / 1e12
+    return {
+        "metric": "pulse-propagator gradient evaluations/s (8192 sets, 2 qubits, 2000 Magnus-2 steps, FP64)",
+        "value": round(n_sets * steps / (t1 - t0), 1),
+        "unit": "parameter sets/s (propagator + infidelity + 4-parameter gradient)",
+        "steps": steps,
+        "ms_per_step": round((t1 - t0) / steps * 1e3, 3),
+        "dtype": "f64",
+        "config": {
+            "workload": "8192 sets: A~U[0.1,1], sigma~U[5,20] ns, beta~U[-1,1], phi~U[0,2pi), w_q~U[4.5,5.5]*2pi "
+            "rad/ns (seed 1005); J=2pi*0.005 rad/ns; T=50 ns, 2000 steps; H(t)=sum_q w_q/2 Z_q + J Z0Z1 + "
+            "DRAG drive on qubit 0; infidelity vs RX(pi/2) x I",
+        },
+        "roofline": {
+            "bound": "fp64",
+            "kernel": "pulse_kernel",
+            "achieved": round(ach, 3),
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(ach / FP64_PEAK_TFLOPS, 4),
+            "traffic": None,
+            "flop_model": "1536 FP64 FLOP per step per set (4x4 complex products: 512 forward + 1024 backward)",
+            "kernel_ms": round(kms, 4),
+        },
+        "sample_infidelity": float(inf[0].cpu()),
+    }
+
+
+def cpu_baseline_c5(budget_s=10.0):
+    from oracle import pulse
+
+    params, omega = pulse.workload(8192, 1005)
+    t0 = time.perf_counter()
+    done = 0
+    while time.perf_counter() - t0 < budget_s and done < 8192:
+        pulse.propagate(params[done], omega[done], pulse.J_C5, pulse.T_C5, pulse.N_C5)
+        done += 1
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(done / dt, 3),
+        "unit": "parameter sets/s (forward propagator only)",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"the first {done} sets, forward Magnus-2 product only (SciPy expm per step, the reference step); "
+        "no gradient",
+        "seconds": round(dt, 2),
+    }
+
+
 def cpu_baseline_c2(theta):
     from oracle import cpu
 
@@ -371,6 +471,7 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the 4096x14-qubit ragged-batch leg")
     ap.add_argument("--c3-steps", type=int, default=5)
     ap.add_argument("--no-c4", action="store_true", help="skip the 12-qubit noisy density-matrix leg")
+    ap.add_argument("--no-c5", action="store_true", help="skip the pulse-propagator leg")
     ap.add_argument(
         "--copy-probe",
         type=int,
@@ -451,6 +552,11 @@ def main():
         line["c3"] = c3
     if not args.no_c4:
         line["c4"] = bench_c4(3, 1)
+    if not args.no_c5:
+        c5 = bench_c5(3, 1)
+        if not args.no_cpu:
+            c5["cpu_baseline"] = cpu_baseline_c5()
+        line["c5"] = c5
     print(json.dumps(line), flush=True)
 
 

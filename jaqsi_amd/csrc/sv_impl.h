@@ -381,17 +381,27 @@ __device__ __forceinline__ void sweep_grad(const R* pr, const R* pi, const R* lr
   }
 }
 
+// In-place exchange of two registers with one v_swap_b32 per 32 bits: a
+// swap inside a runtime-dispatched branch cannot be renamed away by the
+// register allocator, and three moves per pair would otherwise be emitted.
+__device__ __forceinline__ void vswap(float& a, float& b) { asm("v_swap_b32 %0, %1" : "+v"(a), "+v"(b)); }
+__device__ __forceinline__ void vswap(double& a, double& b) {
+  uint64_t x = __double_as_longlong(a), y = __double_as_longlong(b);
+  uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32), yl = (uint32_t)y, yh = (uint32_t)(y >> 32);
+  asm("v_swap_b32 %0, %1" : "+v"(xl), "+v"(yl));
+  asm("v_swap_b32 %0, %1" : "+v"(xh), "+v"(yh));
+  a = __longlong_as_double((int64_t)(((uint64_t)xh << 32) | xl));
+  b = __longlong_as_double((int64_t)(((uint64_t)yh << 32) | yl));
+}
+
 template <typename R, int NR, int A>
 __device__ __forceinline__ void perm_x(R* re, R* im, uint32_t creg) {
 #pragma unroll
   for (int k = 0; k < NR / 2; ++k) {
     const int i = G1<NR, A>::base(k), j = i | (1 << A);
     if (((uint32_t)i & creg) != creg) continue;
-    const R ar = re[i], ai = im[i];
-    re[i] = re[j];
-    im[i] = im[j];
-    re[j] = ar;
-    im[j] = ai;
+    vswap(re[i], re[j]);
+    vswap(im[i], im[j]);
   }
 }
 
@@ -402,11 +412,8 @@ __device__ __forceinline__ void perm_swap(R* re, R* im, uint32_t creg) {
     const int i = G2<NR, A, B>::base(k);
     if (((uint32_t)i & creg) != creg) continue;
     const int p = i | (1 << A), q = i | (1 << B);
-    const R ar = re[p], ai = im[p];
-    re[p] = re[q];
-    im[p] = im[q];
-    re[q] = ar;
-    im[q] = ai;
+    vswap(re[p], re[q]);
+    vswap(im[p], im[q]);
   }
 }
 
@@ -910,14 +917,17 @@ __device__ __forceinline__ bool same_layout(cint* sub, const Layout& L) {
 // Load/store under the canonical layout: registers (2k, 2k+1) hold adjacent
 // amplitudes, moved as one 16-byte (complex64) or two 16-byte (complex128)
 // vector accesses.
+// |0...0>: only register 0 of the thread at global offset 0 (its register
+// bits contribute nothing) holds amplitude 0.
 template <typename R, int RB>
 __device__ __forceinline__ void zero_state(R* re, R* im, const Layout& L, uint64_t base) {
   constexpr int NR = 1 << RB;
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
-    re[i] = (reg_gidx<RB>(L, base, i) == 0) ? R(1) : R(0);
+    re[i] = 0;
     im[i] = 0;
   }
+  if ((base | L.gthr) == 0) re[0] = 1;
 }
 
 template <typename R>
