@@ -473,6 +473,11 @@ def main():
     ap.add_argument("--no-c4", action="store_true", help="skip the 12-qubit noisy density-matrix leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the pulse-propagator leg")
     ap.add_argument(
+        "--legs",
+        default="c2,c3,c4,c5",
+        help="comma-separated workloads to run (profiling runs select one); the headline line needs c2",
+    )
+    ap.add_argument(
         "--copy-probe",
         type=int,
         default=0,
@@ -499,6 +504,20 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": args.copy_probe << 20}), flush=True)
+        return
+    legs = set(args.legs.split(","))
+    if "c2" not in legs:
+        # profiling run of the secondary legs only: one JSON object per leg
+        out = {}
+        if "c3" in legs:
+            r3 = bench_c3(args.c3_steps, 1)
+            out["c3"] = {"circuits_per_s": round(r3["B"] * args.c3_steps / r3["seconds"], 1),
+                         "roofline": roofline_c3(r3, args.c3_steps)}
+        if "c4" in legs:
+            out["c4"] = bench_c4(args.steps, 1)
+        if "c5" in legs:
+            out["c5"] = bench_c5(args.steps, 1)
+        print(json.dumps(out), flush=True)
         return
     res = bench_c2(args.steps, args.warmup)
     ms = res["seconds"] / args.steps * 1e3
@@ -529,7 +548,7 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_c2(res["theta"])
-    if not args.no_c3:
+    if not args.no_c3 and "c3" in legs:
         r3 = bench_c3(args.c3_steps, 1)
         c3 = {
             "metric": "ragged-batch adjoint-grad circuits/s (4096 x 14 qubits, depth U{2..12})",
@@ -550,9 +569,9 @@ def main():
         if not args.no_cpu:
             c3["cpu_baseline"] = cpu_baseline_c3(r3)
         line["c3"] = c3
-    if not args.no_c4:
+    if not args.no_c4 and "c4" in legs:
         line["c4"] = bench_c4(3, 1)
-    if not args.no_c5:
+    if not args.no_c5 and "c5" in legs:
         c5 = bench_c5(3, 1)
         if not args.no_cpu:
             c5["cpu_baseline"] = cpu_baseline_c5()

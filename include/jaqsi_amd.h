@@ -211,7 +211,7 @@ int jq_pulse_propagate(const jq_pulse_desc* desc, const double* params, const do
 
 /* Kernel timing for measurement: when enabled, every tile-pass launch is
  * bracketed by HIP events on its stream; jq_timing_read sums and clears
- * them (which = 0 forward, 1 backward, 2 fused).  No reference counterpart. */
+ * them (which = 0 forward passes, 1 backward passes).  No reference counterpart. */
 int jq_timing_enable(int32_t on);
 int jq_timing_read(int32_t which, double* total_ms, int32_t* count);
 

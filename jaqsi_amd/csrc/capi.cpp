@@ -109,10 +109,10 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   if (desc->dtype != 0 && desc->dtype != 1) return fail(JQ_ERR_ARG, "dtype must be 0 (complex64) or 1 (complex128)");
   if (desc->n_ops < 0 || (desc->n_ops > 0 && !desc->ops)) return fail(JQ_ERR_ARG, "bad op list");
   int T = desc->tile_bits;
-  // forward: 32 (complex64) / 16 (complex128) amplitudes per thread; the
-  // backward holds psi and lambda, so it uses half as many per thread.
-  const int RB = desc->dtype == 0 ? 5 : 4;
-  const int RBb = RB - 1;
+  // amplitudes per thread: forward 16 (2^4), backward 8 (2^3, psi and lambda)
+  // for both dtypes; the workgroup covers one 64 KiB tile (sv_f32 / sv_f64).
+  const int RB = 4;
+  const int RBb = 3;
   if (T == 0) T = desc->dtype == 0 ? 13 : 12;
   const bool okT = desc->dtype == 0 ? (T == 13) : (T == 12);
   if (!okT) return fail(JQ_ERR_ARG, "tile_bits must be 13 (complex64) or 12 (complex128)");
@@ -135,7 +135,7 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   const int* W = p->hp.words.data();
   for (int q = 0; q < p->hp.n_passes; ++q) {
     const int* pass = W + W[jq::HDR_WORDS + q];
-    if (lds_state + 64 * 4 + 128 * 8 + (64 + (size_t)pass[2]) * 8 > 160 * 1024) {
+    if (2 * lds_state + 64 * 4 + 128 * 8 + (64 + (size_t)pass[2]) * 8 > 160 * 1024) {
       delete p;
       return fail(JQ_ERR_UNSUPPORTED, "too many gradient slots in one pass for LDS");
     }
@@ -293,9 +293,9 @@ jq::SvArgs base_args(jq_plan* p) {
   return a;
 }
 
-size_t smem_for(jq_plan* p, int extra_reals) {
-  // state tile | observable table (64 ints + 128 doubles) | 64 expval + slot reals
-  return ((size_t)elem_bytes(p->dtype) << p->hp.T) + 64 * 4 + 128 * 8 + (size_t)(64 + extra_reals) * 8;
+size_t smem_for(jq_plan* p, int extra_reals, int tiles = 1) {
+  // state tile(s) | observable table (64 ints + 128 doubles) | 64 expval + slot reals
+  return (size_t)tiles * ((size_t)elem_bytes(p->dtype) << p->hp.T) + 64 * 4 + 128 * 8 + (size_t)(64 + extra_reals) * 8;
 }
 
 int max_slots(jq_plan* p) {
@@ -430,7 +430,7 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
     }
     if (expvals) HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
     // backward sweep
-    const size_t sm = smem_for(p, max_slots(p));
+    const size_t sm = smem_for(p, max_slots(p), 2);
     a.prog = (const int*)p->prog_b.p;
     for (int q = P - 1; q >= 0; --q) {
       a.pass = q;
@@ -510,7 +510,7 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
     t.batch = G.batch;
     row_begin[k + 1] = row_begin[k] + G.batch;
     maxP = std::max(maxP, P);
-    smem_b = std::max(smem_b, smem_for(p, max_slots(p)));
+    smem_b = std::max(smem_b, smem_for(p, max_slots(p), 2));
   }
   HIP_TRY(p0->groups.ensure(sizeof(jq::SvGroup) * n_groups));
   HIP_TRY(jq::launch_bytes_upload(table.data(), sizeof(jq::SvGroup) * n_groups, p0->groups.p, st));

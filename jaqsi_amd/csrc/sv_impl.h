@@ -69,6 +69,7 @@ __device__ __forceinline__ float dpp_row_sum(float v) {
   return v;
 }
 
+// x + (x moved by DPP control CTRL): one step of a DPP reduction.
 template <int CTRL>
 __device__ __forceinline__ double dpp_step_d(double x) {
   const long long b = __double_as_longlong(x);
@@ -107,8 +108,14 @@ __device__ __forceinline__ R wave_sum(R v) {
 }
 
 // Add a wave-uniform value into an LDS double from one lane.
+// Add a wave-uniform value once per wave (lane 0 issues the atomic).
 __device__ __forceinline__ void lds_add(double* p, double v) {
   if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Per-lane LDS atomic add (each calling lane its own address and value).
+__device__ __forceinline__ void lds_add_lane(double* p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // ---------------------------------------------------------------------------
@@ -478,7 +485,9 @@ __device__ __forceinline__ void dense3(R* re, R* im, cmat<R>* mp, int mode, cons
     case 0: { constexpr int A = 0; __VA_ARGS__; } break; \
     case 1: { constexpr int A = 1; __VA_ARGS__; } break; \
     case 2: { constexpr int A = 2; __VA_ARGS__; } break; \
-    case 3: { constexpr int A = 3; __VA_ARGS__; } break; \
+    case 3:                                            \
+      if constexpr (RB > 3) { constexpr int A = RB > 3 ? 3 : 0; __VA_ARGS__; } \
+      break;                                           \
     default:                                           \
       if constexpr (RB > 4) { constexpr int A = RB > 4 ? 4 : 0; __VA_ARGS__; } \
       break;                                           \
@@ -616,7 +625,12 @@ __device__ __forceinline__ float dpp_f(float x) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, false));
 }
 template <int CTRL>
-__device__ __forceinline__ double dpp_f(double x) { return dpp_step_d<CTRL>(x); }
+__device__ __forceinline__ double dpp_f(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 // v_permlane32_swap: a' = [a.lanes 0-31, b.lanes 0-31], b' = [a.lanes 32-63, b.lanes 32-63]
 __device__ __forceinline__ float swap_add32(float a, float b) {
@@ -664,7 +678,7 @@ __device__ __forceinline__ void flush8(const R* v, int nv, double* slots, Dst ds
   r += dpp_f<0x4E>(r);
   r += dpp_f<0x141>(r);
   const int k = lane >> 3;
-  if ((lane & 7) == 0 && k < nv) lds_add(&slots[dst(k)], (double)r);
+  if ((lane & 7) == 0 && k < nv) lds_add_lane(&slots[dst(k)], (double)r);
 }
 
 // Reduce per-thread accumulators acc[0 .. nv) (nv <= 8) over the workgroup
@@ -1032,8 +1046,12 @@ __device__ __forceinline__ void store_canon(const R* re, const R* im, C2<R>* dst
 //   swz(toff) ^ XOR_{b in i} swz(1 << rpos[b])     (entries)
 // whose second term is wave-uniform: it is formed in SGPRs (one s_xor per
 // register) and each access costs a single v_xor.
+// `region` selects a 2-plane LDS region (0: the first 2 * 2^T entries, 1:
+// the next); `tail_sync` = false skips the closing barrier when the next LDS
+// writes go to the other region (psi and lambda in the backward).
 template <typename R, int RB>
-__device__ __forceinline__ void relayout(R* re, R* im, C2<R>* lds, const Layout& from, const Layout& to) {
+__device__ __forceinline__ void relayout(R* re, R* im, C2<R>* lds, const Layout& from, const Layout& to,
+                                         int region = 0, bool tail_sync = true) {
   constexpr int NR = 1 << RB;
   constexpr uint32_t ES = sizeof(R);
   uint32_t wf[RB], wt[RB];
@@ -1046,7 +1064,7 @@ __device__ __forceinline__ void relayout(R* re, R* im, C2<R>* lds, const Layout&
   // at LDS offset 0 (the kernels declare no static LDS), so the thread term
   // can carry the base and the register term is XORed in.
   constexpr uint32_t IM = ES << kTileBits<R>;  // byte distance of the imaginary plane
-  const uint32_t lb = (uint32_t)(uintptr_t)(JQ_LDS char*)(char*)lds;
+  const uint32_t lb = (uint32_t)(uintptr_t)(JQ_LDS char*)(char*)lds + (uint32_t)region * 2 * IM;
   const uint32_t sf = lb + lds_swz(from.toff) * ES, st = lb + lds_swz(to.toff) * ES;
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
@@ -1069,7 +1087,7 @@ __device__ __forceinline__ void relayout(R* re, R* im, C2<R>* lds, const Layout&
     re[i] = *reinterpret_cast<const JQ_LDS R*>((uintptr_t)a);
     im[i] = *reinterpret_cast<const JQ_LDS R*>((uintptr_t)(a + IM));
   }
-  __syncthreads();
+  if (tail_sync) __syncthreads();
 }
 
 __device__ __forceinline__ uint64_t tile_base(int tile, cint* tbits, int T, int n) {
@@ -1273,7 +1291,7 @@ __device__ __forceinline__ void to_canonical(R* re, R* im, cint* tbits, Layout& 
 }
 
 template <typename R, int RB, int NT, bool FULL>
-__global__ __launch_bounds__(NT, 2) void sv_forward_kernel(SvArgs a0) {
+__global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void sv_forward_kernel(SvArgs a0) {
   extern __shared__ __align__(16) unsigned char smem[];
   int circ;
   const SvArgs a = resolve_args<R>(a0, circ);
@@ -1365,8 +1383,8 @@ __device__ __forceinline__ void to_canonical2(R* pr, R* pi, R* lr, R* li, cint* 
 #pragma unroll
   for (int b = 0; b < RB; ++b) same = same && (cl.rpos[b] == L.rpos[b]);
   if (!same) {
-    relayout<R, RB>(pr, pi, lds, L, cl);
-    relayout<R, RB>(lr, li, lds, L, cl);
+    relayout<R, RB>(pr, pi, lds, L, cl, 0, false);
+    relayout<R, RB>(lr, li, lds, L, cl, 1, true);
     L = cl;
   }
 }
@@ -1404,8 +1422,8 @@ __device__ __forceinline__ void backward_tile(const SvArgs& a, int pass_idx, int
     if (!same_layout<RB>(sub, L)) {
       Layout nl;
       decode_layout<RB, LNT>(sub, tbits, nl);
-      relayout<R, RB>(pr, pi, lds, L, nl);
-      relayout<R, RB>(lr, li, lds, L, nl);
+      relayout<R, RB>(pr, pi, lds, L, nl, 0, false);  // psi: region 0, lambda: region 1
+      relayout<R, RB>(lr, li, lds, L, nl, 1, true);
       L = nl;
     }
     const int ng = sub[0];
@@ -1424,8 +1442,8 @@ __global__ __launch_bounds__(NT) void sv_backward_kernel(SvArgs a0) {
   int circ;
   const SvArgs a = resolve_args<R>(a0, circ);
   const int T = ((cint*)a.prog)[2];
-  C2<R>* lds = reinterpret_cast<C2<R>*>(smem);
-  ObsLds* obs = reinterpret_cast<ObsLds*>(smem + (sizeof(C2<R>) << T));
+  C2<R>* lds = reinterpret_cast<C2<R>*>(smem);  // psi and lambda tiles (relayout2)
+  ObsLds* obs = reinterpret_cast<ObsLds*>(smem + (2 * sizeof(C2<R>) << T));
   double* slots = reinterpret_cast<double*>(obs + 1);
   const int tile = blockIdx.x;
   cint* pass = (cint*)a.prog + ((cint*)a.prog)[HDR_WORDS + a.pass];

@@ -343,3 +343,46 @@ def _lifted_diag(m, wires, n):
     for j, w in enumerate(wires):
         r |= ((idx >> (n - 1 - w)) & 1) << (k - 1 - j)
     return np.diag(m)[r]
+
+
+# ---------------------------------------------------------------------------
+# thin entry points for a custom_vjp bridge on the reference side: the
+# lowered structure is built once, angles and cotangents cross as tensors.
+# ---------------------------------------------------------------------------
+def lower_for_jax(tape: List[Operation], n_qubits: int, use_density: bool):
+    """Lower a recorded tape; returns (lowered structure, float64 angle vector)."""
+    tape = [op for op in tape if not isinstance(op, Barrier)]
+    low = lowering.lower_mixed(tape, n_qubits) if use_density else lowering.lower_pure(tape, n_qubits)
+    angles = np.array([float(a) for a in low.angle_values], dtype=np.float64)
+    return (low, n_qubits, use_density), angles
+
+
+def _runner_for(lowered, obs, batch=1):
+    low, n_qubits, use_density = lowered
+    dtype = _dtype_code()
+    plan = get_plan(low, dtype)
+    diags = diag_observables(obs)
+    if not diags:
+        raise NotImplementedError("the lowered entry points take diagonal single-wire observables")
+    runner = _Runner(plan, batch, use_density, diags, n_qubits, False, dtype)
+    mats = lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, None)
+    if mats is not None:
+        mats = mats.reshape(1, -1).to(device(), torch.complex128).contiguous()
+    return runner, mats
+
+
+def run_lowered(lowered, angles: torch.Tensor, obs) -> torch.Tensor:
+    """Expectation values (n_obs,) of the lowered circuit at `angles` (float64)."""
+    runner, mats = _runner_for(lowered, obs)
+    a = angles.to(device(), torch.float64).reshape(1, -1).contiguous()
+    ev, _ = runner.forward(a, mats, None)
+    return ev[0]
+
+
+def adjoint_lowered(lowered, angles: torch.Tensor, obs, ct: torch.Tensor) -> torch.Tensor:
+    """Gradient of sum_o ct_o <O_o> w.r.t. the angle vector (float64)."""
+    runner, mats = _runner_for(lowered, obs)
+    a = angles.to(device(), torch.float64).reshape(1, -1).contiguous()
+    c = ct.to(device(), torch.float64).reshape(1, -1).contiguous()
+    _, ga, _, _ = runner.adjoint(a, mats, None, c, False)
+    return ga[0]

@@ -117,3 +117,26 @@ def test_initial_state_and_its_cotangent():
     np.testing.assert_allclose(ev.detach().cpu().numpy(), vals, atol=1e-11)
     # PyTorch's cotangent is the conjugate of JAX's
     np.testing.assert_allclose(p0.grad.cpu().numpy(), np.conj(lam0), atol=1e-11)
+
+
+def test_lowered_entry_points_for_a_custom_vjp_bridge():
+    import jaqsi_amd as J
+    from jaqsi_amd import simulation as S
+    from jaqsi_amd.tape import recording
+
+    print("lowered entry points", flush=True)
+    J.enable_x64(True)
+    n = 4
+    theta = hea_theta(n, 2, 77)
+    with recording() as tape:
+        record_hea(theta, n)
+    obs = [J.PauliZ(wires=q, record=False) for q in range(n)]
+    lowered, angles = S.lower_for_jax(tape, n, False)
+    a = torch.tensor(angles, device="cuda")
+    ev = S.run_lowered(lowered, a, obs).cpu().numpy()
+    ct = torch.arange(1, n + 1, dtype=torch.float64, device="cuda")
+    g = S.adjoint_lowered(lowered, a, obs, ct).cpu().numpy()
+    ops = sv.hea_ops(theta, n)
+    vals, grads, _, _ = sv.adjoint(ops, n, sv.z_obs(n), ct=np.arange(1, n + 1, dtype=float))
+    np.testing.assert_allclose(ev, vals, atol=1e-11)
+    np.testing.assert_allclose(g, sv.flat_grads(ops, grads), atol=1e-10)
