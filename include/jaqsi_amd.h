@@ -90,6 +90,12 @@ typedef struct jq_op {
 #define JQ_OBS_NONE 0
 #define JQ_OBS_SV_DIAG 1 /* <psi|O|psi>, O = diag(d0,d1) on one wire (simulation.py:772-801) */
 #define JQ_OBS_DM_DIAG 2 /* Tr(O rho), O = diag(d0,d1) on one ket wire  (simulation.py:850-852) */
+/* <psi|O|psi>, O|x> = d[parity(x & mask)] |x>: a Pauli-Z word (d0 = 1, d1 = -1) on
+ * the wires of `mask`.  For this kind obs_wires holds TWO int32 per observable,
+ * the low and high 32 bits of the wire mask (bit w = wire w).  The GPU Script
+ * path measures Pauli words / Hermitian observables through it after basis
+ * rotations (simulation.py:806-809 is the dense fallback it replaces). */
+#define JQ_OBS_SV_PARITY 3
 
 typedef struct jq_plan_desc {
   int32_t n_wires;        /* qubits of the simulated vector (2n for a density matrix) */

@@ -27,7 +27,7 @@ JQ_ERR_NONFINITE = 5
 JQ_M_ROT, JQ_M_PHASE, JQ_M_MATRIX = 0, 1, 2
 JQ_MF_CONJ, JQ_MF_SUPEROP, JQ_MF_GRAD = 1, 2, 4
 JQ_S_DIAGONAL, JQ_S_PERMUTATION, JQ_S_NONUNITARY = 1, 2, 4
-JQ_OBS_NONE, JQ_OBS_SV_DIAG, JQ_OBS_DM_DIAG = 0, 1, 2
+JQ_OBS_NONE, JQ_OBS_SV_DIAG, JQ_OBS_DM_DIAG, JQ_OBS_SV_PARITY = 0, 1, 2, 3
 MAX_WIRES = 5
 
 

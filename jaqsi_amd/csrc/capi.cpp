@@ -135,7 +135,7 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   const int* W = p->hp.words.data();
   for (int q = 0; q < p->hp.n_passes; ++q) {
     const int* pass = W + W[jq::HDR_WORDS + q];
-    if (2 * lds_state + 64 * 4 + 128 * 8 + (64 + (size_t)pass[2]) * 8 > 160 * 1024) {
+    if (2 * lds_state + sizeof(jq::ObsTable) + (64 + (size_t)pass[2]) * 8 > 160 * 1024) {
       delete p;
       return fail(JQ_ERR_UNSUPPORTED, "too many gradient slots in one pass for LDS");
     }
@@ -238,15 +238,25 @@ int fill_obs(jq_plan* p, jq::SvArgs& a, int n_wires, int obs_kind, int n_obs, co
   a.n_obs = 0;
   a.obs = nullptr;
   if (obs_kind == JQ_OBS_NONE) return JQ_OK;
-  if (obs_kind != JQ_OBS_SV_DIAG && obs_kind != JQ_OBS_DM_DIAG) return fail(JQ_ERR_ARG, "unknown observable kind");
+  if (obs_kind != JQ_OBS_SV_DIAG && obs_kind != JQ_OBS_DM_DIAG && obs_kind != JQ_OBS_SV_PARITY)
+    return fail(JQ_ERR_ARG, "unknown observable kind");
   if (n_obs < 0 || n_obs > 64) return fail(JQ_ERR_ARG, "between 0 and 64 observables are supported");
   if (n_obs > 0 && (!obs_wires || !obs_d)) return fail(JQ_ERR_ARG, "null observable arrays");
   const int span = obs_kind == JQ_OBS_DM_DIAG ? n_wires / 2 : n_wires;
   jq::ObsTable t;
   std::memset(&t, 0, sizeof(t));
   for (int o = 0; o < n_obs; ++o) {
-    if (obs_wires[o] < 0 || obs_wires[o] >= span) return fail(JQ_ERR_ARG, "observable wire out of range");
-    t.gbit[o] = n_wires - 1 - obs_wires[o];
+    if (obs_kind == JQ_OBS_SV_PARITY) {
+      const uint64_t wm = (uint64_t)(uint32_t)obs_wires[2 * o] | ((uint64_t)(uint32_t)obs_wires[2 * o + 1] << 32);
+      if (span < 64 && (wm >> span) != 0) return fail(JQ_ERR_ARG, "observable wire out of range");
+      uint64_t gm = 0;
+      for (int w = 0; w < span; ++w)
+        if ((wm >> w) & 1ull) gm |= 1ull << (n_wires - 1 - w);
+      t.mask[o] = gm;
+    } else {
+      if (obs_wires[o] < 0 || obs_wires[o] >= span) return fail(JQ_ERR_ARG, "observable wire out of range");
+      t.gbit[o] = n_wires - 1 - obs_wires[o];
+    }
     t.d[2 * o] = obs_d[2 * o];
     t.d[2 * o + 1] = obs_d[2 * o + 1];
   }
@@ -295,7 +305,7 @@ jq::SvArgs base_args(jq_plan* p) {
 
 size_t smem_for(jq_plan* p, int extra_reals, int tiles = 1) {
   // state tile(s) | observable table (64 ints + 128 doubles) | 64 expval + slot reals
-  return (size_t)tiles * ((size_t)elem_bytes(p->dtype) << p->hp.T) + 64 * 4 + 128 * 8 + (size_t)(64 + extra_reals) * 8;
+  return (size_t)tiles * ((size_t)elem_bytes(p->dtype) << p->hp.T) + sizeof(jq::ObsTable) + (size_t)(64 + extra_reals) * 8;
 }
 
 int max_slots(jq_plan* p) {

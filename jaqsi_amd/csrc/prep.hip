@@ -425,6 +425,7 @@ __global__ void obs_upload_kernel(ObsTable t, ObsTable* dst) {
   const int i = threadIdx.x;
   if (i < 64) dst->gbit[i] = t.gbit[i];
   if (i < 128) dst->d[i] = t.d[i];
+  if (i < 64) dst->mask[i] = t.mask[i];
 }
 
 hipError_t launch_obs_upload(const ObsTable& t, void* dst, hipStream_t st) {

@@ -1116,6 +1116,7 @@ __device__ __forceinline__ void load_obs(const SvArgs& a, ObsLds* t) {
   if (src != nullptr) {
     for (int o = threadIdx.x; o < 64; o += blockDim.x) t->gbit[o] = src->gbit[o];
     for (int o = threadIdx.x; o < 128; o += blockDim.x) t->d[o] = src->d[o];
+    for (int o = threadIdx.x; o < 64; o += blockDim.x) t->mask[o] = src->mask[o];
   }
   __syncthreads();
 }
@@ -1128,6 +1129,16 @@ __device__ __forceinline__ double obs_weight(uint64_t g, int n_obs, const ObsLds
     w += ct[o] * t->d[2 * o + bit];
   }
   return w;
+}
+
+// Register-index bits whose global bit lies in mask m (wave-uniform).
+template <int RB>
+__device__ __forceinline__ uint32_t parity_reg_mask(const Layout& L, uint64_t m) {
+  uint32_t r = 0;
+#pragma unroll
+  for (int b = 0; b < RB; ++b)
+    if ((m >> L.tbits[L.rpos[b]]) & 1ull) r |= 1u << b;
+  return r;
 }
 
 // Register bit holding global bit gb, or -1 (wave-uniform).
@@ -1174,6 +1185,28 @@ __device__ __forceinline__ void expval_partials(const R* re, const R* im, const 
       } else {
         v = (((tg >> gb) & 1ull) ? d1 : d0) * ptot;
       }
+      lds_add(&ev_lds[o], (double)wave_sum<R>(v));
+    }
+    return;
+  }
+  if (a.obs_kind == JQ_OBS_SV_PARITY) {
+    R p[NR], ptot = 0;
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      p[i] = fma(re[i], re[i], im[i] * im[i]);
+      ptot += p[i];
+    }
+    const uint64_t tg = base | L.gthr;  // register bit positions are zero here
+    for (int o = 0; o < a.n_obs; ++o) {
+      const uint64_t m = t->mask[o];
+      const uint32_t mreg = parity_reg_mask<RB>(L, m);
+      const int pt = __popcll(tg & m) & 1;
+      R se = 0;
+#pragma unroll
+      for (int i = 0; i < NR; ++i) se += (__popc((uint32_t)i & mreg) & 1) ? R(0) : p[i];
+      const R so = ptot - se;
+      const R d0 = (R)t->d[2 * o], d1 = (R)t->d[2 * o + 1];
+      const R v = pt ? fma(d1, se, d0 * so) : fma(d0, se, d1 * so);
       lds_add(&ev_lds[o], (double)wave_sum<R>(v));
     }
     return;
@@ -1358,6 +1391,27 @@ __device__ __forceinline__ void seed_lambda(const R* pr, const R* pi, R* lr, R* 
       for (int b = 0; b < RB; ++b)
         if ((i >> b) & 1) w += A[b];
       const R w2 = (R)(2.0 * w);
+      lr[i] = w2 * pr[i];
+      li[i] = -w2 * pi[i];
+    }
+    return;
+  }
+  if (a.obs_kind == JQ_OBS_SV_PARITY) {
+    const uint64_t tg = base | L.gthr;
+    double w[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) w[i] = 0;
+    for (int o = 0; o < a.n_obs; ++o) {
+      const uint64_t m = t->mask[o];
+      const uint32_t mreg = parity_reg_mask<RB>(L, m);
+      const int pt = __popcll(tg & m) & 1;
+      const double c0 = ct[o] * t->d[2 * o], c1 = ct[o] * t->d[2 * o + 1];
+#pragma unroll
+      for (int i = 0; i < NR; ++i) w[i] += ((pt ^ (__popc((uint32_t)i & mreg) & 1)) ? c1 : c0);
+    }
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const R w2 = (R)(2.0 * w[i]);
       lr[i] = w2 * pr[i];
       li[i] = -w2 * pi[i];
     }

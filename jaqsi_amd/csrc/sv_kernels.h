@@ -63,8 +63,9 @@ struct SvArgs {
 
 // Observable table: global bit and diagonal (d0, d1) of each observable wire.
 struct ObsTable {
-  int gbit[64];
-  double d[128];
+  int gbit[64];         // single-wire kinds: global bit of the observable wire
+  double d[128];        // (d0, d1) per observable
+  uint64_t mask[64];    // JQ_OBS_SV_PARITY: global-bit mask of the Pauli-Z word
 };
 
 // Copy a host-built table to device memory on the stream (a tiny kernel, so

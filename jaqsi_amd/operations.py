@@ -238,6 +238,7 @@ class Operation:
         names = "*".join(o.name for o in all_ops)
         op = Operation(wires=all_wires, matrix=mat, name=f"Prod({names})", record=False)
         op.is_unitary = all(o.is_unitary for o in all_ops)
+        op._factors = all_ops  # kept for float64 re-evaluation of observables
         return op
 
     def __matmul__(self, other):

@@ -95,10 +95,183 @@ def diag_observables(obs: Sequence[Operation]) -> Optional[List[Tuple[int, float
     return out
 
 
+# ---------------------------------------------------------------------------
+# general observables: Pauli decomposition + basis-rotated parity groups
+# ---------------------------------------------------------------------------
+_PAULI_1Q = {
+    "I": np.eye(2, dtype=complex),
+    "X": np.array([[0, 1], [1, 0]], dtype=complex),
+    "Y": np.array([[0, -1j], [1j, 0]], dtype=complex),
+    "Z": np.array([[1, 0], [0, -1]], dtype=complex),
+}
+
+
+def pauli_terms(m: np.ndarray, wires: Sequence[int], tol: float = 1e-12) -> List[Tuple[dict, float]]:
+    """Decompose a Hermitian matrix on `wires` (first wire = most significant
+    index bit) into real-weighted Pauli words: [({wire: letter}, coeff)]."""
+    import itertools
+
+    k = len(wires)
+    m = np.asarray(m, dtype=complex)
+    if m.shape != (2**k, 2**k):
+        raise ValueError("observable matrix does not match its wires")
+    if not np.allclose(m, m.conj().T, atol=1e-10):
+        raise NotImplementedError("gradients of non-Hermitian observables are not supported")
+    out = []
+    for letters in itertools.product("IXYZ", repeat=k):
+        P = np.array([[1.0 + 0j]])
+        for c in letters:
+            P = np.kron(P, _PAULI_1Q[c])
+        c = np.real(np.trace(P @ m)) / 2**k
+        if abs(c) > tol:
+            out.append(({w: l for w, l in zip(wires, letters) if l != "I"}, float(c)))
+    return out
+
+
+def pauli_groups(obs: Sequence[Operation]):
+    """Group the Pauli terms of all observables qubit-wise by measurement
+    basis.  Returns (groups, n_obs) with groups = [(basis {wire: X|Y|Z},
+    [(obs index, coeff, wire mask)])]; a term with no wires (identity) gets
+    mask 0 and contributes coeff * 1."""
+    groups: List[Tuple[dict, list]] = []
+    for oi, ob in enumerate(obs):
+        if len(ob.wires) > 6:
+            raise NotImplementedError("observables on more than 6 wires are not supported by the gradient path")
+        for word, c in pauli_terms(_obs_matrix_f64(ob), ob.wires):
+            mask = 0
+            for w in word:
+                mask |= 1 << w
+            for basis, terms in groups:
+                if all(basis.get(w, l) == l for w, l in word.items()):
+                    basis.update(word)
+                    terms.append((oi, c, mask))
+                    break
+            else:
+                groups.append((dict(word), [(oi, c, mask)]))
+    return groups
+
+
+def _basis_rotations(basis: dict) -> List[Operation]:
+    """Gates mapping each wire's measurement basis onto Z: X -> H, Y -> H S^dagger."""
+    from jaqsi_amd import gateset as G
+
+    ops: List[Operation] = []
+    for w, l in sorted(basis.items()):
+        if l == "X":
+            ops.append(G.H(wires=w, record=False))
+        elif l == "Y":
+            ops.append(G.RZ(-np.pi / 2, wires=w, record=False))  # S^dagger up to a global phase
+            ops.append(G.H(wires=w, record=False))
+    return ops
+
+
+class _ParityRunner:
+    """One adjoint runner per basis group; observables combine the groups'
+    Pauli-word expectation values with their coefficients."""
+
+    def __init__(self, tape, n_qubits, groups, n_obs, batch, psi0_batched, dtype):
+        self.groups = []
+        for basis, terms in groups:
+            low = lowering.lower_pure(list(tape) + _basis_rotations(basis), n_qubits)
+            plan = get_plan(low, dtype)
+            mats = lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, batch)
+            words = np.zeros((len(terms), 2), dtype=np.int32)
+            for t, (_, _, mask) in enumerate(terms):
+                words[t, 0] = np.int32(np.uint32(mask & 0xFFFFFFFF).astype(np.int32))
+                words[t, 1] = np.int32(np.uint32(mask >> 32).astype(np.int32))
+            d = np.tile(np.array([1.0, -1.0]), len(terms))
+            W = np.zeros((n_obs, len(terms)))  # expval_o = sum_t W[o, t] <P_t>
+            for t, (oi, c, _) in enumerate(terms):
+                W[oi, t] = c
+            self.groups.append((plan, mats, words, d, torch.tensor(W, dtype=torch.float64), len(terms)))
+        self.batch = batch
+        self.psi0_batched = psi0_batched
+        self.dtype = dtype
+        self.n_qubits = n_qubits
+        self.n_obs = n_obs
+
+    def _mats(self, mats, dev):
+        if mats is None:
+            return None
+        return mats.reshape(self.batch, -1).to(dev, torch.complex128).contiguous()
+
+    def forward(self, angles, psi0):
+        dev = angles.device
+        out = torch.zeros((self.batch, self.n_obs), dtype=torch.float64, device=dev)
+        for plan, mats, words, d, W, nt in self.groups:
+            state = torch.empty((self.batch, 1 << self.n_qubits), dtype=_cdtype_torch(self.dtype), device=dev)
+            ev = torch.empty((self.batch, nt), dtype=torch.float64, device=dev)
+            nat.check(nat.lib().jq_sv_forward(
+                plan.handle, self.batch, nat.ptr(angles), nat.ptr(self._mats(mats, dev)), nat.ptr(psi0),
+                int(self.psi0_batched), nat.ptr(state), nat.JQ_OBS_SV_PARITY, nt, nat.ptr(words), nat.ptr(d),
+                nat.ptr(ev), _stream()))
+            out += ev @ W.to(dev).T
+        return out
+
+    def adjoint(self, angles, psi0, ct):
+        dev = angles.device
+        grad = torch.zeros_like(angles)
+        for plan, mats, words, d, W, nt in self.groups:
+            ctp = (ct @ W.to(dev)).contiguous()  # cotangent of each Pauli word
+            state = torch.empty((self.batch, 1 << self.n_qubits), dtype=_cdtype_torch(self.dtype), device=dev)
+            ga = torch.empty_like(angles)
+            nat.check(nat.lib().jq_sv_adjoint(
+                plan.handle, self.batch, nat.ptr(angles), nat.ptr(self._mats(mats, dev)), nat.ptr(psi0),
+                int(self.psi0_batched), nat.ptr(state), nat.JQ_OBS_SV_PARITY, nt, nat.ptr(words), nat.ptr(d),
+                nat.ptr(ctp), None, nat.ptr(ga), None, None, _stream()))
+            grad += ga
+        return grad
+
+
+class _ParityExpval(torch.autograd.Function):
+    """Expectation values of general observables, adjoint backward per basis group."""
+
+    @staticmethod
+    def forward(ctx, angles, psi0, runner):
+        ctx.runner = runner
+        ctx.save_for_backward(angles, psi0 if psi0 is not None else torch.empty(0))
+        ctx.has_psi0 = psi0 is not None
+        return runner.forward(angles, psi0)
+
+    @staticmethod
+    def backward(ctx, g):
+        angles, psi0 = ctx.saved_tensors
+        psi0 = psi0 if ctx.has_psi0 else None
+        ga = ctx.runner.adjoint(angles, psi0, g.to(torch.float64).contiguous())
+        return ga, None, None
+
+
 def _obs_arrays(diags):
     wires = np.array([w for w, _, _ in diags], dtype=np.int32)
     d = np.array([[a, b] for _, a, b in diags], dtype=np.float64).reshape(-1)
     return wires, d
+
+
+def _obs_matrix_f64(ob: Operation) -> np.ndarray:
+    """The observable's matrix in complex128, rebuilt from float64 sources
+    (class matrices, Hermitian's stored array, product factors) so that its
+    Pauli coefficients do not inherit complex64 rounding."""
+    factors = getattr(ob, "_factors", None)
+    if factors:
+        wires = list(ob.wires)
+        m = np.eye(2 ** len(wires), dtype=complex)
+        for f in factors:
+            fw = list(f.wires)
+            rest = [w for w in wires if w not in fw]
+            k = len(wires)
+            big = np.kron(_obs_matrix_f64(f), np.eye(2 ** len(rest), dtype=complex))  # wire order fw + rest
+            order = fw + rest
+            perm = [order.index(w) for w in wires]
+            t = big.reshape((2,) * (2 * k)).transpose(perm + [k + q for q in perm])
+            m = m @ t.reshape(2**k, 2**k)
+        return m
+    npm = getattr(ob, "_np_matrix", None)
+    if npm is not None:
+        return np.asarray(npm, dtype=complex)
+    cm = getattr(ob.__class__, "_matrix", None)
+    if cm is not None and not getattr(ob, "_explicit", False) and not getattr(ob, "parameters", None):
+        return np.asarray(cm, dtype=complex)
+    return _obs_matrix(ob)
 
 
 def _obs_matrix(ob: Operation) -> np.ndarray:
@@ -256,11 +429,20 @@ def simulate_and_measure(
             runner = _Runner(plan, B, use_density, diags, n_qubits, psi0_batched, dtype)
             ev = _AdjointExpval.apply(angles, mats, psi0, runner)
             return unbatch(ev.to(rdt))
-        # general observables: forward simulation + matrix expectation values
-        if any(t is not None and t.requires_grad for t in (angles, mats, psi0)):
-            raise NotImplementedError(
-                "gradients through non-diagonal observables are not supported by the GPU path yet"
-            )
+        # general observables: gradients through Pauli-word parity groups
+        if any(t is not None and t.requires_grad for t in (angles, mats, psi0)) and torch.is_grad_enabled():
+            if use_density:
+                raise NotImplementedError(
+                    "gradients of non-diagonal observables of density matrices are not supported yet"
+                )
+            if (mats is not None and mats.requires_grad) or (psi0 is not None and psi0.requires_grad):
+                raise NotImplementedError(
+                    "gradients of non-diagonal observables w.r.t. explicit matrices or initial states are not supported yet"
+                )
+            groups = pauli_groups(obs)
+            runner = _ParityRunner(tape, n_qubits, groups, len(obs), B, psi0_batched, dtype)
+            ev = _ParityExpval.apply(angles, psi0, runner)
+            return unbatch(ev.to(rdt))
         state = _forward_state(plan, B, angles, mats, psi0, psi0_batched, dtype)
         out = torch.zeros((B, len(obs)), dtype=torch.float64, device=angles.device)
         for i, ob in enumerate(obs):

@@ -23,3 +23,12 @@ def test_version_and_error_text():
 
     assert b"gfx950" in nat.lib().jq_version()
     assert isinstance(nat.lib().jq_last_error(), bytes)
+
+
+def test_header_constants_match_bindings():
+    """Every #define JQ_* integer constant of the header is mirrored in _native."""
+    from jaqsi_amd import _native as nat
+
+    header = open(os.path.join(ROOT, "include", "jaqsi_amd.h")).read()
+    for name, val in re.findall(r"^#define (JQ_(?:OBS|ENV)_\w+)\s+(\d+)", header, re.M):
+        assert getattr(nat, name) == int(val), name

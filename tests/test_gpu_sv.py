@@ -140,3 +140,36 @@ def test_lowered_entry_points_for_a_custom_vjp_bridge():
     vals, grads, _, _ = sv.adjoint(ops, n, sv.z_obs(n), ct=np.arange(1, n + 1, dtype=float))
     np.testing.assert_allclose(ev, vals, atol=1e-11)
     np.testing.assert_allclose(g, sv.flat_grads(ops, grads), atol=1e-10)
+
+
+@pytest.mark.parametrize("x64", [False, True])
+def test_general_observables_expval_and_gradient(x64):
+    """Pauli words, products across wires and Hermitian matrices: measured
+    through basis-rotated parity groups, differentiated by the adjoint."""
+    import jaqsi_amd as J
+
+    print(f"general observables x64={x64}", flush=True)
+    J.enable_x64(x64)
+    n = 4
+    theta = hea_theta(n, 2, 4321)
+    rng = np.random.default_rng(5)
+    h2 = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+    h2 = h2 + h2.conj().T
+    h4 = rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4))
+    h4 = h4 + h4.conj().T
+    obs = [
+        J.PauliX(0, record=False),
+        J.PauliZ(0, record=False) @ J.PauliY(2, record=False),
+        J.Hermitian(h2, wires=1, record=False),
+        J.PauliY(1, record=False) @ J.PauliX(3, record=False),
+        J.Hermitian(h4, wires=[2, 3], record=False),
+    ]
+    ref_obs = [(sv.PX, (0,)), (np.kron(sv.PZ, sv.PY), (0, 2)), (h2, (1,)), (np.kron(sv.PY, sv.PX), (1, 3)), (h4, (2, 3))]
+    ct = np.array([1.0, -0.5, 0.7, 1.3, 0.25])
+    th = torch.tensor(theta, device="cuda", requires_grad=True)
+    ev = J.Script(lambda t: record_hea(t, n), n_qubits=n).execute(type="expval", obs=obs, args=(th,))
+    (ev * torch.tensor(ct, device="cuda", dtype=ev.dtype)).sum().backward()
+    ops = sv.hea_ops(theta, n)
+    vals, grads, _, _ = sv.adjoint(ops, n, ref_obs, ct=ct)
+    np.testing.assert_allclose(ev.detach().cpu().numpy(), vals, atol=TOL[x64] * 10)
+    np.testing.assert_allclose(th.grad.cpu().numpy().reshape(-1), sv.flat_grads(ops, grads), atol=TOL[x64] * 40)
