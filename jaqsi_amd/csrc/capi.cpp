@@ -28,8 +28,8 @@ KTimer g_timer;
 // which: 0 forward pass, 1 backward pass; `full` selects the kernel variant
 // that also carries 4x4 register blocks and the dense 3-wire path.
 hipError_t timed_launch(int which, bool full, int dtype, int T, const jq::SvArgs& a, int ntiles, int batch,
-                        size_t smem, hipStream_t st) {
-  const int variant = which + (full ? 2 : 0);
+                        size_t smem, hipStream_t st, bool wide = false) {
+  const int variant = which + (full ? 2 : 0) + (which == 1 && wide ? 4 : 0);
   if (!g_timer.on) return jq::launch_sv(variant, dtype, T, a, ntiles, batch, smem, st);
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
@@ -91,6 +91,7 @@ struct jq_plan {
   int n_slots = 0;
   std::vector<int32_t> slot_words;
   std::vector<char> pass_full;  // pass holds a 4x4 register block or a dense 3-wire block
+  bool wide_bwd = false;        // backward program built for 16 amplitudes x 512 threads (complex64)
   bool uploaded = false;
   // workspaces
   DevBuf pool, partials, bars, ev_part, lam, ckpt, status, obs, fstate, groups;
@@ -111,6 +112,8 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   int T = desc->tile_bits;
   // amplitudes per thread: forward 16 (2^4), backward 8 (2^3, psi and lambda)
   // for both dtypes; the workgroup covers one 64 KiB tile (sv_f32 / sv_f64).
+  // complex64 plans with 4x4 / dense 3-wire blocks are rebuilt below with a
+  // 16-amplitude backward.
   const int RB = 4;
   const int RBb = 3;
   if (T == 0) T = desc->dtype == 0 ? 13 : 12;
@@ -141,6 +144,7 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
     }
   }
   p->pass_full.assign(p->hp.n_passes, 0);
+  bool any_full = false;
   for (int q = 0; q < p->hp.n_passes; ++q) {
     const int* pass = W + W[jq::HDR_WORDS + q];
     const int* soff = pass + 4 + T;
@@ -150,6 +154,23 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
       for (int k = 0; k < sub[0]; ++k, g += jq::GW)
         if (g[jq::G_KIND] == jq::AK_MC2 || g[jq::G_KIND] == jq::AK_DENSEK) p->pass_full[q] = 1;
     }
+    any_full = any_full || p->pass_full[q];
+  }
+  if (any_full && desc->dtype == 0) {
+    // same passes (tile choice does not depend on the register tile), a wider backward program
+    jq::HostProgram wide;
+    try {
+      jq::build_program(*desc, T, RB, 4, wide);
+    } catch (const std::exception& e) {
+      delete p;
+      return fail(JQ_ERR_ARG, e.what());
+    }
+    if (wide.n_passes != p->hp.n_passes) {
+      delete p;
+      return fail(JQ_ERR_ARG, "internal: wide backward program changed the pass structure");
+    }
+    p->hp.words_b = wide.words_b;
+    p->wide_bwd = true;
   }
   std::vector<int32_t> sm;
   std::vector<jq::SlotMap> smv = p->hp.slot_map;
@@ -448,7 +469,7 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
       a.src_stride = (int64_t)a.vec_len;
       a.dst = ck ? nullptr : state;  // without checkpoints psi is un-applied in place
       a.dst_stride = (int64_t)a.vec_len;
-      HIP_TRY(timed_launch(1, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, sm, st));
+      HIP_TRY(timed_launch(1, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, sm, st, p->wide_bwd));
     }
   }
   return finish_grads(p, batch, angles, matrices, a.grad_part, a.part_stride, grad_angles, grad_matrices, st);
@@ -466,8 +487,9 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
   for (int g = 0; g < n_groups; ++g) {
     const jq_group& G = groups[g];
     if (!G.plan || G.batch < 1 || !G.ct || !G.state) return fail(JQ_ERR_ARG, "bad group");
-    if (G.plan->hp.n_wires != p0->hp.n_wires || G.plan->dtype != p0->dtype || G.plan->hp.T != p0->hp.T)
-      return fail(JQ_ERR_ARG, "groups must share width, dtype and tile size");
+    if (G.plan->hp.n_wires != p0->hp.n_wires || G.plan->dtype != p0->dtype || G.plan->hp.T != p0->hp.T ||
+        G.plan->wide_bwd != p0->wide_bwd)
+      return fail(JQ_ERR_ARG, "groups must share width, dtype, tile size and backward register tile");
     for (int h = 0; h < g; ++h)
       if (groups[h].plan == G.plan)
         return fail(JQ_ERR_ARG, "two groups share a plan: merge circuits of one structure into one group");
@@ -557,7 +579,7 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
     const int rows = active_rows(s, full, 1);
     a.step = s;
     a.dir = 1;
-    HIP_TRY(timed_launch(1, full, p0->dtype, p0->hp.T, a, ntiles, rows, smem_b, st));
+    HIP_TRY(timed_launch(1, full, p0->dtype, p0->hp.T, a, ntiles, rows, smem_b, st, p0->wide_bwd));
   }
   for (int k = 0; k < n_groups; ++k) {
     const jq_group& G = groups[order[k]];

@@ -139,20 +139,17 @@ struct Layout {
 // instead of being kept live, which keeps the layouts small in SGPRs).
 __device__ __forceinline__ uint64_t gpos(const Layout& L, int b) { return 1ull << L.tbits[L.rpos[b]]; }
 
+// Substep layouts only need the tile-local thread offset: the global offset
+// gthr is read exclusively under the canonical layout (loads, stores,
+// expectation values, lambda seed), which canonical_layout sets up.
 template <int RB, int LNT>
 __device__ __forceinline__ void decode_layout(cint* sub, cint* tbits, Layout& L) {
   uint32_t toff = 0;
-  uint64_t gthr = 0;
   const uint32_t tid = threadIdx.x;
 #pragma unroll
-  for (int j = 0; j < LNT; ++j) {
-    const int tb = sub[1 + RB + j];
-    const uint32_t bit = (tid >> j) & 1u;
-    toff |= bit << tb;
-    gthr |= (uint64_t)bit << tbits[tb];
-  }
+  for (int j = 0; j < LNT; ++j) toff |= ((tid >> j) & 1u) << sub[1 + RB + j];
   L.toff = toff;
-  L.gthr = gthr;
+  L.gthr = 0;
   L.tbits = tbits;
 #pragma unroll
   for (int b = 0; b < RB; ++b) L.rpos[b] = sub[1 + b];
@@ -366,6 +363,65 @@ __device__ __forceinline__ void sweep_apply(R* re, R* im, const Mat<R, D>& m, ui
     for (int q = 0; q < D; ++q) {
       re[G::at(k, q)] = yr[q];
       im[G::at(k, q)] = yi[q];
+    }
+  }
+}
+
+// 4x4 block applied in two row-halves, so only 8 matrix entries are live
+// at a time (the 1024-thread backward has 128 VGPRs).  The effective matrix
+// is E = M (mode 0), M^dagger (1) or M^T (2) of the pool matrix p, with the
+// two target bits exchanged when `swap` (kernels use register bits A > B).
+template <typename R, int NR, typename G>
+__device__ __forceinline__ void sweep_apply4(R* re, R* im, cmat<R>* p, bool swap, int mode, uint32_t creg) {
+  constexpr int NG = G::n;
+  R tr[NG][2], ti[NG][2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    R er[2][4], ei[2][4];
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int r = 2 * h + rr;
+        const int sr = swap ? (((r & 1) << 1) | (r >> 1)) : r;
+        const int sc = swap ? (((c & 1) << 1) | (c >> 1)) : c;
+        const int src = mode == 0 ? sr * 4 + sc : sc * 4 + sr;
+        const C2<R> v = p[src];
+        er[rr][c] = v.x;
+        ei[rr][c] = mode == 1 ? -v.y : v.y;
+      }
+#pragma unroll
+    for (int k = 0; k < NG; ++k) {
+      if (((uint32_t)G::base(k) & creg) != creg) continue;
+      R xr[4], xi[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        xr[q] = re[G::at(k, q)];
+        xi[q] = im[G::at(k, q)];
+      }
+#pragma unroll
+      for (int rr = 0; rr < 2; ++rr) {
+        R ar = xr[0] * er[rr][0];
+        R ai = xi[0] * er[rr][0];
+        ar = fma(-ei[rr][0], xi[0], ar);
+        ai = fma(ei[rr][0], xr[0], ai);
+#pragma unroll
+        for (int c = 1; c < 4; ++c) cmul_acc(ar, ai, er[rr][c], ei[rr][c], xr[c], xi[c]);
+        if (h == 0) {
+          tr[k][rr] = ar;
+          ti[k][rr] = ai;
+        } else {
+          re[G::at(k, 2 + rr)] = ar;
+          im[G::at(k, 2 + rr)] = ai;
+        }
+      }
+      if (h == 1) {
+#pragma unroll
+        for (int rr = 0; rr < 2; ++rr) {
+          re[G::at(k, rr)] = tr[k][rr];
+          im[G::at(k, rr)] = ti[k][rr];
+        }
+      }
     }
   }
 }
@@ -766,6 +822,37 @@ __device__ __forceinline__ void fwd_gate(R* re, R* im, cint* g, cmat<R>* pool, c
   }
 }
 
+// Backward step of a 4x4 block in one dispatch: psi <- U^{-1} psi, the
+// cotangent in two row halves (flushed per half), lambda <- U^T lambda.
+// `ok` is false for threads whose thread / outside controls are not met:
+// they skip the arithmetic but take part in the wave-wide flushes.
+template <typename R, int NR, typename G>
+__device__ __forceinline__ void bw_mc2(R* pr, R* pi, R* lr, R* li, cmat<R>* mp, cmat<R>* up, bool inv, bool sw,
+                                       bool ok, uint32_t creg, double* slots, int slot) {
+  if (ok) sweep_apply4<R, NR, G>(pr, pi, up, sw, inv ? 0 : 1, creg);
+  if (slot >= 0) {
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      R acc[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[e] = 0;
+      if (ok) sweep_grad<R, NR, 4, G>(pr, pi, lr, li, creg, 2 * half, acc);
+#pragma unroll
+      for (int oo = 0; oo < 2; ++oo) {
+        const int ko = 2 * half + oo;
+        const int bo = sw ? (((ko & 1) << 1) | (ko >> 1)) : ko;
+        // acc[2 (oo 4 + c) + reim] -> slot + 2 (bo 4 + s(c)) + reim (kernel row / column order)
+        flush8<R>(acc + 8 * oo, 8, slots, [=](int k) {
+          const int c = k >> 1;
+          const int bc = sw ? (((c & 1) << 1) | (c >> 1)) : c;
+          return slot + 2 * (bo * 4 + bc) + (k & 1);
+        });
+      }
+    }
+  }
+  if (ok) sweep_apply4<R, NR, G>(lr, li, mp, sw, 2, creg);
+}
+
 // Backward: psi <- U^{-1} psi, bar += lambda (x) psi, lambda <- U^T lambda.
 template <typename R, int RB, bool FULL>
 __device__ __forceinline__ void bwd_gate(R* pr, R* pi, R* lr, R* li, cint* g, cmat<R>* pool,
@@ -810,50 +897,10 @@ __device__ __forceinline__ void bwd_gate(R* pr, R* pi, R* lr, R* li, cint* g, cm
       break;
     }
     case AK_MC2: {
-      if constexpr (!FULL) break;
-      const bool sw = ra < rb;
-      {
-        Mat<R, 4> ui;
-        if (inv) {
-          load4<R>(ip, ui, sw);
-        } else {
-          Mat<R, 4> u;
-          load4<R>(mp, u, sw);
-          adjoint_of<R, 4>(u, ui);
-        }
-        if (cx.ok) {
-          JQ_CASE2(sweep_apply<R, NR, 4, G2<NR, A, B>>(pr, pi, ui, cx.creg, false))
-        }
-      }
-      if (slot >= 0) {
-        // cotangent rows in the block's own target order: kernel row s(o)
-#pragma unroll
-        for (int half = 0; half < 2; ++half) {
-          R acc[16];
-#pragma unroll
-          for (int e = 0; e < 16; ++e) acc[e] = 0;
-          if (cx.ok) {
-            JQ_CASE2(sweep_grad<R, NR, 4, G2<NR, A, B>>(pr, pi, lr, li, cx.creg, 2 * half, acc))
-          }
-          // kernel rows 2h, 2h+1; kernel column c -> block (s(row), s(c)) when swapped
-#pragma unroll
-          for (int oo = 0; oo < 2; ++oo) {
-            const int ko = 2 * half + oo;
-            const int bo = sw ? (((ko & 1) << 1) | (ko >> 1)) : ko;
-            // acc[2 (oo 4 + c) + reim] -> slot + 2 (bo 4 + s(c)) + reim
-            flush8<R>(acc + 8 * oo, 8, slots, [=](int k) {
-              const int c = k >> 1;
-              const int bc = sw ? (((c & 1) << 1) | (c >> 1)) : c;
-              return slot + 2 * (bo * 4 + bc) + (k & 1);
-            });
-          }
-        }
-      }
-      {
-        Mat<R, 4> u;
-        load4<R>(mp, u, sw);
-        if (cx.ok) {
-          JQ_CASE2(sweep_apply<R, NR, 4, G2<NR, A, B>>(lr, li, u, cx.creg, true))
+      if constexpr (FULL) {
+        if (cx.ok || slot >= 0) {
+          JQ_CASE2(bw_mc2<R, NR, G2<NR, A, B>>(pr, pi, lr, li, mp, inv ? ip : mp, inv, ra < rb, cx.ok, cx.creg,
+                                                slots, slot))
         }
       }
       break;
@@ -912,6 +959,8 @@ __device__ __forceinline__ void canonical_layout(cint* tbits, Layout& L) {
   const uint32_t tid = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < LNT; ++j) gthr |= (uint64_t)((tid >> j) & 1u) << tbits[1 + j];
+  // keep the finished offset (2 VGPRs) rather than its per-bit terms
+  asm volatile("" : "+v"(gthr));
   L.toff = tid << 1;
   L.gthr = gthr;
   L.tbits = tbits;
@@ -1317,10 +1366,8 @@ __device__ __forceinline__ void to_canonical(R* re, R* im, cint* tbits, Layout& 
   bool same = true;
 #pragma unroll
   for (int b = 0; b < RB; ++b) same = same && (cl.rpos[b] == L.rpos[b]);
-  if (!same) {
-    relayout<R, RB>(re, im, lds, L, cl);
-    L = cl;
-  }
+  if (!same) relayout<R, RB>(re, im, lds, L, cl);
+  L = cl;  // also when no exchange was needed: substep layouts carry no gthr
 }
 
 template <typename R, int RB, int NT, bool FULL>
@@ -1439,8 +1486,8 @@ __device__ __forceinline__ void to_canonical2(R* pr, R* pi, R* lr, R* li, cint* 
   if (!same) {
     relayout<R, RB>(pr, pi, lds, L, cl, 0, false);
     relayout<R, RB>(lr, li, lds, L, cl, 1, true);
-    L = cl;
   }
+  L = cl;  // also when no exchange was needed: substep layouts carry no gthr
 }
 
 // Backward sweep over one pass (reverse substeps, reverse gates).  When

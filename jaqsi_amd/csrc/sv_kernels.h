@@ -76,7 +76,8 @@ hipError_t launch_obs_upload(const ObsTable& t, void* dst, hipStream_t st);
 // (asynchronous and capturable; for small tables).
 hipError_t launch_bytes_upload(const void* src, size_t n, void* dst, hipStream_t st);
 
-// which: 0 forward pass, 1 backward pass, 2 fused single-tile forward+backward
+// which: 0 forward, 1 backward, +2 full variant (4x4 / dense 3-wire blocks),
+// +4 backward with the wide register tile (complex64 only)
 hipError_t launch_sv_f32(int which, int T, const SvArgs& a, int ntiles, int batch, size_t smem, hipStream_t st);
 hipError_t launch_sv_f64(int which, int T, const SvArgs& a, int ntiles, int batch, size_t smem, hipStream_t st);
 

@@ -169,12 +169,24 @@ class _ParityRunner:
     """One adjoint runner per basis group; observables combine the groups'
     Pauli-word expectation values with their coefficients."""
 
-    def __init__(self, tape, n_qubits, groups, n_obs, batch, psi0_batched, dtype):
+    def __init__(self, tape, n_qubits, groups, n_obs, batch, psi0_batched, dtype, base_low, mats):
         self.groups = []
+        P0, E0 = len(base_low.angle_values), base_low.n_matrix_elems
         for basis, terms in groups:
             low = lowering.lower_pure(list(tape) + _basis_rotations(basis), n_qubits)
             plan = get_plan(low, dtype)
-            mats = lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, batch)
+            # the rotations only append: angles [0, P0) and matrix elements
+            # [0, E0) are the circuit's own, the rest are the rotation constants
+            if len(low.angle_values) < P0 or low.n_matrix_elems < E0:
+                raise RuntimeError("basis rotations changed the circuit's own parameter layout")
+            extra_angles = torch.tensor([float(a) for a in low.angle_values[P0:]], dtype=torch.float64)
+            extra_m = lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, None)
+            extra_m = extra_m.reshape(-1)[E0:] if extra_m is not None else None
+            if mats is not None:
+                gm = torch.cat([mats.reshape(batch, -1).detach().to(torch.complex128).cpu(),
+                                extra_m.reshape(1, -1).expand(batch, -1)], dim=1) if extra_m is not None and extra_m.numel() else mats.reshape(batch, -1)
+            else:
+                gm = extra_m.reshape(1, -1).expand(batch, -1) if extra_m is not None and extra_m.numel() else None
             words = np.zeros((len(terms), 2), dtype=np.int32)
             for t, (_, _, mask) in enumerate(terms):
                 words[t, 0] = np.int32(np.uint32(mask & 0xFFFFFFFF).astype(np.int32))
@@ -183,7 +195,8 @@ class _ParityRunner:
             W = np.zeros((n_obs, len(terms)))  # expval_o = sum_t W[o, t] <P_t>
             for t, (oi, c, _) in enumerate(terms):
                 W[oi, t] = c
-            self.groups.append((plan, mats, words, d, torch.tensor(W, dtype=torch.float64), len(terms)))
+            self.groups.append((plan, gm, words, d, torch.tensor(W, dtype=torch.float64), len(terms), extra_angles))
+        self.P0 = P0
         self.batch = batch
         self.psi0_batched = psi0_batched
         self.dtype = dtype
@@ -195,14 +208,23 @@ class _ParityRunner:
             return None
         return mats.reshape(self.batch, -1).to(dev, torch.complex128).contiguous()
 
+    def _angles(self, angles, extra):
+        if angles.shape[1] != self.P0:
+            raise ValueError("angle vector does not match the circuit")
+        if extra.numel() == 0:
+            return angles
+        return torch.cat([angles, extra.to(angles.device).reshape(1, -1).expand(self.batch, -1)], dim=1).contiguous()
+
     def forward(self, angles, psi0):
         dev = angles.device
         out = torch.zeros((self.batch, self.n_obs), dtype=torch.float64, device=dev)
-        for plan, mats, words, d, W, nt in self.groups:
+        for plan, mats, words, d, W, nt, extra in self.groups:
+            angles_g = self._angles(angles, extra)
             state = torch.empty((self.batch, 1 << self.n_qubits), dtype=_cdtype_torch(self.dtype), device=dev)
             ev = torch.empty((self.batch, nt), dtype=torch.float64, device=dev)
+            m_g = self._mats(mats, dev)
             nat.check(nat.lib().jq_sv_forward(
-                plan.handle, self.batch, nat.ptr(angles), nat.ptr(self._mats(mats, dev)), nat.ptr(psi0),
+                plan.handle, self.batch, nat.ptr(angles_g), nat.ptr(m_g), nat.ptr(psi0),
                 int(self.psi0_batched), nat.ptr(state), nat.JQ_OBS_SV_PARITY, nt, nat.ptr(words), nat.ptr(d),
                 nat.ptr(ev), _stream()))
             out += ev @ W.to(dev).T
@@ -211,15 +233,17 @@ class _ParityRunner:
     def adjoint(self, angles, psi0, ct):
         dev = angles.device
         grad = torch.zeros_like(angles)
-        for plan, mats, words, d, W, nt in self.groups:
+        for plan, mats, words, d, W, nt, extra in self.groups:
+            angles_g = self._angles(angles, extra)
+            m_g = self._mats(mats, dev)
             ctp = (ct @ W.to(dev)).contiguous()  # cotangent of each Pauli word
             state = torch.empty((self.batch, 1 << self.n_qubits), dtype=_cdtype_torch(self.dtype), device=dev)
-            ga = torch.empty_like(angles)
+            ga = torch.empty_like(angles_g)
             nat.check(nat.lib().jq_sv_adjoint(
-                plan.handle, self.batch, nat.ptr(angles), nat.ptr(self._mats(mats, dev)), nat.ptr(psi0),
+                plan.handle, self.batch, nat.ptr(angles_g), nat.ptr(m_g), nat.ptr(psi0),
                 int(self.psi0_batched), nat.ptr(state), nat.JQ_OBS_SV_PARITY, nt, nat.ptr(words), nat.ptr(d),
                 nat.ptr(ctp), None, nat.ptr(ga), None, None, _stream()))
-            grad += ga
+            grad += ga[:, : self.P0]
         return grad
 
 
@@ -440,7 +464,7 @@ def simulate_and_measure(
                     "gradients of non-diagonal observables w.r.t. explicit matrices or initial states are not supported yet"
                 )
             groups = pauli_groups(obs)
-            runner = _ParityRunner(tape, n_qubits, groups, len(obs), B, psi0_batched, dtype)
+            runner = _ParityRunner(tape, n_qubits, groups, len(obs), B, psi0_batched, dtype, low, mats)
             ev = _ParityExpval.apply(angles, psi0, runner)
             return unbatch(ev.to(rdt))
         state = _forward_state(plan, B, angles, mats, psi0, psi0_batched, dtype)
