@@ -25,6 +25,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "adjoint-grad circuits/s at 24q & batch 4096×14q; HBM GB/s vs peak"
+
+# Pass kernels: plan-specialised (compiled once per plan before the warm-up,
+# outside the timed region) unless --interpreter selects the program
+# interpreter kernels.
+USE_JIT = True
+
+
+def specialise(plan):
+    """Compile the plan's specialised pass kernels (when enabled); seconds."""
+    from jaqsi_amd import _native as nat
+
+    if not USE_JIT:
+        plan.jit(nat.JQ_JIT_OFF)
+        return 0.0
+    t0 = time.perf_counter()
+    plan.jit(nat.JQ_JIT_SYNC)
+    return round(time.perf_counter() - t0, 2)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -67,6 +84,7 @@ def bench_c2(steps, warmup, n=24, layers=8):
     rng = np.random.default_rng(1002)
     theta = rng.uniform(0, 2 * np.pi, (layers, n, 3))
     plan, mats = build_plan(n, layers)
+    compile_s = specialise(plan)
     mats = mats.reshape(1, -1).to(dev) if mats is not None else None
     info = plan.info()
     angles = torch.tensor(theta.reshape(1, -1), dtype=torch.float64, device=dev)
@@ -111,6 +129,7 @@ def bench_c2(steps, warmup, n=24, layers=8):
         "expvals": ev.cpu().numpy()[0],
         "grads": ga.cpu().numpy()[0],
         "n": n,
+        "compile_s": compile_s,
     }
 
 
@@ -290,6 +309,7 @@ def bench_c4(steps, warmup, n=12, layers=6, seed=1004):
     with recording() as tape:
         record_noisy_hea(theta, n, 0.01, 0.005)
     low, plan, angles, mats, psi0, psi0_b = S._prepare(tape, n, True, None, None, 0)
+    compile_s = specialise(plan)
     runner = S._Runner(plan, 1, True, [(0, 1.0, -1.0)], n, False, 0)
     ct = torch.ones((1, 1), dtype=torch.float64, device=angles.device)
     for _ in range(warmup):
@@ -329,6 +349,8 @@ def bench_c4(steps, warmup, n=12, layers=6, seed=1004):
             "Tr(Z0 rho) and its gradient by the checkpointed adjoint",
             "passes": info["n_passes"],
             "blocks": info["n_blocks"],
+            "pass_kernels": "plan-specialised" if USE_JIT else "interpreter",
+            "specialise_compile_s": compile_s,
         },
         "roofline": {
             "bound": "hbm",
@@ -477,6 +499,8 @@ def main():
         default="c2,c3,c4,c5",
         help="comma-separated workloads to run (profiling runs select one); the headline line needs c2",
     )
+    ap.add_argument("--interpreter", action="store_true",
+                    help="run the program-interpreter pass kernels instead of the plan-specialised ones")
     ap.add_argument(
         "--copy-probe",
         type=int,
@@ -485,6 +509,8 @@ def main():
         help="only run a plain device copy of MIB MiB three times (calibrates the HBM byte counters)",
     )
     args = ap.parse_args()
+    global USE_JIT
+    USE_JIT = not args.interpreter
     if args.gpus != 1:
         print(
             "bench.py measures one GPU: every run in this build uses a single MI355X; "
@@ -543,6 +569,8 @@ def main():
             "params": 576,
             "passes": res["info"]["n_passes"],
             "tile_bits": res["info"]["tile_bits"],
+            "pass_kernels": "plan-specialised" if USE_JIT else "interpreter",
+            "specialise_compile_s": res["compile_s"],
         },
         "roofline": roofline_c2(res),
     }

@@ -35,7 +35,9 @@
 #ifndef JAQSI_AMD_H
 #define JAQSI_AMD_H
 
+#ifndef __HIPCC_RTC__
 #include <stdint.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -128,6 +130,34 @@ int jq_plan_program(const jq_plan* plan, int32_t* out, int64_t capacity, int64_t
  * 2 member table, 3 cotangent slot map.  No reference counterpart: these
  * let tests execute the scheduled program on the CPU. */
 int jq_plan_tables(const jq_plan* plan, int32_t which, int32_t* out, int64_t capacity, int64_t* size);
+
+/* Plan-specialised pass kernels.  The prebuilt kernels interpret the
+ * program; for one plan the same gate sequence can also be compiled at run
+ * time (hipRTC) into straight-line kernels, one per pass and direction,
+ * which give the same results with less instruction overhead per gate.
+ * No reference counterpart: the reference's jax.jit tracing per structure
+ * (script.py:389-471) plays this role there.
+ *   mode JQ_JIT_OFF       run the prebuilt kernels;
+ *        JQ_JIT_ASYNC     compile on a worker thread, switch once compiled;
+ *        JQ_JIT_SYNC      compile now (or wait for the worker) and load the
+ *                         module; fails with the compiler log on error;
+ *        JQ_JIT_QUERY     change nothing.
+ *   *phase (optional) receives JQ_JIT_NONE / _COMPILING / _COMPILED /
+ *   _LOADED / _FAILED.
+ * Without a call, plans on JQ_JIT_MIN_WIRES or more wires start in
+ * JQ_JIT_ASYNC and smaller ones in JQ_JIT_OFF; the environment variable
+ * JQ_JIT=0 / 1 overrides that default. */
+#define JQ_JIT_OFF 0
+#define JQ_JIT_ASYNC 1
+#define JQ_JIT_SYNC 2
+#define JQ_JIT_QUERY (-1)
+#define JQ_JIT_NONE 0
+#define JQ_JIT_COMPILING 1
+#define JQ_JIT_COMPILED 2
+#define JQ_JIT_LOADED 3
+#define JQ_JIT_FAILED (-1)
+#define JQ_JIT_MIN_WIRES 20
+int jq_plan_jit(jq_plan* plan, int32_t mode, int32_t* phase);
 
 /* Forward simulation of `batch` circuits sharing the plan's structure.
  *   angles   [batch][n_params] float64 (device)

@@ -100,6 +100,11 @@ class PulseDesc(C.Structure):
 
 JQ_ENV_DRAG_GAUSSIAN = 1
 
+# plan-specialised kernels (jq_plan_jit)
+JQ_JIT_OFF, JQ_JIT_ASYNC, JQ_JIT_SYNC, JQ_JIT_QUERY = 0, 1, 2, -1
+JQ_JIT_NONE, JQ_JIT_COMPILING, JQ_JIT_COMPILED, JQ_JIT_LOADED, JQ_JIT_FAILED = 0, 1, 2, 3, -1
+JQ_JIT_MIN_WIRES = 20
+
 
 _lib = None
 _lock = threading.Lock()
@@ -110,6 +115,7 @@ EXPORTS = [
     "jq_plan_info",
     "jq_plan_program",
     "jq_plan_tables",
+    "jq_plan_jit",
     "jq_sv_forward",
     "jq_sv_adjoint",
     "jq_sv_adjoint_groups",
@@ -151,6 +157,7 @@ def lib():
         L.jq_plan_info.argtypes = [vp] + [C.POINTER(C.c_int32)] * 5
         L.jq_plan_program.argtypes = [vp, C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int64)]
         L.jq_plan_tables.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int64)]
+        L.jq_plan_jit.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
         L.jq_sv_forward.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
         L.jq_sv_adjoint.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp]
         L.jq_sv_adjoint_groups.argtypes = [C.c_int32, C.POINTER(Group), C.c_int32, C.c_int32, vp, vp, vp]
@@ -217,6 +224,13 @@ class Plan:
         check(lib().jq_plan_info(self.handle, *[C.byref(v) for v in vals]))
         keys = ["n_passes", "n_substeps", "n_blocks", "n_grad_reals", "tile_bits"]
         return {k: v.value for k, v in zip(keys, vals)}
+
+    def jit(self, mode: int = JQ_JIT_QUERY) -> int:
+        """Select the plan-specialised kernels (JQ_JIT_OFF / _ASYNC / _SYNC);
+        returns the compile phase (JQ_JIT_NONE ... JQ_JIT_LOADED, _FAILED)."""
+        ph = C.c_int32()
+        check(lib().jq_plan_jit(self.handle, mode, C.byref(ph)))
+        return ph.value
 
     def program(self) -> np.ndarray:
         n = C.c_int64()

@@ -3,12 +3,16 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "jaqsi_amd.h"
+#include "jit.h"
 #include "plan_internal.h"
 #include "prep.h"
 #include "pulse.h"
@@ -25,17 +29,34 @@ struct KTimer {
 };
 KTimer g_timer;
 
+// A plan-specialised pass kernel (jit.cpp), or none.
+struct JitFn {
+  hipFunction_t f = nullptr;
+  int nt = 0;            // threads per workgroup
+  int64_t pool_off = 0;  // SvArgs::pool_off of the pass
+};
+
+hipError_t launch_pass(const JitFn& jf, int variant, int dtype, int T, const jq::SvArgs& a, int ntiles, int batch,
+                       size_t smem, hipStream_t st) {
+  if (jf.f == nullptr) return jq::launch_sv(variant, dtype, T, a, ntiles, batch, smem, st);
+  jq::SvArgs arg = a;
+  arg.pool_off = jf.pool_off;
+  void* params[] = {(void*)&arg};
+  return hipModuleLaunchKernel(jf.f, ntiles, batch, 1, jf.nt, 1, 1, (unsigned)smem, st, params, nullptr);
+}
+
 // which: 0 forward pass, 1 backward pass; `full` selects the kernel variant
-// that also carries 4x4 register blocks and the dense 3-wire path.
+// that also carries 4x4 register blocks and the dense 3-wire path.  With a
+// specialised kernel `jf` the pass runs that kernel instead.
 hipError_t timed_launch(int which, bool full, int dtype, int T, const jq::SvArgs& a, int ntiles, int batch,
-                        size_t smem, hipStream_t st, bool wide = false) {
+                        size_t smem, hipStream_t st, bool wide = false, JitFn jf = JitFn()) {
   const int variant = which + (full ? 2 : 0) + (which == 1 && wide ? 4 : 0);
-  if (!g_timer.on) return jq::launch_sv(variant, dtype, T, a, ntiles, batch, smem, st);
+  if (!g_timer.on) return launch_pass(jf, variant, dtype, T, a, ntiles, batch, smem, st);
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   hipEventRecord(e0, st);
-  hipError_t e = jq::launch_sv(variant, dtype, T, a, ntiles, batch, smem, st);
+  hipError_t e = launch_pass(jf, variant, dtype, T, a, ntiles, batch, smem, st);
   hipEventRecord(e1, st);
   g_timer.ev[which].push_back({e0, e1});
   return e;
@@ -81,6 +102,22 @@ hipError_t upload(DevBuf& b, const std::vector<T>& v) {
 }
 }  // namespace
 
+// Plan-specialised kernels: compiled on the host (possibly on a worker
+// thread: hipRTC only, no HIP runtime call), loaded as a module on first use
+// by the launching thread.
+struct JitState {
+  std::thread worker;
+  std::atomic<int> phase{JQ_JIT_NONE};
+  int mode = 0;  // 0 interpreter, 1 specialised kernels once compiled
+  jq::JitPlan plan;                       // distinct pass bodies and the pass -> kernel map
+  std::vector<std::vector<int>> chunks;   // kernel ids of each module
+  std::vector<std::vector<char>> code;    // one code object per chunk
+  std::string log;
+  std::vector<hipModule_t> mods;
+  std::vector<hipFunction_t> fn;          // per kernel id
+  int nt[2] = {0, 0};
+};
+
 struct jq_plan {
   jq::HostProgram hp;
   int dtype = 0;
@@ -93,9 +130,111 @@ struct jq_plan {
   std::vector<char> pass_full;  // pass holds a 4x4 register block or a dense 3-wire block
   bool wide_bwd = false;        // backward program built for 16 amplitudes x 512 threads (complex64)
   bool uploaded = false;
+  JitState jit;
   // workspaces
   DevBuf pool, partials, bars, ev_part, lam, ckpt, status, obs, fstate, groups;
 };
+
+namespace {
+
+// Compile the plan's specialised kernels (host only; any thread).  The
+// kernels are split into chunks compiled concurrently, one module each.
+void jit_build(jq_plan* p) {
+  jq::JitSpec spec;
+  spec.words_f = &p->hp.words;
+  spec.words_b = &p->hp.words_b;
+  spec.dtype = p->dtype;
+  jq::jit_plan(spec, p->jit.plan);
+  const int nk = (int)p->jit.plan.kernels.size();
+  const int nchunk = 1;  // the run-time compiler serialises concurrent compiles
+  std::vector<std::vector<int>> chunks(nchunk);
+  for (int i = 0; i < nk; ++i) chunks[i % nchunk].push_back(i);
+  std::vector<std::vector<char>> code(nchunk);
+  std::vector<std::string> logs(nchunk);
+  std::vector<char> ok(nchunk, 0);
+  const char* dump = std::getenv("JQ_JIT_DUMP");  // debugging aid: keep the generated source
+  auto work = [&](int c) {
+    const std::string src = jq::jit_source(spec, p->jit.plan, chunks[c]);
+    if (dump) {
+      if (FILE* fh = std::fopen((std::string(dump) + "." + std::to_string(c) + ".hip").c_str(), "w")) {
+        std::fwrite(src.data(), 1, src.size(), fh);
+        std::fclose(fh);
+      }
+    }
+    ok[c] = jq::jit_compile(src, code[c], logs[c]) ? 1 : 0;
+  };
+  std::vector<std::thread> th;
+  for (int c = 1; c < nchunk; ++c) th.emplace_back(work, c);
+  work(0);
+  for (auto& t : th) t.join();
+  bool all_ok = true;
+  std::string log;
+  for (int c = 0; c < nchunk; ++c) {
+    all_ok = all_ok && ok[c];
+    if (!ok[c]) log += logs[c];
+  }
+  p->jit.chunks.swap(chunks);
+  p->jit.code.swap(code);
+  p->jit.log = log;
+  p->jit.phase.store(all_ok ? JQ_JIT_COMPILED : JQ_JIT_FAILED, std::memory_order_release);
+}
+
+void jit_start(jq_plan* p) {
+  int expect = JQ_JIT_NONE;
+  if (!p->jit.phase.compare_exchange_strong(expect, JQ_JIT_COMPILING)) return;
+  p->jit.worker = std::thread(jit_build, p);
+}
+
+// Load the compiled module (launching thread).  False on failure.
+bool jit_load(jq_plan* p) {
+  if (p->jit.phase.load(std::memory_order_acquire) != JQ_JIT_COMPILED) return false;
+  if (p->jit.worker.joinable()) p->jit.worker.join();
+  p->jit.fn.assign(p->jit.plan.kernels.size(), nullptr);
+  hipError_t e = hipSuccess;
+  for (size_t c = 0; c < p->jit.code.size() && e == hipSuccess; ++c) {
+    hipModule_t m = nullptr;
+    e = hipModuleLoadData(&m, p->jit.code[c].data());
+    if (e != hipSuccess) break;
+    p->jit.mods.push_back(m);
+    for (int id : p->jit.chunks[c]) {
+      e = hipModuleGetFunction(&p->jit.fn[id], m, jq::jit_kernel_name(p->jit.plan.kernels[id].dir, id).c_str());
+      if (e != hipSuccess) break;
+    }
+  }
+  if (e != hipSuccess) {
+    for (hipModule_t m : p->jit.mods) hipModuleUnload(m);
+    p->jit.mods.clear();
+    p->jit.log = std::string("module load: ") + hipGetErrorString(e);
+    p->jit.phase.store(JQ_JIT_FAILED);
+    return false;
+  }
+  p->jit.nt[0] = 1 << (p->hp.words[2] - p->hp.words[3]);
+  p->jit.nt[1] = 1 << (p->hp.words_b[2] - p->hp.words_b[3]);
+  p->jit.code.clear();
+  p->jit.phase.store(JQ_JIT_LOADED);
+  return true;
+}
+
+// Whether this call's passes run the specialised kernels.
+bool jit_ready(jq_plan* p) {
+  if (p->jit.mode == JQ_JIT_OFF) return false;
+  const int ph = p->jit.phase.load(std::memory_order_acquire);
+  if (ph == JQ_JIT_LOADED) return true;
+  if (ph == JQ_JIT_COMPILED) return jit_load(p);
+  return false;
+}
+
+JitFn jit_fn(jq_plan* p, bool on, int dir, int q) {
+  JitFn f;
+  if (on) {
+    f.f = p->jit.fn[p->jit.plan.kernel_of[dir][q]];
+    f.nt = p->jit.nt[dir];
+    f.pool_off = p->jit.plan.pool_off[dir][q];
+  }
+  return f;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -181,12 +320,46 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   }
   p->n_slots = (int)smv.size();
   p->slot_words = sm;
+  {
+    const char* env = std::getenv("JQ_JIT");
+    bool on = p->hp.n_wires >= JQ_JIT_MIN_WIRES;
+    if (env && env[0] == '0') on = false;
+    if (env && env[0] == '1') on = true;
+    if (on) {
+      p->jit.mode = JQ_JIT_ASYNC;
+      jit_start(p);
+    }
+  }
   *out = p;
   return JQ_OK;
 }
 
+int jq_plan_jit(jq_plan* p, int32_t mode, int32_t* phase) {
+  if (!p) return fail(JQ_ERR_ARG, "null plan");
+  if (mode < JQ_JIT_QUERY || mode > JQ_JIT_SYNC) return fail(JQ_ERR_ARG, "unknown specialisation mode");
+  int rc = JQ_OK;
+  if (mode == JQ_JIT_OFF) {
+    p->jit.mode = JQ_JIT_OFF;
+  } else if (mode == JQ_JIT_ASYNC) {
+    p->jit.mode = JQ_JIT_ASYNC;
+    jit_start(p);
+  } else if (mode == JQ_JIT_SYNC) {
+    p->jit.mode = JQ_JIT_ASYNC;
+    int expect = JQ_JIT_NONE;
+    if (p->jit.phase.compare_exchange_strong(expect, JQ_JIT_COMPILING)) jit_build(p);
+    if (p->jit.worker.joinable()) p->jit.worker.join();
+    const int ph = p->jit.phase.load(std::memory_order_acquire);
+    if (ph == JQ_JIT_COMPILED) jit_load(p);
+    if (p->jit.phase.load() != JQ_JIT_LOADED) rc = fail(JQ_ERR_DEVICE, "plan-specialised kernels: " + p->jit.log);
+  }
+  if (phase) *phase = p->jit.phase.load(std::memory_order_acquire);
+  return rc;
+}
+
 int jq_plan_destroy(jq_plan* p) {
   if (!p) return JQ_OK;
+  if (p->jit.worker.joinable()) p->jit.worker.join();
+  for (hipModule_t m : p->jit.mods) hipModuleUnload(m);
   for (DevBuf* b : {&p->prog, &p->prog_b, &p->block_table, &p->member_list, &p->member_table, &p->member_values, &p->slot_map,
                     &p->pool, &p->partials, &p->bars, &p->ev_part, &p->lam, &p->ckpt, &p->status, &p->obs, &p->fstate, &p->groups})
     b->release();
@@ -399,13 +572,15 @@ int jq_sv_forward(jq_plan* p, int32_t batch, const double* angles, const void* m
     HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * a.n_obs) * 8));
     a.ev_part = (double*)p->ev_part.p;
   }
+  const bool jit = jit_ready(p);
   for (int q = 0; q < p->hp.n_passes; ++q) {
     a.pass = q;
     a.src = q == 0 ? psi0 : state;
     a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
     a.dst = state;
     a.dst_stride = (int64_t)a.vec_len;
-    HIP_TRY(timed_launch(0, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st));
+    HIP_TRY(timed_launch(0, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st, false,
+                         jit_fn(p, jit, 0, q)));
   }
   if (a.obs_kind != JQ_OBS_NONE) HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
   return JQ_OK;
@@ -451,13 +626,15 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
       if (q == P - 1 || !ck) return state;
       return (char*)p->ckpt.p + vbytes * (size_t)q;
     };
+    const bool jit = jit_ready(p);
     for (int q = 0; q < P; ++q) {
       a.pass = q;
       a.src = q == 0 ? psi0 : pass_out(q - 1);
       a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
       a.dst = pass_out(q);
       a.dst_stride = (int64_t)a.vec_len;
-      HIP_TRY(timed_launch(0, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st));
+      HIP_TRY(timed_launch(0, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st, false,
+                           jit_fn(p, jit, 0, q)));
     }
     if (expvals) HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
     // backward sweep
@@ -469,7 +646,8 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
       a.src_stride = (int64_t)a.vec_len;
       a.dst = ck ? nullptr : state;  // without checkpoints psi is un-applied in place
       a.dst_stride = (int64_t)a.vec_len;
-      HIP_TRY(timed_launch(1, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, sm, st, p->wide_bwd));
+      HIP_TRY(timed_launch(1, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, sm, st, p->wide_bwd,
+                           jit_fn(p, jit, 1, q)));
     }
   }
   return finish_grads(p, batch, angles, matrices, a.grad_part, a.part_stride, grad_angles, grad_matrices, st);

@@ -1,0 +1,327 @@
+// Plan-specialised tile-pass kernels: source generation and hipRTC compile.
+//
+// The generated body of a pass repeats, gate by gate, what the interpreter
+// (InterpBody in sv_impl.h) does with the same program words: the layout
+// check before each substep, the control tests, the gate arithmetic and the
+// gradient flushes, in the same order and through the same device
+// functions.  Only the record decoding and the dispatch disappear, so the
+// two paths produce the same amplitudes and gradients.
+#include "jit.h"
+
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <sstream>
+
+#include "program.h"
+
+namespace jq {
+
+namespace {
+
+#include "jit_headers.inc"  // kJitHeaderNames / kJitHeaderTexts (build.py embeds the kernel headers)
+
+// Types of <stdint.h> for the run-time compiler, which declares them in its
+// own namespace.
+const char* kPrelude =
+    "typedef __hip_internal::int8_t int8_t;\n"
+    "typedef __hip_internal::uint8_t uint8_t;\n"
+    "typedef __hip_internal::int16_t int16_t;\n"
+    "typedef __hip_internal::uint16_t uint16_t;\n"
+    "typedef __hip_internal::int32_t int32_t;\n"
+    "typedef __hip_internal::uint32_t uint32_t;\n"
+    "typedef __hip_internal::int64_t int64_t;\n"
+    "typedef __hip_internal::uint64_t uint64_t;\n"
+    "typedef unsigned long uintptr_t;\n"
+    "#include \"sv_impl.h\"\n"
+    "namespace jq {\n";
+
+struct ProgView {
+  const int32_t* W;
+  int T, RB, n_passes;
+  const int32_t* pass(int q) const { return W + W[HDR_WORDS + q]; }
+  const int32_t* sub(int q, int s) const { return W + pass(q)[4 + T + s]; }
+};
+
+std::string hex32(uint32_t v) {
+  char b[24];
+  std::snprintf(b, sizeof(b), "0x%xu", v);
+  return b;
+}
+std::string hex64(uint64_t v) {
+  char b[32];
+  std::snprintf(b, sizeof(b), "0x%llxull", (unsigned long long)v);
+  return b;
+}
+
+// layout words of a substep record: RB register sources, then the thread sources
+std::string layout_args(const ProgView& pv, const int32_t* sub) {
+  std::string o;
+  for (int j = 0; j < pv.T; ++j) o += ", " + std::to_string(sub[1 + j]);
+  return o;
+}
+
+bool same_rpos(const ProgView& pv, const int32_t* sub, const std::vector<int>& cur) {
+  for (int b = 0; b < pv.RB; ++b)
+    if (sub[1 + b] != cur[b]) return false;
+  return true;
+}
+
+std::vector<int> canonical_rpos(const ProgView& pv) {
+  std::vector<int> r(pv.RB);
+  r[0] = 0;
+  for (int b = 1; b < pv.RB; ++b) r[b] = pv.T - pv.RB + b;
+  return r;
+}
+
+struct GateLit {
+  int kind, ra, rb, hi, lo, k, mat, inv, slot, w0, w1;
+  uint32_t creg, cthr;
+  uint64_t cout;
+  GateLit(const int32_t* g, int rebase) {
+    kind = g[G_KIND];
+    ra = g[G_RA];
+    rb = g[G_RB];
+    hi = ra > rb ? ra : rb;
+    lo = ra > rb ? rb : ra;
+    k = g[G_K];
+    mat = g[G_MAT] - rebase;
+    inv = g[G_INV] >= 0 ? g[G_INV] - rebase : -1;
+    slot = g[G_SLOT];
+    w0 = g[G_DW0];
+    w1 = g[G_DW0 + 1];
+    creg = (uint32_t)g[G_CREG];
+    cthr = (uint32_t)g[G_CTHR];
+    cout = ((uint64_t)(uint32_t)g[G_COUT_HI] << 32) | (uint32_t)g[G_COUT_LO];
+  }
+  bool ctl() const { return cthr != 0 || cout != 0; }
+  std::string ok() const { return ctl() ? "ctl_ok(L, base, " + hex32(cthr) + ", " + hex64(cout) + ")" : "true"; }
+};
+
+void emit_fwd_gate(std::ostringstream& o, const GateLit& g) {
+  const std::string mp = "pool + " + std::to_string(g.mat);
+  const std::string nr = "(1 << RB)";
+  std::string call;
+  switch (g.kind) {
+    case AK_MC1:
+      call = "jf_mc1<R, RB, " + std::to_string(g.ra) + ", " + hex32(g.creg) + ">(re, im, " + mp + ");";
+      break;
+    case AK_MC2:
+      call = "jf_mc2<R, RB, " + std::to_string(g.hi) + ", " + std::to_string(g.lo) + ", " + hex32(g.creg) + ", " +
+             (g.ra < g.rb ? "true" : "false") + ">(re, im, " + mp + ");";
+      break;
+    case AK_MCX:
+      call = "ren_x<R, " + nr + ", " + std::to_string(g.ra) + ", " + hex32(g.creg) + ">(re, im);";
+      break;
+    case AK_MCSWAP:
+      call = "ren_swap<R, " + nr + ", " + std::to_string(g.hi) + ", " + std::to_string(g.lo) + ", " + hex32(g.creg) +
+             ">(re, im);";
+      break;
+    case AK_DIAG:  // controls are part of the diagonal
+      o << "    jf_diag<R, RB, " << g.k << ", " << g.w0 << ", " << g.w1 << ">(re, im, " << mp << ", L, base);\n";
+      return;
+    case AK_DENSEK:  // no controls; contains barriers, so every thread enters
+      o << "    if constexpr (RB >= 3) dense3<R, RB>(re, im, " << mp << ", 0, L, lds);\n";
+      return;
+    default:
+      return;
+  }
+  if (g.ctl())
+    o << "    if (" << g.ok() << ") " << call << "\n";
+  else
+    o << "    " << call << "\n";
+}
+
+void emit_bwd_gate(std::ostringstream& o, const GateLit& g) {
+  const std::string mp = "pool + " + std::to_string(g.mat);
+  const std::string ip = "pool + " + std::to_string(g.inv >= 0 ? g.inv : g.mat);
+  const std::string inv = g.inv >= 0 ? "true" : "false";
+  const std::string nr = "(1 << RB)";
+  const std::string st = std::to_string(g.slot);
+  switch (g.kind) {
+    case AK_MC1:
+      o << "    jb_mc1<R, RB, " << g.ra << ", " << hex32(g.creg) << ", " << inv << ">(pr, pi, lr, li, " << mp << ", "
+        << ip << ", " << g.ok() << ", slots, " << st << ");\n";
+      return;
+    case AK_MC2:
+      o << "    jb_mc2<R, RB, " << g.hi << ", " << g.lo << ", " << hex32(g.creg) << ", " << inv << ", "
+        << (g.ra < g.rb ? "true" : "false") << ">(pr, pi, lr, li, " << mp << ", " << ip << ", " << g.ok()
+        << ", slots, " << st << ");\n";
+      return;
+    case AK_MCX: {
+      const std::string t = "<R, " + nr + ", " + std::to_string(g.ra) + ", " + hex32(g.creg) + ">";
+      o << "    if (" << g.ok() << ") { ren_x" << t << "(pr, pi); ren_x" << t << "(lr, li); }\n";
+      return;
+    }
+    case AK_MCSWAP: {
+      const std::string t =
+          "<R, " + nr + ", " + std::to_string(g.hi) + ", " + std::to_string(g.lo) + ", " + hex32(g.creg) + ">";
+      o << "    if (" << g.ok() << ") { ren_swap" << t << "(pr, pi); ren_swap" << t << "(lr, li); }\n";
+      return;
+    }
+    case AK_DIAG:
+      o << "    jb_diag<R, RB, " << g.k << ", " << g.w0 << ", " << g.w1 << ", " << inv << ">(pr, pi, lr, li, " << mp
+        << ", " << ip << ", L, base, slots, " << st << ");\n";
+      return;
+    case AK_DENSEK:
+      o << "    if constexpr (RB >= 3) jb_dense3<R, RB, " << inv << ">(pr, pi, lr, li, " << mp << ", " << ip
+        << ", L, lds);\n";
+      return;
+    default:
+      return;
+  }
+}
+
+// Smallest matrix offset referenced by pass q (its gates' offsets are
+// written relative to it).
+int pass_rebase(const ProgView& pv, int q) {
+  int m = -1;
+  const int nsub = pv.pass(q)[0];
+  for (int s = 0; s < nsub; ++s) {
+    const int32_t* sub = pv.sub(q, s);
+    const int32_t* g = sub + 1 + pv.T;
+    for (int k = 0; k < sub[0]; ++k, g += GW) {
+      const int kind = g[G_KIND];
+      if (kind == AK_NOP || kind == AK_MCX || kind == AK_MCSWAP) continue;
+      for (int v : {g[G_MAT], g[G_INV]})
+        if (v >= 0 && (m < 0 || v < m)) m = v;
+    }
+  }
+  return m < 0 ? 0 : m;
+}
+
+void emit_forward(std::ostringstream& o, const ProgView& pv, int q, int rebase) {
+  std::vector<int> cur = canonical_rpos(pv);
+  const int nsub = pv.pass(q)[0];
+  for (int s = 0; s < nsub; ++s) {
+    const int32_t* sub = pv.sub(q, s);
+    if (!same_rpos(pv, sub, cur)) {
+      o << "    jit_relayout<R, RB, LNT" << layout_args(pv, sub) << ">(re, im, lds, L);\n";
+      for (int b = 0; b < pv.RB; ++b) cur[b] = sub[1 + b];
+    }
+    const int32_t* g = sub + 1 + pv.T;
+    for (int k = 0; k < sub[0]; ++k, g += GW) emit_fwd_gate(o, GateLit(g, rebase));
+  }
+}
+
+void emit_backward(std::ostringstream& o, const ProgView& pv, int q, int rebase) {
+  std::vector<int> cur = canonical_rpos(pv);
+  const int nsub = pv.pass(q)[0];
+  for (int s = nsub - 1; s >= 0; --s) {
+    const int32_t* sub = pv.sub(q, s);
+    if (!same_rpos(pv, sub, cur)) {
+      o << "    jit_relayout2<R, RB, LNT" << layout_args(pv, sub) << ">(pr, pi, lr, li, lds, L);\n";
+      for (int b = 0; b < pv.RB; ++b) cur[b] = sub[1 + b];
+    }
+    const int32_t* g0 = sub + 1 + pv.T;
+    for (int k = sub[0] - 1; k >= 0; --k) emit_bwd_gate(o, GateLit(g0 + k * GW, rebase));
+  }
+}
+
+ProgView view(const std::vector<int32_t>& w) {
+  ProgView pv;
+  pv.W = w.data();
+  pv.n_passes = w[1];
+  pv.T = w[2];
+  pv.RB = w[3];
+  return pv;
+}
+
+}  // namespace
+
+std::string jit_kernel_name(int dir, int id) {
+  return std::string(dir == 0 ? "jq_sv_forward_s" : "jq_sv_backward_s") + std::to_string(id);
+}
+
+void jit_plan(const JitSpec& spec, JitPlan& out) {
+  out = JitPlan();
+  for (int dir = 0; dir < 2; ++dir) {
+    const ProgView pv = view(dir == 0 ? *spec.words_f : *spec.words_b);
+    for (int q = 0; q < pv.n_passes; ++q) {
+      const int rebase = pass_rebase(pv, q);
+      std::ostringstream o;
+      if (dir == 0)
+        emit_forward(o, pv, q, rebase);
+      else
+        emit_backward(o, pv, q, rebase);
+      const std::string body = o.str();
+      int id = -1;
+      for (size_t k = 0; k < out.kernels.size(); ++k)
+        if (out.kernels[k].dir == dir && out.kernels[k].body == body) id = (int)k;
+      if (id < 0) {
+        id = (int)out.kernels.size();
+        out.kernels.push_back({dir, body});
+      }
+      out.kernel_of[dir].push_back(id);
+      out.pool_off[dir].push_back(rebase);
+    }
+  }
+}
+
+std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vector<int>& ids) {
+  std::ostringstream o;
+  o << kPrelude;
+  const char* rt = spec.dtype == 0 ? "float" : "double";
+  const ProgView f = view(*spec.words_f), b = view(*spec.words_b);
+  const int occ = spec.dtype == 0 ? 4 : 2;  // the interpreter's forward launch bounds
+  for (int id : ids) {
+    const JitPlan::Kernel& k = plan.kernels[id];
+    const ProgView& pv = k.dir == 0 ? f : b;
+    const int nt = 1 << (pv.T - pv.RB);
+    if (k.dir == 0) {
+      o << "struct JF" << id << " {\n"
+        << "  template <typename R, int RB, int NT, bool FULL>\n"
+        << "  static __device__ __forceinline__ void fwd(cint*, int, R* re, R* im, Layout& L, uint64_t base, "
+           "C2<R>* lds, cmat<R>* pool) {\n"
+        << "    constexpr int LNT = __builtin_ctz(NT);\n"
+        << k.body << "    (void)lds; (void)base; (void)pool;\n  }\n};\n"
+        << "extern \"C\" __global__ __launch_bounds__(" << nt << ", " << occ << ") void "
+        << jit_kernel_name(0, id) << "(SvArgs a) {\n  sv_forward_impl<" << rt << ", " << pv.RB << ", " << nt
+        << ", false, JF" << id << ">(a);\n}\n\n";
+    } else {
+      o << "struct JB" << id << " {\n"
+        << "  template <typename R, int RB, int NT, bool FULL>\n"
+        << "  static __device__ __forceinline__ void bwd(cint*, int, R* pr, R* pi, R* lr, R* li, Layout& L, "
+           "uint64_t base, C2<R>* lds, cmat<R>* pool, double* slots) {\n"
+        << "    constexpr int LNT = __builtin_ctz(NT);\n"
+        << k.body << "    (void)lds; (void)base; (void)pool; (void)slots;\n  }\n};\n"
+        << "extern \"C\" __global__ __launch_bounds__(" << nt << ") void " << jit_kernel_name(1, id)
+        << "(SvArgs a) {\n  sv_backward_impl<" << rt << ", " << pv.RB << ", " << nt << ", false, JB" << id
+        << ">(a);\n}\n\n";
+    }
+  }
+  o << "}  // namespace jq\n";
+  return o.str();
+}
+bool jit_compile(const std::string& src, std::vector<char>& code, std::string& log) {
+  hiprtcProgram prog;
+  const int nh = (int)(sizeof(kJitHeaderNames) / sizeof(kJitHeaderNames[0]));
+  hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), "jaqsi_amd_plan.hip", nh, kJitHeaderTexts, kJitHeaderNames);
+  if (r != HIPRTC_SUCCESS) {
+    log = std::string("hiprtcCreateProgram: ") + hiprtcGetErrorString(r);
+    return false;
+  }
+  // packed FP32 instructions stay off, as for the prebuilt kernels
+  const char* opts[] = {"--offload-arch=gfx950", "-O3",   "-std=c++17",       "-Xclang",
+                        "-target-feature",       "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics"};
+  r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  size_t ls = 0;
+  hiprtcGetProgramLogSize(prog, &ls);
+  log.assign(ls, '\0');
+  if (ls) hiprtcGetProgramLog(prog, &log[0]);
+  bool ok = r == HIPRTC_SUCCESS;
+  if (ok) {
+    size_t cs = 0;
+    ok = hiprtcGetCodeSize(prog, &cs) == HIPRTC_SUCCESS && cs > 0;
+    if (ok) {
+      code.resize(cs);
+      ok = hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS;
+    }
+  } else {
+    log = std::string("hiprtcCompileProgram: ") + hiprtcGetErrorString(r) + "\n" + log;
+  }
+  hiprtcDestroyProgram(&prog);
+  return ok;
+}
+
+}  // namespace jq

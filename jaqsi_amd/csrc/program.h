@@ -20,7 +20,9 @@
 //           [1+RB .. 1+T) thr_src (tile bit of thread bit j)
 //           then n_gates gate records of GW words
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <stdint.h>
+#endif
 
 namespace jq {
 

@@ -14,8 +14,10 @@
 // (simulation.py:616-652): psi <- U^dagger psi (or the inverse of a
 // non-unitary superoperator), bar[o,i] += lambda[o] psi[i], lambda <- U^T lambda.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "jaqsi_amd.h"
 #include "program.h"
@@ -142,8 +144,8 @@ __device__ __forceinline__ uint64_t gpos(const Layout& L, int b) { return 1ull <
 // Substep layouts only need the tile-local thread offset: the global offset
 // gthr is read exclusively under the canonical layout (loads, stores,
 // expectation values, lambda seed), which canonical_layout sets up.
-template <int RB, int LNT>
-__device__ __forceinline__ void decode_layout(cint* sub, cint* tbits, Layout& L) {
+template <int RB, int LNT, typename SubPtr>
+__device__ __forceinline__ void decode_layout(SubPtr sub, cint* tbits, Layout& L) {
   uint32_t toff = 0;
   const uint32_t tid = threadIdx.x;
 #pragma unroll
@@ -569,15 +571,14 @@ struct DiagLoc {
   int k;
 };
 
-__device__ __forceinline__ void decode_diag(cint* g, const Layout& L, uint64_t base, DiagLoc& dl) {
-  const int k = g[G_K];
+__device__ __forceinline__ void decode_diag_w(int k, int w0, int w1, const Layout& L, uint64_t base, DiagLoc& dl) {
   dl.k = k;
   dl.uni = 0;
 #pragma unroll
   for (int j = 0; j < 5; ++j) {
     dl.rbit[j] = -1;
     if (j >= k) continue;
-    const int loc = (g[G_DW0 + j / 4] >> (8 * (j % 4))) & 0xff;
+    const int loc = ((j < 4 ? w0 : w1) >> (8 * (j % 4))) & 0xff;
     const int type = loc >> 6, pos = loc & 63;
     uint32_t bit = 0;
     if (type == DL_REG)
@@ -588,6 +589,10 @@ __device__ __forceinline__ void decode_diag(cint* g, const Layout& L, uint64_t b
       bit = (uint32_t)((base >> pos) & 1ull);
     dl.uni |= bit << (k - 1 - j);
   }
+}
+
+__device__ __forceinline__ void decode_diag(cint* g, const Layout& L, uint64_t base, DiagLoc& dl) {
+  decode_diag_w(g[G_K], g[G_DW0], g[G_DW0 + 1], L, base, dl);
 }
 
 __device__ __forceinline__ uint32_t diag_index(const DiagLoc& dl, int i) {
@@ -1277,6 +1282,164 @@ __device__ __forceinline__ void expval_partials(const R* re, const R* im, const 
   }
 }
 
+// ---------------------------------------------------------------------------
+// building blocks of plan-specialised pass bodies
+// ---------------------------------------------------------------------------
+// A generated body (jit.cpp) calls these with its register bits, control
+// masks, pool offsets and layouts as literals.  Gates follow one another in
+// straight-line code, so a permutation of registers (CX, SWAP on register
+// bits) compiles to renaming only, and no dispatch merges register states.
+
+// Layout of a substep from literal words: RB register sources, then LNT
+// thread sources (tile bits), as in the substep record.
+template <int RB, int LNT, int... W>
+__device__ __forceinline__ void layout_lit(cint* tbits, Layout& L) {
+  constexpr int w[] = {W...};
+  static_assert(sizeof...(W) == RB + LNT, "layout words");
+  uint32_t toff = 0;
+  const uint32_t tid = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < LNT; ++j) toff |= ((tid >> j) & 1u) << w[RB + j];
+  L.toff = toff;
+  L.gthr = 0;
+  L.tbits = tbits;
+#pragma unroll
+  for (int b = 0; b < RB; ++b) L.rpos[b] = w[b];
+}
+
+template <typename R, int RB, int LNT, int... W>
+__device__ __forceinline__ void jit_relayout(R* re, R* im, C2<R>* lds, Layout& L) {
+  Layout nl;
+  layout_lit<RB, LNT, W...>(L.tbits, nl);
+  relayout<R, RB>(re, im, lds, L, nl);
+  L = nl;
+}
+
+template <typename R, int RB, int LNT, int... W>
+__device__ __forceinline__ void jit_relayout2(R* pr, R* pi, R* lr, R* li, C2<R>* lds, Layout& L) {
+  Layout nl;
+  layout_lit<RB, LNT, W...>(L.tbits, nl);
+  relayout<R, RB>(pr, pi, lds, L, nl, 0, false);
+  relayout<R, RB>(lr, li, lds, L, nl, 1, true);
+  L = nl;
+}
+
+// thread / outside-tile controls of a gate
+__device__ __forceinline__ bool ctl_ok(const Layout& L, uint64_t base, uint32_t cthr, uint64_t cout) {
+  return ((L.toff & cthr) == cthr) && ((base & cout) == cout);
+}
+
+// X on register bit A / exchange of register bits A, B, as plain moves
+template <typename R, int NR, int A, uint32_t CREG>
+__device__ __forceinline__ void ren_x(R* re, R* im) {
+#pragma unroll
+  for (int k = 0; k < NR / 2; ++k) {
+    const int i = G1<NR, A>::base(k), j = i | (1 << A);
+    if (((uint32_t)i & CREG) != CREG) continue;
+    const R tr = re[i], ti = im[i];
+    re[i] = re[j];
+    im[i] = im[j];
+    re[j] = tr;
+    im[j] = ti;
+  }
+}
+
+template <typename R, int NR, int A, int B, uint32_t CREG>
+__device__ __forceinline__ void ren_swap(R* re, R* im) {
+#pragma unroll
+  for (int k = 0; k < NR / 4; ++k) {
+    const int i = G2<NR, A, B>::base(k);
+    if (((uint32_t)i & CREG) != CREG) continue;
+    const int p = i | (1 << A), q = i | (1 << B);
+    const R tr = re[p], ti = im[p];
+    re[p] = re[q];
+    im[p] = im[q];
+    re[q] = tr;
+    im[q] = ti;
+  }
+}
+
+// forward gates
+template <typename R, int RB, int A, uint32_t CREG>
+__device__ __forceinline__ void jf_mc1(R* re, R* im, cmat<R>* mp) {
+  constexpr int NR = 1 << RB;
+  Mat<R, 2> m;
+  load2<R>(mp, m);
+  fw_dense<R, NR, 2, G1<NR, A>, (CREG != 0)>(re, im, m, CREG);
+}
+
+template <typename R, int RB, int A, int B, uint32_t CREG, bool SW>
+__device__ __forceinline__ void jf_mc2(R* re, R* im, cmat<R>* mp) {
+  constexpr int NR = 1 << RB;
+  Mat<R, 4> m;
+  load4<R>(mp, m, SW);
+  fw_dense<R, NR, 4, G2<NR, A, B>, (CREG != 0)>(re, im, m, CREG);
+}
+
+template <typename R, int RB, int K, int W0, int W1>
+__device__ __forceinline__ void jf_diag(R* re, R* im, cmat<R>* d, const Layout& L, uint64_t base) {
+  DiagLoc dl;
+  decode_diag_w(K, W0, W1, L, base, dl);
+  apply_diag<R, RB>(re, im, d, dl, false);
+}
+
+// backward gates: psi <- U^{-1} psi, bar += lambda (x) psi, lambda <- U^T lambda
+template <typename R, int RB, int A, uint32_t CREG, bool INV>
+__device__ __forceinline__ void jb_mc1(R* pr, R* pi, R* lr, R* li, cmat<R>* mp, cmat<R>* ip, bool ok, double* slots,
+                                       int slot) {
+  constexpr int NR = 1 << RB;
+  Mat<R, 2> u, ui;
+  load2<R>(mp, u);
+  if constexpr (INV)
+    load2<R>(ip, ui);
+  else
+    adjoint_of<R, 2>(u, ui);
+  R acc[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) acc[e] = 0;
+  if (ok) bw_dense<R, NR, 2, G1<NR, A>, (CREG != 0)>(pr, pi, lr, li, u, ui, CREG, acc);
+  if (slot >= 0) flush_vals<R>(acc, 8, slots, slot);
+}
+
+template <typename R, int RB, int A, int B, uint32_t CREG, bool INV, bool SW>
+__device__ __forceinline__ void jb_mc2(R* pr, R* pi, R* lr, R* li, cmat<R>* mp, cmat<R>* ip, bool ok, double* slots,
+                                       int slot) {
+  constexpr int NR = 1 << RB;
+  bw_mc2<R, NR, G2<NR, A, B>>(pr, pi, lr, li, mp, INV ? ip : mp, INV, SW, ok, CREG, slots, slot);
+}
+
+template <typename R, int RB, int K, int W0, int W1, bool INV>
+__device__ __forceinline__ void jb_diag(R* pr, R* pi, R* lr, R* li, cmat<R>* mp, cmat<R>* ip, const Layout& L,
+                                        uint64_t base, double* slots, int slot) {
+  constexpr int NR = 1 << RB;
+  DiagLoc dl;
+  decode_diag_w(K, W0, W1, L, base, dl);
+  if constexpr (INV)
+    apply_diag<R, RB>(pr, pi, ip, dl, false);
+  else
+    apply_diag<R, RB>(pr, pi, mp, dl, true);
+  if (slot >= 0) {
+    R acc[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) acc[e] = 0;
+    if (dl.k == 1 && dl.rbit[0] >= 0) {
+      constexpr int A = (K == 1 && ((W0 & 0xff) >> 6) == DL_REG) ? (W0 & 63) : 0;
+      diag1_grad<R, NR, A>(pr, pi, lr, li, acc);
+    } else {
+      diag_grad<R, NR>(pr, pi, lr, li, dl, acc);
+    }
+    flush_vals<R>(acc, 2 << K, slots, slot);
+  }
+  apply_diag<R, RB>(lr, li, mp, dl, false);
+}
+
+template <typename R, int RB, bool INV>
+__device__ __forceinline__ void jb_dense3(R* pr, R* pi, R* lr, R* li, cmat<R>* mp, cmat<R>* ip, const Layout& L,
+                                          C2<R>* lds) {
+  dense3<R, RB>(pr, pi, INV ? ip : mp, INV ? 0 : 1, L, lds);
+  dense3<R, RB>(lr, li, mp, 2, L, lds);
+}
+
 // Per-circuit arguments of grid row blockIdx.y: the launch arguments
 // themselves, or (multi-plan launch) the row's group record.
 template <typename R>
@@ -1323,18 +1486,72 @@ __device__ __forceinline__ SvArgs resolve_args(const SvArgs& a0, int& circ) {
   return a;
 }
 
-template <typename R, int RB, int NT, bool FULL>
+// Gate sequence of a pass.  The interpreter walks the substep and gate
+// records of the program; a body generated for one plan at run time
+// (jit.cpp) spells the same sequence out with literal register bits, masks
+// and pool offsets, so no record is decoded and no dispatch is taken.
+struct InterpBody {
+  template <typename R, int RB, int NT, bool FULL>
+  static __device__ __forceinline__ void fwd(cint* P, int pass_idx, R* re, R* im, Layout& L, uint64_t base,
+                                             C2<R>* lds, cmat<R>* pool) {
+    const int T = P[2];
+    cint* pass = P + P[HDR_WORDS + pass_idx];
+    const int nsub = pass[0];
+    cint* tbits = pass + 4;
+    cint* soff = pass + 4 + T;
+    constexpr int LNT = __builtin_ctz(NT);
+    for (int s = 0; s < nsub; ++s) {
+      cint* sub = P + soff[s];
+      const int ng = sub[0];
+      if (!same_layout<RB>(sub, L)) {
+        Layout nl;
+        decode_layout<RB, LNT>(sub, tbits, nl);
+        relayout<R, RB>(re, im, lds, L, nl);
+        L = nl;
+      }
+      cint* g = sub + 1 + T;
+      for (int q = 0; q < ng; ++q, g += GW) fwd_gate<R, RB, FULL>(re, im, g, pool, L, base, lds);
+    }
+  }
+
+  template <typename R, int RB, int NT, bool FULL>
+  static __device__ __forceinline__ void bwd(cint* P, int pass_idx, R* pr, R* pi, R* lr, R* li, Layout& L,
+                                             uint64_t base, C2<R>* lds, cmat<R>* pool, double* slots) {
+    const int T = P[2];
+    cint* pass = P + P[HDR_WORDS + pass_idx];
+    const int nsub = pass[0];
+    cint* tbits = pass + 4;
+    cint* soff = pass + 4 + T;
+    constexpr int LNT = __builtin_ctz(NT);
+    for (int s = nsub - 1; s >= 0; --s) {
+      cint* sub = P + soff[s];
+      if (!same_layout<RB>(sub, L)) {
+        Layout nl;
+        decode_layout<RB, LNT>(sub, tbits, nl);
+        relayout<R, RB>(pr, pi, lds, L, nl, 0, false);  // psi: region 0, lambda: region 1
+        relayout<R, RB>(lr, li, lds, L, nl, 1, true);
+        L = nl;
+      }
+      const int ng = sub[0];
+      cint* g0 = sub + 1 + T;
+      for (int q = ng - 1; q >= 0; --q) {
+        cint* g = g0 + q * GW;
+        bwd_gate<R, RB, FULL>(pr, pi, lr, li, g, pool, L, base, lds, slots);
+      }
+    }
+  }
+};
+
+template <typename R, int RB, int NT, bool FULL, typename Body>
 __device__ __forceinline__ void forward_tile(const SvArgs& a, int pass_idx, int circ, int tile, R* re, R* im,
                                              Layout& L, uint64_t& base, C2<R>* lds, bool do_load) {
   cint* P = (cint*)a.prog;
   const int T = P[2];
   cint* pass = P + P[HDR_WORDS + pass_idx];
-  const int nsub = pass[0];
   cint* tbits = pass + 4;
   constexpr int LNT = __builtin_ctz(NT);
   base = tile_base(tile, tbits, T, a.n_wires);
-  cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride;
-  cint* soff = pass + 4 + T;
+  cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride + a.pool_off;
   canonical_layout<RB, LNT>(tbits, L);
   if (do_load) {
     if (a.src == nullptr)
@@ -1343,18 +1560,7 @@ __device__ __forceinline__ void forward_tile(const SvArgs& a, int pass_idx, int 
       load_canon<R, RB>(re, im, reinterpret_cast<const C2<R>*>(a.src) + (int64_t)circ * a.src_stride, a.vec_len, L,
                         base, a.n_wires >= T);
   }
-  for (int s = 0; s < nsub; ++s) {
-    cint* sub = P + soff[s];
-    const int ng = sub[0];
-    if (!same_layout<RB>(sub, L)) {
-      Layout nl;
-      decode_layout<RB, LNT>(sub, tbits, nl);
-      relayout<R, RB>(re, im, lds, L, nl);
-      L = nl;
-    }
-    cint* g = sub + 1 + T;
-    for (int q = 0; q < ng; ++q, g += GW) fwd_gate<R, RB, FULL>(re, im, g, pool, L, base, lds);
-  }
+  Body::template fwd<R, RB, NT, FULL>(P, pass_idx, re, im, L, base, lds, pool);
 }
 
 // Bring the registers back to the canonical layout.
@@ -1370,8 +1576,8 @@ __device__ __forceinline__ void to_canonical(R* re, R* im, cint* tbits, Layout& 
   L = cl;  // also when no exchange was needed: substep layouts carry no gthr
 }
 
-template <typename R, int RB, int NT, bool FULL>
-__global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void sv_forward_kernel(SvArgs a0) {
+template <typename R, int RB, int NT, bool FULL, typename Body>
+__device__ __forceinline__ void sv_forward_impl(const SvArgs& a0) {
   extern __shared__ __align__(16) unsigned char smem[];
   int circ;
   const SvArgs a = resolve_args<R>(a0, circ);
@@ -1390,7 +1596,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void sv_forward_kernel(
   R re[1 << RB], im[1 << RB];
   Layout L;
   uint64_t base;
-  forward_tile<R, RB, NT, FULL>(a, a.pass, circ, tile, re, im, L, base, lds, true);
+  forward_tile<R, RB, NT, FULL, Body>(a, a.pass, circ, tile, re, im, L, base, lds, true);
   to_canonical<R, RB, NT>(re, im, pass + 4, L, lds);
   store_canon<R, RB>(re, im, reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride, a.vec_len, L, base,
                      a.n_wires >= T);
@@ -1401,6 +1607,11 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void sv_forward_kernel(
     for (int o = threadIdx.x; o < a.n_obs; o += NT)
       a.ev_part[((int64_t)circ * a.n_obs + o) * gridDim.x + tile] = ev[o];
   }
+}
+
+template <typename R, int RB, int NT, bool FULL>
+__global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void sv_forward_kernel(SvArgs a0) {
+  sv_forward_impl<R, RB, NT, FULL, InterpBody>(a0);
 }
 
 // Lambda seed: the JAX cotangent of the measurement (simulation.py:617),
@@ -1493,20 +1704,18 @@ __device__ __forceinline__ void to_canonical2(R* pr, R* pi, R* lr, R* li, cint* 
 // Backward sweep over one pass (reverse substeps, reverse gates).  When
 // `loaded` is false, psi (the pass output) and lambda are read under the
 // canonical layout; otherwise they arrive in registers under layout L.
-template <typename R, int RB, int NT, bool FULL>
+template <typename R, int RB, int NT, bool FULL, typename Body>
 __device__ __forceinline__ void backward_tile(const SvArgs& a, int pass_idx, int circ, int tile, R* pr, R* pi, R* lr,
                                               R* li, Layout& L, uint64_t& base, C2<R>* lds, double* slots,
                                               const ObsLds* obs, bool loaded) {
   cint* P = (cint*)a.prog;
   const int T = P[2];
   cint* pass = P + P[HDR_WORDS + pass_idx];
-  const int nsub = pass[0];
   const int flags = pass[1];
   cint* tbits = pass + 4;
   constexpr int LNT = __builtin_ctz(NT);
   base = tile_base(tile, tbits, T, a.n_wires);
-  cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride;
-  cint* soff = pass + 4 + T;
+  cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride + a.pool_off;
   if (!loaded) {
     canonical_layout<RB, LNT>(tbits, L);
     const C2<R>* ps = reinterpret_cast<const C2<R>*>(a.src) + (int64_t)circ * a.src_stride;
@@ -1518,27 +1727,12 @@ __device__ __forceinline__ void backward_tile(const SvArgs& a, int pass_idx, int
       load_canon<R, RB>(lr, li, ls, a.vec_len, L, base, a.n_wires >= T);
     }
   }
-  for (int s = nsub - 1; s >= 0; --s) {
-    cint* sub = P + soff[s];
-    if (!same_layout<RB>(sub, L)) {
-      Layout nl;
-      decode_layout<RB, LNT>(sub, tbits, nl);
-      relayout<R, RB>(pr, pi, lds, L, nl, 0, false);  // psi: region 0, lambda: region 1
-      relayout<R, RB>(lr, li, lds, L, nl, 1, true);
-      L = nl;
-    }
-    const int ng = sub[0];
-    cint* g0 = sub + 1 + T;
-    for (int q = ng - 1; q >= 0; --q) {
-      cint* g = g0 + q * GW;
-      bwd_gate<R, RB, FULL>(pr, pi, lr, li, g, pool, L, base, lds, slots);
-    }
-  }
+  Body::template bwd<R, RB, NT, FULL>(P, pass_idx, pr, pi, lr, li, L, base, lds, pool, slots);
   to_canonical2<R, RB, NT>(pr, pi, lr, li, tbits, L, lds);
 }
 
-template <typename R, int RB, int NT, bool FULL>
-__global__ __launch_bounds__(NT) void sv_backward_kernel(SvArgs a0) {
+template <typename R, int RB, int NT, bool FULL, typename Body>
+__device__ __forceinline__ void sv_backward_impl(const SvArgs& a0) {
   extern __shared__ __align__(16) unsigned char smem[];
   int circ;
   const SvArgs a = resolve_args<R>(a0, circ);
@@ -1555,7 +1749,7 @@ __global__ __launch_bounds__(NT) void sv_backward_kernel(SvArgs a0) {
   R pr[1 << RB], pi[1 << RB], lr[1 << RB], li[1 << RB];
   Layout L;
   uint64_t base;
-  backward_tile<R, RB, NT, FULL>(a, a.pass, circ, tile, pr, pi, lr, li, L, base, lds, slots, obs, false);
+  backward_tile<R, RB, NT, FULL, Body>(a, a.pass, circ, tile, pr, pi, lr, li, L, base, lds, slots, obs, false);
   C2<R>* lam = reinterpret_cast<C2<R>*>(a.lam) + (int64_t)circ * a.lam_stride;
   store_canon<R, RB>(lr, li, lam, a.vec_len, L, base, a.n_wires >= T);
   if (a.dst != nullptr)
@@ -1566,6 +1760,12 @@ __global__ __launch_bounds__(NT) void sv_backward_kernel(SvArgs a0) {
   for (int e = threadIdx.x; e < nslots; e += NT) gp[(int64_t)e * gridDim.x] = slots[e];
 }
 
+template <typename R, int RB, int NT, bool FULL>
+__global__ __launch_bounds__(NT) void sv_backward_kernel(SvArgs a0) {
+  sv_backward_impl<R, RB, NT, FULL, InterpBody>(a0);
+}
+
+#ifndef __HIPCC_RTC__
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1584,5 +1784,7 @@ hipError_t launch_backward(const SvArgs& a, int ntiles, int batch, size_t smem, 
   hipLaunchKernelGGL((sv_backward_kernel<R, RB, NT, FULL>), dim3(ntiles, batch), dim3(NT), smem, st, a);
   return hipGetLastError();
 }
+
+#endif
 
 }  // namespace jq

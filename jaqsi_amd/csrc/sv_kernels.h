@@ -1,7 +1,9 @@
 // Launch interface of the tile-pass kernels (sv_kernels.hip).
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 namespace jq {
 
@@ -37,6 +39,9 @@ struct SvArgs {
   uint64_t vec_len;    // 2^n_wires
   const void* mats;    // block matrices [batch][pool_stride] (plan dtype)
   int64_t pool_stride;
+  int64_t pool_off;    // element offset added to every matrix reference of the launch
+                       // (0 for the program interpreter; plan-specialised kernels are
+                       // shared by passes that differ only by this offset)
   const void* src;     // input vectors; nullptr => |0...0>
   int64_t src_stride;  // elements between circuits (0 = broadcast)
   void* dst;           // output vectors (forward) / running psi (backward, may be nullptr)
@@ -68,6 +73,7 @@ struct ObsTable {
   uint64_t mask[64];    // JQ_OBS_SV_PARITY: global-bit mask of the Pauli-Z word
 };
 
+#ifndef __HIPCC_RTC__  // host launch interface (absent from run-time compiled kernels)
 // Copy a host-built table to device memory on the stream (a tiny kernel, so
 // the call stays asynchronous and capturable).
 hipError_t launch_obs_upload(const ObsTable& t, void* dst, hipStream_t st);
@@ -86,5 +92,7 @@ inline hipError_t launch_sv(int which, int dtype, int T, const SvArgs& a, int nt
   return dtype == 0 ? launch_sv_f32(which, T, a, ntiles, batch, smem, st)
                     : launch_sv_f64(which, T, a, ntiles, batch, smem, st);
 }
+
+#endif
 
 }  // namespace jq

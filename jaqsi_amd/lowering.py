@@ -158,13 +158,19 @@ def lower_mixed(tape: Sequence[Operation], n_qubits: int) -> Lowered:
 
 
 def angle_tensor(values: list, batch: Optional[int]) -> torch.Tensor:
-    """Stack the angle values into (P,) or (B, P) float64, keeping autograd."""
+    """Stack the angle values into (P,) or (B, P) float64, keeping autograd.
+
+    Constant angles (Python numbers, host tensors) join the device of the
+    first device tensor among the values, so a tape may mix both."""
+    dev = next((v.device for v in values if isinstance(v, torch.Tensor) and v.device.type != "cpu"), None)
     cols = []
     for v in values:
         if isinstance(v, torch.Tensor):
             t = v.to(torch.float64)
         else:
             t = torch.as_tensor(np.asarray(v, dtype=np.float64))
+        if dev is not None and t.device != dev:
+            t = t.to(dev)
         if batch is not None:
             t = t.expand(batch) if t.dim() == 0 else t.reshape(batch)
         cols.append(t)

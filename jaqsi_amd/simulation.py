@@ -27,6 +27,25 @@ from jaqsi_amd.operations import Operation, x64_enabled
 _plans: dict = {}
 _plans_lock = threading.Lock()
 _PLAN_CACHE_MAX = 256
+_jit_mode: Optional[int] = None  # None: the library's default per plan
+
+
+def specialise(mode: Optional[int]) -> None:
+    """Choose the kernels of plans created from now on: nat.JQ_JIT_OFF (the
+    program interpreter), JQ_JIT_ASYNC (plan-specialised kernels once their
+    background compile ends), JQ_JIT_SYNC (compile when the plan is made) or
+    None (the library default: specialise plans of JQ_JIT_MIN_WIRES or more
+    wires in the background).  Cached plans are dropped."""
+    global _jit_mode
+    with _plans_lock:
+        _jit_mode = mode
+        _plans.clear()
+
+
+def cached_plans() -> list:
+    """The plans currently cached (for inspection)."""
+    with _plans_lock:
+        return list(_plans.values())
 
 
 def infer_n_qubits(ops: List[Operation], obs: List[Operation]) -> int:
@@ -67,6 +86,8 @@ def get_plan(low: lowering.Lowered, dtype: int, tile_bits: int = 0, checkpoints:
         if p is None:
             p = nat.Plan(low.n_wires, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, dtype,
                          tile_bits, checkpoints)
+            if _jit_mode is not None:
+                p.jit(_jit_mode)
             if len(_plans) >= _PLAN_CACHE_MAX:
                 _plans.pop(next(iter(_plans)))
             _plans[key] = p
