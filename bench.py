@@ -32,6 +32,13 @@ METRIC = "adjoint-grad circuits/s at 24q & batch 4096×14q; HBM GB/s vs peak"
 USE_JIT = True
 
 
+def kernel_names():
+    """Labels of the forward / backward pass kernels (as rocprofv3 names them)."""
+    if USE_JIT:
+        return ("jq_sv_forward_s*", "jq_sv_backward_s*")
+    return ("sv_forward_kernel", "sv_backward_kernel")
+
+
 def specialise(plan):
     """Compile the plan's specialised pass kernels (when enabled); seconds."""
     from jaqsi_amd import _native as nat
@@ -117,7 +124,7 @@ def bench_c2(steps, warmup, n=24, layers=8):
     import ctypes as C
 
     kt = {}
-    for w, name in ((0, "sv_forward_kernel"), (1, "sv_backward_kernel")):
+    for w, name in enumerate(kernel_names()):
         tot, cnt = C.c_double(), C.c_int32()
         nat.check(L.jq_timing_read(w, C.byref(tot), C.byref(cnt)))
         kt[name] = (tot.value, cnt.value)
@@ -136,7 +143,7 @@ def bench_c2(steps, warmup, n=24, layers=8):
 def roofline_c2(res):
     n = res["n"]
     sweep = 2 * 8 * (1 << n)  # read + write of one complex64 vector
-    bytes_per = {"sv_forward_kernel": sweep, "sv_backward_kernel": sweep * 3 // 2}
+    bytes_per = dict(zip(kernel_names(), (sweep, sweep * 3 // 2)))
     dom = max(res["kernels"], key=lambda k: res["kernels"][k][0])
     tot, cnt = res["kernels"][dom]
     avg_s = tot / 1e3 / max(cnt, 1)
@@ -326,13 +333,13 @@ def bench_c4(steps, warmup, n=12, layers=6, seed=1004):
     t1 = time.perf_counter()
     nat.check(L.jq_timing_enable(0))
     kt = {}
-    for w, name in ((0, "sv_forward_kernel"), (1, "sv_backward_kernel")):
+    for w, name in enumerate(kernel_names()):
         tot, cnt = C.c_double(), C.c_int32()
         nat.check(L.jq_timing_read(w, C.byref(tot), C.byref(cnt)))
         kt[name] = (tot.value, cnt.value)
     info = plan.info()
     sweep = 2 * 8 * (1 << (2 * n))
-    per = {"sv_forward_kernel": sweep, "sv_backward_kernel": sweep * 3 // 2}
+    per = dict(zip(kernel_names(), (sweep, sweep * 3 // 2)))
     kern = {k: {"avg_us": round(v[0] / max(v[1], 1) * 1e3, 2), "launches": v[1],
                 "GB/s": round(per[k] * v[1] / max(v[0] / 1e3, 1e-12) / 1e9, 1)} for k, v in kt.items()}
     dom = max(kt, key=lambda k: kt[k][0])

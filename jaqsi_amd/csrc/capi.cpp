@@ -34,6 +34,7 @@ struct JitFn {
   hipFunction_t f = nullptr;
   int nt = 0;            // threads per workgroup
   int64_t pool_off = 0;  // SvArgs::pool_off of the pass
+  int resident = 0;      // workgroups of the device resident at once (persistent kernels)
 };
 
 hipError_t launch_pass(const JitFn& jf, int variant, int dtype, int T, const jq::SvArgs& a, int ntiles, int batch,
@@ -42,7 +43,10 @@ hipError_t launch_pass(const JitFn& jf, int variant, int dtype, int T, const jq:
   jq::SvArgs arg = a;
   arg.pool_off = jf.pool_off;
   void* params[] = {(void*)&arg};
-  return hipModuleLaunchKernel(jf.f, ntiles, batch, 1, jf.nt, 1, 1, (unsigned)smem, st, params, nullptr);
+  // persistent kernels: one wave of resident workgroups over all rows
+  const int per_row = std::max(1, (jf.resident + batch - 1) / batch);
+  const int gx = jf.resident > 0 ? std::min(ntiles, per_row) : ntiles;
+  return hipModuleLaunchKernel(jf.f, gx, batch, 1, jf.nt, 1, 1, (unsigned)smem, st, params, nullptr);
 }
 
 // which: 0 forward pass, 1 backward pass; `full` selects the kernel variant
@@ -115,6 +119,8 @@ struct JitState {
   std::string log;
   std::vector<hipModule_t> mods;
   std::vector<hipFunction_t> fn;          // per kernel id
+  std::vector<int> resident;              // per kernel id: resident workgroups on the device
+  bool persist[2] = {false, true};        // persistent tile loops (forward, backward)
   int nt[2] = {0, 0};
 };
 
@@ -137,6 +143,9 @@ struct jq_plan {
 
 namespace {
 
+size_t smem_for(jq_plan* p, int extra_reals, int tiles = 1);
+int max_slots(jq_plan* p);
+
 // Compile the plan's specialised kernels (host only; any thread).  The
 // kernels are split into chunks compiled concurrently, one module each.
 void jit_build(jq_plan* p) {
@@ -144,6 +153,8 @@ void jit_build(jq_plan* p) {
   spec.words_f = &p->hp.words;
   spec.words_b = &p->hp.words_b;
   spec.dtype = p->dtype;
+  p->jit.persist[0] = spec.persist[0];
+  p->jit.persist[1] = spec.persist[1];
   jq::jit_plan(spec, p->jit.plan);
   const int nk = (int)p->jit.plan.kernels.size();
   const int nchunk = 1;  // the run-time compiler serialises concurrent compiles
@@ -210,6 +221,25 @@ bool jit_load(jq_plan* p) {
   }
   p->jit.nt[0] = 1 << (p->hp.words[2] - p->hp.words[3]);
   p->jit.nt[1] = 1 << (p->hp.words_b[2] - p->hp.words_b[3]);
+  int dev = 0, cus = 1;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  p->jit.resident.assign(p->jit.fn.size(), cus);
+  for (size_t id = 0; id < p->jit.fn.size(); ++id) {
+    const int dir = p->jit.plan.kernels[id].dir;
+    const size_t sm = dir == 0 ? smem_for(p, 0) : smem_for(p, max_slots(p), 2);
+    int nb = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, p->jit.fn[id], p->jit.nt[dir], sm) == hipSuccess &&
+        nb > 0)
+      p->jit.resident[id] = nb * cus;
+  }
+  // cap on the resident workgroups of a persistent launch (tests use it to
+  // make every workgroup walk several tiles on small vectors)
+  if (const char* e = std::getenv("JQ_JIT_MAX_BLOCKS")) {
+    const int cap = std::atoi(e);
+    if (cap > 0)
+      for (int& r : p->jit.resident) r = std::min(r, cap);
+  }
   p->jit.code.clear();
   p->jit.phase.store(JQ_JIT_LOADED);
   return true;
@@ -230,6 +260,7 @@ JitFn jit_fn(jq_plan* p, bool on, int dir, int q) {
     f.f = p->jit.fn[p->jit.plan.kernel_of[dir][q]];
     f.nt = p->jit.nt[dir];
     f.pool_off = p->jit.plan.pool_off[dir][q];
+    f.resident = p->jit.persist[dir] ? p->jit.resident[p->jit.plan.kernel_of[dir][q]] : 0;
   }
   return f;
 }
@@ -295,7 +326,18 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
     }
     any_full = any_full || p->pass_full[q];
   }
-  if (any_full && desc->dtype == 0) {
+  // Default kernel choice: plan-specialised kernels for plans of
+  // JQ_JIT_MIN_WIRES or more wires (JQ_JIT=0 / 1 overrides).
+  bool jit_on = desc->n_wires >= JQ_JIT_MIN_WIRES;
+  if (const char* env = std::getenv("JQ_JIT")) {
+    if (env[0] == '0') jit_on = false;
+    if (env[0] == '1') jit_on = true;
+  }
+  // complex64 plans with 4x4 / dense 3-wire blocks need the 16-amplitude
+  // backward; so do plans that start specialised (their straight-line
+  // backward keeps 16 amplitudes of psi and lambda in registers without
+  // spilling, and fewer LDS re-layouts beat the lower occupancy).
+  if ((any_full || jit_on) && desc->dtype == 0) {
     // same passes (tile choice does not depend on the register tile), a wider backward program
     jq::HostProgram wide;
     try {
@@ -320,15 +362,9 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   }
   p->n_slots = (int)smv.size();
   p->slot_words = sm;
-  {
-    const char* env = std::getenv("JQ_JIT");
-    bool on = p->hp.n_wires >= JQ_JIT_MIN_WIRES;
-    if (env && env[0] == '0') on = false;
-    if (env && env[0] == '1') on = true;
-    if (on) {
-      p->jit.mode = JQ_JIT_ASYNC;
-      jit_start(p);
-    }
+  if (jit_on) {
+    p->jit.mode = JQ_JIT_ASYNC;
+    jit_start(p);
   }
   *out = p;
   return JQ_OK;
@@ -497,7 +533,7 @@ jq::SvArgs base_args(jq_plan* p) {
   return a;
 }
 
-size_t smem_for(jq_plan* p, int extra_reals, int tiles = 1) {
+size_t smem_for(jq_plan* p, int extra_reals, int tiles) {
   // state tile(s) | observable table (64 ints + 128 doubles) | 64 expval + slot reals
   return (size_t)tiles * ((size_t)elem_bytes(p->dtype) << p->hp.T) + sizeof(jq::ObsTable) + (size_t)(64 + extra_reals) * 8;
 }

@@ -275,9 +275,10 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
            "C2<R>* lds, cmat<R>* pool) {\n"
         << "    constexpr int LNT = __builtin_ctz(NT);\n"
         << k.body << "    (void)lds; (void)base; (void)pool;\n  }\n};\n"
-        << "extern \"C\" __global__ __launch_bounds__(" << nt << ", " << occ << ") void "
-        << jit_kernel_name(0, id) << "(SvArgs a) {\n  sv_forward_impl<" << rt << ", " << pv.RB << ", " << nt
-        << ", false, JF" << id << ">(a);\n}\n\n";
+        << "extern \"C\" __global__ __launch_bounds__(" << nt << ", " << (spec.persist[0] ? 2 : occ) << ") void "
+        << jit_kernel_name(0, id) << "(SvArgs a) {\n  "
+        << (spec.persist[0] ? "sv_forward_loop<" : "sv_forward_impl<") << rt << ", " << pv.RB << ", " << nt
+        << (spec.persist[0] ? "" : ", false") << ", JF" << id << ">(a);\n}\n\n";
     } else {
       o << "struct JB" << id << " {\n"
         << "  template <typename R, int RB, int NT, bool FULL>\n"
@@ -285,9 +286,9 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
            "uint64_t base, C2<R>* lds, cmat<R>* pool, double* slots) {\n"
         << "    constexpr int LNT = __builtin_ctz(NT);\n"
         << k.body << "    (void)lds; (void)base; (void)pool; (void)slots;\n  }\n};\n"
-        << "extern \"C\" __global__ __launch_bounds__(" << nt << ") void " << jit_kernel_name(1, id)
-        << "(SvArgs a) {\n  sv_backward_impl<" << rt << ", " << pv.RB << ", " << nt << ", false, JB" << id
-        << ">(a);\n}\n\n";
+        << "extern \"C\" __global__ __launch_bounds__(" << nt << ") void " << jit_kernel_name(1, id) << "(SvArgs a) {\n  "
+        << (spec.persist[1] ? "sv_backward_loop<" : "sv_backward_impl<") << rt << ", " << pv.RB << ", " << nt
+        << (spec.persist[1] ? "" : ", false") << ", JB" << id << ">(a);\n}\n\n";
     }
   }
   o << "}  // namespace jq\n";

@@ -13,11 +13,16 @@ pytestmark = pytest.mark.gpu
 TOL = {False: 2e-5, True: 1e-11}
 
 
-@pytest.fixture
-def specialised():
+@pytest.fixture(params=["all-resident", "3-workgroups"])
+def specialised(request, monkeypatch):
+    """Plans compiled at creation.  With "3-workgroups" each persistent
+    launch keeps at most three workgroups resident, so every workgroup walks
+    several tiles (next-tile prefetch, slot and partial reuse)."""
     from jaqsi_amd import _native as nat
     from jaqsi_amd import simulation
 
+    if request.param == "3-workgroups":
+        monkeypatch.setenv("JQ_JIT_MAX_BLOCKS", "3")
     simulation.specialise(nat.JQ_JIT_SYNC)
     yield simulation
     simulation.specialise(None)
