@@ -362,7 +362,8 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   }
   p->n_slots = (int)smv.size();
   p->slot_words = sm;
-  if (jit_on) {
+  int ndev = 0;
+  if (jit_on && hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) {  // nothing to run them on otherwise
     p->jit.mode = JQ_JIT_ASYNC;
     jit_start(p);
   }

@@ -196,7 +196,13 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
   int T = T_req;
   if (T < RB) throw PlanError(JQ_ERR_ARG, "tile smaller than the register tile");
   const int ntiles_log = std::max(0, n - T);
-  const int lowfix = std::min(n, std::min(T, 6 + (T >= 13 ? 1 : 0)));
+  // Every tile holds the lowest global bits, so that loads and stores move
+  // contiguous runs: 5 bits for complex64 (256-byte runs, 32 amplitudes) and
+  // 6 for complex128 (1 KiB).  Fewer fixed bits leave more tile bits to the
+  // gates and cut passes (the 24-qubit HEA: 24 passes with 7 fixed bits, 21
+  // with 5); measured on MI355X, 5 is the best trade: 4 and 3 cost more in
+  // access efficiency than they save in passes.
+  const int lowfix = std::min(n, std::min(T, T >= 13 ? 5 : 6));
   // Global bits >= n are phantom (n < T): their amplitudes do not exist.
 
   out = HostProgram();

@@ -954,9 +954,10 @@ __device__ __forceinline__ void bwd_gate(R* pr, R* pi, R* lr, R* li, cint* g, cm
 // global <-> register, register <-> LDS
 // ---------------------------------------------------------------------------
 // Canonical layout: register bit 0 <-> tile bit 0, thread bit j <-> tile bit
-// 1+j, register bits 1.. <-> the top tile bits.  Tile bits 0..6 are the
-// lowest global bits, so a wave reads 64 consecutive amplitude pairs (16-byte
-// vector accesses, 1 KiB contiguous for complex64) per register pair.
+// 1+j, register bits 1.. <-> the top tile bits.  The lowest tile bits are the
+// lowest global bits (5 for complex64, 6 for complex128), so the lanes of a
+// wave read 16-byte vectors in contiguous runs of 256 B (complex64) or 1 KiB
+// (complex128) per register pair.
 template <int RB, int LNT>
 __device__ __forceinline__ void canonical_layout(cint* tbits, Layout& L) {
   constexpr int T = RB + LNT;

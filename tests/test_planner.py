@@ -59,13 +59,21 @@ def test_gate_zoo_program_matches_oracle(dtype):
 
 
 def test_c2_schedule_shape():
-    """The 24-qubit, 8-layer workload schedules into 3 passes per layer."""
+    """The 24-qubit, 8-layer workload schedules into 21 passes (the CX ring
+    advances about 8 wires per pass), and every tile holds global bits 0..4
+    (256-byte contiguous runs of complex64)."""
     from circuits import record_hea
 
     theta = np.zeros((8, 24, 3))
     plan, _, _ = _plan(_record(lambda: record_hea(theta, 24)), 24)
     info = plan.info()
-    assert info["n_passes"] == 24 and info["n_blocks"] == 192 and info["tile_bits"] == 13
+    assert info["n_passes"] == 21 and info["n_blocks"] == 192 and info["tile_bits"] == 13
+    w = plan.program()
+    T = int(w[2])
+    for q in range(info["n_passes"]):
+        rec = int(w[16 + q])
+        tbits = [int(x) for x in w[rec + 4: rec + 4 + T]]
+        assert tbits[:5] == [0, 1, 2, 3, 4] and tbits == sorted(tbits)
 
 
 def test_planner_errors_map_to_exceptions():
