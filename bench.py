@@ -188,6 +188,9 @@ def bench_c3(steps, warmup, B=4096, n=14, seed=1003):
     members = {d: [b for b in range(B) if depths[b] == d] for d in uniq}
     tapes = [record(record_hea, np.zeros((d, n, 3)), n, np.zeros(n)) for d in uniq]
     rb = RaggedBatch(tapes, [len(members[d]) for d in uniq], n, [(q, 1.0, -1.0) for q in range(n)], dtype=0)
+    t_c = time.perf_counter()
+    rb.specialise(nat.JQ_JIT_SYNC if USE_JIT else nat.JQ_JIT_OFF)
+    compile_s = round(time.perf_counter() - t_c, 2) if USE_JIT else 0.0
     angles = [
         torch.tensor(np.stack([np.concatenate([feats[b], thetas[b].reshape(-1)]) for b in members[d]]), device=dev)
         for d in uniq
@@ -207,12 +210,13 @@ def bench_c3(steps, warmup, B=4096, n=14, seed=1003):
     t1 = time.perf_counter()
     nat.check(L.jq_timing_enable(0))
     kt = {}
-    for w, name in ((0, "sv_forward_kernel"), (1, "sv_backward_kernel")):
+    for w, name in enumerate(kernel_names()):
         tot, cnt = C.c_double(), C.c_int32()
         nat.check(L.jq_timing_read(w, C.byref(tot), C.byref(cnt)))
         kt[name] = (tot.value, cnt.value)
     circuit_passes = sum(len(members[d]) * p.info()["n_passes"] for d, p in zip(uniq, rb.plans))
     return {
+        "compile_s": compile_s,
         "seconds": t1 - t0,
         "kernels": kt,
         "circuit_passes": circuit_passes,
@@ -229,7 +233,7 @@ def bench_c3(steps, warmup, B=4096, n=14, seed=1003):
 def roofline_c3(res, steps):
     n = res["n"]
     sweep = 2 * 8 * (1 << n)
-    per = {"sv_forward_kernel": sweep, "sv_backward_kernel": sweep * 3 // 2}
+    per = dict(zip(kernel_names(), (sweep, sweep * 3 // 2)))
     out = {}
     for k, (tot, cnt) in res["kernels"].items():
         s_ = tot / 1e3
@@ -598,6 +602,9 @@ def main():
                 "14 Z expvals + adjoint gradient of sum_q <Z_q> per circuit",
                 "groups": r3["groups"],
                 "circuit_passes": r3["circuit_passes"],
+                "pass_kernels": "plan-specialised (one launch sequence per group)" if USE_JIT
+                else "interpreter (one launch per pass step over all groups)",
+                "specialise_compile_s": r3["compile_s"],
             },
             "roofline": roofline_c3(r3, args.c3_steps),
         }

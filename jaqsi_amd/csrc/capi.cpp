@@ -4,6 +4,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -114,10 +117,8 @@ struct JitState {
   std::atomic<int> phase{JQ_JIT_NONE};
   int mode = 0;  // 0 interpreter, 1 specialised kernels once compiled
   jq::JitPlan plan;                       // distinct pass bodies and the pass -> kernel map
-  std::vector<std::vector<int>> chunks;   // kernel ids of each module
-  std::vector<std::vector<char>> code;    // one code object per chunk
+  std::vector<std::pair<int, std::string>> kref;  // per kernel id: (cache code object, kernel name)
   std::string log;
-  std::vector<hipModule_t> mods;
   std::vector<hipFunction_t> fn;          // per kernel id
   std::vector<int> resident;              // per kernel id: resident workgroups on the device
   bool persist[2] = {false, true};        // persistent tile loops (forward, backward)
@@ -135,6 +136,8 @@ struct jq_plan {
   std::vector<int32_t> slot_words;
   std::vector<char> pass_full;  // pass holds a 4x4 register block or a dense 3-wire block
   bool wide_bwd = false;        // backward program built for 16 amplitudes x 512 threads (complex64)
+  std::vector<int32_t> words_bj;  // backward program of the plan-specialised kernels
+  DevBuf prog_bj;
   bool uploaded = false;
   JitState jit;
   // workspaces
@@ -146,48 +149,74 @@ namespace {
 size_t smem_for(jq_plan* p, int extra_reals, int tiles = 1);
 int max_slots(jq_plan* p);
 
-// Compile the plan's specialised kernels (host only; any thread).  The
-// kernels are split into chunks compiled concurrently, one module each.
+// Process-wide cache of compiled pass kernels.  A kernel serves every plan
+// whose pass body (and register tile, dtype, loop variant) is the same, so
+// plans of one ansatz at different depths compile their shared passes once.
+// Compiles are serialised under the lock (the run-time compiler serialises
+// concurrent compiles anyway); code objects and modules live until the
+// process ends.
+struct JitCache {
+  std::mutex mu;
+  std::unordered_map<std::string, std::pair<int, std::string>> kernels;  // key -> (code object, name)
+  std::vector<std::vector<char>> code;
+  std::map<std::pair<int, int>, hipModule_t> mods;  // (device, code object) -> module
+  int next = 0;                                     // kernel name counter
+};
+JitCache& jit_cache() {
+  static JitCache c;
+  return c;
+}
+
+// Compile the plan's specialised kernels that the cache lacks (host only;
+// any thread).
 void jit_build(jq_plan* p) {
   jq::JitSpec spec;
   spec.words_f = &p->hp.words;
-  spec.words_b = &p->hp.words_b;
+  spec.words_b = &p->words_bj;
   spec.dtype = p->dtype;
   p->jit.persist[0] = spec.persist[0];
   p->jit.persist[1] = spec.persist[1];
   jq::jit_plan(spec, p->jit.plan);
   const int nk = (int)p->jit.plan.kernels.size();
-  const int nchunk = 1;  // the run-time compiler serialises concurrent compiles
-  std::vector<std::vector<int>> chunks(nchunk);
-  for (int i = 0; i < nk; ++i) chunks[i % nchunk].push_back(i);
-  std::vector<std::vector<char>> code(nchunk);
-  std::vector<std::string> logs(nchunk);
-  std::vector<char> ok(nchunk, 0);
-  const char* dump = std::getenv("JQ_JIT_DUMP");  // debugging aid: keep the generated source
-  auto work = [&](int c) {
-    const std::string src = jq::jit_source(spec, p->jit.plan, chunks[c]);
-    if (dump) {
-      if (FILE* fh = std::fopen((std::string(dump) + "." + std::to_string(c) + ".hip").c_str(), "w")) {
+  JitCache& C = jit_cache();
+  std::lock_guard<std::mutex> lock(C.mu);
+  std::vector<std::string> keys(nk);
+  std::vector<int> todo;
+  std::vector<std::string> names;
+  p->jit.kref.assign(nk, {-1, std::string()});
+  for (int id = 0; id < nk; ++id) {
+    keys[id] = jq::jit_key(spec, p->jit.plan, id);
+    auto it = C.kernels.find(keys[id]);
+    if (it != C.kernels.end()) {
+      p->jit.kref[id] = it->second;
+    } else {
+      todo.push_back(id);
+      names.push_back(jq::jit_kernel_name(p->jit.plan.kernels[id].dir, C.next++));
+    }
+  }
+  bool ok = true;
+  std::string log;
+  if (!todo.empty()) {
+    const std::string src = jq::jit_source(spec, p->jit.plan, todo, names);
+    if (const char* dump = std::getenv("JQ_JIT_DUMP")) {  // debugging aid: keep the generated source
+      if (FILE* fh = std::fopen((std::string(dump) + "." + std::to_string(C.code.size()) + ".hip").c_str(), "w")) {
         std::fwrite(src.data(), 1, src.size(), fh);
         std::fclose(fh);
       }
     }
-    ok[c] = jq::jit_compile(src, code[c], logs[c]) ? 1 : 0;
-  };
-  std::vector<std::thread> th;
-  for (int c = 1; c < nchunk; ++c) th.emplace_back(work, c);
-  work(0);
-  for (auto& t : th) t.join();
-  bool all_ok = true;
-  std::string log;
-  for (int c = 0; c < nchunk; ++c) {
-    all_ok = all_ok && ok[c];
-    if (!ok[c]) log += logs[c];
+    std::vector<char> code;
+    ok = jq::jit_compile(src, code, log);
+    if (ok) {
+      const int ci = (int)C.code.size();
+      C.code.push_back(std::move(code));
+      for (size_t j = 0; j < todo.size(); ++j) {
+        C.kernels[keys[todo[j]]] = {ci, names[j]};
+        p->jit.kref[todo[j]] = {ci, names[j]};
+      }
+    }
   }
-  p->jit.chunks.swap(chunks);
-  p->jit.code.swap(code);
   p->jit.log = log;
-  p->jit.phase.store(all_ok ? JQ_JIT_COMPILED : JQ_JIT_FAILED, std::memory_order_release);
+  p->jit.phase.store(ok ? JQ_JIT_COMPILED : JQ_JIT_FAILED, std::memory_order_release);
 }
 
 void jit_start(jq_plan* p) {
@@ -202,25 +231,30 @@ bool jit_load(jq_plan* p) {
   if (p->jit.worker.joinable()) p->jit.worker.join();
   p->jit.fn.assign(p->jit.plan.kernels.size(), nullptr);
   hipError_t e = hipSuccess;
-  for (size_t c = 0; c < p->jit.code.size() && e == hipSuccess; ++c) {
-    hipModule_t m = nullptr;
-    e = hipModuleLoadData(&m, p->jit.code[c].data());
-    if (e != hipSuccess) break;
-    p->jit.mods.push_back(m);
-    for (int id : p->jit.chunks[c]) {
-      e = hipModuleGetFunction(&p->jit.fn[id], m, jq::jit_kernel_name(p->jit.plan.kernels[id].dir, id).c_str());
-      if (e != hipSuccess) break;
+  {
+    JitCache& C = jit_cache();
+    std::lock_guard<std::mutex> lock(C.mu);
+    int dev = 0;
+    e = hipGetDevice(&dev);
+    for (size_t id = 0; id < p->jit.kref.size() && e == hipSuccess; ++id) {
+      const int ci = p->jit.kref[id].first;
+      auto it = C.mods.find({dev, ci});
+      if (it == C.mods.end()) {
+        hipModule_t m = nullptr;
+        e = hipModuleLoadData(&m, C.code[ci].data());
+        if (e != hipSuccess) break;
+        it = C.mods.emplace(std::make_pair(dev, ci), m).first;
+      }
+      e = hipModuleGetFunction(&p->jit.fn[id], it->second, p->jit.kref[id].second.c_str());
     }
   }
   if (e != hipSuccess) {
-    for (hipModule_t m : p->jit.mods) hipModuleUnload(m);
-    p->jit.mods.clear();
     p->jit.log = std::string("module load: ") + hipGetErrorString(e);
     p->jit.phase.store(JQ_JIT_FAILED);
     return false;
   }
   p->jit.nt[0] = 1 << (p->hp.words[2] - p->hp.words[3]);
-  p->jit.nt[1] = 1 << (p->hp.words_b[2] - p->hp.words_b[3]);
+  p->jit.nt[1] = 1 << (p->words_bj[2] - p->words_bj[3]);
   int dev = 0, cus = 1;
   hipGetDevice(&dev);
   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -240,7 +274,6 @@ bool jit_load(jq_plan* p) {
     if (cap > 0)
       for (int& r : p->jit.resident) r = std::min(r, cap);
   }
-  p->jit.code.clear();
   p->jit.phase.store(JQ_JIT_LOADED);
   return true;
 }
@@ -333,12 +366,15 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
     if (env[0] == '0') jit_on = false;
     if (env[0] == '1') jit_on = true;
   }
-  // complex64 plans with 4x4 / dense 3-wire blocks need the 16-amplitude
-  // backward; so do plans that start specialised (their straight-line
-  // backward keeps 16 amplitudes of psi and lambda in registers without
-  // spilling, and fewer LDS re-layouts beat the lower occupancy).
-  if ((any_full || jit_on) && desc->dtype == 0) {
-    // same passes (tile choice does not depend on the register tile), a wider backward program
+  // complex64: a second backward program for 16 amplitudes x 512 threads
+  // (same passes: the tile choice does not depend on the register tile).
+  // The plan-specialised backward always runs it (its straight-line code
+  // keeps 16 amplitudes of psi and lambda in registers without spilling, and
+  // fewer LDS re-layouts beat the lower occupancy); the interpreter runs it
+  // for plans with 4x4 / dense 3-wire blocks, which would spill under the
+  // 1024-thread register cap.
+  p->words_bj = p->hp.words_b;
+  if (desc->dtype == 0) {
     jq::HostProgram wide;
     try {
       jq::build_program(*desc, T, RB, 4, wide);
@@ -350,8 +386,11 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
       delete p;
       return fail(JQ_ERR_ARG, "internal: wide backward program changed the pass structure");
     }
-    p->hp.words_b = wide.words_b;
-    p->wide_bwd = true;
+    p->words_bj = wide.words_b;
+    if (any_full) {
+      p->hp.words_b = wide.words_b;
+      p->wide_bwd = true;
+    }
   }
   std::vector<int32_t> sm;
   std::vector<jq::SlotMap> smv = p->hp.slot_map;
@@ -396,8 +435,7 @@ int jq_plan_jit(jq_plan* p, int32_t mode, int32_t* phase) {
 int jq_plan_destroy(jq_plan* p) {
   if (!p) return JQ_OK;
   if (p->jit.worker.joinable()) p->jit.worker.join();
-  for (hipModule_t m : p->jit.mods) hipModuleUnload(m);
-  for (DevBuf* b : {&p->prog, &p->prog_b, &p->block_table, &p->member_list, &p->member_table, &p->member_values, &p->slot_map,
+  for (DevBuf* b : {&p->prog, &p->prog_b, &p->prog_bj, &p->block_table, &p->member_list, &p->member_table, &p->member_values, &p->slot_map,
                     &p->pool, &p->partials, &p->bars, &p->ev_part, &p->lam, &p->ckpt, &p->status, &p->obs, &p->fstate, &p->groups})
     b->release();
   delete p;
@@ -453,6 +491,7 @@ int ensure_uploaded(jq_plan* p) {
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = upload(p->prog, p->hp.words);
   if (e == hipSuccess) e = upload(p->prog_b, p->hp.words_b);
+  if (e == hipSuccess) e = upload(p->prog_bj, p->words_bj);
   if (e == hipSuccess) e = upload(p->block_table, p->hp.block_table);
   if (e == hipSuccess) e = upload(p->member_list, p->hp.member_list);
   if (e == hipSuccess) e = upload(p->member_table, p->hp.member_table);
@@ -676,7 +715,7 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
     if (expvals) HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
     // backward sweep
     const size_t sm = smem_for(p, max_slots(p), 2);
-    a.prog = (const int*)p->prog_b.p;
+    a.prog = (const int*)(jit ? p->prog_bj.p : p->prog_b.p);
     for (int q = P - 1; q >= 0; --q) {
       a.pass = q;
       a.src = pass_out(q);
@@ -712,6 +751,21 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
     total += G.batch;
   }
   if (total > 65535) return fail(JQ_ERR_ARG, "more than 65535 circuits per call");
+  // Every plan has its specialised kernels: run the groups one after the
+  // other through the single-plan adjoint (a few launches per pass and group
+  // instead of one per pass step, but each pass runs its straight-line
+  // kernel).  Otherwise all groups share the interpreter's launches below.
+  bool all_jit = true;
+  for (int g = 0; g < n_groups && all_jit; ++g) all_jit = jit_ready(groups[g].plan);
+  if (all_jit) {
+    for (int g = 0; g < n_groups; ++g) {
+      const jq_group& G = groups[g];
+      const int rc = jq_sv_adjoint(G.plan, G.batch, G.angles, G.matrices, G.psi0, G.psi0_batched, G.state, obs_kind,
+                                   n_obs, obs_wires, obs_d, G.ct, G.expvals, G.grad_angles, nullptr, nullptr, stream);
+      if (rc) return rc;
+    }
+    return JQ_OK;
+  }
   // groups with more passes first: the circuits still active at a step are a prefix
   std::stable_sort(order.begin(), order.end(),
                    [&](int x, int y) { return groups[x].plan->hp.n_passes > groups[y].plan->hp.n_passes; });

@@ -258,14 +258,24 @@ void jit_plan(const JitSpec& spec, JitPlan& out) {
   }
 }
 
-std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vector<int>& ids) {
+std::string jit_key(const JitSpec& spec, const JitPlan& plan, int id) {
+  const JitPlan::Kernel& k = plan.kernels[id];
+  const ProgView pv = view(k.dir == 0 ? *spec.words_f : *spec.words_b);
+  return std::to_string(k.dir) + " " + std::to_string(spec.dtype) + " " + std::to_string(pv.T) + " " +
+         std::to_string(pv.RB) + " " + std::to_string(spec.persist[k.dir] ? 1 : 0) + "\n" + k.body;
+}
+
+std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vector<int>& ids,
+                       const std::vector<std::string>& names) {
   std::ostringstream o;
   o << kPrelude;
   const char* rt = spec.dtype == 0 ? "float" : "double";
   const ProgView f = view(*spec.words_f), b = view(*spec.words_b);
   const int occ = spec.dtype == 0 ? 4 : 2;  // the interpreter's forward launch bounds
-  for (int id : ids) {
-    const JitPlan::Kernel& k = plan.kernels[id];
+  for (size_t j = 0; j < ids.size(); ++j) {
+    const int id = (int)j;  // struct names are local to this source
+    const JitPlan::Kernel& k = plan.kernels[ids[j]];
+    const std::string& kname = names[j];
     const ProgView& pv = k.dir == 0 ? f : b;
     const int nt = 1 << (pv.T - pv.RB);
     if (k.dir == 0) {
@@ -276,7 +286,7 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
         << "    constexpr int LNT = __builtin_ctz(NT);\n"
         << k.body << "    (void)lds; (void)base; (void)pool;\n  }\n};\n"
         << "extern \"C\" __global__ __launch_bounds__(" << nt << ", " << (spec.persist[0] ? 2 : occ) << ") void "
-        << jit_kernel_name(0, id) << "(SvArgs a) {\n  "
+        << kname << "(SvArgs a) {\n  "
         << (spec.persist[0] ? "sv_forward_loop<" : "sv_forward_impl<") << rt << ", " << pv.RB << ", " << nt
         << (spec.persist[0] ? "" : ", false") << ", JF" << id << ">(a);\n}\n\n";
     } else {
@@ -286,7 +296,7 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
            "uint64_t base, C2<R>* lds, cmat<R>* pool, double* slots) {\n"
         << "    constexpr int LNT = __builtin_ctz(NT);\n"
         << k.body << "    (void)lds; (void)base; (void)pool; (void)slots;\n  }\n};\n"
-        << "extern \"C\" __global__ __launch_bounds__(" << nt << ") void " << jit_kernel_name(1, id) << "(SvArgs a) {\n  "
+        << "extern \"C\" __global__ __launch_bounds__(" << nt << ") void " << kname << "(SvArgs a) {\n  "
         << (spec.persist[1] ? "sv_backward_loop<" : "sv_backward_impl<") << rt << ", " << pv.RB << ", " << nt
         << (spec.persist[1] ? "" : ", false") << ", JB" << id << ">(a);\n}\n\n";
     }

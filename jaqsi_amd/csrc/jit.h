@@ -46,11 +46,18 @@ struct JitPlan {
 
 void jit_plan(const JitSpec& spec, JitPlan& out);
 
-// Kernel names: jq_sv_forward_s<id>, jq_sv_backward_s<id>.
-std::string jit_kernel_name(int dir, int id);
+// Kernel names: jq_sv_forward_s<n>, jq_sv_backward_s<n> (n unique in the process).
+std::string jit_kernel_name(int dir, int n);
 
-// HIP C++ source of the listed kernels of `plan`.
-std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vector<int>& ids);
+// HIP C++ source of the listed kernels of `plan`; kernel ids[i] is emitted
+// as the extern "C" kernel names[i].
+std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vector<int>& ids,
+                       const std::vector<std::string>& names);
+
+// Key under which a compiled kernel can serve any plan: everything the
+// generated kernel depends on (direction, dtype, register tile, loop variant,
+// body text).
+std::string jit_key(const JitSpec& spec, const JitPlan& plan, int id);
 
 // Compile `src` for gfx950.  Returns false and fills `log` on failure.
 bool jit_compile(const std::string& src, std::vector<char>& code, std::string& log);

@@ -44,6 +44,13 @@ class RaggedBatch:
             self.mats.append(m.reshape(1, -1).expand(B, -1).contiguous().to(dev) if m is not None else None)
         self.state = [torch.empty((B, 1 << n_qubits), dtype=_cdtype_torch(self.dtype), device=dev) for B in self.batch]
 
+    def specialise(self, mode: int = nat.JQ_JIT_SYNC) -> List[int]:
+        """Plan-specialised kernels for every group's plan (nat.JQ_JIT_*).
+        Once all groups have them loaded, ``adjoint`` runs each group through
+        them; until then all groups share the interpreter's launches.
+        Returns the compile phase of each plan."""
+        return [p.jit(mode) for p in self.plans]
+
     def angle_shapes(self) -> List[Tuple[int, int]]:
         return [(B, P) for B, P in zip(self.batch, self.n_params)]
 

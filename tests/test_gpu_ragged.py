@@ -13,19 +13,28 @@ pytestmark = pytest.mark.gpu
 TOL = {False: 2e-5, True: 1e-11}
 
 
+@pytest.mark.parametrize("specialised", [False, True])
 @pytest.mark.parametrize("x64", [False, True])
 @pytest.mark.parametrize("n", [5, 14])
-def test_uneven_depth_batch(n, x64):
+def test_uneven_depth_batch(n, x64, specialised):
+    """Grouped interpreter launches, or (specialised) every group through its
+    plan-specialised kernels."""
     import jaqsi_amd as J
+    from jaqsi_amd import _native as nat
+    from jaqsi_amd import simulation
     from jaqsi_amd.ragged import RaggedBatch, record
 
-    print(f"ragged n={n} x64={x64}", flush=True)
+    print(f"ragged n={n} x64={x64} specialised={specialised}", flush=True)
     J.enable_x64(x64)
+    simulation.specialise(None)  # fresh plans for this case
     rng = np.random.default_rng(3000 + n)
     depths = [1, 3, 2]
     per = [3, 2, 4]
     tapes = [record(record_hea, np.zeros((d, n, 3)), n, np.zeros(n)) for d in depths]
     rb = RaggedBatch(tapes, per, n, [(q, 1.0, -1.0) for q in range(n)])
+    phases = rb.specialise(nat.JQ_JIT_SYNC if specialised else nat.JQ_JIT_OFF)
+    if specialised:
+        assert all(ph == nat.JQ_JIT_LOADED for ph in phases)
     thetas, feats, angles, cts = [], [], [], []
     for d, B in zip(depths, per):
         th = rng.uniform(0, 2 * np.pi, (B, d, n, 3))
