@@ -122,6 +122,7 @@ struct JitState {
   std::vector<hipFunction_t> fn;          // per kernel id
   std::vector<int> resident;              // per kernel id: resident workgroups on the device
   bool persist[2] = {false, true};        // persistent tile loops (forward, backward)
+  bool dma = false;                       // forward streams tiles into LDS (two tiles of LDS)
   int nt[2] = {0, 0};
 };
 
@@ -174,6 +175,7 @@ void jit_build(jq_plan* p) {
   spec.words_f = &p->hp.words;
   spec.words_b = &p->words_bj;
   spec.dtype = p->dtype;
+  p->jit.dma = spec.dma && p->dtype == 0;
   p->jit.persist[0] = spec.persist[0];
   p->jit.persist[1] = spec.persist[1];
   jq::jit_plan(spec, p->jit.plan);

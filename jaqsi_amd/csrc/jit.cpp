@@ -190,18 +190,34 @@ int pass_rebase(const ProgView& pv, int q) {
   return m < 0 ? 0 : m;
 }
 
-void emit_forward(std::ostringstream& o, const ProgView& pv, int q, int rebase) {
+// Forward pass body.  `first` receives the layout words of the first
+// substep when the pass starts with a re-layout (empty otherwise); `o`
+// receives the statements after that re-layout.
+void emit_forward(std::ostringstream& o, std::string& first, const ProgView& pv, int q, int rebase) {
   std::vector<int> cur = canonical_rpos(pv);
   const int nsub = pv.pass(q)[0];
+  first.clear();
   for (int s = 0; s < nsub; ++s) {
     const int32_t* sub = pv.sub(q, s);
     if (!same_rpos(pv, sub, cur)) {
-      o << "    jit_relayout<R, RB, LNT" << layout_args(pv, sub) << ">(re, im, lds, L);\n";
+      if (s == 0)
+        first = layout_args(pv, sub);
+      else
+        o << "    jit_relayout<R, RB, LNT" << layout_args(pv, sub) << ">(re, im, lds, L);\n";
       for (int b = 0; b < pv.RB; ++b) cur[b] = sub[1 + b];
     }
     const int32_t* g = sub + 1 + pv.T;
     for (int k = 0; k < sub[0]; ++k, g += GW) emit_fwd_gate(o, GateLit(g, rebase));
   }
+}
+
+// layout words of the canonical layout (register bits 0, T-RB+1 .. T-1;
+// thread bit j <-> tile bit 1 + j)
+std::string canonical_args(const ProgView& pv) {
+  std::string o;
+  for (int r : canonical_rpos(pv)) o += ", " + std::to_string(r);
+  for (int j = 0; j < pv.T - pv.RB; ++j) o += ", " + std::to_string(1 + j);
+  return o;
 }
 
 void emit_backward(std::ostringstream& o, const ProgView& pv, int q, int rebase) {
@@ -240,17 +256,20 @@ void jit_plan(const JitSpec& spec, JitPlan& out) {
     for (int q = 0; q < pv.n_passes; ++q) {
       const int rebase = pass_rebase(pv, q);
       std::ostringstream o;
+      std::string first;
       if (dir == 0)
-        emit_forward(o, pv, q, rebase);
+        emit_forward(o, first, pv, q, rebase);
       else
         emit_backward(o, pv, q, rebase);
-      const std::string body = o.str();
+      const std::string body0 = o.str();
+      const std::string body =
+          first.empty() ? body0 : "    jit_relayout<R, RB, LNT" + first + ">(re, im, lds, L);\n" + body0;
       int id = -1;
       for (size_t k = 0; k < out.kernels.size(); ++k)
         if (out.kernels[k].dir == dir && out.kernels[k].body == body) id = (int)k;
       if (id < 0) {
         id = (int)out.kernels.size();
-        out.kernels.push_back({dir, body});
+        out.kernels.push_back({dir, body, first, body0});
       }
       out.kernel_of[dir].push_back(id);
       out.pool_off[dir].push_back(rebase);
@@ -262,7 +281,8 @@ std::string jit_key(const JitSpec& spec, const JitPlan& plan, int id) {
   const JitPlan::Kernel& k = plan.kernels[id];
   const ProgView pv = view(k.dir == 0 ? *spec.words_f : *spec.words_b);
   return std::to_string(k.dir) + " " + std::to_string(spec.dtype) + " " + std::to_string(pv.T) + " " +
-         std::to_string(pv.RB) + " " + std::to_string(spec.persist[k.dir] ? 1 : 0) + "\n" + k.body;
+         std::to_string(pv.RB) + " " + std::to_string(spec.persist[k.dir] ? 1 : 0) + " " +
+         std::to_string(spec.dma ? 1 : 0) + "\n" + k.body;
 }
 
 std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vector<int>& ids,
@@ -279,16 +299,28 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
     const ProgView& pv = k.dir == 0 ? f : b;
     const int nt = 1 << (pv.T - pv.RB);
     if (k.dir == 0) {
+      const bool dma = spec.dma && spec.dtype == 0;
+      const char* sig =
+          "(cint*, int, R* re, R* im, Layout& L, uint64_t base, C2<R>* lds, cmat<R>* pool) {\n"
+          "    constexpr int LNT = __builtin_ctz(NT);\n";
       o << "struct JF" << id << " {\n"
         << "  template <typename R, int RB, int NT, bool FULL>\n"
-        << "  static __device__ __forceinline__ void fwd(cint*, int, R* re, R* im, Layout& L, uint64_t base, "
-           "C2<R>* lds, cmat<R>* pool) {\n"
-        << "    constexpr int LNT = __builtin_ctz(NT);\n"
-        << k.body << "    (void)lds; (void)base; (void)pool;\n  }\n};\n"
-        << "extern \"C\" __global__ __launch_bounds__(" << nt << ", " << (spec.persist[0] ? 2 : occ) << ") void "
-        << kname << "(SvArgs a) {\n  "
-        << (spec.persist[0] ? "sv_forward_loop<" : "sv_forward_impl<") << rt << ", " << pv.RB << ", " << nt
-        << (spec.persist[0] ? "" : ", false") << ", JF" << id << ">(a);\n}\n\n";
+        << "  static __device__ __forceinline__ void fwd" << sig << k.body
+        << "    (void)lds; (void)base; (void)pool;\n  }\n"
+        << "  template <typename R, int RB, int NT, bool FULL>\n"
+        << "  static __device__ __forceinline__ void fwd0" << sig << k.body0
+        << "    (void)lds; (void)base; (void)pool;\n  }\n"
+        << "  template <int RB, int LNT>\n"
+        << "  static __device__ __forceinline__ void first_layout(cint* tb, Layout& L) {\n"
+        << "    layout_lit<RB, LNT" << (k.first.empty() ? canonical_args(pv) : k.first) << ">(tb, L);\n  }\n};\n";
+      if (dma)
+        o << "extern \"C\" __global__ __launch_bounds__(" << nt << ", " << occ << ") void " << kname
+          << "(SvArgs a) {\n  sv_forward_dma<" << rt << ", " << pv.RB << ", " << nt << ", JF" << id
+          << ", false>(a);\n}\n\n";
+      else
+        o << "extern \"C\" __global__ __launch_bounds__(" << nt << ", " << (spec.persist[0] ? 2 : occ) << ") void "
+          << kname << "(SvArgs a) {\n  " << (spec.persist[0] ? "sv_forward_loop<" : "sv_forward_impl<") << rt
+          << ", " << pv.RB << ", " << nt << (spec.persist[0] ? "" : ", false") << ", JF" << id << ">(a);\n}\n\n";
     } else {
       o << "struct JB" << id << " {\n"
         << "  template <typename R, int RB, int NT, bool FULL>\n"

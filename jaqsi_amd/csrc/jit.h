@@ -29,6 +29,11 @@ struct JitSpec {
   // lengthens the forward (its prefetch registers halve the resident
   // workgroups).
   bool persist[2] = {false, true};
+  // Forward (complex64): the tile streams into LDS and the pass starts in its
+  // first substep's layout (sv_forward_dma, one tile per workgroup so two
+  // workgroups fit per CU; a persistent variant with a separate prefetch
+  // tile measured slower).
+  bool dma = true;
 };
 
 // Unique pass bodies of a plan.  Passes whose gate sequences differ only by
@@ -36,8 +41,10 @@ struct JitSpec {
 // with SvArgs::pool_off = pool_off[dir][q].
 struct JitPlan {
   struct Kernel {
-    int dir;           // 0 forward, 1 backward
-    std::string body;  // generated statements of the pass body
+    int dir;            // 0 forward, 1 backward
+    std::string body;   // generated statements of the pass body (from the canonical layout)
+    std::string first;  // forward: layout words of the first substep, empty when canonical
+    std::string body0;  // forward: the statements after the initial re-layout
   };
   std::vector<Kernel> kernels;
   std::vector<int> kernel_of[2];

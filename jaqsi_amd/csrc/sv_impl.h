@@ -1848,6 +1848,115 @@ __device__ __forceinline__ void sv_forward_loop(const SvArgs& a) {
   }
 }
 
+// Persistent forward whose next tile streams into LDS (global_load_lds:
+// HBM -> LDS without registers) while the current tile's gates run.  LDS:
+// the re-layout tile, then the prefetch tile, then the observable table.
+// Each lane of the canonical layout moves its 16-byte amplitude pair to
+// LDS byte 16 * (pair index * NT + thread), which is byte 8 * x of the
+// amplitude at tile offset x: the prefetch tile lies in natural order, so a
+// thread reads its amplitudes under ANY layout with one 8-byte LDS read
+// each -- the pass starts directly in its first substep's layout
+// (Body::first_layout, Body::fwd0) instead of loading canonically and
+// re-laying out.  Complex64 with whole tiles only (vector >= tile).
+// SEP = false: one tile per workgroup and no separate prefetch tile -- the
+// tile streams into the re-layout region itself (read out before the first
+// re-layout), so the workgroup needs one tile of LDS and two fit per CU.
+template <typename R, int RB, int NT, typename Body, bool SEP = true>
+__device__ __forceinline__ void sv_forward_dma(const SvArgs& a) {
+  static_assert(sizeof(R) == 4, "the natural-order prefetch tile is laid out for complex64");
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NR = 1 << RB;
+  constexpr int LNT = __builtin_ctz(NT);
+  const int circ = blockIdx.y;
+  cint* P = (cint*)a.prog;
+  const int T = P[2];
+  const int ntiles = P[8];
+  C2<R>* lds = reinterpret_cast<C2<R>*>(smem);
+  unsigned char* pre = SEP ? smem + (sizeof(C2<R>) << T) : smem;
+  ObsLds* obs = reinterpret_cast<ObsLds*>(pre + (sizeof(C2<R>) << T));
+  double* ev = reinterpret_cast<double*>(obs + 1);
+  cint* pass = P + P[HDR_WORDS + a.pass];
+  cint* tbits = pass + 4;
+  const bool last = (pass[1] & PF_LAST) != 0;
+  const bool want_ev = last && a.obs_kind != JQ_OBS_NONE && a.ev_part != nullptr;
+  int tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  if (want_ev) {
+    for (int o = threadIdx.x; o < a.n_obs; o += NT) ev[o] = 0;
+    load_obs(a, obs);
+  }
+  cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride + a.pool_off;
+  const bool full = a.n_wires >= T;
+  const bool from_zero = a.src == nullptr;
+  const bool dma = full && !from_zero && a.vec_len <= (1ull << 27);  // 32-bit per-thread byte offsets
+  const C2<R>* src = from_zero ? nullptr : reinterpret_cast<const C2<R>*>(a.src) + (int64_t)circ * a.src_stride;
+  C2<R>* dst = reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride;
+  // issue the LDS loads of one tile (canonical layout: pair r of thread t)
+  auto prefetch = [&](uint64_t b) {
+    Layout Lc;
+    canonical_layout<RB, LNT>(tbits, Lc);
+    const uint32_t tb = (uint32_t)Lc.gthr * (uint32_t)sizeof(C2<R>);
+    const uint32_t wave_off = (threadIdx.x & ~63u) * 16u;
+#pragma unroll
+    for (int r = 0; r < NR / 2; ++r) {
+      const char* up = reinterpret_cast<const char*>(src + (b | reg_goff<RB>(Lc, 2 * r))) + tb;
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)up,
+                                       (JQ_LDS void*)(JQ_LDS char*)(char*)(pre + (uint32_t)r * NT * 16u + wave_off),
+                                       16, 0, 0);
+    }
+  };
+  uint64_t nbase = tile_base(tile, tbits, T, a.n_wires);
+  if (dma) prefetch(nbase);
+  for (;;) {
+    const uint64_t base = nbase;
+    R re[NR], im[NR];
+    Layout L;
+    bool started = false;  // registers already hold the first substep's layout
+    if (dma) {
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's LDS loads have landed
+      __syncthreads();                     // ... and every other wave's
+      Body::template first_layout<RB, LNT>(tbits, L);
+      const uint32_t lb = (uint32_t)(uintptr_t)(JQ_LDS char*)(char*)pre + L.toff * 8u;
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        const float2 v = *reinterpret_cast<const JQ_LDS float2*>((uintptr_t)(lb + reg_roff<RB>(L, i) * 8u));
+        re[i] = v.x;
+        im[i] = v.y;
+      }
+      __syncthreads();  // the prefetch tile is free again
+      started = true;
+    } else {
+      canonical_layout<RB, LNT>(tbits, L);
+      if (from_zero)
+        zero_state<R, RB>(re, im, L, base);
+      else
+        load_canon<R, RB>(re, im, src, a.vec_len, L, base, full);
+    }
+    const int next = SEP ? tile + (int)gridDim.x : ntiles;
+    if (next < ntiles) {
+      nbase = tile_base(next, tbits, T, a.n_wires);
+      if (dma) prefetch(nbase);
+    }
+    if (started)
+      Body::template fwd0<R, RB, NT, false>(P, a.pass, re, im, L, base, lds, pool);
+    else
+      Body::template fwd<R, RB, NT, false>(P, a.pass, re, im, L, base, lds, pool);
+    to_canonical<R, RB, NT>(re, im, tbits, L, lds);
+    store_canon<R, RB>(re, im, dst, a.vec_len, L, base, full);
+    if (want_ev) {
+      __syncthreads();
+      expval_partials<R, RB>(re, im, a, obs, L, base, ev);
+      __syncthreads();
+      for (int o = threadIdx.x; o < a.n_obs; o += NT) {
+        a.ev_part[((int64_t)circ * a.n_obs + o) * ntiles + tile] = ev[o];
+        ev[o] = 0;
+      }
+    }
+    if (next >= ntiles) break;
+    tile = next;
+  }
+}
+
 template <typename R, int RB, int NT, typename Body>
 __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
   extern __shared__ __align__(16) unsigned char smem[];
