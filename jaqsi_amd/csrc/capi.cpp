@@ -46,10 +46,15 @@ hipError_t launch_pass(const JitFn& jf, int variant, int dtype, int T, const jq:
   jq::SvArgs arg = a;
   arg.pool_off = jf.pool_off;
   void* params[] = {(void*)&arg};
-  // persistent kernels: one wave of resident workgroups over all rows
-  const int per_row = std::max(1, (jf.resident + batch - 1) / batch);
-  const int gx = jf.resident > 0 ? std::min(ntiles, per_row) : ntiles;
-  return hipModuleLaunchKernel(jf.f, gx, batch, 1, jf.nt, 1, 1, (unsigned)smem, st, params, nullptr);
+  if (jf.resident > 0) {
+    // persistent kernels: one wave of resident workgroups over the flat
+    // (circuit, tile) items of all rows
+    arg.rows = batch;
+    const int64_t items = (int64_t)batch * ntiles;
+    const int gx = (int)std::min<int64_t>(items, jf.resident);
+    return hipModuleLaunchKernel(jf.f, gx, 1, 1, jf.nt, 1, 1, (unsigned)smem, st, params, nullptr);
+  }
+  return hipModuleLaunchKernel(jf.f, ntiles, batch, 1, jf.nt, 1, 1, (unsigned)smem, st, params, nullptr);
 }
 
 // which: 0 forward pass, 1 backward pass; `full` selects the kernel variant
@@ -176,8 +181,8 @@ void jit_build(jq_plan* p) {
   spec.words_b = &p->words_bj;
   spec.dtype = p->dtype;
   p->jit.dma = spec.dma && p->dtype == 0;
-  p->jit.persist[0] = spec.persist[0];
-  p->jit.persist[1] = spec.persist[1];
+  p->jit.persist[0] = false;
+  p->jit.persist[1] = spec.persist_bwd;
   jq::jit_plan(spec, p->jit.plan);
   const int nk = (int)p->jit.plan.kernels.size();
   JitCache& C = jit_cache();

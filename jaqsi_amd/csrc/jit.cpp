@@ -281,7 +281,7 @@ std::string jit_key(const JitSpec& spec, const JitPlan& plan, int id) {
   const JitPlan::Kernel& k = plan.kernels[id];
   const ProgView pv = view(k.dir == 0 ? *spec.words_f : *spec.words_b);
   return std::to_string(k.dir) + " " + std::to_string(spec.dtype) + " " + std::to_string(pv.T) + " " +
-         std::to_string(pv.RB) + " " + std::to_string(spec.persist[k.dir] ? 1 : 0) + " " +
+         std::to_string(pv.RB) + " " + std::to_string(k.dir == 1 && spec.persist_bwd ? 1 : 0) + " " +
          std::to_string(spec.dma ? 1 : 0) + "\n" + k.body;
 }
 
@@ -318,9 +318,9 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
           << "(SvArgs a) {\n  sv_forward_dma<" << rt << ", " << pv.RB << ", " << nt << ", JF" << id
           << ", false>(a);\n}\n\n";
       else
-        o << "extern \"C\" __global__ __launch_bounds__(" << nt << ", " << (spec.persist[0] ? 2 : occ) << ") void "
-          << kname << "(SvArgs a) {\n  " << (spec.persist[0] ? "sv_forward_loop<" : "sv_forward_impl<") << rt
-          << ", " << pv.RB << ", " << nt << (spec.persist[0] ? "" : ", false") << ", JF" << id << ">(a);\n}\n\n";
+        o << "extern \"C\" __global__ __launch_bounds__(" << nt << ", " << occ << ") void " << kname
+          << "(SvArgs a) {\n  sv_forward_impl<" << rt << ", " << pv.RB << ", " << nt << ", false, JF" << id
+          << ">(a);\n}\n\n";
     } else {
       o << "struct JB" << id << " {\n"
         << "  template <typename R, int RB, int NT, bool FULL>\n"
@@ -329,8 +329,8 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
         << "    constexpr int LNT = __builtin_ctz(NT);\n"
         << k.body << "    (void)lds; (void)base; (void)pool; (void)slots;\n  }\n};\n"
         << "extern \"C\" __global__ __launch_bounds__(" << nt << ") void " << kname << "(SvArgs a) {\n  "
-        << (spec.persist[1] ? "sv_backward_loop<" : "sv_backward_impl<") << rt << ", " << pv.RB << ", " << nt
-        << (spec.persist[1] ? "" : ", false") << ", JB" << id << ">(a);\n}\n\n";
+        << (spec.persist_bwd ? "sv_backward_loop<" : "sv_backward_impl<") << rt << ", " << pv.RB << ", " << nt
+        << (spec.persist_bwd ? "" : ", false") << ", JB" << id << ">(a);\n}\n\n";
     }
   }
   o << "}  // namespace jq\n";

@@ -23,12 +23,11 @@ struct JitSpec {
   const std::vector<int32_t>* words_f = nullptr;  // forward program
   const std::vector<int32_t>* words_b = nullptr;  // backward program
   int dtype = 0;                                  // 0 complex64, 1 complex128
-  // Persistent tile loop with next-tile prefetch, per direction.  Measured
-  // on the 24-qubit workloads: it shortens the backward (one resident
-  // workgroup per CU, so the loads would otherwise not overlap any work) and
-  // lengthens the forward (its prefetch registers halve the resident
-  // workgroups).
-  bool persist[2] = {false, true};
+  // Backward: persistent loop over (circuit, tile) items with next-item
+  // prefetch (one resident workgroup per CU, so the loads would otherwise not
+  // overlap any work).  A persistent forward measured slower: its prefetch
+  // registers halve the resident workgroups.
+  bool persist_bwd = true;
   // Forward (complex64): the tile streams into LDS and the pass starts in its
   // first substep's layout (sv_forward_dma, one tile per workgroup so two
   // workgroups fit per CU; a persistent variant with a separate prefetch

@@ -1774,80 +1774,12 @@ __global__ __launch_bounds__(NT) void sv_backward_kernel(SvArgs a0) {
 // ---------------------------------------------------------------------------
 // persistent pass kernels
 // ---------------------------------------------------------------------------
-// A workgroup walks the tiles blockIdx.x, blockIdx.x + gridDim.x, ... of its
-// circuit row (single-plan launches).  The HBM loads of the next tile are
+// A workgroup walks the work items blockIdx.x, blockIdx.x + gridDim.x, ...
+// (single-plan launches: (circuit, tile) pairs of all rows).  The HBM loads of the next tile are
 // issued into a second register set before the current tile's gates run;
 // barriers inside a pass wait on LDS only, so the loads stay in flight
 // behind the gate arithmetic and the relayouts.  Results and partial sums
 // are those of the one-tile-per-workgroup kernels.
-template <typename R, int RB, int NT, typename Body>
-__device__ __forceinline__ void sv_forward_loop(const SvArgs& a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int NR = 1 << RB;
-  constexpr int LNT = __builtin_ctz(NT);
-  const int circ = blockIdx.y;
-  cint* P = (cint*)a.prog;
-  const int T = P[2];
-  const int ntiles = P[8];
-  C2<R>* lds = reinterpret_cast<C2<R>*>(smem);
-  ObsLds* obs = reinterpret_cast<ObsLds*>(smem + (sizeof(C2<R>) << T));
-  double* ev = reinterpret_cast<double*>(obs + 1);
-  cint* pass = P + P[HDR_WORDS + a.pass];
-  cint* tbits = pass + 4;
-  const bool last = (pass[1] & PF_LAST) != 0;
-  const bool want_ev = last && a.obs_kind != JQ_OBS_NONE && a.ev_part != nullptr;
-  int tile = blockIdx.x;
-  if (tile >= ntiles) return;
-  if (want_ev) {
-    for (int o = threadIdx.x; o < a.n_obs; o += NT) ev[o] = 0;
-    load_obs(a, obs);
-  }
-  cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride + a.pool_off;
-  const bool full = a.n_wires >= T;
-  const bool from_zero = a.src == nullptr;
-  const C2<R>* src = from_zero ? nullptr : reinterpret_cast<const C2<R>*>(a.src) + (int64_t)circ * a.src_stride;
-  C2<R>* dst = reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride;
-  Layout Lc;
-  canonical_layout<RB, LNT>(tbits, Lc);
-  R nre[NR], nim[NR];
-  uint64_t nbase = tile_base(tile, tbits, T, a.n_wires);
-  if (!from_zero) load_canon<R, RB>(nre, nim, src, a.vec_len, Lc, nbase, full);
-  for (;;) {
-    const uint64_t base = nbase;
-    R re[NR], im[NR];
-    Layout L;
-    canonical_layout<RB, LNT>(tbits, L);
-    if (from_zero) {
-      zero_state<R, RB>(re, im, L, base);
-    } else {
-#pragma unroll
-      for (int i = 0; i < NR; ++i) {
-        re[i] = nre[i];
-        im[i] = nim[i];
-      }
-    }
-    const int next = tile + gridDim.x;
-    if (next < ntiles) {
-      nbase = tile_base(next, tbits, T, a.n_wires);
-      if (!from_zero) load_canon<R, RB>(nre, nim, src, a.vec_len, Lc, nbase, full);
-    }
-    Body::template fwd<R, RB, NT, false>(P, a.pass, re, im, L, base, lds, pool);
-    to_canonical<R, RB, NT>(re, im, tbits, L, lds);
-    store_canon<R, RB>(re, im, dst, a.vec_len, L, base, full);
-    if (want_ev) {
-      __syncthreads();
-      expval_partials<R, RB>(re, im, a, obs, L, base, ev);
-      __syncthreads();
-      for (int o = threadIdx.x; o < a.n_obs; o += NT) {
-        a.ev_part[((int64_t)circ * a.n_obs + o) * ntiles + tile] = ev[o];
-        ev[o] = 0;
-      }
-    }
-    if (next >= ntiles) break;
-    tile = next;
-  }
-}
-
 // Persistent forward whose next tile streams into LDS (global_load_lds:
 // HBM -> LDS without registers) while the current tile's gates run.  LDS:
 // the re-layout tile, then the prefetch tile, then the observable table.
@@ -1962,10 +1894,10 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NR = 1 << RB;
   constexpr int LNT = __builtin_ctz(NT);
-  const int circ = blockIdx.y;
   cint* P = (cint*)a.prog;
   const int T = P[2];
   const int ntiles = P[8];
+  const int tlog = __builtin_ctz((unsigned)ntiles);
   C2<R>* lds = reinterpret_cast<C2<R>*>(smem);  // psi and lambda tiles (relayout2)
   ObsLds* obs = reinterpret_cast<ObsLds*>(smem + (2 * sizeof(C2<R>) << T));
   double* slots = reinterpret_cast<double*>(obs + 1);
@@ -1974,20 +1906,24 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
   const bool seed = (pass[1] & PF_LAST) != 0;
   const int nslots = pass[2];
   const int slot_base = pass[3];
-  int tile = blockIdx.x;
-  if (tile >= ntiles) return;
+  // work items w = circuit * ntiles + tile over all rows of the launch, so
+  // the resident workgroups balance across circuits when a row has few tiles
+  const int64_t total = (int64_t)a.rows << tlog;
+  int64_t w = blockIdx.x;
+  if (w >= total) return;
   load_obs(a, obs);
-  cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride + a.pool_off;
   const bool full = a.n_wires >= T;
-  const C2<R>* ps = reinterpret_cast<const C2<R>*>(a.src) + (int64_t)circ * a.src_stride;
-  C2<R>* lam = reinterpret_cast<C2<R>*>(a.lam) + (int64_t)circ * a.lam_stride;
+  const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src);
+  C2<R>* lam = reinterpret_cast<C2<R>*>(a.lam);
   Layout Lc;
   canonical_layout<RB, LNT>(tbits, Lc);
   R npr[NR], npi[NR], nlr[NR], nli[NR];
-  uint64_t nbase = tile_base(tile, tbits, T, a.n_wires);
-  load_canon<R, RB>(npr, npi, ps, a.vec_len, Lc, nbase, full);
-  if (!seed) load_canon<R, RB>(nlr, nli, lam, a.vec_len, Lc, nbase, full);
+  int ncirc = (int)(w >> tlog), ntile = (int)(w & (ntiles - 1));
+  uint64_t nbase = tile_base(ntile, tbits, T, a.n_wires);
+  load_canon<R, RB>(npr, npi, src + (int64_t)ncirc * a.src_stride, a.vec_len, Lc, nbase, full);
+  if (!seed) load_canon<R, RB>(nlr, nli, lam + (int64_t)ncirc * a.lam_stride, a.vec_len, Lc, nbase, full);
   for (;;) {
+    const int circ = ncirc, tile = ntile;
     const uint64_t base = nbase;
     R pr[NR], pi[NR], lr[NR], li[NR];
 #pragma unroll
@@ -2002,20 +1938,23 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
         li[i] = nli[i];
       }
     }
-    const int next = tile + gridDim.x;
-    if (next < ntiles) {
-      nbase = tile_base(next, tbits, T, a.n_wires);
-      load_canon<R, RB>(npr, npi, ps, a.vec_len, Lc, nbase, full);
-      if (!seed) load_canon<R, RB>(nlr, nli, lam, a.vec_len, Lc, nbase, full);
+    const int64_t next = w + gridDim.x;
+    if (next < total) {
+      ncirc = (int)(next >> tlog);
+      ntile = (int)(next & (ntiles - 1));
+      nbase = tile_base(ntile, tbits, T, a.n_wires);
+      load_canon<R, RB>(npr, npi, src + (int64_t)ncirc * a.src_stride, a.vec_len, Lc, nbase, full);
+      if (!seed) load_canon<R, RB>(nlr, nli, lam + (int64_t)ncirc * a.lam_stride, a.vec_len, Lc, nbase, full);
     }
     for (int e = threadIdx.x; e < nslots; e += NT) slots[e] = 0;
     __syncthreads();
+    cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride + a.pool_off;
     Layout L;
     canonical_layout<RB, LNT>(tbits, L);
     if (seed) seed_lambda<R, RB>(pr, pi, lr, li, a, obs, circ, L, base);
     Body::template bwd<R, RB, NT, false>(P, a.pass, pr, pi, lr, li, L, base, lds, pool, slots);
     to_canonical2<R, RB, NT>(pr, pi, lr, li, tbits, L, lds);
-    store_canon<R, RB>(lr, li, lam, a.vec_len, L, base, full);
+    store_canon<R, RB>(lr, li, lam + (int64_t)circ * a.lam_stride, a.vec_len, L, base, full);
     if (a.dst != nullptr)
       store_canon<R, RB>(pr, pi, reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride, a.vec_len, L, base,
                          full);
@@ -2023,8 +1962,8 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
     // partials are real-major ([real][tile]) so the reduction reads them contiguously
     double* gp = a.grad_part + (int64_t)circ * a.part_stride + slot_base + tile;
     for (int e = threadIdx.x; e < nslots; e += NT) gp[(int64_t)e * ntiles] = slots[e];
-    if (next >= ntiles) break;
-    tile = next;
+    if (next >= total) break;
+    w = next;
     __syncthreads();  // the slots are read above before the next tile clears them
   }
 }

@@ -39,6 +39,7 @@ struct SvArgs {
   uint64_t vec_len;    // 2^n_wires
   const void* mats;    // block matrices [batch][pool_stride] (plan dtype)
   int64_t pool_stride;
+  int rows;            // persistent backward: circuits of the launch (grid is flat)
   int64_t pool_off;    // element offset added to every matrix reference of the launch
                        // (0 for the program interpreter; plan-specialised kernels are
                        // shared by passes that differ only by this offset)
