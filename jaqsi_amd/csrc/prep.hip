@@ -168,6 +168,95 @@ __device__ bool invert_d(const cd* A, cd* Ainv, int D) {
   return true;
 }
 
+// Effective matrix of a member with compile-time storage: DIAG keeps the
+// diagonal (DM >= D entries used), dense blocks use D == DM (row stride DM),
+// so every index is a constant and the arrays stay in registers.
+template <bool DIAG, int DM>
+__device__ __forceinline__ void member_eff_t(const int* mt, double th, const cd* xm, int D, int offs, bool deriv,
+                                             cd* E) {
+  if constexpr (DIAG) {
+#pragma unroll
+    for (int r = 0; r < DM; ++r)
+      if (r < D) E[r] = member_entry(mt, th, xm, offs + r, offs + r, deriv);
+  } else {
+#pragma unroll
+    for (int r = 0; r < DM; ++r)
+#pragma unroll
+      for (int c = 0; c < DM; ++c) E[r * DM + c] = member_entry(mt, th, xm, offs + r, offs + c, deriv);
+  }
+}
+
+// M <- E M (dense, D == DM) or elementwise (diagonal)
+template <bool DIAG, int DM>
+__device__ __forceinline__ void left_mul_t(const cd* E, cd* M, int D) {
+  if constexpr (DIAG) {
+#pragma unroll
+    for (int r = 0; r < DM; ++r)
+      if (r < D) M[r] = cmul(E[r], M[r]);
+  } else {
+    cd T[DM * DM];
+#pragma unroll
+    for (int r = 0; r < DM; ++r)
+#pragma unroll
+      for (int c = 0; c < DM; ++c) {
+        cd acc = {0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < DM; ++k) acc = cadd(acc, cmul(E[r * DM + k], M[k * DM + c]));
+        T[r * DM + c] = acc;
+      }
+#pragma unroll
+    for (int e = 0; e < DM * DM; ++e) M[e] = T[e];
+  }
+}
+
+template <bool DIAG, int DM>
+__device__ __forceinline__ void identity_t(cd* M) {
+#pragma unroll
+  for (int e = 0; e < (DIAG ? DM : DM * DM); ++e)
+    M[e] = {DIAG ? 1.0 : ((e / DM) == (e % DM) ? 1.0 : 0.0), 0.0};
+}
+
+template <typename R, bool DIAG, int DM>
+__device__ __forceinline__ void prep_block(const PrepArgs& a, int circ, const int* bt) {
+  constexpr int N = DIAG ? DM : DM * DM;
+  const int D = bt[B_D], offs = bt[B_OFFS];
+  const int ne = DIAG ? D : D * D;
+  const double* angles = a.angles + (int64_t)circ * a.n_params;
+  const cd* xm = reinterpret_cast<const cd*>(a.matrices) + (a.matrices ? (int64_t)circ * a.n_matrix_elems : 0);
+  cd M[N], E[N];
+  identity_t<DIAG, DM>(M);
+  for (int mi = bt[B_MBEG]; mi < bt[B_MEND]; ++mi) {
+    const int m = a.member_list[mi];
+    const int* mt = a.member_table + m * MW;
+    const double th = member_theta(mt, a.member_values[m], angles);
+    member_eff_t<DIAG, DM>(mt, th, xm, D, offs, false, E);
+    left_mul_t<DIAG, DM>(E, M, D);
+  }
+  Out2<R>* pool = reinterpret_cast<Out2<R>*>(a.pool) + (int64_t)circ * a.pool_stride;
+#pragma unroll
+  for (int e = 0; e < N; ++e)
+    if (e < ne) pool[bt[B_MAT] + e] = {(R)M[e].x, (R)M[e].y};
+  if (bt[B_INV] >= 0) {
+    cd T[N];
+    bool ok = true;
+    if constexpr (DIAG) {
+#pragma unroll
+      for (int r = 0; r < DM; ++r) {
+        if (r >= D) continue;
+        const double den = M[r].x * M[r].x + M[r].y * M[r].y;
+        if (!(den > 1e-300)) ok = false;
+        T[r] = {M[r].x / den, -M[r].y / den};
+      }
+    } else {
+      ok = invert_d(M, T, DM);
+    }
+    if (!ok) atomicOr(a.status, 1);
+#pragma unroll
+    for (int e = 0; e < N; ++e)
+      if (e < ne) pool[bt[B_INV] + e] = {(R)T[e].x, (R)T[e].y};
+  }
+}
+
 template <typename R>
 __global__ void prep_kernel(PrepArgs a) {
   const int nb = a.n_blocks;
@@ -176,40 +265,18 @@ __global__ void prep_kernel(PrepArgs a) {
   const int circ = (int)(gid / nb);
   const int b = (int)(gid % nb);
   const int* bt = a.block_table + b * BW;
-  const int D = bt[B_D], offs = bt[B_OFFS];
-  const bool diagf = bt[B_DIAGF] != 0;
-  const double* angles = a.angles + (int64_t)circ * a.n_params;
-  const cd* xm = reinterpret_cast<const cd*>(a.matrices) + (a.matrices ? (int64_t)circ * a.n_matrix_elems : 0);
-  cd M[64], E[64], T[64];
-  const int ne = diagf ? D : D * D;
-  for (int e = 0; e < ne; ++e) M[e] = {diagf ? 1.0 : ((e / D) == (e % D) ? 1.0 : 0.0), 0.0};
-  for (int mi = bt[B_MBEG]; mi < bt[B_MEND]; ++mi) {
-    const int m = a.member_list[mi];
-    const int* mt = a.member_table + m * MW;
-    const double th = member_theta(mt, a.member_values[m], angles);
-    member_eff(mt, th, xm, D, offs, diagf, false, E);
-    if (diagf) {
-      for (int r = 0; r < D; ++r) M[r] = cmul(E[r], M[r]);
-    } else {
-      matmul_d(E, M, T, D);
-      for (int e = 0; e < ne; ++e) M[e] = T[e];
-    }
-  }
-  Out2<R>* pool = reinterpret_cast<Out2<R>*>(a.pool) + (int64_t)circ * a.pool_stride;
-  for (int e = 0; e < ne; ++e) pool[bt[B_MAT] + e] = {(R)M[e].x, (R)M[e].y};
-  if (bt[B_INV] >= 0) {
-    bool ok = true;
-    if (diagf) {
-      for (int r = 0; r < D; ++r) {
-        const double den = M[r].x * M[r].x + M[r].y * M[r].y;
-        if (!(den > 1e-300)) ok = false;
-        T[r] = {M[r].x / den, -M[r].y / den};
-      }
-    } else {
-      ok = invert_d(M, T, D);
-    }
-    if (!ok) atomicOr(a.status, 1);
-    for (int e = 0; e < ne; ++e) pool[bt[B_INV] + e] = {(R)T[e].x, (R)T[e].y};
+  const int D = bt[B_D];
+  if (bt[B_DIAGF] != 0) {
+    if (D <= 4)
+      prep_block<R, true, 4>(a, circ, bt);
+    else
+      prep_block<R, true, 32>(a, circ, bt);
+  } else if (D == 2) {
+    prep_block<R, false, 2>(a, circ, bt);
+  } else if (D == 4) {
+    prep_block<R, false, 4>(a, circ, bt);
+  } else {
+    prep_block<R, false, 8>(a, circ, bt);
   }
 }
 
@@ -248,6 +315,26 @@ __global__ void reduce_grad_kernel(ReduceArgs a) {
   if ((threadIdx.x & 63) == 0) a.bars[(int64_t)circ * a.n_reals + e] = s;
 }
 
+// The same reduction with one thread per output real, summing its tiles in
+// order: for circuits of few tiles a wave per output would mostly idle.
+__global__ void reduce_grad_thread_kernel(ReduceArgs a) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (int64_t)a.n_reals * a.batch) return;
+  const int circ = (int)(id / a.n_reals);
+  const int e = (int)(id % a.n_reals);
+  const int* sm = a.slot_map;
+  int lo = 0, hi = a.n_slots - 1;
+  while (lo < hi) {  // entries are sorted by bar offset
+    const int mid = (lo + hi + 1) / 2;
+    if (sm[mid * 5 + 3] <= e) lo = mid; else hi = mid - 1;
+  }
+  const int base = sm[lo * 5 + 0], local = sm[lo * 5 + 2], bar = sm[lo * 5 + 3];
+  const double* row = a.partials + (int64_t)circ * a.part_stride + base + (int64_t)(local + (e - bar)) * a.ntiles;
+  double s = 0;
+  for (int t = 0; t < a.ntiles; ++t) s += row[t];
+  a.bars[(int64_t)circ * a.n_reals + e] = s;
+}
+
 // partials: [batch][n_obs][ntiles]
 __global__ void reduce_ev_kernel(const double* part, double* out, int ntiles, int n_obs, int batch) {
   const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -257,25 +344,21 @@ __global__ void reduce_ev_kernel(const double* part, double* out, int ntiles, in
 }
 
 // block cotangent -> member cotangents W_j = L_j^T bar R_j^T -> Re sum W . dE
-__global__ void chain_kernel(ChainArgs a) {
-  const int nb = a.n_blocks;
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (int64_t)nb * a.batch) return;
-  const int circ = (int)(gid / nb);
-  const int b = (int)(gid % nb);
-  const int* bt = a.block_table + b * BW;
-  if (bt[B_BAR] < 0) return;
+template <bool DIAG, int DM>
+__device__ __forceinline__ void chain_block(const ChainArgs& a, int circ, const int* bt) {
+  constexpr int N = DIAG ? DM : DM * DM;
   const int D = bt[B_D], offs = bt[B_OFFS];
-  const bool diagf = bt[B_DIAGF] != 0;
-  const int ne = diagf ? D : D * D;
+  const int ne = DIAG ? D : D * D;
   const double* angles = a.angles + (int64_t)circ * a.n_params;
   const cd* xm = reinterpret_cast<const cd*>(a.matrices) + (a.matrices ? (int64_t)circ * a.n_matrix_elems : 0);
   const cd* bar = reinterpret_cast<const cd*>(a.bars) + (int64_t)circ * a.bars_stride + bt[B_BAR];
   double* gang = a.grad_angles ? a.grad_angles + (int64_t)circ * a.n_params : nullptr;
   cd* gmat = a.grad_matrices ? reinterpret_cast<cd*>(a.grad_matrices) + (int64_t)circ * a.n_matrix_elems : nullptr;
   const int mb = bt[B_MBEG], me = bt[B_MEND];
-  cd Rp[64], Lm[64], E[64], T[64], W[64];
-  for (int e = 0; e < ne; ++e) Rp[e] = {diagf ? 1.0 : ((e / D) == (e % D) ? 1.0 : 0.0), 0.0};
+  cd Rp[N], Lm[N], E[N], W[N], B[N];
+#pragma unroll
+  for (int e = 0; e < N; ++e) B[e] = e < ne ? bar[e] : cd{0.0, 0.0};
+  identity_t<DIAG, DM>(Rp);
   for (int j = mb; j < me; ++j) {
     const int m = a.member_list[j];
     const int* mt = a.member_table + m * MW;
@@ -284,47 +367,56 @@ __global__ void chain_kernel(ChainArgs a) {
     const bool want_mat = mt[M_KIND] == JQ_M_MATRIX && (mt[M_FLAGS] & JQ_MF_GRAD) && gmat != nullptr;
     if (want_theta || want_mat) {
       // L = E_last ... E_{j+1}
-      for (int e = 0; e < ne; ++e) Lm[e] = {diagf ? 1.0 : ((e / D) == (e % D) ? 1.0 : 0.0), 0.0};
+      identity_t<DIAG, DM>(Lm);
       for (int q = j + 1; q < me; ++q) {
         const int mq = a.member_list[q];
         const int* mtq = a.member_table + mq * MW;
-        member_eff(mtq, member_theta(mtq, a.member_values[mq], angles), xm, D, offs, diagf, false, E);
-        if (diagf) {
-          for (int r = 0; r < D; ++r) Lm[r] = cmul(E[r], Lm[r]);
-        } else {
-          matmul_d(E, Lm, T, D);
-          for (int e = 0; e < ne; ++e) Lm[e] = T[e];
-        }
+        member_eff_t<DIAG, DM>(mtq, member_theta(mtq, a.member_values[mq], angles), xm, D, offs, false, E);
+        left_mul_t<DIAG, DM>(E, Lm, D);
       }
       // W = L^T bar R^T
-      if (diagf) {
-        for (int r = 0; r < D; ++r) W[r] = cmul(cmul(Lm[r], bar[r]), Rp[r]);
+      if constexpr (DIAG) {
+#pragma unroll
+        for (int r = 0; r < DM; ++r)
+          if (r < D) W[r] = cmul(cmul(Lm[r], B[r]), Rp[r]);
       } else {
-        for (int r = 0; r < D; ++r)
-          for (int c = 0; c < D; ++c) {
+        cd T[N];
+#pragma unroll
+        for (int r = 0; r < DM; ++r)
+#pragma unroll
+          for (int c = 0; c < DM; ++c) {
             cd acc = {0.0, 0.0};
-            for (int k = 0; k < D; ++k) acc = cadd(acc, cmul(Lm[k * D + r], bar[k * D + c]));
-            T[r * D + c] = acc;
+#pragma unroll
+            for (int k = 0; k < DM; ++k) acc = cadd(acc, cmul(Lm[k * DM + r], B[k * DM + c]));
+            T[r * DM + c] = acc;
           }
-        for (int r = 0; r < D; ++r)
-          for (int c = 0; c < D; ++c) {
+#pragma unroll
+        for (int r = 0; r < DM; ++r)
+#pragma unroll
+          for (int c = 0; c < DM; ++c) {
             cd acc = {0.0, 0.0};
-            for (int k = 0; k < D; ++k) acc = cadd(acc, cmul(T[r * D + k], Rp[c * D + k]));
-            W[r * D + c] = acc;
+#pragma unroll
+            for (int k = 0; k < DM; ++k) acc = cadd(acc, cmul(T[r * DM + k], Rp[c * DM + k]));
+            W[r * DM + c] = acc;
           }
       }
       if (want_theta) {
-        member_eff(mt, th, xm, D, offs, diagf, true, E);
+        member_eff_t<DIAG, DM>(mt, th, xm, D, offs, true, E);
         double g = 0;
-        for (int e = 0; e < ne; ++e) g += W[e].x * E[e].x - W[e].y * E[e].y;
+#pragma unroll
+        for (int e = 0; e < N; ++e)
+          if (e < ne) g += W[e].x * E[e].x - W[e].y * E[e].y;
         atomicAdd(&gang[mt[M_PARAM]], g);
       }
       if (want_mat) {
         const int dim = 1 << mt[M_KB];
         // explicit matrices are unstructured: D == dim, offs == 0
-        for (int r = 0; r < D; ++r)
-          for (int c = 0; c < D; ++c) {
-            const cd w = diagf ? (r == c ? W[r] : cd{0.0, 0.0}) : W[r * D + c];
+#pragma unroll
+        for (int r = 0; r < DM; ++r)
+#pragma unroll
+          for (int c = 0; c < DM; ++c) {
+            if (r >= D || c >= D) continue;
+            const cd w = DIAG ? (r == c ? W[r] : cd{0.0, 0.0}) : W[r * DM + c];
             cd* dst = &gmat[mt[M_MATRIX] + r * dim + c];
             atomicAdd(&dst->x, w.x);
             atomicAdd(&dst->y, w.y);
@@ -332,13 +424,31 @@ __global__ void chain_kernel(ChainArgs a) {
       }
     }
     // R <- E_j R
-    member_eff(mt, th, xm, D, offs, diagf, false, E);
-    if (diagf) {
-      for (int r = 0; r < D; ++r) Rp[r] = cmul(E[r], Rp[r]);
-    } else {
-      matmul_d(E, Rp, T, D);
-      for (int e = 0; e < ne; ++e) Rp[e] = T[e];
-    }
+    member_eff_t<DIAG, DM>(mt, th, xm, D, offs, false, E);
+    left_mul_t<DIAG, DM>(E, Rp, D);
+  }
+}
+
+__global__ void chain_kernel(ChainArgs a) {
+  const int nb = a.n_blocks;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (int64_t)nb * a.batch) return;
+  const int circ = (int)(gid / nb);
+  const int b = (int)(gid % nb);
+  const int* bt = a.block_table + b * BW;
+  if (bt[B_BAR] < 0) return;
+  const int D = bt[B_D];
+  if (bt[B_DIAGF] != 0) {
+    if (D <= 4)
+      chain_block<true, 4>(a, circ, bt);
+    else
+      chain_block<true, 32>(a, circ, bt);
+  } else if (D == 2) {
+    chain_block<false, 2>(a, circ, bt);
+  } else if (D == 4) {
+    chain_block<false, 4>(a, circ, bt);
+  } else {
+    chain_block<false, 8>(a, circ, bt);
   }
 }
 
@@ -476,7 +586,10 @@ hipError_t launch_prep(int dtype, const PrepArgs& a, hipStream_t st) {
 hipError_t launch_reduce_grad(const ReduceArgs& a, hipStream_t st) {
   const int64_t n = (int64_t)a.n_reals * a.batch;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(reduce_grad_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, a);
+  if (a.ntiles < 64)  // few tiles per circuit (many small circuits): one thread per output
+    hipLaunchKernelGGL(reduce_grad_thread_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(reduce_grad_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
