@@ -145,6 +145,7 @@ struct jq_plan {
   std::vector<int32_t> words_bj;  // backward program of the plan-specialised kernels
   DevBuf prog_bj;
   bool uploaded = false;
+  int class_mask = 0;  // block classes of the prep / chain kernels (jq::block_class)
   JitState jit;
   // workspaces
   DevBuf pool, partials, bars, ev_part, lam, ckpt, status, obs, fstate, groups;
@@ -408,6 +409,8 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   }
   p->n_slots = (int)smv.size();
   p->slot_words = sm;
+  for (size_t r = 0; r + jq::BW <= p->hp.block_table.size(); r += jq::BW)
+    p->class_mask |= 1 << jq::block_class(p->hp.block_table[r + jq::B_DIAGF] != 0, p->hp.block_table[r + jq::B_D]);
   int ndev = 0;
   if (jit_on && hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) {  // nothing to run them on otherwise
     p->jit.mode = JQ_JIT_ASYNC;
@@ -562,6 +565,7 @@ int run_prep(jq_plan* p, int batch, const double* angles, const void* matrices, 
   pa.pool = p->pool.p;
   pa.pool_stride = p->hp.pool;
   pa.status = (int*)p->status.p;
+  pa.class_mask = p->class_mask;
   if (p->n_params > 0 && !angles) return fail(JQ_ERR_ARG, "angles required");
   if (p->n_matrix_elems > 0 && !matrices) return fail(JQ_ERR_ARG, "explicit matrices required");
   HIP_TRY(hipMemsetAsync(p->status.p, 0, sizeof(int), st));
@@ -629,6 +633,7 @@ int finish_grads(jq_plan* p, int batch, const double* angles, const void* matric
     c.bars_stride = p->hp.bars;
     c.grad_angles = grad_angles;
     c.grad_matrices = grad_matrices;
+    c.class_mask = p->class_mask;
     if (grad_angles || grad_matrices) HIP_TRY(jq::launch_chain(c, st));
   }
   return JQ_OK;

@@ -18,7 +18,13 @@ struct PrepArgs {
   void* pool;               // [batch][pool_stride] plan dtype
   int64_t pool_stride;
   int* status;              // bit 0: singular non-unitary block
+  int class_mask;           // block classes present (block_class), one kernel instance each
 };
+
+// Block classes of the prep / chain kernels, each compiled with matrices of
+// a fixed size: 0 diagonal <= 4, 1 diagonal <= 32, 2 dense 2x2, 3 dense 4x4,
+// 4 dense 8x8.
+__host__ __device__ inline int block_class(bool diagf, int D) { return diagf ? (D <= 4 ? 0 : 1) : (D == 2 ? 2 : D == 4 ? 3 : 4); }
 
 struct ReduceArgs {
   const int* slot_map;      // n_slots x {base, nslots, local, bar, size}, sorted by bar
@@ -45,6 +51,7 @@ struct ChainArgs {
   int64_t bars_stride;      // complex elements
   double* grad_angles;      // [batch][n_params], pre-zeroed
   void* grad_matrices;      // [batch][n_matrix_elems] complex128, pre-zeroed, may be null
+  int class_mask;           // block classes present (block_class)
 };
 
 struct ExpvalArgs {

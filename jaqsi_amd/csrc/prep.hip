@@ -257,7 +257,10 @@ __device__ __forceinline__ void prep_block(const PrepArgs& a, int circ, const in
   }
 }
 
-template <typename R>
+// One instance per block class (launch_prep starts those the plan holds):
+// a kernel compiled for one matrix size keeps it in registers, and the
+// scratch of the large classes is not reserved for the common small ones.
+template <typename R, int CLS>
 __global__ void prep_kernel(PrepArgs a) {
   const int nb = a.n_blocks;
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -265,19 +268,10 @@ __global__ void prep_kernel(PrepArgs a) {
   const int circ = (int)(gid / nb);
   const int b = (int)(gid % nb);
   const int* bt = a.block_table + b * BW;
-  const int D = bt[B_D];
-  if (bt[B_DIAGF] != 0) {
-    if (D <= 4)
-      prep_block<R, true, 4>(a, circ, bt);
-    else
-      prep_block<R, true, 32>(a, circ, bt);
-  } else if (D == 2) {
-    prep_block<R, false, 2>(a, circ, bt);
-  } else if (D == 4) {
-    prep_block<R, false, 4>(a, circ, bt);
-  } else {
-    prep_block<R, false, 8>(a, circ, bt);
-  }
+  if (block_class(bt[B_DIAGF] != 0, bt[B_D]) != CLS) return;
+  constexpr bool DIAG = CLS <= 1;
+  constexpr int DM = CLS == 0 ? 4 : CLS == 1 ? 32 : CLS == 2 ? 2 : CLS == 3 ? 4 : 8;
+  prep_block<R, DIAG, DM>(a, circ, bt);
 }
 
 // sum over tiles of each cotangent slot -> bars (FP64 reals)
@@ -429,6 +423,7 @@ __device__ __forceinline__ void chain_block(const ChainArgs& a, int circ, const 
   }
 }
 
+template <int CLS>
 __global__ void chain_kernel(ChainArgs a) {
   const int nb = a.n_blocks;
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -436,20 +431,10 @@ __global__ void chain_kernel(ChainArgs a) {
   const int circ = (int)(gid / nb);
   const int b = (int)(gid % nb);
   const int* bt = a.block_table + b * BW;
-  if (bt[B_BAR] < 0) return;
-  const int D = bt[B_D];
-  if (bt[B_DIAGF] != 0) {
-    if (D <= 4)
-      chain_block<true, 4>(a, circ, bt);
-    else
-      chain_block<true, 32>(a, circ, bt);
-  } else if (D == 2) {
-    chain_block<false, 2>(a, circ, bt);
-  } else if (D == 4) {
-    chain_block<false, 4>(a, circ, bt);
-  } else {
-    chain_block<false, 8>(a, circ, bt);
-  }
+  if (bt[B_BAR] < 0 || block_class(bt[B_DIAGF] != 0, bt[B_D]) != CLS) return;
+  constexpr bool DIAG = CLS <= 1;
+  constexpr int DM = CLS == 0 ? 4 : CLS == 1 ? 32 : CLS == 2 ? 2 : CLS == 3 ? 4 : 8;
+  chain_block<DIAG, DM>(a, circ, bt);
 }
 
 // ---------------------------------------------------------------------------
@@ -576,10 +561,15 @@ hipError_t launch_prep(int dtype, const PrepArgs& a, hipStream_t st) {
   if (n == 0) return hipSuccess;
   const int tpb = 64;
   dim3 grid((unsigned)((n + tpb - 1) / tpb));
-  if (dtype == 0)
-    hipLaunchKernelGGL(prep_kernel<float>, grid, dim3(tpb), 0, st, a);
-  else
-    hipLaunchKernelGGL(prep_kernel<double>, grid, dim3(tpb), 0, st, a);
+#define JQ_PREP(C)                                                                      \
+  if (a.class_mask & (1 << C)) {                                                      \
+    if (dtype == 0)                                                                   \
+      hipLaunchKernelGGL((prep_kernel<float, C>), grid, dim3(tpb), 0, st, a);         \
+    else                                                                              \
+      hipLaunchKernelGGL((prep_kernel<double, C>), grid, dim3(tpb), 0, st, a);        \
+  }
+  JQ_PREP(0) JQ_PREP(1) JQ_PREP(2) JQ_PREP(3) JQ_PREP(4)
+#undef JQ_PREP
   return hipGetLastError();
 }
 
@@ -604,7 +594,12 @@ hipError_t launch_reduce_ev(const double* part, double* out, int ntiles, int n_o
 hipError_t launch_chain(const ChainArgs& a, hipStream_t st) {
   const int64_t n = (int64_t)a.n_blocks * a.batch;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(chain_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, a);
+  const dim3 grid((unsigned)((n + 63) / 64));
+  if (a.class_mask & 1) hipLaunchKernelGGL(chain_kernel<0>, grid, dim3(64), 0, st, a);
+  if (a.class_mask & 2) hipLaunchKernelGGL(chain_kernel<1>, grid, dim3(64), 0, st, a);
+  if (a.class_mask & 4) hipLaunchKernelGGL(chain_kernel<2>, grid, dim3(64), 0, st, a);
+  if (a.class_mask & 8) hipLaunchKernelGGL(chain_kernel<3>, grid, dim3(64), 0, st, a);
+  if (a.class_mask & 16) hipLaunchKernelGGL(chain_kernel<4>, grid, dim3(64), 0, st, a);
   return hipGetLastError();
 }
 
