@@ -231,25 +231,30 @@ def bench_c3(steps, warmup, B=4096, n=14, seed=1003):
 
 
 def roofline_c3(res, steps):
+    """Groups run concurrently on side streams, so launch durations overlap:
+    `achieved` is the algorithmic bytes of every pass launch of the timed
+    steps over their wall time (the per-kernel entries keep the summed,
+    overlapping event times for reference)."""
     n = res["n"]
     sweep = 2 * 8 * (1 << n)
     per = dict(zip(kernel_names(), (sweep, sweep * 3 // 2)))
     out = {}
+    total_bytes = 0
     for k, (tot, cnt) in res["kernels"].items():
-        s_ = tot / 1e3
         bytes_ = per[k] * res["circuit_passes"] * steps
-        out[k] = {"ms_total": round(tot, 3), "launches": cnt, "GB/s": round(bytes_ / max(s_, 1e-12) / 1e9, 1)}
-    dom = max(res["kernels"], key=lambda k: res["kernels"][k][0])
-    ach = out[dom]["GB/s"]
+        total_bytes += bytes_
+        out[k] = {"ms_total_summed": round(tot, 3), "launches": cnt}
+    ach = round(total_bytes / res["seconds"] / 1e9, 1)
     return {
         "bound": "hbm",
-        "kernel": dom,
+        "kernel": "all pass kernels (" + ", ".join(kernel_names()) + ")",
         "achieved": ach,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(ach / HBM_PEAK_GBS, 4),
         "traffic": None,
-        "algorithmic_bytes_per_launch_note": "sum over active circuits of the per-circuit pass bytes (2*8*2^14 forward, 3*8*2^14 backward)",
+        "algorithmic_bytes_note": "per circuit pass 2*8*2^14 B forward, 3*8*2^14 B backward, summed over all "
+        "circuit passes of the timed steps, divided by their wall time",
         "kernels": out,
     }
 

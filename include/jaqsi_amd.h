@@ -199,10 +199,14 @@ typedef struct jq_group {
 
 /* Forward + adjoint gradient of a ragged batch: several circuit structures
  * (e.g. uneven depths) of the same width and dtype, observables shared.
- * Every pass step is ONE launch over all circuits that still have a pass at
- * that step.  The reference runs one vmapped program per structure group
- * (script.py:406-434, the vmap in _vectorize script.py:43-127); this call
- * replaces that loop of execute() calls. */
+ * With the interpreter, every pass step is ONE launch over all circuits that
+ * still have a pass at that step.  When every group's plan has its
+ * plan-specialised kernels, each group runs its own adjoint on one of four
+ * side streams forked from `stream` and joined back to it with events (groups
+ * sharing a plan share a side stream), so work enqueued on `stream` after
+ * the call sees every output.  The reference runs one vmapped
+ * program per structure group (script.py:406-434, the vmap in _vectorize
+ * script.py:43-127); this call replaces that loop of execute() calls. */
 int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_kind, int32_t n_obs,
                          const int32_t* obs_wires, const double* obs_d, void* stream);
 

@@ -286,6 +286,38 @@ bool jit_load(jq_plan* p) {
   return true;
 }
 
+// Side streams of ragged-batch calls, per device (created once, default
+// priority, non-blocking), with the events that fork and join them.
+struct SideStreams {
+  static constexpr int kStreams = 4;
+  hipStream_t s[kStreams];
+  hipEvent_t fork;
+  hipEvent_t join[kStreams];
+  std::mutex mu;  // held while a call records on the shared events
+};
+
+hipError_t side_streams(SideStreams** out) {
+  static std::mutex mu;
+  static std::map<int, SideStreams*> per_dev;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = per_dev.find(dev);
+  if (it == per_dev.end()) {
+    SideStreams* ss = new SideStreams();
+    for (int k = 0; k < SideStreams::kStreams && e == hipSuccess; ++k)
+      e = hipStreamCreateWithFlags(&ss->s[k], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ss->fork, hipEventDisableTiming);
+    for (int k = 0; k < SideStreams::kStreams && e == hipSuccess; ++k)
+      e = hipEventCreateWithFlags(&ss->join[k], hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+    it = per_dev.emplace(dev, ss).first;
+  }
+  *out = it->second;
+  return hipSuccess;
+}
+
 // Whether this call's passes run the specialised kernels.
 bool jit_ready(jq_plan* p) {
   if (p->jit.mode == JQ_JIT_OFF) return false;
@@ -770,13 +802,38 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
   bool all_jit = true;
   for (int g = 0; g < n_groups && all_jit; ++g) all_jit = jit_ready(groups[g].plan);
   if (all_jit) {
-    for (int g = 0; g < n_groups; ++g) {
-      const jq_group& G = groups[g];
-      const int rc = jq_sv_adjoint(G.plan, G.batch, G.angles, G.matrices, G.psi0, G.psi0_batched, G.state, obs_kind,
-                                   n_obs, obs_wires, obs_d, G.ct, G.expvals, G.grad_angles, nullptr, nullptr, stream);
-      if (rc) return rc;
+    // Groups share nothing (each plan owns its workspaces), so they run on
+    // side streams forked from and joined back to the caller's stream with
+    // events: small groups' launches overlap instead of leaving the device
+    // partly idle.  Event fork / join keeps the call capturable.
+    SideStreams* ss = nullptr;
+    hipError_t e = side_streams(&ss);
+    if (e != hipSuccess) return fail(JQ_ERR_DEVICE, std::string("side streams: ") + hipGetErrorString(e));
+    const int ns = SideStreams::kStreams;
+    std::lock_guard<std::mutex> lock(ss->mu);
+    HIP_TRY(hipEventRecord(ss->fork, st));
+    for (int k = 0; k < ns; ++k) HIP_TRY(hipStreamWaitEvent(ss->s[k], ss->fork, 0));
+    // largest groups first, one per stream in turn
+    std::vector<int> by_work(n_groups);
+    for (int g = 0; g < n_groups; ++g) by_work[g] = g;
+    std::stable_sort(by_work.begin(), by_work.end(), [&](int x, int y) {
+      return (int64_t)groups[x].batch * groups[x].plan->hp.n_passes > (int64_t)groups[y].batch * groups[y].plan->hp.n_passes;
+    });
+    // A plan's workspaces serve one launch sequence at a time: groups that
+    // share a plan share its stream.
+    std::map<const jq_plan*, int> stream_of;
+    int rc = JQ_OK;
+    for (int j = 0; j < n_groups && rc == JQ_OK; ++j) {
+      const jq_group& G = groups[by_work[j]];
+      const int k = stream_of.emplace(G.plan, (int)stream_of.size() % ns).first->second;
+      rc = jq_sv_adjoint(G.plan, G.batch, G.angles, G.matrices, G.psi0, G.psi0_batched, G.state, obs_kind, n_obs,
+                         obs_wires, obs_d, G.ct, G.expvals, G.grad_angles, nullptr, nullptr, ss->s[k]);
     }
-    return JQ_OK;
+    for (int k = 0; k < ns; ++k) {
+      HIP_TRY(hipEventRecord(ss->join[k], ss->s[k]));
+      HIP_TRY(hipStreamWaitEvent(st, ss->join[k], 0));
+    }
+    return rc;
   }
   // groups with more passes first: the circuits still active at a step are a prefix
   std::stable_sort(order.begin(), order.end(),
