@@ -190,25 +190,52 @@ int pass_rebase(const ProgView& pv, int q) {
   return m < 0 ? 0 : m;
 }
 
-// Forward pass body.  `first` receives the layout words of the first
-// substep when the pass starts with a re-layout (empty otherwise); `o`
-// receives the statements after that re-layout.
-void emit_forward(std::ostringstream& o, std::string& first, const ProgView& pv, int q, int rebase) {
+// Layout words (`first` / `last`, with the tile bit of each register and
+// thread bit) at the ends of a pass body, empty for the canonical layout.
+struct Ends {
+  std::string first, last;
+  const int32_t* first_sub = nullptr;
+  const int32_t* last_sub = nullptr;
+};
+
+// Forward pass body.  `e.first` receives the layout of the first substep
+// when the pass starts with a re-layout; `o` receives the statements after
+// that re-layout.
+void emit_forward(std::ostringstream& o, Ends& e, const ProgView& pv, int q, int rebase) {
   std::vector<int> cur = canonical_rpos(pv);
   const int nsub = pv.pass(q)[0];
-  first.clear();
+  e = Ends();
   for (int s = 0; s < nsub; ++s) {
     const int32_t* sub = pv.sub(q, s);
     if (!same_rpos(pv, sub, cur)) {
-      if (s == 0)
-        first = layout_args(pv, sub);
-      else
+      if (s == 0) {
+        e.first = layout_args(pv, sub);
+        e.first_sub = sub;
+      } else {
         o << "    jit_relayout<R, RB, LNT" << layout_args(pv, sub) << ">(re, im, lds, L);\n";
+      }
+      e.last = layout_args(pv, sub);
+      e.last_sub = sub;
       for (int b = 0; b < pv.RB; ++b) cur[b] = sub[1 + b];
     }
     const int32_t* g = sub + 1 + pv.T;
     for (int k = 0; k < sub[0]; ++k, g += GW) emit_fwd_gate(o, GateLit(g, rebase));
   }
+}
+
+// How a tile can move between HBM and registers under the layout of `sub`:
+// 1 when register bit 0 and thread bits 0, 1 hold tile bits 0, 1, 2 (16-byte
+// accesses, 4 lanes per 64 contiguous bytes), 2 when thread bits 0, 1, 2 do
+// (8-byte accesses, 8 lanes per 64 bytes), else 0 (exchange through LDS into
+// the canonical layout).  Tile bits 0..2 must be global bits 0..2 in every
+// pass (`lowrun`).
+int access_mode(const ProgView& pv, const int32_t* sub, bool lowrun) {
+  if (sub == nullptr || !lowrun || pv.T - pv.RB < 3) return 0;
+  const int32_t* w = sub + 1;
+  const int rb = pv.RB;
+  if (w[0] == 0 && w[rb] == 1 && w[rb + 1] == 2) return 1;
+  if (w[rb] == 0 && w[rb + 1] == 1 && w[rb + 2] == 2) return 2;
+  return 0;
 }
 
 // layout words of the canonical layout (register bits 0, T-RB+1 .. T-1;
@@ -220,13 +247,23 @@ std::string canonical_args(const ProgView& pv) {
   return o;
 }
 
-void emit_backward(std::ostringstream& o, const ProgView& pv, int q, int rebase) {
+// Backward pass body (substeps and gates in reverse); `e` and `o` as for
+// the forward.
+void emit_backward(std::ostringstream& o, Ends& e, const ProgView& pv, int q, int rebase) {
   std::vector<int> cur = canonical_rpos(pv);
   const int nsub = pv.pass(q)[0];
+  e = Ends();
   for (int s = nsub - 1; s >= 0; --s) {
     const int32_t* sub = pv.sub(q, s);
     if (!same_rpos(pv, sub, cur)) {
-      o << "    jit_relayout2<R, RB, LNT" << layout_args(pv, sub) << ">(pr, pi, lr, li, lds, L);\n";
+      if (s == nsub - 1) {
+        e.first = layout_args(pv, sub);
+        e.first_sub = sub;
+      } else {
+        o << "    jit_relayout2<R, RB, LNT" << layout_args(pv, sub) << ">(pr, pi, lr, li, lds, L);\n";
+      }
+      e.last = layout_args(pv, sub);
+      e.last_sub = sub;
       for (int b = 0; b < pv.RB; ++b) cur[b] = sub[1 + b];
     }
     const int32_t* g0 = sub + 1 + pv.T;
@@ -251,25 +288,44 @@ std::string jit_kernel_name(int dir, int id) {
 
 void jit_plan(const JitSpec& spec, JitPlan& out) {
   out = JitPlan();
+  // Direct HBM access under substep layouts needs full tiles, 32-bit byte
+  // offsets (2^27 amplitudes at most) and tile bits 0..2 = global bits 0..2
+  // in every pass of both programs.
+  const ProgView pf = view(*spec.words_f);
+  const bool full = pf.W[4] >= pf.T;
+  bool lowrun = full && pf.W[4] <= 27 && pf.T >= 3;
+  for (int dir = 0; dir < 2 && lowrun; ++dir) {
+    const ProgView pv = view(dir == 0 ? *spec.words_f : *spec.words_b);
+    for (int q = 0; q < pv.n_passes && lowrun; ++q) {
+      const int32_t* tb = pv.pass(q) + 4;
+      lowrun = tb[0] == 0 && tb[1] == 1 && tb[2] == 2;
+    }
+  }
   for (int dir = 0; dir < 2; ++dir) {
     const ProgView pv = view(dir == 0 ? *spec.words_f : *spec.words_b);
     for (int q = 0; q < pv.n_passes; ++q) {
       const int rebase = pass_rebase(pv, q);
       std::ostringstream o;
-      std::string first;
+      Ends e;
       if (dir == 0)
-        emit_forward(o, first, pv, q, rebase);
+        emit_forward(o, e, pv, q, rebase);
       else
-        emit_backward(o, pv, q, rebase);
+        emit_backward(o, e, pv, q, rebase);
       const std::string body0 = o.str();
-      const std::string body =
-          first.empty() ? body0 : "    jit_relayout<R, RB, LNT" + first + ">(re, im, lds, L);\n" + body0;
+      const std::string relayout = dir == 0 ? "    jit_relayout<R, RB, LNT" + e.first + ">(re, im, lds, L);\n"
+                                            : "    jit_relayout2<R, RB, LNT" + e.first + ">(pr, pi, lr, li, lds, L);\n";
+      const std::string body = e.first.empty() ? body0 : relayout + body0;
       int id = -1;
       for (size_t k = 0; k < out.kernels.size(); ++k)
         if (out.kernels[k].dir == dir && out.kernels[k].body == body) id = (int)k;
       if (id < 0) {
         id = (int)out.kernels.size();
-        out.kernels.push_back({dir, body, first, body0});
+        JitPlan::Kernel k{dir, body, e.first, body0, e.last};
+        k.full = full;
+        // the forward reads through LDS already (sv_forward_dma); only its store changes
+        k.load = dir == 1 ? access_mode(pv, e.first_sub, lowrun) : 0;
+        k.store = access_mode(pv, e.last_sub, lowrun);
+        out.kernels.push_back(k);
       }
       out.kernel_of[dir].push_back(id);
       out.pool_off[dir].push_back(rebase);
@@ -282,7 +338,8 @@ std::string jit_key(const JitSpec& spec, const JitPlan& plan, int id) {
   const ProgView pv = view(k.dir == 0 ? *spec.words_f : *spec.words_b);
   return std::to_string(k.dir) + " " + std::to_string(spec.dtype) + " " + std::to_string(pv.T) + " " +
          std::to_string(pv.RB) + " " + std::to_string(k.dir == 1 && spec.persist_bwd ? 1 : 0) + " " +
-         std::to_string(spec.dma ? 1 : 0) + "\n" + k.body;
+         std::to_string(spec.dma ? 1 : 0) + " " + std::to_string(k.full ? 1 : 0) + " " + std::to_string(k.load) +
+         " " + std::to_string(k.store) + "\n" + k.body;
 }
 
 std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vector<int>& ids,
@@ -304,6 +361,11 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
           "(cint*, int, R* re, R* im, Layout& L, uint64_t base, C2<R>* lds, cmat<R>* pool) {\n"
           "    constexpr int LNT = __builtin_ctz(NT);\n";
       o << "struct JF" << id << " {\n"
+        << "  static constexpr bool kFull = " << (k.full ? "true" : "false") << ";\n"
+        << "  static constexpr int kStore = " << (dma ? k.store : 0) << ";\n"
+        << "  template <int RB, int LNT>\n"
+        << "  static __device__ __forceinline__ void last_layout(cint* tb, Layout& L) {\n"
+        << "    layout_lit_g<RB, LNT" << (k.last.empty() ? canonical_args(pv) : k.last) << ">(tb, L);\n  }\n"
         << "  template <typename R, int RB, int NT, bool FULL>\n"
         << "  static __device__ __forceinline__ void fwd" << sig << k.body
         << "    (void)lds; (void)base; (void)pool;\n  }\n"
@@ -322,12 +384,28 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
           << "(SvArgs a) {\n  sv_forward_impl<" << rt << ", " << pv.RB << ", " << nt << ", false, JF" << id
           << ">(a);\n}\n\n";
     } else {
+      const char* sig =
+          "(cint*, int, R* pr, R* pi, R* lr, R* li, Layout& L, uint64_t base, C2<R>* lds, cmat<R>* pool, "
+          "double* slots) {\n    constexpr int LNT = __builtin_ctz(NT);\n";
+      const int load = spec.persist_bwd ? k.load : 0, store = spec.persist_bwd ? k.store : 0;
       o << "struct JB" << id << " {\n"
+        << "  static constexpr bool kFull = " << (k.full ? "true" : "false") << ";\n"
+        << "  static constexpr int kLoad = " << load << ";\n"
+        << "  static constexpr int kStore = " << store << ";\n"
+        << "  template <int RB, int LNT>\n"
+        << "  static __device__ __forceinline__ void first_layout(cint* tb, Layout& L) {\n"
+        << "    layout_lit_g<RB, LNT" << (k.first.empty() ? canonical_args(pv) : k.first) << ">(tb, L);\n  }\n"
+        << "  template <int RB, int LNT>\n"
+        << "  static __device__ __forceinline__ void last_layout(cint* tb, Layout& L) {\n"
+        << "    layout_lit_g<RB, LNT" << (k.last.empty() ? canonical_args(pv) : k.last) << ">(tb, L);\n  }\n"
         << "  template <typename R, int RB, int NT, bool FULL>\n"
-        << "  static __device__ __forceinline__ void bwd(cint*, int, R* pr, R* pi, R* lr, R* li, Layout& L, "
-           "uint64_t base, C2<R>* lds, cmat<R>* pool, double* slots) {\n"
-        << "    constexpr int LNT = __builtin_ctz(NT);\n"
-        << k.body << "    (void)lds; (void)base; (void)pool; (void)slots;\n  }\n};\n"
+        << "  static __device__ __forceinline__ void bwd" << sig << k.body
+        << "    (void)lds; (void)base; (void)pool; (void)slots;\n  }\n";
+      if (load != 0)
+        o << "  template <typename R, int RB, int NT, bool FULL>\n"
+          << "  static __device__ __forceinline__ void bwd0" << sig << k.body0
+          << "    (void)lds; (void)base; (void)pool; (void)slots;\n  }\n";
+      o << "};\n"
         << "extern \"C\" __global__ __launch_bounds__(" << nt << ") void " << kname << "(SvArgs a) {\n  "
         << (spec.persist_bwd ? "sv_backward_loop<" : "sv_backward_impl<") << rt << ", " << pv.RB << ", " << nt
         << (spec.persist_bwd ? "" : ", false") << ", JB" << id << ">(a);\n}\n\n";

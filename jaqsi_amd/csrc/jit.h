@@ -42,8 +42,14 @@ struct JitPlan {
   struct Kernel {
     int dir;            // 0 forward, 1 backward
     std::string body;   // generated statements of the pass body (from the canonical layout)
-    std::string first;  // forward: layout words of the first substep, empty when canonical
-    std::string body0;  // forward: the statements after the initial re-layout
+    std::string first;  // layout words of the first substep visited, empty when canonical
+    std::string body0;  // the statements after the initial re-layout
+    std::string last;   // layout words at the end of the body, empty when canonical
+    // HBM access under the first / last layout instead of the canonical one
+    // (0: canonical, 1: register pairs of adjacent amplitudes, 2: single
+    // amplitudes); `full`: n_wires >= T.
+    int load = 0, store = 0;
+    bool full = false;
   };
   std::vector<Kernel> kernels;
   std::vector<int> kernel_of[2];

@@ -1313,6 +1313,77 @@ __device__ __forceinline__ void layout_lit(cint* tbits, Layout& L) {
   for (int b = 0; b < RB; ++b) L.rpos[b] = w[b];
 }
 
+// The same layout with its global thread offset, for the direct accesses
+// below (substep layouts otherwise carry none).
+template <int RB, int LNT, int... W>
+__device__ __forceinline__ void layout_lit_g(cint* tbits, Layout& L) {
+  constexpr int w[] = {W...};
+  layout_lit<RB, LNT, W...>(tbits, L);
+  uint32_t tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  uint64_t g = 0;
+#pragma unroll
+  for (int j = 0; j < LNT; ++j) g |= (uint64_t)((tid >> j) & 1u) << tbits[w[RB + j]];
+  asm volatile("" : "+v"(g));
+  L.gthr = g;
+}
+
+template <typename R>
+__device__ __forceinline__ void ld_one(const C2<R>* p, R& r, R& i) {
+  if constexpr (sizeof(R) == 4) {
+    const float2 v = *reinterpret_cast<const float2*>(p);
+    r = v.x;
+    i = v.y;
+  } else {
+    const double2 v = *reinterpret_cast<const double2*>(p);
+    r = v.x;
+    i = v.y;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void st_one(C2<R>* p, R r, R i) {
+  if constexpr (sizeof(R) == 4)
+    *reinterpret_cast<float2*>(p) = make_float2(r, i);
+  else
+    *reinterpret_cast<double2*>(p) = make_double2(r, i);
+}
+
+// Full-tile HBM access under a substep layout (with gthr) whose lanes run
+// over the lowest global bits, so that every 8 lanes cover 64 contiguous
+// bytes: a pass can start or end in its own substep layout instead of
+// exchanging the tile through LDS into the canonical one.  PAIRS: register
+// bit 0 is global bit 0, so registers (2k, 2k+1) hold adjacent amplitudes and
+// move as one access.  Byte offsets are 32-bit (vectors of up to 2^27
+// amplitudes); the generator selects these only for such plans.
+template <typename R, int RB, bool PAIRS>
+__device__ __forceinline__ void load_direct(R* re, R* im, const C2<R>* src, const Layout& L, uint64_t base) {
+  constexpr int NR = 1 << RB;
+  const uint32_t tb = (uint32_t)L.gthr * (uint32_t)sizeof(C2<R>);
+#pragma unroll
+  for (int i = 0; i < NR; i += PAIRS ? 2 : 1) {
+    const char* up = reinterpret_cast<const char*>(src + (base | reg_goff<RB>(L, i))) + tb;
+    if constexpr (PAIRS)
+      ld_pair<R>(reinterpret_cast<const C2<R>*>(up), re[i], im[i], re[i + 1], im[i + 1]);
+    else
+      ld_one<R>(reinterpret_cast<const C2<R>*>(up), re[i], im[i]);
+  }
+}
+
+template <typename R, int RB, bool PAIRS>
+__device__ __forceinline__ void store_direct(const R* re, const R* im, C2<R>* dst, const Layout& L, uint64_t base) {
+  constexpr int NR = 1 << RB;
+  const uint32_t tb = (uint32_t)opaque(L.gthr) * (uint32_t)sizeof(C2<R>);
+#pragma unroll
+  for (int i = 0; i < NR; i += PAIRS ? 2 : 1) {
+    char* up = reinterpret_cast<char*>(dst + (base | reg_goff<RB>(L, i))) + tb;
+    if constexpr (PAIRS)
+      st_pair<R>(reinterpret_cast<C2<R>*>(up), re[i], im[i], re[i + 1], im[i + 1]);
+    else
+      st_one<R>(reinterpret_cast<C2<R>*>(up), re[i], im[i]);
+  }
+}
+
 template <typename R, int RB, int LNT, int... W>
 __device__ __forceinline__ void jit_relayout(R* re, R* im, C2<R>* lds, Layout& L) {
   Layout nl;
@@ -1818,7 +1889,7 @@ __device__ __forceinline__ void sv_forward_dma(const SvArgs& a) {
     load_obs(a, obs);
   }
   cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride + a.pool_off;
-  const bool full = a.n_wires >= T;
+  const bool full = Body::kFull;  // n_wires >= T, fixed when the kernel is generated
   const bool from_zero = a.src == nullptr;
   const bool dma = full && !from_zero && a.vec_len <= (1ull << 27);  // 32-bit per-thread byte offsets
   const C2<R>* src = from_zero ? nullptr : reinterpret_cast<const C2<R>*>(a.src) + (int64_t)circ * a.src_stride;
@@ -1873,8 +1944,13 @@ __device__ __forceinline__ void sv_forward_dma(const SvArgs& a) {
       Body::template fwd0<R, RB, NT, false>(P, a.pass, re, im, L, base, lds, pool);
     else
       Body::template fwd<R, RB, NT, false>(P, a.pass, re, im, L, base, lds, pool);
-    to_canonical<R, RB, NT>(re, im, tbits, L, lds);
-    store_canon<R, RB>(re, im, dst, a.vec_len, L, base, full);
+    if constexpr (Body::kStore != 0) {
+      Body::template last_layout<RB, LNT>(tbits, L);
+      store_direct<R, RB, Body::kStore == 1>(re, im, dst, L, base);
+    } else {
+      to_canonical<R, RB, NT>(re, im, tbits, L, lds);
+      store_canon<R, RB>(re, im, dst, a.vec_len, L, base, full);
+    }
     if (want_ev) {
       __syncthreads();
       expval_partials<R, RB>(re, im, a, obs, L, base, ev);
@@ -1912,16 +1988,26 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
   int64_t w = blockIdx.x;
   if (w >= total) return;
   load_obs(a, obs);
-  const bool full = a.n_wires >= T;
+  const bool full = Body::kFull;  // n_wires >= T, fixed when the kernel is generated
   const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src);
   C2<R>* lam = reinterpret_cast<C2<R>*>(a.lam);
+  // Tiles are read under the canonical layout, or (Body::kLoad) directly
+  // under the layout of the first substep the sweep visits.
   Layout Lc;
   canonical_layout<RB, LNT>(tbits, Lc);
+  Layout Lf;
+  if constexpr (Body::kLoad != 0) Body::template first_layout<RB, LNT>(tbits, Lf);
+  auto fetch = [&](R* xr, R* xi, const C2<R>* p, uint64_t b) {
+    if constexpr (Body::kLoad != 0)
+      load_direct<R, RB, Body::kLoad == 1>(xr, xi, p, Lf, b);
+    else
+      load_canon<R, RB>(xr, xi, p, a.vec_len, Lc, b, full);
+  };
   R npr[NR], npi[NR], nlr[NR], nli[NR];
   int ncirc = (int)(w >> tlog), ntile = (int)(w & (ntiles - 1));
   uint64_t nbase = tile_base(ntile, tbits, T, a.n_wires);
-  load_canon<R, RB>(npr, npi, src + (int64_t)ncirc * a.src_stride, a.vec_len, Lc, nbase, full);
-  if (!seed) load_canon<R, RB>(nlr, nli, lam + (int64_t)ncirc * a.lam_stride, a.vec_len, Lc, nbase, full);
+  fetch(npr, npi, src + (int64_t)ncirc * a.src_stride, nbase);
+  if (!seed) fetch(nlr, nli, lam + (int64_t)ncirc * a.lam_stride, nbase);
   for (;;) {
     const int circ = ncirc, tile = ntile;
     const uint64_t base = nbase;
@@ -1943,21 +2029,36 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
       ncirc = (int)(next >> tlog);
       ntile = (int)(next & (ntiles - 1));
       nbase = tile_base(ntile, tbits, T, a.n_wires);
-      load_canon<R, RB>(npr, npi, src + (int64_t)ncirc * a.src_stride, a.vec_len, Lc, nbase, full);
-      if (!seed) load_canon<R, RB>(nlr, nli, lam + (int64_t)ncirc * a.lam_stride, a.vec_len, Lc, nbase, full);
+      fetch(npr, npi, src + (int64_t)ncirc * a.src_stride, nbase);
+      if (!seed) fetch(nlr, nli, lam + (int64_t)ncirc * a.lam_stride, nbase);
     }
     for (int e = threadIdx.x; e < nslots; e += NT) slots[e] = 0;
     __syncthreads();
     cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride + a.pool_off;
     Layout L;
-    canonical_layout<RB, LNT>(tbits, L);
+    if constexpr (Body::kLoad != 0)
+      L = Lf;
+    else
+      canonical_layout<RB, LNT>(tbits, L);
     if (seed) seed_lambda<R, RB>(pr, pi, lr, li, a, obs, circ, L, base);
-    Body::template bwd<R, RB, NT, false>(P, a.pass, pr, pi, lr, li, L, base, lds, pool, slots);
-    to_canonical2<R, RB, NT>(pr, pi, lr, li, tbits, L, lds);
-    store_canon<R, RB>(lr, li, lam + (int64_t)circ * a.lam_stride, a.vec_len, L, base, full);
-    if (a.dst != nullptr)
-      store_canon<R, RB>(pr, pi, reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride, a.vec_len, L, base,
-                         full);
+    if constexpr (Body::kLoad != 0)
+      Body::template bwd0<R, RB, NT, false>(P, a.pass, pr, pi, lr, li, L, base, lds, pool, slots);
+    else
+      Body::template bwd<R, RB, NT, false>(P, a.pass, pr, pi, lr, li, L, base, lds, pool, slots);
+    C2<R>* pout = a.dst != nullptr ? reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride : nullptr;
+    if constexpr (Body::kStore != 0) {
+      Body::template last_layout<RB, LNT>(tbits, L);
+      store_direct<R, RB, Body::kStore == 1>(lr, li, lam + (int64_t)circ * a.lam_stride, L, base);
+      if (pout != nullptr) store_direct<R, RB, Body::kStore == 1>(pr, pi, pout, L, base);
+    } else if (pout != nullptr) {
+      to_canonical2<R, RB, NT>(pr, pi, lr, li, tbits, L, lds);
+      store_canon<R, RB>(lr, li, lam + (int64_t)circ * a.lam_stride, a.vec_len, L, base, full);
+      store_canon<R, RB>(pr, pi, pout, a.vec_len, L, base, full);
+    } else {
+      // psi is not kept: only lambda goes back to the canonical layout
+      to_canonical<R, RB, NT>(lr, li, tbits, L, lds);
+      store_canon<R, RB>(lr, li, lam + (int64_t)circ * a.lam_stride, a.vec_len, L, base, full);
+    }
     __syncthreads();
     // partials are real-major ([real][tile]) so the reduction reads them contiguously
     double* gp = a.grad_part + (int64_t)circ * a.part_stride + slot_base + tile;
