@@ -1,7 +1,10 @@
 """``Gates.<Name>(...)``: the recording entry point (mirrors jaqsi/gates.py).
 
-Routes a call to ``UnitaryGates`` or, with ``pulse=True``, to
-``PulseGates`` (gates.py:88-176), filtering keyword arguments the same way.
+Routes a call to ``UnitaryGates`` (gates.py:88-176), filtering keyword
+arguments the same way.  Pulse-level gates (``pulse=True``, the reference's
+``PulseGates``, jaqsi/pulses.py:1370-1665) are not lowered by this backend:
+such calls raise ``NotImplementedError``.  Batched pulse propagators of the
+two-qubit drive Hamiltonian are available from ``jaqsi_amd.pulse``.
 """
 
 from __future__ import annotations
@@ -14,6 +17,9 @@ from jaqsi_amd.gateset import Barrier as BarrierOp
 from jaqsi_amd.unitary import UnitaryGates
 
 log = logging.getLogger(__name__)
+
+_PULSE_GATES = ("pulse-level gates (pulse=True) are not lowered by this backend; batched two-qubit pulse "
+                "propagators with gradients are available from jaqsi_amd.pulse")
 
 
 class _GateHandler:
@@ -45,8 +51,6 @@ def Barrier(wires: Union[int, List[int]], *args, **kwargs):
 class Gates(metaclass=GatesMeta):
     """Entry point for applying gates inside a circuit function."""
 
-    _pulse_mgr = None
-
     @classmethod
     def _inner_getattr(cls, gate_name, *args, **kwargs):
         if gate_name == "Barrier":
@@ -54,23 +58,14 @@ class Gates(metaclass=GatesMeta):
         pulse = kwargs.pop("pulse", False)
         if not isinstance(pulse, bool):
             raise TypeError(f"'pulse' must be a bool, got {type(pulse).__name__}.")
-        allowed = ["w", "wires", "phi", "theta", "omega", "noise_params", "random_key"]
         if pulse:
-            from jaqsi_amd.pulses import PulseGates
-
-            backend = PulseGates
-            allowed += ["pulse_params"]
-        else:
-            backend = UnitaryGates
+            raise NotImplementedError(_PULSE_GATES)
+        allowed = ["w", "wires", "phi", "theta", "omega", "noise_params", "random_key"]
+        backend = UnitaryGates
         extra = kwargs.keys() - allowed
         if extra:
             log.debug(f"Unsupported keyword arguments: {list(extra)}")
         kwargs = {k: v for k, v in kwargs.items() if k in allowed}
-        if pulse and cls._pulse_mgr is not None:
-            from jaqsi_amd.pulses import PulseInformation
-
-            n_params = PulseInformation.gate_by_name(gate_name).size
-            kwargs["pulse_params"] = PulseInformation.gate_by_name(gate_name).params * cls._pulse_mgr.get(n_params)
         gate = getattr(backend, gate_name, None)
         if gate is None:
             raise AttributeError(f"'{backend.__name__}' object has no attribute '{gate_name}'")
@@ -79,13 +74,8 @@ class Gates(metaclass=GatesMeta):
     @classmethod
     @contextmanager
     def pulse_manager_context(cls, pulse_params):
-        from jaqsi_amd.pulses import PulseParamManager
-
-        cls._pulse_mgr = PulseParamManager(pulse_params)
-        try:
-            yield
-        finally:
-            cls._pulse_mgr = None
+        raise NotImplementedError(_PULSE_GATES)
+        yield  # pragma: no cover  (keeps this a generator for contextmanager)
 
     @classmethod
     def parse_gates(cls, gates, set_of_gates=None):

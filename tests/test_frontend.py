@@ -21,6 +21,28 @@ def test_recording_and_wire_validation():
         J.Gates.RX(0.1, wires=0, pulse="yes")
 
 
+def test_pulse_gates_raise_and_propagator_validates():
+    """Pulse-level circuit gates are not lowered: a clear NotImplementedError.
+    The batched propagator API checks its arguments before any GPU work."""
+    import jaqsi_amd as J
+    from jaqsi_amd import pulse
+
+    with pytest.raises(NotImplementedError, match="jaqsi_amd.pulse"):
+        J.Gates.RX(0.1, wires=0, pulse=True)
+    with pytest.raises(NotImplementedError):
+        with J.Gates.pulse_manager_context([1.0]):
+            pass
+    p, w = torch.zeros(3, 4, dtype=torch.float64), torch.zeros(3, 2, dtype=torch.float64)
+    with pytest.raises(ValueError, match="CUDA"):
+        pulse.propagate(p, w, np.eye(4))
+    with pytest.raises(ValueError, match="omega"):
+        pulse.propagate(p, w[:2], np.eye(4))
+    with pytest.raises(ValueError, match="params"):
+        pulse.propagate(p[:, :3], w, np.eye(4))
+    with pytest.raises(ValueError, match="4x4"):
+        pulse._target_host(np.eye(2))
+
+
 def test_noise_params_attach_channels():
     import jaqsi_amd as J
     from jaqsi_amd.noise import BitFlip, DepolarizingChannel, QubitChannel

@@ -69,3 +69,27 @@ def test_zero_drive_is_diagonal_phase():
     z = np.array([1, -1])
     e = np.array([15.0 * z0 + 15.5 * z1 + pulse.J_C5 * z0 * z1 for z0 in z for z1 in z])
     np.testing.assert_allclose(U[0], np.diag(np.exp(-1j * 50.0 * e)), atol=1e-12)
+
+
+def test_autograd_api_matches_oracle_and_kernel_gradient():
+    """jaqsi_amd.pulse: propagators / infidelities against the oracle, and the
+    autograd gradient of sum(w * infidelity) equal to w times the kernel's
+    own per-set gradient."""
+    from jaqsi_amd import pulse as P
+
+    print("pulse autograd api", flush=True)
+    params, omega = pulse.workload(4, seed=13)
+    n_steps = 120
+    V = pulse.target_rx_half_pi()
+    pt = torch.tensor(params, dtype=torch.float64, device="cuda", requires_grad=True)
+    wt = torch.tensor(omega, dtype=torch.float64, device="cuda")
+    U, inf, g = P.propagate(pt, wt, V, n_steps=n_steps, coupling=pulse.J_C5)
+    for s in range(4):
+        ref = pulse.propagate(params[s], omega[s], pulse.J_C5, 50.0, n_steps)
+        np.testing.assert_allclose(U[s].cpu().numpy(), ref, atol=1e-11)
+        np.testing.assert_allclose(inf[s].item(), pulse.infidelity(ref, V), atol=1e-11)
+    w = torch.tensor([1.0, -2.0, 0.5, 3.0], dtype=torch.float64, device="cuda")
+    f = P.infidelity(pt, wt, torch.tensor(V), n_steps=n_steps, coupling=pulse.J_C5)
+    np.testing.assert_allclose(f.detach().cpu().numpy(), inf.cpu().numpy(), atol=1e-15, rtol=0)
+    (w * f).sum().backward()
+    np.testing.assert_allclose(pt.grad.cpu().numpy(), (w[:, None] * g).cpu().numpy(), atol=1e-15, rtol=1e-13)
