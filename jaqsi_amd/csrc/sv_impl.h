@@ -1105,23 +1105,21 @@ __device__ __forceinline__ void store_canon(const R* re, const R* im, C2<R>* dst
 // `region` selects a 2-plane LDS region (0: the first 2 * 2^T entries, 1:
 // the next); `tail_sync` = false skips the closing barrier when the next LDS
 // writes go to the other region (psi and lambda in the backward).
+// The two halves of an exchange: every thread stores its registers under
+// layout `from`, and (after a barrier) loads them back under layout `to`.
 template <typename R, int RB>
-__device__ __forceinline__ void relayout(R* re, R* im, C2<R>* lds, const Layout& from, const Layout& to,
-                                         int region = 0, bool tail_sync = true) {
+__device__ __forceinline__ void relayout_put(const R* re, const R* im, C2<R>* lds, const Layout& from, int region) {
   constexpr int NR = 1 << RB;
   constexpr uint32_t ES = sizeof(R);
-  uint32_t wf[RB], wt[RB];
+  uint32_t wf[RB];
 #pragma unroll
-  for (int b = 0; b < RB; ++b) {
-    wf[b] = __builtin_amdgcn_readfirstlane(lds_swz(1u << from.rpos[b]) * ES);
-    wt[b] = __builtin_amdgcn_readfirstlane(lds_swz(1u << to.rpos[b]) * ES);
-  }
+  for (int b = 0; b < RB; ++b) wf[b] = __builtin_amdgcn_readfirstlane(lds_swz(1u << from.rpos[b]) * ES);
   // 32-bit LDS addresses.  The tile starts the dynamic LDS block, which is
   // at LDS offset 0 (the kernels declare no static LDS), so the thread term
   // can carry the base and the register term is XORed in.
   constexpr uint32_t IM = ES << kTileBits<R>;  // byte distance of the imaginary plane
   const uint32_t lb = (uint32_t)(uintptr_t)(JQ_LDS char*)(char*)lds + (uint32_t)region * 2 * IM;
-  const uint32_t sf = lb + lds_swz(from.toff) * ES, st = lb + lds_swz(to.toff) * ES;
+  const uint32_t sf = lb + lds_swz(from.toff) * ES;
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
     uint32_t o = 0;
@@ -1132,7 +1130,18 @@ __device__ __forceinline__ void relayout(R* re, R* im, C2<R>* lds, const Layout&
     *reinterpret_cast<JQ_LDS R*>((uintptr_t)a) = re[i];
     *reinterpret_cast<JQ_LDS R*>((uintptr_t)(a + IM)) = im[i];
   }
-  __syncthreads();
+}
+
+template <typename R, int RB>
+__device__ __forceinline__ void relayout_get(R* re, R* im, C2<R>* lds, const Layout& to, int region) {
+  constexpr int NR = 1 << RB;
+  constexpr uint32_t ES = sizeof(R);
+  uint32_t wt[RB];
+#pragma unroll
+  for (int b = 0; b < RB; ++b) wt[b] = __builtin_amdgcn_readfirstlane(lds_swz(1u << to.rpos[b]) * ES);
+  constexpr uint32_t IM = ES << kTileBits<R>;
+  const uint32_t lb = (uint32_t)(uintptr_t)(JQ_LDS char*)(char*)lds + (uint32_t)region * 2 * IM;
+  const uint32_t st = lb + lds_swz(to.toff) * ES;
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
     uint32_t o = 0;
@@ -1143,7 +1152,28 @@ __device__ __forceinline__ void relayout(R* re, R* im, C2<R>* lds, const Layout&
     re[i] = *reinterpret_cast<const JQ_LDS R*>((uintptr_t)a);
     im[i] = *reinterpret_cast<const JQ_LDS R*>((uintptr_t)(a + IM));
   }
+}
+
+template <typename R, int RB>
+__device__ __forceinline__ void relayout(R* re, R* im, C2<R>* lds, const Layout& from, const Layout& to,
+                                         int region = 0, bool tail_sync = true) {
+  relayout_put<R, RB>(re, im, lds, from, region);
+  __syncthreads();
+  relayout_get<R, RB>(re, im, lds, to, region);
   if (tail_sync) __syncthreads();
+}
+
+// psi and lambda of the backward together (regions 0 and 1): both stores,
+// one barrier, both loads, one barrier.
+template <typename R, int RB>
+__device__ __forceinline__ void relayout_pair(R* pr, R* pi, R* lr, R* li, C2<R>* lds, const Layout& from,
+                                              const Layout& to) {
+  relayout_put<R, RB>(pr, pi, lds, from, 0);
+  relayout_put<R, RB>(lr, li, lds, from, 1);
+  __syncthreads();
+  relayout_get<R, RB>(pr, pi, lds, to, 0);
+  relayout_get<R, RB>(lr, li, lds, to, 1);
+  __syncthreads();
 }
 
 __device__ __forceinline__ uint64_t tile_base(int tile, cint* tbits, int T, int n) {
@@ -1396,8 +1426,7 @@ template <typename R, int RB, int LNT, int... W>
 __device__ __forceinline__ void jit_relayout2(R* pr, R* pi, R* lr, R* li, C2<R>* lds, Layout& L) {
   Layout nl;
   layout_lit<RB, LNT, W...>(L.tbits, nl);
-  relayout<R, RB>(pr, pi, lds, L, nl, 0, false);
-  relayout<R, RB>(lr, li, lds, L, nl, 1, true);
+  relayout_pair<R, RB>(pr, pi, lr, li, lds, L, nl);
   L = nl;
 }
 
@@ -1605,8 +1634,7 @@ struct InterpBody {
       if (!same_layout<RB>(sub, L)) {
         Layout nl;
         decode_layout<RB, LNT>(sub, tbits, nl);
-        relayout<R, RB>(pr, pi, lds, L, nl, 0, false);  // psi: region 0, lambda: region 1
-        relayout<R, RB>(lr, li, lds, L, nl, 1, true);
+        relayout_pair<R, RB>(pr, pi, lr, li, lds, L, nl);  // psi: region 0, lambda: region 1
         L = nl;
       }
       const int ng = sub[0];
@@ -1772,8 +1800,7 @@ __device__ __forceinline__ void to_canonical2(R* pr, R* pi, R* lr, R* li, cint* 
 #pragma unroll
   for (int b = 0; b < RB; ++b) same = same && (cl.rpos[b] == L.rpos[b]);
   if (!same) {
-    relayout<R, RB>(pr, pi, lds, L, cl, 0, false);
-    relayout<R, RB>(lr, li, lds, L, cl, 1, true);
+    relayout_pair<R, RB>(pr, pi, lr, li, lds, L, cl);
   }
   L = cl;  // also when no exchange was needed: substep layouts carry no gthr
 }
