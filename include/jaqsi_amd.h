@@ -143,7 +143,9 @@ int jq_plan_tables(const jq_plan* plan, int32_t which, int32_t* out, int64_t cap
  *                         module; fails with the compiler log on error;
  *        JQ_JIT_QUERY     change nothing.
  *   *phase (optional) receives JQ_JIT_NONE / _COMPILING / _COMPILED /
- *   _LOADED / _FAILED.
+ *   _LOADED / _FAILED.  A plan with a pass of more than 256 gates (a deep
+ *   circuit on few wires) declines specialisation: _FAILED without an
+ *   error, and the prebuilt kernels run it.
  * Without a call, plans on JQ_JIT_MIN_WIRES or more wires start in
  * JQ_JIT_ASYNC and smaller ones in JQ_JIT_OFF; the environment variable
  * JQ_JIT=0 / 1 overrides that default. */

@@ -128,6 +128,7 @@ struct JitState {
   std::vector<int> resident;              // per kernel id: resident workgroups on the device
   bool persist[2] = {false, true};        // persistent tile loops (forward, backward)
   bool dma = false;                       // forward streams tiles into LDS (two tiles of LDS)
+  bool declined = false;                  // passes too long to specialise: the interpreter runs the plan
   int nt[2] = {0, 0};
 };
 
@@ -176,7 +177,29 @@ JitCache& jit_cache() {
 
 // Compile the plan's specialised kernels that the cache lacks (host only;
 // any thread).
+// Longest pass (in gates) that is specialised: straight-line code grows
+// with the gate count and so does the compile time; longer passes (deep
+// circuits on few wires) stay with the interpreter.
+constexpr int kJitMaxPassGates = 256;
+
 void jit_build(jq_plan* p) {
+  {
+    const int* W = p->hp.words.data();
+    int longest = 0;
+    for (int q = 0; q < p->hp.n_passes; ++q) {
+      const int* pass = W + W[jq::HDR_WORDS + q];
+      int ng = 0;
+      for (int s = 0; s < pass[0]; ++s) ng += W[pass[4 + p->hp.T + s]];
+      longest = std::max(longest, ng);
+    }
+    if (longest > kJitMaxPassGates) {
+      p->jit.log = "a pass of " + std::to_string(longest) + " gates exceeds the specialisation limit (" +
+                   std::to_string(kJitMaxPassGates) + "); the interpreter runs this plan";
+      p->jit.declined = true;
+      p->jit.phase.store(JQ_JIT_FAILED, std::memory_order_release);
+      return;
+    }
+  }
   jq::JitSpec spec;
   spec.words_f = &p->hp.words;
   spec.words_b = &p->words_bj;
@@ -362,9 +385,13 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   if (T == 0) T = desc->dtype == 0 ? 13 : 12;
   const bool okT = desc->dtype == 0 ? (T == 13) : (T == 12);
   if (!okT) return fail(JQ_ERR_ARG, "tile_bits must be 13 (complex64) or 12 (complex128)");
+  // LDS per backward workgroup: two state tiles, the observable table, 64
+  // expectation reals and the pass's cotangent slots.
+  const size_t lds_state = (size_t)elem_bytes(desc->dtype) << T;
+  const int slot_budget = (int)((160 * 1024 - 2 * lds_state - sizeof(jq::ObsTable)) / 8) - 64;
   jq_plan* p = new jq_plan();
   try {
-    jq::build_program(*desc, T, RB, RBb, p->hp);
+    jq::build_program(*desc, T, RB, RBb, p->hp, slot_budget);
   } catch (const jq::PlanError& e) {
     delete p;
     return fail(e.code, e.what());
@@ -376,8 +403,7 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   p->n_params = desc->n_params;
   p->n_matrix_elems = desc->n_matrix_elems;
   p->checkpoints = desc->checkpoints;
-  // per-pass slot capacity in LDS
-  const size_t lds_state = (size_t)elem_bytes(p->dtype) << T;
+  // per-pass slot capacity in LDS (the planner keeps every pass within it)
   const int* W = p->hp.words.data();
   for (int q = 0; q < p->hp.n_passes; ++q) {
     const int* pass = W + W[jq::HDR_WORDS + q];
@@ -417,7 +443,7 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   if (desc->dtype == 0) {
     jq::HostProgram wide;
     try {
-      jq::build_program(*desc, T, RB, 4, wide);
+      jq::build_program(*desc, T, RB, 4, wide, slot_budget);
     } catch (const std::exception& e) {
       delete p;
       return fail(JQ_ERR_ARG, e.what());
@@ -468,7 +494,8 @@ int jq_plan_jit(jq_plan* p, int32_t mode, int32_t* phase) {
     if (p->jit.worker.joinable()) p->jit.worker.join();
     const int ph = p->jit.phase.load(std::memory_order_acquire);
     if (ph == JQ_JIT_COMPILED) jit_load(p);
-    if (p->jit.phase.load() != JQ_JIT_LOADED) rc = fail(JQ_ERR_DEVICE, "plan-specialised kernels: " + p->jit.log);
+    if (p->jit.phase.load() != JQ_JIT_LOADED && !p->jit.declined)
+      rc = fail(JQ_ERR_DEVICE, "plan-specialised kernels: " + p->jit.log);
   }
   if (phase) *phase = p->jit.phase.load(std::memory_order_acquire);
   return rc;

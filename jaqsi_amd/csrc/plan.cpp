@@ -67,7 +67,7 @@ int gbit(int n, int wire) { return n - 1 - wire; }
 
 }  // namespace
 
-void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgram& out) {
+void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgram& out, int slot_budget) {
   const int n = d.n_wires;
   if (n < 1 || n > 40) throw PlanError(JQ_ERR_ARG, "n_wires must be in [1, 40]");
   // ---- fusion (simulation.py:214-240) ---------------------------------
@@ -277,12 +277,14 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
     }
     Pass P;
     std::vector<int> left;
+    int pslots = 0;  // cotangent reals flushed by this pass
     for (int bi : remaining) {
       HB& b = blocks[bi];
       bool blk = false;
       for (int w : b.wires)
         if (blocked[gbit(n, w)]) blk = true;
-      if (!blk) {
+      const int reals = b.grad ? b.bar_size * 2 : 0;
+      if (!blk && pslots + reals <= slot_budget) {
         int need = 0;
         for (int w : b.targets)
           if (!in_tile[gbit(n, w)]) ++need;
@@ -290,6 +292,7 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
           for (int w : b.targets)
             if (!in_tile[gbit(n, w)]) in_tile[gbit(n, w)] = 1, ++tcount;
           P.blocks.push_back(bi);
+          pslots += reals;
           continue;
         }
       }

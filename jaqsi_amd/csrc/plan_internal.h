@@ -36,6 +36,8 @@ struct HostProgram {
   int n_dev_blocks = 0, n_passes = 0, n_substeps = 0;
 };
 
-void build_program(const jq_plan_desc& d, int T, int RB, int RBb, HostProgram& out);
+// slot_budget: most cotangent reals one pass may flush (they live in LDS
+// next to the tiles); a pass closes early rather than exceed it.
+void build_program(const jq_plan_desc& d, int T, int RB, int RBb, HostProgram& out, int slot_budget);
 
 }  // namespace jq

@@ -221,6 +221,24 @@ def test_dense_three_wire_block_and_state_specialised(specialised):
     np.testing.assert_allclose(st, ref, atol=1e-11)
 
 
+def test_long_passes_stay_with_the_interpreter():
+    """A deep circuit on few wires makes passes of hundreds of gates: the
+    plan declines specialisation (no long compile) and runs interpreted."""
+    import jaqsi_amd as J
+    from jaqsi_amd import _native as nat
+    from jaqsi_amd import lowering
+    from jaqsi_amd.tape import recording
+
+    print("specialised long pass", flush=True)
+    J.enable_x64(False)
+    theta = hea_theta(13, 40, 6)
+    with recording() as tape:
+        record_hea(theta, 13)
+    low = lowering.lower_pure(tape, 13)
+    plan = nat.Plan(13, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0)
+    assert plan.jit(nat.JQ_JIT_SYNC) == nat.JQ_JIT_FAILED
+
+
 def test_specialisation_modes_and_phases():
     import jaqsi_amd as J
     from jaqsi_amd import _native as nat

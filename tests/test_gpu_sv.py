@@ -37,6 +37,20 @@ def test_hea_expval_and_adjoint(n, layers, x64):
 
 
 @pytest.mark.parametrize("x64", [False, True])
+def test_deep_circuit_split_by_slot_budget(x64):
+    """13 wires, 40 layers: one tile holds every wire, so only the LDS budget
+    of the cotangent slots splits the passes."""
+    n, layers = 13, 40
+    print(f"deep hea n={n} layers={layers} x64={x64}", flush=True)
+    theta = hea_theta(n, layers, 4000 + n)
+    ev, grad = _script_expval(n, theta, x64)
+    ops = sv.hea_ops(theta, n)
+    vals, grads, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
+    np.testing.assert_allclose(ev, vals, atol=TOL[x64], rtol=0)
+    np.testing.assert_allclose(grad, sv.flat_grads(ops, grads), atol=TOL[x64] * 10, rtol=0)
+
+
+@pytest.mark.parametrize("x64", [False, True])
 def test_gate_zoo_state_and_gradients(x64):
     import jaqsi_amd as J
 

@@ -86,3 +86,26 @@ def test_planner_errors_map_to_exceptions():
         nat.Plan(4, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0)
     with pytest.raises(ValueError):
         nat.Plan(2, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0)
+
+
+def test_deep_circuit_passes_stay_within_the_slot_budget():
+    """A deep circuit on few wires (one tile holds every wire) is split into
+    passes whose cotangent slots fit the LDS next to the two state tiles, and
+    the split program still reproduces the oracle."""
+    from circuits import record_hea
+    from plan_interp import Program
+
+    n, layers = 11, 48  # 528 fused 2x2 blocks x 8 reals > the ~3.8K reals of one pass
+    theta = np.random.default_rng(77).uniform(0, 2 * np.pi, (layers, n, 3))
+    plan, angles, xm = _plan(_record(lambda: record_hea(theta, n)), n, 0)
+    prog = Program(plan)
+    budget = (160 * 1024 - 2 * (8 << 13) - 1792) // 8 - 64
+    nslots = [ns for _, _, ns, _, _, _ in prog.passes()]
+    assert len(nslots) >= 2 and max(nslots) <= budget
+    assert sum(nslots) == 2 * 4 * n * layers
+    ops = sv.hea_ops(theta, n)
+    np.testing.assert_allclose(prog.forward(angles, xm), sv.simulate(ops, n), atol=1e-11)
+    zs = sv.z_signs(n).sum(0)
+    _, _, grads, _ = prog.adjoint(angles, lambda f: 2 * zs * np.conj(f), xm)
+    _, og, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
+    np.testing.assert_allclose(grads, sv.flat_grads(ops, og), atol=1e-10)
