@@ -2022,13 +2022,16 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
   // under the layout of the first substep the sweep visits.
   Layout Lc;
   canonical_layout<RB, LNT>(tbits, Lc);
-  Layout Lf;
-  if constexpr (Body::kLoad != 0) Body::template first_layout<RB, LNT>(tbits, Lf);
+  // (the first layout is rebuilt per use from the opaque thread index, so
+  // its offsets are not kept live across the tile loop)
   auto fetch = [&](R* xr, R* xi, const C2<R>* p, uint64_t b) {
-    if constexpr (Body::kLoad != 0)
+    if constexpr (Body::kLoad != 0) {
+      Layout Lf;
+      Body::template first_layout<RB, LNT>(tbits, Lf);
       load_direct<R, RB, Body::kLoad == 1>(xr, xi, p, Lf, b);
-    else
+    } else {
       load_canon<R, RB>(xr, xi, p, a.vec_len, Lc, b, full);
+    }
   };
   R npr[NR], npi[NR], nlr[NR], nli[NR];
   int ncirc = (int)(w >> tlog), ntile = (int)(w & (ntiles - 1));
@@ -2064,7 +2067,7 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
     cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride + a.pool_off;
     Layout L;
     if constexpr (Body::kLoad != 0)
-      L = Lf;
+      Body::template first_layout<RB, LNT>(tbits, L);
     else
       canonical_layout<RB, LNT>(tbits, L);
     if (seed) seed_lambda<R, RB>(pr, pi, lr, li, a, obs, circ, L, base);
