@@ -251,9 +251,13 @@ typedef struct jq_pulse_desc {
 int jq_pulse_propagate(const jq_pulse_desc* desc, const double* params, const double* target,
                        double* unitaries, double* infidelity, double* grad, void* stream);
 
-/* Kernel timing for measurement: when enabled, every tile-pass launch is
- * bracketed by HIP events on its stream; jq_timing_read sums and clears
- * them (which = 0 forward passes, 1 backward passes).  No reference counterpart. */
+/* Kernel timing for measurement: when enabled, each run of consecutive
+ * tile-pass launches of one kind (the forward passes of a call, its backward
+ * passes) is bracketed by a pair of HIP events on its stream; jq_timing_read
+ * returns the summed run durations and the number of launches they cover,
+ * and clears them (which = 0 forward passes, 1 backward passes).  One event
+ * pair per run, not per launch, keeps the measurement from stretching the
+ * step.  For use from one host thread.  No reference counterpart. */
 int jq_timing_enable(int32_t on);
 int jq_timing_read(int32_t which, double* total_ms, int32_t* count);
 

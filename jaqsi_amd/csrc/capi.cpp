@@ -25,10 +25,30 @@
 namespace {
 thread_local std::string g_err;
 
-// Optional per-kernel timing: HIP events around every tile-pass launch.
+// Optional kernel timing: HIP events around each run of consecutive
+// tile-pass launches of one kind (the forward passes of a call, its backward
+// passes).  An event pair per run rather than per launch: event records
+// between kernels stretch a 24-qubit gradient step by about 5%.
 struct KTimer {
+  struct Run {
+    hipEvent_t e0, e1;
+    int launches;
+  };
   bool on = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
+  std::vector<Run> runs[3];
+  Run open[3] = {};          // run in progress per kind (e0 recorded)
+  bool is_open[3] = {false, false, false};
+  std::vector<hipEvent_t> pool;  // events reused across reads
+  hipEvent_t take() {
+    hipEvent_t e = nullptr;
+    if (!pool.empty()) {
+      e = pool.back();
+      pool.pop_back();
+    } else {
+      (void)hipEventCreate(&e);
+    }
+    return e;
+  }
 };
 KTimer g_timer;
 
@@ -60,17 +80,28 @@ hipError_t launch_pass(const JitFn& jf, int variant, int dtype, int T, const jq:
 // which: 0 forward pass, 1 backward pass; `full` selects the kernel variant
 // that also carries 4x4 register blocks and the dense 3-wire path.  With a
 // specialised kernel `jf` the pass runs that kernel instead.
+// `first` / `last`: this launch opens / closes a run of launches of kind
+// `which` on stream `st` (timing only).
 hipError_t timed_launch(int which, bool full, int dtype, int T, const jq::SvArgs& a, int ntiles, int batch,
-                        size_t smem, hipStream_t st, bool wide = false, JitFn jf = JitFn()) {
+                        size_t smem, hipStream_t st, bool wide, JitFn jf, bool first, bool last) {
   const int variant = which + (full ? 2 : 0) + (which == 1 && wide ? 4 : 0);
   if (!g_timer.on) return launch_pass(jf, variant, dtype, T, a, ntiles, batch, smem, st);
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  hipEventRecord(e0, st);
+  KTimer::Run& r = g_timer.open[which];
+  if (first) {
+    if (g_timer.is_open[which]) g_timer.pool.push_back(r.e0);  // a run left open by a failed launch
+    r.e0 = g_timer.take();
+    r.launches = 0;
+    g_timer.is_open[which] = true;
+    hipEventRecord(r.e0, st);
+  }
   hipError_t e = launch_pass(jf, variant, dtype, T, a, ntiles, batch, smem, st);
-  hipEventRecord(e1, st);
-  g_timer.ev[which].push_back({e0, e1});
+  if (g_timer.is_open[which]) ++r.launches;
+  if (last && g_timer.is_open[which]) {
+    r.e1 = g_timer.take();
+    hipEventRecord(r.e1, st);
+    g_timer.runs[which].push_back(r);
+    g_timer.is_open[which] = false;
+  }
   return e;
 }
 
@@ -727,7 +758,7 @@ int jq_sv_forward(jq_plan* p, int32_t batch, const double* angles, const void* m
     a.dst = state;
     a.dst_stride = (int64_t)a.vec_len;
     HIP_TRY(timed_launch(0, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st, false,
-                         jit_fn(p, jit, 0, q)));
+                         jit_fn(p, jit, 0, q), q == 0, q == p->hp.n_passes - 1));
   }
   if (a.obs_kind != JQ_OBS_NONE) HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
   return JQ_OK;
@@ -781,7 +812,7 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
       a.dst = pass_out(q);
       a.dst_stride = (int64_t)a.vec_len;
       HIP_TRY(timed_launch(0, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st, false,
-                           jit_fn(p, jit, 0, q)));
+                           jit_fn(p, jit, 0, q), q == 0, q == P - 1));
     }
     if (expvals) HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
     // backward sweep
@@ -794,7 +825,7 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
       a.dst = ck ? nullptr : state;  // without checkpoints psi is un-applied in place
       a.dst_stride = (int64_t)a.vec_len;
       HIP_TRY(timed_launch(1, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, sm, st, p->wide_bwd,
-                           jit_fn(p, jit, 1, q)));
+                           jit_fn(p, jit, 1, q), q == P - 1, q == 0));
     }
   }
   return finish_grads(p, batch, angles, matrices, a.grad_part, a.part_stride, grad_angles, grad_matrices, st);
@@ -932,7 +963,8 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
     const int rows = active_rows(s, full, 0);
     a.step = s;
     a.dir = 0;
-    HIP_TRY(timed_launch(0, full, p0->dtype, p0->hp.T, a, ntiles, rows, smem_f, st));
+    HIP_TRY(timed_launch(0, full, p0->dtype, p0->hp.T, a, ntiles, rows, smem_f, st, false, JitFn(), s == 0,
+                         s == maxP - 1));
   }
   for (int k = 0; k < n_groups; ++k) {
     const jq_group& G = groups[order[k]];
@@ -944,7 +976,8 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
     const int rows = active_rows(s, full, 1);
     a.step = s;
     a.dir = 1;
-    HIP_TRY(timed_launch(1, full, p0->dtype, p0->hp.T, a, ntiles, rows, smem_b, st, p0->wide_bwd));
+    HIP_TRY(timed_launch(1, full, p0->dtype, p0->hp.T, a, ntiles, rows, smem_b, st, p0->wide_bwd, JitFn(), s == 0,
+                         s == maxP - 1));
   }
   for (int k = 0; k < n_groups; ++k) {
     const jq_group& G = groups[order[k]];
@@ -1007,16 +1040,16 @@ int jq_timing_read(int32_t which, double* total_ms, int32_t* count) {
   if (which < 0 || which > 2) return fail(JQ_ERR_ARG, "which must be 0 (forward), 1 (backward) or 2 (fused)");
   double tot = 0;
   int n = 0;
-  for (auto& pr : g_timer.ev[which]) {
-    HIP_TRY(hipEventSynchronize(pr.second));
+  for (auto& r : g_timer.runs[which]) {
+    HIP_TRY(hipEventSynchronize(r.e1));
     float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second));
+    HIP_TRY(hipEventElapsedTime(&ms, r.e0, r.e1));
     tot += ms;
-    ++n;
-    hipEventDestroy(pr.first);
-    hipEventDestroy(pr.second);
+    n += r.launches;
+    g_timer.pool.push_back(r.e0);
+    g_timer.pool.push_back(r.e1);
   }
-  g_timer.ev[which].clear();
+  g_timer.runs[which].clear();
   if (total_ms) *total_ms = tot;
   if (count) *count = n;
   return JQ_OK;
