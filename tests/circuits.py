@@ -1,6 +1,8 @@
 """Shared circuits for the tests: the same gates recorded through jaqsi_amd
 and described for the oracle."""
 
+import functools
+
 import numpy as np
 
 from oracle import sv
@@ -72,3 +74,14 @@ def zoo_ops(p):
 
 # parameter index of every oracle parameter in the flat zoo parameter vector
 ZOO_GRAD_MAP = [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 0]
+
+
+@functools.lru_cache(maxsize=None)
+def noisy_hea_reference(n, layers, seed, p_dep=0.01, gamma=0.005):
+    """Oracle value and parameter-shift gradient of <Z_0> for the noisy ansatz
+    (cached: the density-matrix shift rule is the slow part of a GPU test)."""
+    from oracle import dm
+
+    theta = hea_theta(n, layers, seed)
+    v, g = dm.value_and_shift_grad(dm.noisy_hea(theta, n, p_dep, gamma), n, [(sv.PZ, (0,))], np.array([1.0]))
+    return v, g

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 import torch
 
-from circuits import hea_theta
+from circuits import hea_theta, noisy_hea_reference
 from oracle import dm, sv
 
 pytestmark = pytest.mark.gpu
@@ -38,7 +38,7 @@ def _record_noisy_hea(theta, n, p_dep, gamma):
 
 
 @pytest.mark.parametrize("x64", [False, True])
-@pytest.mark.parametrize("n,layers", [(2, 1), (3, 2), (5, 1)])
+@pytest.mark.parametrize("n,layers", [(2, 1), (3, 2), (5, 1), (7, 1)])
 def test_noisy_hea_expval_and_gradient(n, layers, x64):
     import jaqsi_amd as J
 
@@ -49,7 +49,7 @@ def test_noisy_hea_expval_and_gradient(n, layers, x64):
     script = J.Script(lambda t: _record_noisy_hea(t, n, 0.01, 0.005), n_qubits=n)
     ev = script.execute(type="expval", obs=[J.PauliZ(wires=0, record=False)], args=(th,))
     ev.sum().backward()
-    v, g = dm.value_and_shift_grad(dm.noisy_hea(theta, n, 0.01, 0.005), n, [(sv.PZ, (0,))], np.array([1.0]))
+    v, g = noisy_hea_reference(n, layers, 2000 + n)
     np.testing.assert_allclose(ev.detach().cpu().numpy()[0], v, atol=TOL[x64], rtol=0)
     np.testing.assert_allclose(th.grad.cpu().numpy().reshape(-1), g, atol=TOL[x64] * 10, rtol=0)
 

@@ -5,8 +5,8 @@ import numpy as np
 import pytest
 import torch
 
-from circuits import hea_theta, record_hea, zoo_params
-from oracle import dm, sv
+from circuits import hea_theta, noisy_hea_reference, record_hea, zoo_params
+from oracle import sv
 
 pytestmark = pytest.mark.gpu
 
@@ -174,10 +174,11 @@ def _record_noisy(theta, n, p_dep, gamma):
 
 
 @pytest.mark.parametrize("x64", [False, True])
-def test_noisy_density_matrix_specialised(specialised, x64):
+@pytest.mark.parametrize("n", [5, 7])  # 7 qubits = 14 wires: full tiles, HBM accessed under substep layouts
+def test_noisy_density_matrix_specialised(specialised, n, x64):
     import jaqsi_amd as J
 
-    n, layers = 5, 1
+    layers = 1
     print(f"specialised dm n={n} x64={x64}", flush=True)
     J.enable_x64(x64)
     theta = hea_theta(n, layers, 2000 + n)
@@ -186,7 +187,7 @@ def test_noisy_density_matrix_specialised(specialised, x64):
     ev = script.execute(type="expval", obs=[J.PauliZ(wires=0, record=False)], args=(th,))
     ev.sum().backward()
     _all_loaded(specialised)
-    v, g = dm.value_and_shift_grad(dm.noisy_hea(theta, n, 0.01, 0.005), n, [(sv.PZ, (0,))], np.array([1.0]))
+    v, g = noisy_hea_reference(n, layers, 2000 + n)
     np.testing.assert_allclose(ev.detach().cpu().numpy()[0], v, atol=TOL[x64], rtol=0)
     np.testing.assert_allclose(th.grad.cpu().numpy().reshape(-1), g, atol=TOL[x64] * 10, rtol=0)
 
