@@ -500,6 +500,78 @@ def cpu_baseline_c2(theta):
     }
 
 
+def naive_hea_expval(theta, n):
+    """The plain GPU path the speed-up target is stated against: gate-by-gate
+    complex64 statevector in PyTorch (RX, RY, RZ on every wire in the slice
+    form of the reference's _apply_gate, then the CX ring), the gradient by
+    autograd taping every intermediate state.  Returns sum_q <Z_q>."""
+    import torch
+
+    dev = theta.device
+    psi = torch.zeros(1 << n, dtype=torch.complex64, device=dev)
+    psi[0] = 1
+    eye = torch.eye(2, dtype=torch.complex64, device=dev)
+    X = torch.tensor([[0, 1], [1, 0]], dtype=torch.complex64, device=dev)
+    Y = torch.tensor([[0, -1j], [1j, 0]], dtype=torch.complex64, device=dev)
+    Z = torch.tensor([[1, 0], [0, -1]], dtype=torch.complex64, device=dev)
+
+    def rot(P, t):  # cos(t/2) I - i sin(t/2) P, differentiable in t
+        c, sn = torch.cos(t / 2).to(torch.complex64), torch.sin(t / 2).to(torch.complex64)
+        return c * eye - 1j * sn * P
+
+    def apply1(psi, G, q):  # wire q = bit n-1-q; out_i = sum_j G[i, j] slice_j (simulation.py:70-112)
+        v = psi.reshape(1 << q, 2, 1 << (n - 1 - q))
+        v0, v1 = v[:, 0, :], v[:, 1, :]
+        return torch.stack([G[0, 0] * v0 + G[0, 1] * v1, G[1, 0] * v0 + G[1, 1] * v1], dim=1).reshape(-1)
+
+    def cx(psi, c, t):
+        v = psi.reshape([2] * n)
+        flipped = torch.flip(v, dims=[t])
+        idx = [slice(None)] * n
+        idx[c] = 1
+        out = v.clone()
+        out[tuple(idx)] = flipped[tuple(idx)]
+        return out.reshape(-1)
+
+    for layer in range(theta.shape[0]):
+        for q in range(n):
+            psi = apply1(psi, rot(X, theta[layer, q, 0]), q)
+            psi = apply1(psi, rot(Y, theta[layer, q, 1]), q)
+            psi = apply1(psi, rot(Z, theta[layer, q, 2]), q)
+        for q in range(n):
+            psi = cx(psi, q, (q + 1) % n)
+    p = (psi.real ** 2 + psi.imag ** 2).reshape([2] * n)
+    tot = 0
+    for q in range(n):
+        m = p.sum(dim=[d for d in range(n) if d != q])
+        tot = tot + (m[0] - m[1])
+    return tot
+
+
+def naive_gpu_c2(theta, steps=2):
+    import torch
+
+    n = theta.shape[1]
+    th = torch.tensor(theta, dtype=torch.float32, device="cuda", requires_grad=True)
+    naive_hea_expval(th, n).backward()  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        th.grad = None
+        naive_hea_expval(th, n).backward()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return {
+        "value": round(1.0 / dt, 3),
+        "unit": "gradient evaluations/s",
+        "kind": "plain PyTorch on the same GPU",
+        "sample": f"the full 24-qubit, 8-layer workload, complex64: every gate applied to the whole state in the "
+        f"reference's slice form (out_i = sum_j G_ij slice_j), the CX ring as flips, autograd over the taped states; "
+        f"{steps} timed gradients after one warm-up",
+        "seconds_per_gradient": round(dt, 4),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -510,6 +582,7 @@ def main():
     ap.add_argument("--c3-steps", type=int, default=5)
     ap.add_argument("--no-c4", action="store_true", help="skip the 12-qubit noisy density-matrix leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the pulse-propagator leg")
+    ap.add_argument("--no-naive", action="store_true", help="skip the plain-PyTorch GPU comparison of the headline")
     ap.add_argument(
         "--legs",
         default="c2,c3,c4,c5",
@@ -590,8 +663,15 @@ def main():
         },
         "roofline": roofline_c2(res),
     }
+    print(f"c2 leg done: {value:.3f} gradient evaluations/s", file=sys.stderr, flush=True)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_c2(res["theta"])
+        print("c2 cpu baseline done", file=sys.stderr, flush=True)
+    if not args.no_naive:
+        nb = naive_gpu_c2(res["theta"])
+        nb["speedup"] = round(value / nb["value"], 1)
+        line["naive_gpu_baseline"] = nb
+        print("naive gpu comparison done", file=sys.stderr, flush=True)
     if not args.no_c3 and "c3" in legs:
         r3 = bench_c3(args.c3_steps, 1)
         c3 = {
@@ -616,8 +696,10 @@ def main():
         if not args.no_cpu:
             c3["cpu_baseline"] = cpu_baseline_c3(r3)
         line["c3"] = c3
+        print("c3 leg done", file=sys.stderr, flush=True)
     if not args.no_c4 and "c4" in legs:
         line["c4"] = bench_c4(3, 1)
+        print("c4 leg done", file=sys.stderr, flush=True)
     if not args.no_c5 and "c5" in legs:
         c5 = bench_c5(3, 1)
         if not args.no_cpu:

@@ -136,3 +136,24 @@ def test_parity_groups_extend_the_parameter_layout():
             [b for b, _ in groups][runner.groups.index((plan, mats, words, d, W, nt, extra))]), 2).angle_values)
     # the Y rotation is the only source of an extra angle
     assert sum(e.numel() for *_, e in runner.groups) == 1
+
+
+def test_naive_gpu_comparator_matches_the_oracle():
+    """The bench's plain-PyTorch comparator (gate by gate, autograd) computes
+    the same objective and gradient as the oracle (run here on the CPU)."""
+    import sys
+    from pathlib import Path
+
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
+    import bench
+    from oracle import sv
+
+    n, layers = 6, 2
+    theta = np.random.default_rng(5).uniform(0, 2 * np.pi, (layers, n, 3))
+    th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    f = bench.naive_hea_expval(th, n)
+    f.backward()
+    ops = sv.hea_ops(theta, n)
+    vals, grads, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
+    np.testing.assert_allclose(f.item(), vals.sum(), atol=1e-6)
+    np.testing.assert_allclose(th.grad.numpy().reshape(-1), sv.flat_grads(ops, grads), atol=1e-6)
