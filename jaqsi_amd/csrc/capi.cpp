@@ -658,7 +658,7 @@ int run_prep(jq_plan* p, int batch, const double* angles, const void* matrices, 
   pa.class_mask = p->class_mask;
   if (p->n_params > 0 && !angles) return fail(JQ_ERR_ARG, "angles required");
   if (p->n_matrix_elems > 0 && !matrices) return fail(JQ_ERR_ARG, "explicit matrices required");
-  HIP_TRY(hipMemsetAsync(p->status.p, 0, sizeof(int), st));
+  HIP_TRY(jq::launch_zero(p->status.p, sizeof(int), st));
   HIP_TRY(jq::launch_prep(p->dtype, pa, st));
   return JQ_OK;
 }
@@ -693,9 +693,9 @@ int finish_grads(jq_plan* p, int batch, const double* angles, const void* matric
   const int ntiles = p->hp.ntiles;
   const int nreals = p->hp.bars * 2;
   if (grad_angles && p->n_params > 0)
-    HIP_TRY(hipMemsetAsync(grad_angles, 0, (size_t)batch * p->n_params * 8, st));
+    HIP_TRY(jq::launch_zero(grad_angles, (size_t)batch * p->n_params * 8, st));
   if (grad_matrices && p->n_matrix_elems > 0)
-    HIP_TRY(hipMemsetAsync(grad_matrices, 0, (size_t)batch * p->n_matrix_elems * 16, st));
+    HIP_TRY(jq::launch_zero(grad_matrices, (size_t)batch * p->n_matrix_elems * 16, st));
   if (nreals > 0 && (grad_angles || grad_matrices)) {
     HIP_TRY(p->bars.ensure((size_t)batch * nreals * 8));
     jq::ReduceArgs r;
@@ -1026,7 +1026,7 @@ int jq_expval_matrix(int32_t n_qubits, int32_t batch, int32_t dtype, int32_t den
   a.mat = matrix;
   a.out = out;
   hipStream_t st = (hipStream_t)stream;
-  HIP_TRY(hipMemsetAsync(out, 0, (size_t)batch * 8, st));
+  HIP_TRY(jq::launch_zero(out, (size_t)batch * 8, st));
   HIP_TRY(jq::launch_expval_matrix(dtype, a, batch, st));
   return JQ_OK;
 }

@@ -67,6 +67,8 @@ struct ExpvalArgs {
 hipError_t launch_prep(int dtype, const PrepArgs& a, hipStream_t st);
 hipError_t launch_reduce_grad(const ReduceArgs& a, hipStream_t st);
 hipError_t launch_reduce_ev(const double* part, double* out, int ntiles, int n_obs, int batch, hipStream_t st);
+// Zero `bytes` (a multiple of 4, 4-byte aligned) with a kernel.
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t st);
 hipError_t launch_chain(const ChainArgs& a, hipStream_t st);
 hipError_t launch_probs(int dtype, const void* s, void* p, int64_t total, hipStream_t st);
 hipError_t launch_outer(int dtype, const void* s, void* rho, int64_t dim, int64_t batch, hipStream_t st);

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "jaqsi_amd.h"
 #include "prep.h"
 #include "sv_kernels.h"
@@ -580,6 +582,23 @@ hipError_t launch_reduce_grad(const ReduceArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(reduce_grad_thread_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
   else
     hipLaunchKernelGGL(reduce_grad_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// Zero fill by kernel rather than hipMemsetAsync: the entry points stay
+// safe to record into a HIP graph (memset nodes have given wrong results on
+// replays after the first).
+__global__ void zero_words_kernel(uint32_t* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return hipSuccess;
+  if (bytes % 4 != 0 || ((uintptr_t)p & 3u) != 0) return hipErrorInvalidValue;
+  const int64_t n = (int64_t)(bytes / 4);
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (uint32_t*)p, n);
   return hipGetLastError();
 }
 
