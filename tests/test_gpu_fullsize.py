@@ -135,3 +135,41 @@ def test_pulse_batch_at_full_size_sampled_against_the_oracle():
         fm = pulse.infidelity(pulse.propagate(pm, omega[s], pulse.J_C5, pulse.T_C5, pulse.N_C5), V)
         np.testing.assert_allclose(g[s, k], (fp - fm) / (2 * h), atol=2e-7, rtol=1e-6)
     print(f"  max |dU| {worst:.2e}", flush=True)
+
+
+def test_noisy_density_matrix_at_full_size_against_finite_differences():
+    """12 qubits, 6 layers, 720 Kraus channels (rho 4096 x 4096 as a 24-wire
+    vector), complex128: the adjoint gradient of <Z_0> against central
+    differences of the forward for three angles, and Tr rho = 1."""
+    import jaqsi_amd as J
+    from test_gpu_dm import _record_noisy_hea
+
+    print("full size noisy dm 12q", flush=True)
+    n, layers = 12, 6
+    J.enable_x64(True)
+    theta = np.random.default_rng(1004).uniform(0, 2 * np.pi, (layers, n, 3))
+    script = J.Script(lambda t: _record_noisy_hea(t, n, 0.01, 0.005), n_qubits=n)
+    obs = [J.PauliZ(wires=0, record=False)]
+    th = torch.tensor(theta, dtype=torch.float64, device="cuda", requires_grad=True)
+    ev = script.execute(type="expval", obs=obs, args=(th,))
+    ev.sum().backward()
+    grad = th.grad.cpu().numpy().reshape(-1)
+    probs = script.execute(type="probs", args=(th.detach(),))
+    assert abs(float(probs.sum()) - 1.0) < 1e-10
+
+    def f(x):
+        t = torch.tensor(x, dtype=torch.float64, device="cuda")
+        return float(script.execute(type="expval", obs=obs, args=(t,)).sum())
+
+    flat = theta.reshape(-1)
+    worst = 0.0
+    for k in (0, flat.size // 2, flat.size - 1):
+        h = 1e-5
+        xp, xm = flat.copy(), flat.copy()
+        xp[k] += h
+        xm[k] -= h
+        fd = (f(xp.reshape(theta.shape)) - f(xm.reshape(theta.shape))) / (2 * h)
+        worst = max(worst, abs(grad[k] - fd))
+        assert abs(grad[k] - fd) < 1e-8, (k, grad[k], fd)
+    J.enable_x64(False)
+    print(f"  <Z0> {float(ev.detach().sum()):.6e}, max |adjoint - fd| {worst:.2e}", flush=True)
