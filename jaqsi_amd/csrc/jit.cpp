@@ -378,7 +378,7 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
       if (dma)
         o << "extern \"C\" __global__ __launch_bounds__(" << nt << ", " << occ << ") void " << kname
           << "(SvArgs a) {\n  sv_forward_dma<" << rt << ", " << pv.RB << ", " << nt << ", JF" << id
-          << ", false>(a);\n}\n\n";
+          << ">(a);\n}\n\n";
       else
         o << "extern \"C\" __global__ __launch_bounds__(" << nt << ", " << occ << ") void " << kname
           << "(SvArgs a) {\n  sv_forward_impl<" << rt << ", " << pv.RB << ", " << nt << ", false, JF" << id

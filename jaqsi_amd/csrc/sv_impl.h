@@ -1878,20 +1878,18 @@ __global__ __launch_bounds__(NT) void sv_backward_kernel(SvArgs a0) {
 // barriers inside a pass wait on LDS only, so the loads stay in flight
 // behind the gate arithmetic and the relayouts.  Results and partial sums
 // are those of the one-tile-per-workgroup kernels.
-// Persistent forward whose next tile streams into LDS (global_load_lds:
-// HBM -> LDS without registers) while the current tile's gates run.  LDS:
-// the re-layout tile, then the prefetch tile, then the observable table.
-// Each lane of the canonical layout moves its 16-byte amplitude pair to
-// LDS byte 16 * (pair index * NT + thread), which is byte 8 * x of the
-// amplitude at tile offset x: the prefetch tile lies in natural order, so a
-// thread reads its amplitudes under ANY layout with one 8-byte LDS read
-// each -- the pass starts directly in its first substep's layout
-// (Body::first_layout, Body::fwd0) instead of loading canonically and
-// re-laying out.  Complex64 with whole tiles only (vector >= tile).
-// SEP = false: one tile per workgroup and no separate prefetch tile -- the
-// tile streams into the re-layout region itself (read out before the first
-// re-layout), so the workgroup needs one tile of LDS and two fit per CU.
-template <typename R, int RB, int NT, typename Body, bool SEP = true>
+// Forward of one tile per workgroup whose tile streams into LDS
+// (global_load_lds: HBM -> LDS without registers).  Each lane of the
+// canonical layout moves its 16-byte amplitude pair to LDS byte
+// 16 * (pair index * NT + thread), which is byte 8 * x of the amplitude at
+// tile offset x: the tile lies in natural order, so a thread reads its
+// amplitudes under ANY layout with one 8-byte LDS read each -- the pass
+// starts directly in its first substep's layout (Body::first_layout,
+// Body::fwd0) instead of loading canonically and re-laying out.  The tile
+// lands in the re-layout region itself (read out before the first
+// re-layout), so a workgroup needs one tile of LDS and two fit per CU.
+// Complex64 with whole tiles only (vector >= tile).
+template <typename R, int RB, int NT, typename Body>
 __device__ __forceinline__ void sv_forward_dma(const SvArgs& a) {
   static_assert(sizeof(R) == 4, "the natural-order prefetch tile is laid out for complex64");
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1902,14 +1900,14 @@ __device__ __forceinline__ void sv_forward_dma(const SvArgs& a) {
   const int T = P[2];
   const int ntiles = P[8];
   C2<R>* lds = reinterpret_cast<C2<R>*>(smem);
-  unsigned char* pre = SEP ? smem + (sizeof(C2<R>) << T) : smem;
+  unsigned char* pre = smem;
   ObsLds* obs = reinterpret_cast<ObsLds*>(pre + (sizeof(C2<R>) << T));
   double* ev = reinterpret_cast<double*>(obs + 1);
   cint* pass = P + P[HDR_WORDS + a.pass];
   cint* tbits = pass + 4;
   const bool last = (pass[1] & PF_LAST) != 0;
   const bool want_ev = last && a.obs_kind != JQ_OBS_NONE && a.ev_part != nullptr;
-  int tile = blockIdx.x;
+  const int tile = blockIdx.x;
   if (tile >= ntiles) return;
   if (want_ev) {
     for (int o = threadIdx.x; o < a.n_obs; o += NT) ev[o] = 0;
@@ -1921,7 +1919,7 @@ __device__ __forceinline__ void sv_forward_dma(const SvArgs& a) {
   const bool dma = full && !from_zero && a.vec_len <= (1ull << 27);  // 32-bit per-thread byte offsets
   const C2<R>* src = from_zero ? nullptr : reinterpret_cast<const C2<R>*>(a.src) + (int64_t)circ * a.src_stride;
   C2<R>* dst = reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride;
-  // issue the LDS loads of one tile (canonical layout: pair r of thread t)
+  // issue the LDS loads of the tile (canonical layout: pair r of thread t)
   auto prefetch = [&](uint64_t b) {
     Layout Lc;
     canonical_layout<RB, LNT>(tbits, Lc);
@@ -1935,10 +1933,9 @@ __device__ __forceinline__ void sv_forward_dma(const SvArgs& a) {
                                        16, 0, 0);
     }
   };
-  uint64_t nbase = tile_base(tile, tbits, T, a.n_wires);
-  if (dma) prefetch(nbase);
-  for (;;) {
-    const uint64_t base = nbase;
+  const uint64_t base = tile_base(tile, tbits, T, a.n_wires);
+  if (dma) prefetch(base);
+  {
     R re[NR], im[NR];
     Layout L;
     bool started = false;  // registers already hold the first substep's layout
@@ -1962,11 +1959,6 @@ __device__ __forceinline__ void sv_forward_dma(const SvArgs& a) {
       else
         load_canon<R, RB>(re, im, src, a.vec_len, L, base, full);
     }
-    const int next = SEP ? tile + (int)gridDim.x : ntiles;
-    if (next < ntiles) {
-      nbase = tile_base(next, tbits, T, a.n_wires);
-      if (dma) prefetch(nbase);
-    }
     if (started)
       Body::template fwd0<R, RB, NT, false>(P, a.pass, re, im, L, base, lds, pool);
     else
@@ -1987,8 +1979,6 @@ __device__ __forceinline__ void sv_forward_dma(const SvArgs& a) {
         ev[o] = 0;
       }
     }
-    if (next >= ntiles) break;
-    tile = next;
   }
 }
 
