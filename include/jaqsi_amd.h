@@ -203,7 +203,7 @@ typedef struct jq_group {
  * (e.g. uneven depths) of the same width and dtype, observables shared.
  * With the interpreter, every pass step is ONE launch over all circuits that
  * still have a pass at that step.  When every group's plan has its
- * plan-specialised kernels, each group runs its own adjoint on one of four
+ * plan-specialised kernels, each group runs its own adjoint on one of eight
  * side streams forked from `stream` and joined back to it with events (groups
  * sharing a plan share a side stream), so work enqueued on `stream` after
  * the call sees every output.  The reference runs one vmapped

@@ -343,7 +343,7 @@ bool jit_load(jq_plan* p) {
 // Side streams of ragged-batch calls, per device (created once, default
 // priority, non-blocking), with the events that fork and join them.
 struct SideStreams {
-  static constexpr int kStreams = 4;
+  static constexpr int kStreams = 8;
   hipStream_t s[kStreams];
   hipEvent_t fork;
   hipEvent_t join[kStreams];
