@@ -571,6 +571,96 @@ def naive_hea_expval(theta, n):
     return tot
 
 
+def naive_hea_batched(theta, n, features=None):
+    """The same plain path for a batch of circuits of one structure (the
+    reference vmaps a structure group): state (B, 2^n), the RX(x) encoding
+    layer when features (B, n) are given, then the ansatz layers of theta
+    (B, layers, n, 3).  Returns sum_q <Z_q> per circuit, shape (B,)."""
+    import torch
+
+    dev = theta.device
+    B = theta.shape[0]
+    psi = torch.zeros((B, 1 << n), dtype=torch.complex64, device=dev)
+    psi[:, 0] = 1
+
+    def rot(axis, t):  # (B, 2, 2) rotations cos(t/2) I - i sin(t/2) P
+        c, sn = torch.cos(t / 2).to(torch.complex64), torch.sin(t / 2).to(torch.complex64)
+        z = torch.zeros_like(c)
+        if axis == 0:
+            rows = [[c, -1j * sn], [-1j * sn, c]]
+        elif axis == 1:
+            rows = [[c, -sn], [sn, c]]
+        else:
+            rows = [[c - 1j * sn, z], [z, c + 1j * sn]]
+        return rows
+
+    def apply1(psi, G, q):
+        v = psi.reshape(B, 1 << q, 2, 1 << (n - 1 - q))
+        v0, v1 = v[:, :, 0, :], v[:, :, 1, :]
+        g = [[x[:, None, None] for x in r] for r in G]
+        return torch.stack([g[0][0] * v0 + g[0][1] * v1, g[1][0] * v0 + g[1][1] * v1], dim=2).reshape(B, -1)
+
+    def cx(psi, c, t):
+        v = psi.reshape([B] + [2] * n)
+        flipped = torch.flip(v, dims=[1 + t])
+        idx = [slice(None)] * (n + 1)
+        idx[1 + c] = 1
+        out = v.clone()
+        out[tuple(idx)] = flipped[tuple(idx)]
+        return out.reshape(B, -1)
+
+    if features is not None:
+        for q in range(n):
+            psi = apply1(psi, rot(0, features[:, q]), q)
+    for layer in range(theta.shape[1]):
+        for q in range(n):
+            for axis in range(3):
+                psi = apply1(psi, rot(axis, theta[:, layer, q, axis]), q)
+        for q in range(n):
+            psi = cx(psi, q, (q + 1) % n)
+    p = (psi.real ** 2 + psi.imag ** 2).reshape([B] + [2] * n)
+    tot = 0
+    for q in range(n):
+        m = p.sum(dim=[1 + d for d in range(n) if d != q])
+        tot = tot + (m[:, 0] - m[:, 1])
+    return tot
+
+
+def naive_gpu_c3(res, steps=2):
+    """The plain path over the ragged batch: one batched autograd program per
+    depth group, groups one after the other."""
+    import torch
+
+    n, depths, feats, thetas = res["n"], res["depths"], res["feats"], res["thetas"]
+    groups = []
+    for d in sorted(set(int(x) for x in depths)):
+        idx = [b for b in range(len(depths)) if depths[b] == d]
+        th = torch.tensor(np.stack([thetas[b] for b in idx]), dtype=torch.float32, device="cuda", requires_grad=True)
+        x = torch.tensor(feats[idx], dtype=torch.float32, device="cuda")
+        groups.append((th, x))
+
+    def step():
+        for th, x in groups:
+            th.grad = None
+            naive_hea_batched(th, n, x).sum().backward()
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return {
+        "value": round(len(depths) / dt, 1),
+        "unit": "circuits/s",
+        "kind": "plain PyTorch on the same GPU",
+        "sample": f"the full 4096-circuit batch, one batched autograd program per depth group (gates in the "
+        f"reference's slice form, CX as flips), complex64; {steps} timed batch steps after one warm-up",
+        "seconds_per_batch": round(dt, 4),
+    }
+
+
 def naive_gpu_c2(theta, steps=2):
     import torch
 
@@ -721,6 +811,10 @@ def main():
         }
         if not args.no_cpu:
             c3["cpu_baseline"] = cpu_baseline_c3(r3)
+        if not args.no_naive:
+            nb = naive_gpu_c3(r3)
+            nb["speedup"] = round(c3["value"] / nb["value"], 1)
+            c3["naive_gpu_baseline"] = nb
         line["c3"] = c3
         print("c3 leg done", file=sys.stderr, flush=True)
     if not args.no_c4 and "c4" in legs:

@@ -157,3 +157,27 @@ def test_naive_gpu_comparator_matches_the_oracle():
     vals, grads, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
     np.testing.assert_allclose(f.item(), vals.sum(), atol=1e-6)
     np.testing.assert_allclose(th.grad.numpy().reshape(-1), sv.flat_grads(ops, grads), atol=1e-6)
+
+
+def test_naive_batched_comparator_matches_the_oracle():
+    """The bench's batched plain-PyTorch comparator (one structure group, with
+    the RX(x) encoding layer) against the oracle, circuit by circuit."""
+    import sys
+    from pathlib import Path
+
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
+    import bench
+    from oracle import sv
+
+    n, layers, B = 5, 2, 3
+    rng = np.random.default_rng(6)
+    theta = rng.uniform(0, 2 * np.pi, (B, layers, n, 3))
+    x = rng.uniform(-np.pi, np.pi, (B, n))
+    th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    f = bench.naive_hea_batched(th, n, torch.tensor(x))
+    f.sum().backward()
+    for b in range(B):
+        ops = sv.hea_ops(theta[b], n, features=x[b])
+        vals, grads, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
+        np.testing.assert_allclose(f[b].item(), vals.sum(), atol=1e-6)
+        np.testing.assert_allclose(th.grad[b].numpy().reshape(-1), sv.flat_grads(ops, grads)[n:], atol=1e-6)
