@@ -81,15 +81,20 @@ def test_sv_adjoint_replays_from_a_graph(specialised):
             np.testing.assert_allclose(ga.cpu().numpy(), eager[k][1].cpu().numpy(), atol=1e-5, rtol=0)
 
 
-def test_ragged_groups_replay_from_a_graph():
+@pytest.mark.parametrize("specialised", [False, True])
+def test_ragged_groups_replay_from_a_graph(specialised):
+    """Prebuilt kernels: one launch per pass step over all groups (group table
+    in kernel arguments); specialised: each group on a side stream."""
     from jaqsi_amd import _native as nat
     from jaqsi_amd.ragged import RaggedBatch, record
 
-    print("graph ragged groups", flush=True)
+    print(f"graph ragged groups specialised={specialised}", flush=True)
     depths, per = (2, 3), (5, 3)
     tapes = [record(record_hea, np.zeros((dd, N, 3)), N) for dd in depths]
     rb = RaggedBatch(tapes, list(per), N, [(q, 1.0, -1.0) for q in range(N)])
-    assert all(p == nat.JQ_JIT_LOADED for p in rb.specialise(nat.JQ_JIT_SYNC))
+    phases = rb.specialise(nat.JQ_JIT_SYNC if specialised else nat.JQ_JIT_OFF)
+    if specialised:
+        assert all(p == nat.JQ_JIT_LOADED for p in phases)
     dev = torch.device("cuda")
     rng = np.random.default_rng(9)
     sets = [[torch.tensor(rng.uniform(0, 2 * np.pi, (b, dd * N * 3)), dtype=torch.float64, device=dev)
