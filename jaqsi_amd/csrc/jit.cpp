@@ -342,6 +342,15 @@ std::string jit_key(const JitSpec& spec, const JitPlan& plan, int id) {
          " " + std::to_string(k.store) + "\n" + k.body;
 }
 
+// `body` with its last occurrence of `from` replaced by `to`; unchanged when
+// absent or when a dense 3-wire gate (which exchanges through the same LDS
+// region) follows it.
+std::string replace_last(std::string body, const std::string& from, const std::string& to) {
+  const size_t at = body.rfind(from);
+  if (at != std::string::npos && body.find("dense3", at) == std::string::npos) body.replace(at, from.size(), to);
+  return body;
+}
+
 std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vector<int>& ids,
                        const std::vector<std::string>& names) {
   std::ostringstream o;
@@ -360,6 +369,11 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
       const char* sig =
           "(cint*, int, R* re, R* im, Layout& L, uint64_t base, C2<R>* lds, cmat<R>* pool) {\n"
           "    constexpr int LNT = __builtin_ctz(NT);\n";
+      // with a direct store nothing reads or writes the tile region after the
+      // last exchange (one tile per workgroup): its closing barrier goes
+      const bool nt_last = dma && k.store != 0;
+      const std::string fbody = nt_last ? replace_last(k.body, "jit_relayout<", "jit_relayout_nt<") : k.body;
+      const std::string fbody0 = nt_last ? replace_last(k.body0, "jit_relayout<", "jit_relayout_nt<") : k.body0;
       o << "struct JF" << id << " {\n"
         << "  static constexpr bool kFull = " << (k.full ? "true" : "false") << ";\n"
         << "  static constexpr int kStore = " << (dma ? k.store : 0) << ";\n"
@@ -367,10 +381,10 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
         << "  static __device__ __forceinline__ void last_layout(cint* tb, Layout& L) {\n"
         << "    layout_lit_g<RB, LNT" << (k.last.empty() ? canonical_args(pv) : k.last) << ">(tb, L);\n  }\n"
         << "  template <typename R, int RB, int NT, bool FULL>\n"
-        << "  static __device__ __forceinline__ void fwd" << sig << k.body
+        << "  static __device__ __forceinline__ void fwd" << sig << fbody
         << "    (void)lds; (void)base; (void)pool;\n  }\n"
         << "  template <typename R, int RB, int NT, bool FULL>\n"
-        << "  static __device__ __forceinline__ void fwd0" << sig << k.body0
+        << "  static __device__ __forceinline__ void fwd0" << sig << fbody0
         << "    (void)lds; (void)base; (void)pool;\n  }\n"
         << "  template <int RB, int LNT>\n"
         << "  static __device__ __forceinline__ void first_layout(cint* tb, Layout& L) {\n"
@@ -388,6 +402,10 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
           "(cint*, int, R* pr, R* pi, R* lr, R* li, Layout& L, uint64_t base, C2<R>* lds, cmat<R>* pool, "
           "double* slots) {\n    constexpr int LNT = __builtin_ctz(NT);\n";
       const int load = spec.persist_bwd ? k.load : 0, store = spec.persist_bwd ? k.store : 0;
+      // direct store: the tile loop's barriers after the stores protect the
+      // regions, so the last exchange needs no closing barrier
+      const std::string bbody = store != 0 ? replace_last(k.body, "jit_relayout2<", "jit_relayout2_nt<") : k.body;
+      const std::string bbody0 = store != 0 ? replace_last(k.body0, "jit_relayout2<", "jit_relayout2_nt<") : k.body0;
       o << "struct JB" << id << " {\n"
         << "  static constexpr bool kFull = " << (k.full ? "true" : "false") << ";\n"
         << "  static constexpr int kLoad = " << load << ";\n"
@@ -399,11 +417,11 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
         << "  static __device__ __forceinline__ void last_layout(cint* tb, Layout& L) {\n"
         << "    layout_lit_g<RB, LNT" << (k.last.empty() ? canonical_args(pv) : k.last) << ">(tb, L);\n  }\n"
         << "  template <typename R, int RB, int NT, bool FULL>\n"
-        << "  static __device__ __forceinline__ void bwd" << sig << k.body
+        << "  static __device__ __forceinline__ void bwd" << sig << bbody
         << "    (void)lds; (void)base; (void)pool; (void)slots;\n  }\n";
       if (load != 0)
         o << "  template <typename R, int RB, int NT, bool FULL>\n"
-          << "  static __device__ __forceinline__ void bwd0" << sig << k.body0
+          << "  static __device__ __forceinline__ void bwd0" << sig << bbody0
           << "    (void)lds; (void)base; (void)pool; (void)slots;\n  }\n";
       o << "};\n"
         << "extern \"C\" __global__ __launch_bounds__(" << nt << ") void " << kname << "(SvArgs a) {\n  "

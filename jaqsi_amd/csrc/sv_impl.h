@@ -1164,16 +1164,17 @@ __device__ __forceinline__ void relayout(R* re, R* im, C2<R>* lds, const Layout&
 }
 
 // psi and lambda of the backward together (regions 0 and 1): both stores,
-// one barrier, both loads, one barrier.
+// one barrier, both loads, and (tail_sync) one barrier before the regions
+// may be written again.
 template <typename R, int RB>
 __device__ __forceinline__ void relayout_pair(R* pr, R* pi, R* lr, R* li, C2<R>* lds, const Layout& from,
-                                              const Layout& to) {
+                                              const Layout& to, bool tail_sync = true) {
   relayout_put<R, RB>(pr, pi, lds, from, 0);
   relayout_put<R, RB>(lr, li, lds, from, 1);
   __syncthreads();
   relayout_get<R, RB>(pr, pi, lds, to, 0);
   relayout_get<R, RB>(lr, li, lds, to, 1);
-  __syncthreads();
+  if (tail_sync) __syncthreads();
 }
 
 __device__ __forceinline__ uint64_t tile_base(int tile, cint* tbits, int T, int n) {
@@ -1427,6 +1428,25 @@ __device__ __forceinline__ void jit_relayout2(R* pr, R* pi, R* lr, R* li, C2<R>*
   Layout nl;
   layout_lit<RB, LNT, W...>(L.tbits, nl);
   relayout_pair<R, RB>(pr, pi, lr, li, lds, L, nl);
+  L = nl;
+}
+
+// The last exchange of a pass body whose tile leaves for HBM directly (no
+// later LDS use of the tile region before the next barrier): its closing
+// barrier is dropped.
+template <typename R, int RB, int LNT, int... W>
+__device__ __forceinline__ void jit_relayout_nt(R* re, R* im, C2<R>* lds, Layout& L) {
+  Layout nl;
+  layout_lit<RB, LNT, W...>(L.tbits, nl);
+  relayout<R, RB>(re, im, lds, L, nl, 0, false);
+  L = nl;
+}
+
+template <typename R, int RB, int LNT, int... W>
+__device__ __forceinline__ void jit_relayout2_nt(R* pr, R* pi, R* lr, R* li, C2<R>* lds, Layout& L) {
+  Layout nl;
+  layout_lit<RB, LNT, W...>(L.tbits, nl);
+  relayout_pair<R, RB>(pr, pi, lr, li, lds, L, nl, false);
   L = nl;
 }
 
