@@ -323,7 +323,7 @@ bool jit_load(jq_plan* p) {
   p->jit.resident.assign(p->jit.fn.size(), cus);
   for (size_t id = 0; id < p->jit.fn.size(); ++id) {
     const int dir = p->jit.plan.kernels[id].dir;
-    const size_t sm = dir == 0 ? smem_for(p, 0) : smem_for(p, max_slots(p), 2);
+    const size_t sm = dir == 0 ? smem_for(p, 0) : smem_for(p, 2 * max_slots(p), 2);
     int nb = 0;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, p->jit.fn[id], p->jit.nt[dir], sm) == hipSuccess &&
         nb > 0)
@@ -417,9 +417,10 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   const bool okT = desc->dtype == 0 ? (T == 13) : (T == 12);
   if (!okT) return fail(JQ_ERR_ARG, "tile_bits must be 13 (complex64) or 12 (complex128)");
   // LDS per backward workgroup: two state tiles, the observable table, 64
-  // expectation reals and the pass's cotangent slots.
+  // expectation reals and two buffers of the pass's cotangent slots (the
+  // persistent backward alternates them from tile to tile).
   const size_t lds_state = (size_t)elem_bytes(desc->dtype) << T;
-  const int slot_budget = (int)((160 * 1024 - 2 * lds_state - sizeof(jq::ObsTable)) / 8) - 64;
+  const int slot_budget = ((int)((160 * 1024 - 2 * lds_state - sizeof(jq::ObsTable)) / 8) - 64) / 2;
   jq_plan* p = new jq_plan();
   try {
     jq::build_program(*desc, T, RB, RBb, p->hp, slot_budget);
@@ -438,7 +439,7 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   const int* W = p->hp.words.data();
   for (int q = 0; q < p->hp.n_passes; ++q) {
     const int* pass = W + W[jq::HDR_WORDS + q];
-    if (2 * lds_state + sizeof(jq::ObsTable) + (64 + (size_t)pass[2]) * 8 > 160 * 1024) {
+    if (2 * lds_state + sizeof(jq::ObsTable) + (64 + 2 * (size_t)pass[2]) * 8 > 160 * 1024) {
       delete p;
       return fail(JQ_ERR_UNSUPPORTED, "too many gradient slots in one pass for LDS");
     }
@@ -816,7 +817,7 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
     }
     if (expvals) HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
     // backward sweep
-    const size_t sm = smem_for(p, max_slots(p), 2);
+    const size_t sm = smem_for(p, 2 * max_slots(p), 2);
     a.prog = (const int*)(jit ? p->prog_bj.p : p->prog_b.p);
     for (int q = P - 1; q >= 0; --q) {
       a.pass = q;
@@ -938,7 +939,7 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
     t.batch = G.batch;
     row_begin[k + 1] = row_begin[k] + G.batch;
     maxP = std::max(maxP, P);
-    smem_b = std::max(smem_b, smem_for(p, max_slots(p), 2));
+    smem_b = std::max(smem_b, smem_for(p, 2 * max_slots(p), 2));
   }
   HIP_TRY(p0->groups.ensure(sizeof(jq::SvGroup) * n_groups));
   HIP_TRY(jq::launch_bytes_upload(table.data(), sizeof(jq::SvGroup) * n_groups, p0->groups.p, st));

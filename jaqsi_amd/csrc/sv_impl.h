@@ -2013,7 +2013,10 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
   const int tlog = __builtin_ctz((unsigned)ntiles);
   C2<R>* lds = reinterpret_cast<C2<R>*>(smem);  // psi and lambda tiles (relayout2)
   ObsLds* obs = reinterpret_cast<ObsLds*>(smem + (2 * sizeof(C2<R>) << T));
-  double* slots = reinterpret_cast<double*>(obs + 1);
+  // two cotangent slot buffers, alternating from item to item: a buffer is
+  // read (and cleared) right after its item's flushes and next written two
+  // items later, so no barrier is needed to clear it
+  double* slot_buf = reinterpret_cast<double*>(obs + 1);
   cint* pass = P + P[HDR_WORDS + a.pass];
   cint* tbits = pass + 4;
   const bool seed = (pass[1] & PF_LAST) != 0;
@@ -2024,7 +2027,8 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
   const int64_t total = (int64_t)a.rows << tlog;
   int64_t w = blockIdx.x;
   if (w >= total) return;
-  load_obs(a, obs);
+  for (int e = threadIdx.x; e < 2 * nslots; e += NT) slot_buf[e] = 0;
+  load_obs(a, obs);  // (ends with a barrier)
   const bool full = Body::kFull;  // n_wires >= T, fixed when the kernel is generated
   const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src);
   C2<R>* lam = reinterpret_cast<C2<R>*>(a.lam);
@@ -2048,8 +2052,10 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
   uint64_t nbase = tile_base(ntile, tbits, T, a.n_wires);
   fetch(npr, npi, src + (int64_t)ncirc * a.src_stride, nbase);
   if (!seed) fetch(nlr, nli, lam + (int64_t)ncirc * a.lam_stride, nbase);
+  int par = 0;
   for (;;) {
     const int circ = ncirc, tile = ntile;
+    double* slots = slot_buf + par * nslots;
     const uint64_t base = nbase;
     R pr[NR], pi[NR], lr[NR], li[NR];
 #pragma unroll
@@ -2072,8 +2078,6 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
       fetch(npr, npi, src + (int64_t)ncirc * a.src_stride, nbase);
       if (!seed) fetch(nlr, nli, lam + (int64_t)ncirc * a.lam_stride, nbase);
     }
-    for (int e = threadIdx.x; e < nslots; e += NT) slots[e] = 0;
-    __syncthreads();
     cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride + a.pool_off;
     Layout L;
     if constexpr (Body::kLoad != 0)
@@ -2099,13 +2103,16 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
       to_canonical<R, RB, NT>(lr, li, tbits, L, lds);
       store_canon<R, RB>(lr, li, lam + (int64_t)circ * a.lam_stride, a.vec_len, L, base, full);
     }
-    __syncthreads();
+    __syncthreads();  // every flush of this item has landed
     // partials are real-major ([real][tile]) so the reduction reads them contiguously
     double* gp = a.grad_part + (int64_t)circ * a.part_stride + slot_base + tile;
-    for (int e = threadIdx.x; e < nslots; e += NT) gp[(int64_t)e * ntiles] = slots[e];
+    for (int e = threadIdx.x; e < nslots; e += NT) {
+      gp[(int64_t)e * ntiles] = slots[e];
+      slots[e] = 0;
+    }
     if (next >= total) break;
     w = next;
-    __syncthreads();  // the slots are read above before the next tile clears them
+    par ^= 1;
   }
 }
 
