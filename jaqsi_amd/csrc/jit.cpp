@@ -402,14 +402,40 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
           "(cint*, int, R* pr, R* pi, R* lr, R* li, Layout& L, uint64_t base, C2<R>* lds, cmat<R>* pool, "
           "double* slots) {\n    constexpr int LNT = __builtin_ctz(NT);\n";
       const int load = spec.persist_bwd ? k.load : 0, store = spec.persist_bwd ? k.store : 0;
-      // direct store: the tile loop's barriers after the stores protect the
-      // regions, so the last exchange needs no closing barrier
-      const std::string bbody = store != 0 ? replace_last(k.body, "jit_relayout2<", "jit_relayout2_nt<") : k.body;
-      const std::string bbody0 = store != 0 ? replace_last(k.body0, "jit_relayout2<", "jit_relayout2_nt<") : k.body0;
+      // persistent loop: every exchange after the first of a body opens with
+      // the barrier that the previous one would have closed with; a dense
+      // 3-wire gate (which writes the same LDS region) gets one of its own
+      const bool split = spec.persist_bwd;
+      auto split_body = [&](const std::string& b) {
+        std::string out;
+        bool first = true;
+        size_t pos = 0;
+        for (;;) {
+          const size_t nl = b.find('\n', pos);
+          std::string line = b.substr(pos, nl == std::string::npos ? std::string::npos : nl - pos + 1);
+          if (line.empty()) break;
+          const size_t at = line.find("jit_relayout2<");
+          if (at != std::string::npos) {
+            line.replace(at, 14, std::string("jit_relayout2_split<") + (first ? "false" : "true") + ", ");
+            first = false;
+          } else if (line.find("dense3") != std::string::npos && !first) {
+            out += "    __syncthreads();\n";
+          }
+          out += line;
+          if (nl == std::string::npos) break;
+          pos = nl + 1;
+        }
+        return out;
+      };
+      const std::string bbody = split ? split_body(k.body)
+                                      : (store != 0 ? replace_last(k.body, "jit_relayout2<", "jit_relayout2_nt<") : k.body);
+      const std::string bbody0 = split ? split_body(k.body0)
+                                       : (store != 0 ? replace_last(k.body0, "jit_relayout2<", "jit_relayout2_nt<") : k.body0);
       o << "struct JB" << id << " {\n"
         << "  static constexpr bool kFull = " << (k.full ? "true" : "false") << ";\n"
         << "  static constexpr int kLoad = " << load << ";\n"
         << "  static constexpr int kStore = " << store << ";\n"
+        << "  static constexpr bool kSplit = " << (split ? "true" : "false") << ";\n"
         << "  template <int RB, int LNT>\n"
         << "  static __device__ __forceinline__ void first_layout(cint* tb, Layout& L) {\n"
         << "    layout_lit_g<RB, LNT" << (k.first.empty() ? canonical_args(pv) : k.first) << ">(tb, L);\n  }\n"

@@ -1450,6 +1450,21 @@ __device__ __forceinline__ void jit_relayout2_nt(R* pr, R* pi, R* lr, R* li, C2<
   L = nl;
 }
 
+// Exchanges of the persistent backward with the barrier that frees the LDS
+// regions moved from the end of one exchange to the start of the next
+// (LEAD): the waves run their gates while the slowest finishes its loads.
+// The first exchange of an item needs none (the tile loop's barrier after
+// the previous item's stores precedes it); writers of the regions other
+// than exchanges issue their own barrier first.
+template <bool LEAD, typename R, int RB, int LNT, int... W>
+__device__ __forceinline__ void jit_relayout2_split(R* pr, R* pi, R* lr, R* li, C2<R>* lds, Layout& L) {
+  Layout nl;
+  layout_lit<RB, LNT, W...>(L.tbits, nl);
+  if (LEAD) __syncthreads();
+  relayout_pair<R, RB>(pr, pi, lr, li, lds, L, nl, false);
+  L = nl;
+}
+
 // thread / outside-tile controls of a gate
 __device__ __forceinline__ bool ctl_ok(const Layout& L, uint64_t base, uint32_t cthr, uint64_t cout) {
   return ((L.toff & cthr) == cthr) && ((base & cout) == cout);
@@ -2095,11 +2110,13 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
       store_direct<R, RB, Body::kStore == 1>(lr, li, lam + (int64_t)circ * a.lam_stride, L, base);
       if (pout != nullptr) store_direct<R, RB, Body::kStore == 1>(pr, pi, pout, L, base);
     } else if (pout != nullptr) {
+      if constexpr (Body::kSplit) __syncthreads();  // the body's last exchange has no closing barrier
       to_canonical2<R, RB, NT>(pr, pi, lr, li, tbits, L, lds);
       store_canon<R, RB>(lr, li, lam + (int64_t)circ * a.lam_stride, a.vec_len, L, base, full);
       store_canon<R, RB>(pr, pi, pout, a.vec_len, L, base, full);
     } else {
       // psi is not kept: only lambda goes back to the canonical layout
+      if constexpr (Body::kSplit) __syncthreads();  // the body's last exchange has no closing barrier
       to_canonical<R, RB, NT>(lr, li, tbits, L, lds);
       store_canon<R, RB>(lr, li, lam + (int64_t)circ * a.lam_stride, a.vec_len, L, base, full);
     }
