@@ -342,6 +342,31 @@ std::string jit_key(const JitSpec& spec, const JitPlan& plan, int id) {
          " " + std::to_string(k.store) + "\n" + k.body;
 }
 
+// `body` with every exchange `name<` rewritten to `name_split<LEAD, `: no
+// leading barrier for the first (a barrier precedes the body), one for the
+// others; a dense 3-wire gate after an exchange gets a barrier of its own.
+std::string split_exchanges(const std::string& b, const std::string& name) {
+  std::string out;
+  bool first = true;
+  size_t pos = 0;
+  const std::string from = name + "<";
+  while (pos < b.size()) {
+    const size_t nl = b.find('\n', pos);
+    std::string line = b.substr(pos, nl == std::string::npos ? std::string::npos : nl - pos + 1);
+    const size_t at = line.find(from);
+    if (at != std::string::npos) {
+      line.replace(at, from.size(), name + "_split<" + (first ? "false" : "true") + ", ");
+      first = false;
+    } else if (line.find("dense3") != std::string::npos && !first) {
+      out += "    __syncthreads();\n";
+    }
+    out += line;
+    if (nl == std::string::npos) break;
+    pos = nl + 1;
+  }
+  return out;
+}
+
 // `body` with its last occurrence of `from` replaced by `to`; unchanged when
 // absent or when a dense 3-wire gate (which exchanges through the same LDS
 // region) follows it.
@@ -369,14 +394,15 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
       const char* sig =
           "(cint*, int, R* re, R* im, Layout& L, uint64_t base, C2<R>* lds, cmat<R>* pool) {\n"
           "    constexpr int LNT = __builtin_ctz(NT);\n";
-      // with a direct store nothing reads or writes the tile region after the
-      // last exchange (one tile per workgroup): its closing barrier goes
-      const bool nt_last = dma && k.store != 0;
-      const std::string fbody = nt_last ? replace_last(k.body, "jit_relayout<", "jit_relayout_nt<") : k.body;
-      const std::string fbody0 = nt_last ? replace_last(k.body0, "jit_relayout<", "jit_relayout_nt<") : k.body0;
+      // one tile per workgroup: every exchange after the first opens with the
+      // barrier the previous one would have closed with (a store through the
+      // canonical layout opens with one too)
+      const std::string fbody = dma ? split_exchanges(k.body, "jit_relayout") : k.body;
+      const std::string fbody0 = dma ? split_exchanges(k.body0, "jit_relayout") : k.body0;
       o << "struct JF" << id << " {\n"
         << "  static constexpr bool kFull = " << (k.full ? "true" : "false") << ";\n"
         << "  static constexpr int kStore = " << (dma ? k.store : 0) << ";\n"
+        << "  static constexpr bool kSplit = " << (dma ? "true" : "false") << ";\n"
         << "  template <int RB, int LNT>\n"
         << "  static __device__ __forceinline__ void last_layout(cint* tb, Layout& L) {\n"
         << "    layout_lit_g<RB, LNT" << (k.last.empty() ? canonical_args(pv) : k.last) << ">(tb, L);\n  }\n"
@@ -403,33 +429,11 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
           "double* slots) {\n    constexpr int LNT = __builtin_ctz(NT);\n";
       const int load = spec.persist_bwd ? k.load : 0, store = spec.persist_bwd ? k.store : 0;
       // persistent loop: every exchange after the first of a body opens with
-      // the barrier that the previous one would have closed with; a dense
-      // 3-wire gate (which writes the same LDS region) gets one of its own
+      // the barrier that the previous one would have closed with
       const bool split = spec.persist_bwd;
-      auto split_body = [&](const std::string& b) {
-        std::string out;
-        bool first = true;
-        size_t pos = 0;
-        for (;;) {
-          const size_t nl = b.find('\n', pos);
-          std::string line = b.substr(pos, nl == std::string::npos ? std::string::npos : nl - pos + 1);
-          if (line.empty()) break;
-          const size_t at = line.find("jit_relayout2<");
-          if (at != std::string::npos) {
-            line.replace(at, 14, std::string("jit_relayout2_split<") + (first ? "false" : "true") + ", ");
-            first = false;
-          } else if (line.find("dense3") != std::string::npos && !first) {
-            out += "    __syncthreads();\n";
-          }
-          out += line;
-          if (nl == std::string::npos) break;
-          pos = nl + 1;
-        }
-        return out;
-      };
-      const std::string bbody = split ? split_body(k.body)
+      const std::string bbody = split ? split_exchanges(k.body, "jit_relayout2")
                                       : (store != 0 ? replace_last(k.body, "jit_relayout2<", "jit_relayout2_nt<") : k.body);
-      const std::string bbody0 = split ? split_body(k.body0)
+      const std::string bbody0 = split ? split_exchanges(k.body0, "jit_relayout2")
                                        : (store != 0 ? replace_last(k.body0, "jit_relayout2<", "jit_relayout2_nt<") : k.body0);
       o << "struct JB" << id << " {\n"
         << "  static constexpr bool kFull = " << (k.full ? "true" : "false") << ";\n"

@@ -1450,6 +1450,18 @@ __device__ __forceinline__ void jit_relayout2_nt(R* pr, R* pi, R* lr, R* li, C2<
   L = nl;
 }
 
+// The forward's single-vector exchange with the same barrier placement
+// (one tile per workgroup: the first exchange follows the tile's LDS read
+// and its barrier).
+template <bool LEAD, typename R, int RB, int LNT, int... W>
+__device__ __forceinline__ void jit_relayout_split(R* re, R* im, C2<R>* lds, Layout& L) {
+  Layout nl;
+  layout_lit<RB, LNT, W...>(L.tbits, nl);
+  if (LEAD) __syncthreads();
+  relayout<R, RB>(re, im, lds, L, nl, 0, false);
+  L = nl;
+}
+
 // Exchanges of the persistent backward with the barrier that frees the LDS
 // regions moved from the end of one exchange to the start of the next
 // (LEAD): the waves run their gates while the slowest finishes its loads.
@@ -2002,6 +2014,7 @@ __device__ __forceinline__ void sv_forward_dma(const SvArgs& a) {
       Body::template last_layout<RB, LNT>(tbits, L);
       store_direct<R, RB, Body::kStore == 1>(re, im, dst, L, base);
     } else {
+      if constexpr (Body::kSplit) __syncthreads();  // the body's last exchange has no closing barrier
       to_canonical<R, RB, NT>(re, im, tbits, L, lds);
       store_canon<R, RB>(re, im, dst, a.vec_len, L, base, full);
     }
