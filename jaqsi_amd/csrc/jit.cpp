@@ -76,6 +76,7 @@ std::vector<int> canonical_rpos(const ProgView& pv) {
 
 struct GateLit {
   int kind, ra, rb, hi, lo, k, mat, inv, slot, w0, w1;
+  bool quat;
   uint32_t creg, cthr;
   uint64_t cout;
   GateLit(const int32_t* g, int rebase) {
@@ -88,6 +89,7 @@ struct GateLit {
     mat = g[G_MAT] - rebase;
     inv = g[G_INV] >= 0 ? g[G_INV] - rebase : -1;
     slot = g[G_SLOT];
+    quat = kind == AK_MC1 && g[G_QUAT] != 0;
     w0 = g[G_DW0];
     w1 = g[G_DW0 + 1];
     creg = (uint32_t)g[G_CREG];
@@ -140,8 +142,8 @@ void emit_bwd_gate(std::ostringstream& o, const GateLit& g) {
   const std::string st = std::to_string(g.slot);
   switch (g.kind) {
     case AK_MC1:
-      o << "    jb_mc1<R, RB, " << g.ra << ", " << hex32(g.creg) << ", " << inv << ">(pr, pi, lr, li, " << mp << ", "
-        << ip << ", " << g.ok() << ", slots, " << st << ");\n";
+      o << "    jb_mc1<R, RB, " << g.ra << ", " << hex32(g.creg) << ", " << inv << ", " << (g.quat ? "true" : "false")
+        << ">(pr, pi, lr, li, " << mp << ", " << ip << ", " << g.ok() << ", slots, " << st << ");\n";
       return;
     case AK_MC2:
       o << "    jb_mc2<R, RB, " << g.hi << ", " << g.lo << ", " << hex32(g.creg) << ", " << inv << ", "

@@ -32,6 +32,7 @@ struct HB {  // host block
   int k = 0, D = 0, diagf = 0, offs = 0;
   bool grad = false;
   bool nonunitary = false;
+  bool quat = false;  // MC1 block of the form [[a, -b*], [b, a*]] for every angle
   int mat = -1, inv = -1, bar = -1, bar_size = 0;
   std::vector<int> targets;  // wires that must sit in registers, in matrix order
   std::vector<int> ctrls;    // control wires
@@ -155,6 +156,16 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
       if (t == 1) {
         b.kind = AK_MC1;
         b.D = 2;
+        // Products of Pauli rotations of the target (cos I - i sin P, P in
+        // {X, Y, Z}, conjugated or not) and their angle derivatives all have
+        // the form [[a, -b*], [b, a*]], so Re sum bar_ij dU_ij/dt only needs
+        // bar00 + conj(bar11) and bar10 - conj(bar01).
+        b.quat = !b.nonunitary;
+        for (int m : b.members) {
+          const jq_member& mm = d.members[m];
+          b.quat = b.quat && mm.kind == JQ_M_ROT && !(mm.flags & JQ_MF_SUPEROP) && mm.word >= 1 && mm.word <= 3 &&
+                   mm.nctrl == b.k - 1;
+        }
       } else if (t == 2) {
         b.kind = AK_MC2;
         b.D = 4;
@@ -496,6 +507,7 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
         g[G_INV] = b.inv;
         g[G_SLOT] = slot_of[bi];
         g[G_K] = b.k;
+        g[G_QUAT] = b.kind == AK_MC1 && b.quat ? 1 : 0;
         if (b.kind == AK_DIAG) {
           for (int j = 0; j < b.k; ++j) {
             int gb = gbit(n, b.wires[j]);

@@ -57,6 +57,7 @@ constexpr int G_INV = 9;      // element offset of the inverse (non-unitary), -1
 constexpr int G_SLOT = 10;    // grad slot (reals, within the pass tile area), -1 none
 constexpr int G_K = 11;       // DIAG / DENSEK: number of wires
 constexpr int G_DW0 = 12;     // DIAG wire locators (4 words, 8 bits each: [type:2][pos:6])
+constexpr int G_QUAT = 15;    // MC1: 1 when every factor is a Pauli rotation of the target (quaternion form)
 // DIAG locator types
 constexpr int DL_REG = 0;     // pos = register bit
 constexpr int DL_THR = 1;     // pos = tile-local bit (held by thread bits)

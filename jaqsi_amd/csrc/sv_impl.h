@@ -742,6 +742,23 @@ __device__ __forceinline__ void flush8(const R* v, int nv, double* slots, Dst ds
   if ((lane & 7) == 0 && k < nv) lds_add_lane(&slots[dst(k)], (double)r);
 }
 
+// The same for 4 values: lane bit 5 by permlane32 swaps, bit 4 by one
+// permlane16 swap, then the sum over the 16 lanes of a row (row_ror:8,
+// quad_perm xor 1, xor 2, row_half_mirror).  Lane 16k holds the wave total of
+// v[k] and adds it to slots[dst(k)].
+template <typename R, typename Dst>
+__device__ __forceinline__ void flush4(const R* v, double* slots, Dst dst) {
+  const R w0 = swap_add32(v[0], v[2]);
+  const R w1 = swap_add32(v[1], v[3]);
+  R r = swap_add16(w0, w1);
+  r += dpp_f<0x128>(r);
+  r += dpp_f<0xB1>(r);
+  r += dpp_f<0x4E>(r);
+  r += dpp_f<0x141>(r);
+  const int lane = threadIdx.x & 63;
+  if ((lane & 15) == 0) lds_add_lane(&slots[dst(lane >> 4)], (double)r);
+}
+
 // Reduce per-thread accumulators acc[0 .. nv) (nv <= 8) over the workgroup
 // into the LDS slots base .. base + nv.
 template <typename R>
@@ -1537,7 +1554,11 @@ __device__ __forceinline__ void jf_diag(R* re, R* im, cmat<R>* d, const Layout& 
 }
 
 // backward gates: psi <- U^{-1} psi, bar += lambda (x) psi, lambda <- U^T lambda
-template <typename R, int RB, int A, uint32_t CREG, bool INV>
+// QUAT (blocks of Pauli rotations, whose angle derivatives have the form
+// [[a, -b*], [b, a*]]): only S = bar00 + conj(bar11) and D = bar10 - conj(bar01)
+// enter the angle gradients, so S goes to the bar00 slot reals, D to the
+// bar10 ones, and the bar01 / bar11 reals stay zero.
+template <typename R, int RB, int A, uint32_t CREG, bool INV, bool QUAT = false>
 __device__ __forceinline__ void jb_mc1(R* pr, R* pi, R* lr, R* li, cmat<R>* mp, cmat<R>* ip, bool ok, double* slots,
                                        int slot) {
   constexpr int NR = 1 << RB;
@@ -1551,7 +1572,13 @@ __device__ __forceinline__ void jb_mc1(R* pr, R* pi, R* lr, R* li, cmat<R>* mp, 
 #pragma unroll
   for (int e = 0; e < 8; ++e) acc[e] = 0;
   if (ok) bw_dense<R, NR, 2, G1<NR, A>, (CREG != 0)>(pr, pi, lr, li, u, ui, CREG, acc);
-  if (slot >= 0) flush_vals<R>(acc, 8, slots, slot);
+  if (slot < 0) return;
+  if constexpr (QUAT) {
+    const R v[4] = {acc[0] + acc[6], acc[1] - acc[7], acc[4] - acc[2], acc[5] + acc[3]};
+    flush4<R>(v, slots, [slot](int k) { return slot + (k < 2 ? k : k + 2); });
+  } else {
+    flush_vals<R>(acc, 8, slots, slot);
+  }
 }
 
 template <typename R, int RB, int A, int B, uint32_t CREG, bool INV, bool SW>

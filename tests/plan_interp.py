@@ -64,7 +64,11 @@ def member_full(mt, theta, xm, deriv=False):
 
 
 class Program:
-    def __init__(self, plan):
+    def __init__(self, plan, quat=False):
+        # quat: reduce the cotangent of MC1 gates flagged in word 15 to
+        # S = bar00 + conj(bar11) (bar00 slot) and D = bar10 - conj(bar01)
+        # (bar10 slot), as the plan-specialised backward does
+        self.quat = quat
         self.w = plan.program()
         self.wb = plan.table(4)
         self.bt = plan.table(0).reshape(-1, BW)
@@ -342,6 +346,8 @@ class Program:
                 for c in range(1 << k):
                     vals[o, c] = np.sum(lam_out[idx[o]] * psi_in[idx[c]])
             vals = vals.reshape(-1)
+            if self.quat and kind == AK_MC1 and int(g[15]):
+                vals = np.array([vals[0] + np.conj(vals[3]), 0, vals[2] - np.conj(vals[1]), 0])
         key = (p, int(g[10]))
         flat = np.stack([vals.real, vals.imag], axis=-1).reshape(-1)
         partial[key] = partial.get(key, 0) + flat

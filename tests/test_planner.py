@@ -58,6 +58,94 @@ def test_gate_zoo_program_matches_oracle(dtype):
     np.testing.assert_allclose(grads, sv.flat_grads(ops, og), atol=1e-12)
 
 
+@pytest.mark.parametrize("n,layers", [(3, 2), (9, 2)])
+def test_rotation_blocks_reduced_cotangent_matches_oracle(n, layers):
+    """The specialised backward keeps only bar00 + conj(bar11) and
+    bar10 - conj(bar01) of blocks the planner flags as Pauli-rotation
+    products; the angle gradients through the chain rule stay exact."""
+    from circuits import record_hea
+    from plan_interp import Program
+
+    theta = np.random.default_rng(n).uniform(0, 2 * np.pi, (layers, n, 3))
+    plan, angles, xm = _plan(_record(lambda: record_hea(theta, n)), n)
+    prog = Program(plan, quat=True)
+    zs = sv.z_signs(n).sum(0)
+    _, bars, grads, _ = prog.adjoint(angles, lambda f: 2 * zs * np.conj(f), xm)
+    ops = sv.hea_ops(theta, n)
+    _, og, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
+    np.testing.assert_allclose(grads, sv.flat_grads(ops, og), atol=1e-12)
+    # the same with a generic complex cotangent seed, against the full bars
+    seed = np.random.default_rng(n + 1).normal(size=(2, 1 << n)).T @ np.array([1, 1j])
+    _, _, g_full, _ = Program(plan).adjoint(angles, lambda f: seed, xm)
+    _, _, g_red, _ = prog.adjoint(angles, lambda f: seed, xm)
+    np.testing.assert_allclose(g_red, g_full, atol=1e-12)
+    # every HEA rotation block is flagged: half of its bar reals are zero
+    _, full, _, _ = Program(plan).adjoint(angles, lambda f: 2 * zs * np.conj(f), xm)
+    assert np.count_nonzero(bars) * 2 == np.count_nonzero(full)
+
+
+def test_rotation_block_flags_on_the_gate_zoo():
+    """The zoo holds H, phase shifts, explicit matrices and controlled
+    rotations: with the reduced cotangent of the flagged blocks the
+    gradients stay the oracle's."""
+    from plan_interp import Program
+
+    p = zoo_params()
+    plan, angles, xm = _plan(_record(lambda: record_zoo(p)), 5)
+    zs = sv.z_signs(5).sum(0)
+    ops = zoo_ops(p)
+    _, og, _, _ = sv.adjoint(ops, 5, sv.z_obs(5))
+    _, _, grads, _ = Program(plan, quat=True).adjoint(angles, lambda f: 2 * zs * np.conj(f), xm)
+    np.testing.assert_allclose(grads, sv.flat_grads(ops, og), atol=1e-12)
+
+
+def test_rotations_fused_with_fixed_gates_are_not_flagged():
+    """S . RY . H fuses into a 2x2 block outside the quaternion form: the
+    planner leaves it unflagged (gradients exact), and flagging every 2x2
+    block anyway (a planted mistake) breaks the gradients.  (The block acts
+    after an entangling gate: on |0> the bar01 / bar11 entries vanish and the
+    mistake would not show.)"""
+    import jaqsi_amd as J
+    from plan_interp import Program
+
+    a, b, c = 0.7, -1.1, 0.4
+
+    def circ():
+        J.RY(a, wires=0)
+        J.RX(b, wires=1)
+        J.CX(wires=[0, 1])
+        J.H(1)
+        J.RY(c, wires=1)
+        J.S(wires=1)
+        J.CX(wires=[1, 0])
+
+    ops = [sv.Op("RY", (0,), (a,)), sv.Op("RX", (1,), (b,)), sv.Op("CX", (0, 1)), sv.Op("H", (1,)),
+           sv.Op("RY", (1,), (c,)), sv.Op("S", (1,)), sv.Op("CX", (1, 0))]
+    plan, angles, xm = _plan(_record(circ), 2)
+    zs = sv.z_signs(2).sum(0)
+    _, og, _, _ = sv.adjoint(ops, 2, sv.z_obs(2))
+    expect = sv.flat_grads(ops, og)
+    _, _, grads, _ = Program(plan, quat=True).adjoint(angles, lambda f: 2 * zs * np.conj(f), xm)
+    np.testing.assert_allclose(grads, expect, atol=1e-12)
+    bad = Program(plan, quat=True)
+    orig = bad._grad
+
+    def flag_all(psi_in, lam_out, g, *rest):
+        g = np.array(g)
+        if int(g[0]) == 1:
+            g[15] = 1
+        return orig(psi_in, lam_out, g, *rest)
+
+    bad._grad = flag_all
+    # with a generic complex cotangent seed every bar entry matters
+    seed = np.random.default_rng(7).normal(size=(4, 2)) @ np.array([1, 1j])
+    _, _, full, _ = Program(plan).adjoint(angles, lambda f: seed, xm)
+    _, _, reduced, _ = Program(plan, quat=True).adjoint(angles, lambda f: seed, xm)
+    np.testing.assert_allclose(reduced, full, atol=1e-12)
+    _, _, wrong, _ = bad.adjoint(angles, lambda f: seed, xm)
+    assert np.abs(wrong - full).max() > 1e-2
+
+
 def test_c2_schedule_shape():
     """The 24-qubit, 8-layer workload schedules into 21 passes (the CX ring
     advances about 8 wires per pass), and every tile holds global bits 0..4
