@@ -10,7 +10,8 @@
 //  - the adjoint of simulation.py:616-652: psi <- U^dagger psi,
 //    bar[o][i] = sum lambda[o] psi[i], lambda <- U^T lambda, permutations by
 //    their inverse gather; bars chained to the angles.
-// Arithmetic is complex64 (the reference's default dtype), reductions in FP64.
+// Arithmetic is complex64 (the reference's default dtype) or complex128 (its
+// jax_enable_x64 mode, operations.py:26-43), reductions in FP64.
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -19,7 +20,6 @@
 
 #include <omp.h>
 
-typedef std::complex<float> cf;
 typedef std::complex<double> cdd;
 
 namespace {
@@ -57,6 +57,7 @@ M2 rot(int axis, double t, bool deriv) {
   return m;
 }
 
+template <typename cf>
 void apply2(cf* psi, int64_t dim, int bit, const M2& m) {
   const cf m0(m.a[0]), m1(m.a[1]), m2(m.a[2]), m3(m.a[3]);
   const int64_t stride = 1LL << bit;
@@ -70,21 +71,20 @@ void apply2(cf* psi, int64_t dim, int bit, const M2& m) {
   }
 }
 
+template <typename cf>
 void gather(cf* psi, cf* tmp, const int32_t* idx, int64_t dim) {
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < dim; ++i) tmp[i] = psi[idx[i]];
   std::memcpy(psi, tmp, dim * sizeof(cf));
 }
 
-}  // namespace
-
-extern "C" {
-
 // Forward + adjoint gradient of sum_q <Z_q> for the hardware-efficient
 // ansatz: theta[layers][n][3] (RX, RY, RZ), optional features[n] (encoding RX).
 // Returns the expectation values (n) and d/dtheta (layers*n*3), in FP64.
-int cpu_hea_grad(int n, int layers, const double* theta, const double* features, int threads, double* expvals,
-                 double* grads) {
+template <typename cf>
+int hea_grad(int n, int layers, const double* theta, const double* features, int threads, double* expvals,
+             double* grads) {
+  typedef typename cf::value_type rf;
   if (threads > 0) omp_set_num_threads(threads);
   const int64_t dim = 1LL << n;
   std::vector<cf> psi(dim), lam(dim), tmp(dim);
@@ -132,7 +132,7 @@ int cpu_hea_grad(int n, int layers, const double* theta, const double* features,
         loc[q] += p * z;
         w += z;
       }
-      lam[i] = cf(2.0f * (float)w) * std::conj(psi[i]);
+      lam[i] = cf(rf(2) * (rf)w) * std::conj(psi[i]);
     }
 #pragma omp critical
     for (int q = 0; q < n; ++q) ev[q] += loc[q];
@@ -195,6 +195,21 @@ int cpu_hea_grad(int n, int layers, const double* theta, const double* features,
       }
     }
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpu_hea_grad(int n, int layers, const double* theta, const double* features, int threads, double* expvals,
+                 double* grads) {
+  return hea_grad<std::complex<float>>(n, layers, theta, features, threads, expvals, grads);
+}
+
+// The same algorithm in complex128 (the reference's x64 mode).
+int cpu_hea_grad_c128(int n, int layers, const double* theta, const double* features, int threads, double* expvals,
+                      double* grads) {
+  return hea_grad<std::complex<double>>(n, layers, theta, features, threads, expvals, grads);
 }
 
 }  // extern "C"

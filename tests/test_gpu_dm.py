@@ -7,10 +7,10 @@ import torch
 
 from circuits import hea_theta, noisy_hea_reference
 from oracle import dm, sv
+from parity import assert_parity
 
 pytestmark = pytest.mark.gpu
 
-TOL = {False: 2e-5, True: 1e-11}
 
 
 def _record_noisy_hea(theta, n, p_dep, gamma):
@@ -50,8 +50,8 @@ def test_noisy_hea_expval_and_gradient(n, layers, x64):
     ev = script.execute(type="expval", obs=[J.PauliZ(wires=0, record=False)], args=(th,))
     ev.sum().backward()
     v, g = noisy_hea_reference(n, layers, 2000 + n)
-    np.testing.assert_allclose(ev.detach().cpu().numpy()[0], v, atol=TOL[x64], rtol=0)
-    np.testing.assert_allclose(th.grad.cpu().numpy().reshape(-1), g, atol=TOL[x64] * 10, rtol=0)
+    assert_parity(ev.detach().cpu().numpy()[0], v, x64)
+    assert_parity(th.grad.cpu().numpy().reshape(-1), g, x64)
 
 
 @pytest.mark.parametrize("x64", [False, True])
@@ -85,11 +85,11 @@ def test_density_and_probs_outputs(x64):
     ]
     ref = dm.simulate(items, n)
     rho = J.Script(circ, n_qubits=n).execute(type="density").cpu().numpy()
-    np.testing.assert_allclose(rho, ref, atol=TOL[x64])
+    assert_parity(rho, ref, x64)
     probs = J.Script(circ, n_qubits=n).execute(type="probs").cpu().numpy()
-    np.testing.assert_allclose(probs, dm.probs(ref), atol=TOL[x64])
+    assert_parity(probs, dm.probs(ref), x64)
     ev = J.Script(circ, n_qubits=n).execute(type="expval", obs=[J.PauliX(wires=2, record=False), J.PauliZ(wires=1, record=False)])
-    np.testing.assert_allclose(ev.cpu().numpy(), [dm.expval(ref, sv.PX, (2,), n), dm.expval(ref, sv.PZ, (1,), n)], atol=TOL[x64])
+    assert_parity(ev.cpu().numpy(), [dm.expval(ref, sv.PX, (2,), n), dm.expval(ref, sv.PZ, (1,), n)], x64)
 
 
 def test_noise_params_attach_channels_on_gpu():
@@ -113,4 +113,4 @@ def test_noise_params_attach_channels_on_gpu():
     ]
     ref = dm.simulate(items, n)
     rho = J.Script(circ, n_qubits=n).execute(type="density").cpu().numpy()
-    np.testing.assert_allclose(rho, ref, atol=1e-11)
+    assert_parity(rho, ref, True)

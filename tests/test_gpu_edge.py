@@ -7,6 +7,7 @@ import pytest
 import torch
 
 from oracle import sv
+from parity import assert_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -21,10 +22,10 @@ def test_empty_circuit_is_the_zero_state(x64):
     script = J.Script(lambda: None, n_qubits=n)
     ev = script.execute(type="expval", obs=[J.PauliZ(wires=q, record=False) for q in range(n)])
     probs = script.execute(type="probs")
-    np.testing.assert_allclose(ev.cpu().numpy(), np.ones(n), atol=1e-7)
+    assert_parity(ev.cpu().numpy(), np.ones(n), x64)
     expect = np.zeros(1 << n)
     expect[0] = 1
-    np.testing.assert_allclose(probs.cpu().numpy(), expect, atol=1e-7)
+    assert_parity(probs.cpu().numpy(), expect, x64)
 
 
 @pytest.mark.parametrize("x64", [False, True])
@@ -39,9 +40,8 @@ def test_single_qubit_rx_closed_form(x64):
         ev = J.Script(lambda a: J.RX(a, wires=0), n_qubits=1).execute(
             type="expval", obs=[J.PauliZ(wires=0, record=False)], args=(th,))
         ev.sum().backward()
-        tol = 1e-11 if x64 else 2e-6
-        np.testing.assert_allclose(ev.item(), np.cos(t), atol=tol)
-        np.testing.assert_allclose(th.grad.item(), -np.sin(t), atol=tol * 10)
+        assert_parity(ev.item(), np.cos(t), x64, scale=1.0)
+        assert_parity(th.grad.item(), -np.sin(t), x64, scale=1.0)
 
 
 def test_parameter_free_circuit():
@@ -59,9 +59,9 @@ def test_parameter_free_circuit():
     st = J.Script(circ, n_qubits=n).execute(type="state").cpu().numpy()
     ghz = np.zeros(1 << n, dtype=complex)
     ghz[0] = ghz[-1] = 1 / np.sqrt(2)
-    np.testing.assert_allclose(st, ghz, atol=1e-12)
+    assert_parity(st, ghz, True)
     ops = [sv.Op("H", (0,))] + [sv.Op("CX", (q, q + 1)) for q in range(n - 1)]
-    np.testing.assert_allclose(st, sv.simulate(ops, n), atol=1e-12)
+    assert_parity(st, sv.simulate(ops, n), True)
 
 
 def test_batch_of_one_keeps_its_axis():
@@ -78,4 +78,4 @@ def test_batch_of_one_keeps_its_axis():
     ev = J.Script(circ, n_qubits=2).execute(type="expval", obs=[J.PauliZ(wires=0, record=False)],
                                             args=(th,), in_axes=(0,))
     assert tuple(ev.shape)[0] == 1
-    np.testing.assert_allclose(ev.cpu().numpy().reshape(-1), [np.cos(0.3)], atol=1e-12)
+    assert_parity(ev.cpu().numpy().reshape(-1), [np.cos(0.3)], True)

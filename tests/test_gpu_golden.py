@@ -7,11 +7,11 @@ import pytest
 import torch
 
 from circuits import record_hea
+from parity import assert_parity
 
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-TOL = {False: 2e-5, True: 1e-11}
 
 
 def _load(name):
@@ -30,8 +30,8 @@ def test_c1_fixture(x64):
     ev = J.Script(lambda t: record_hea(t, n), n_qubits=n).execute(
         type="expval", obs=[J.PauliZ(wires=q, record=False) for q in range(n)], args=(th,))
     ev.sum().backward()
-    np.testing.assert_allclose(ev.detach().cpu().numpy(), g["expvals"], atol=TOL[x64])
-    np.testing.assert_allclose(th.grad.cpu().numpy().reshape(-1), g["grads"], atol=TOL[x64] * 10)
+    assert_parity(ev.detach().cpu().numpy(), g["expvals"], x64)
+    assert_parity(th.grad.cpu().numpy().reshape(-1), g["grads"], x64)
 
 
 @pytest.mark.parametrize("x64", [False, True])
@@ -54,8 +54,8 @@ def test_c3_head_fixture(x64):
     evs, grads = rb.adjoint(angles, cts)
     for gi, d in enumerate(uniq):
         for r, b in enumerate(members[d]):
-            np.testing.assert_allclose(evs[gi][r].cpu().numpy(), g[f"expvals_{b}"], atol=TOL[x64])
-            np.testing.assert_allclose(grads[gi][r].cpu().numpy(), g[f"grads_{b}"], atol=TOL[x64] * 20)
+            assert_parity(evs[gi][r].cpu().numpy(), g[f"expvals_{b}"], x64)
+            assert_parity(grads[gi][r].cpu().numpy(), g[f"grads_{b}"], x64)
 
 
 @pytest.mark.parametrize("x64", [False, True])
@@ -71,8 +71,8 @@ def test_dm_fixture(x64):
     ev = J.Script(lambda t: _record_noisy_hea(t, n, 0.01, 0.005), n_qubits=n).execute(
         type="expval", obs=[J.PauliZ(wires=0, record=False)], args=(th,))
     ev.sum().backward()
-    np.testing.assert_allclose(ev.detach().cpu().numpy()[0], g["value"], atol=TOL[x64])
-    np.testing.assert_allclose(th.grad.cpu().numpy().reshape(-1), g["grads"], atol=TOL[x64] * 10)
+    assert_parity(ev.detach().cpu().numpy()[0], g["value"], x64)
+    assert_parity(th.grad.cpu().numpy().reshape(-1), g["grads"], x64)
 
 
 def test_c5_fixture():
@@ -81,6 +81,6 @@ def test_c5_fixture():
     print("golden c5", flush=True)
     g = _load("c5_small")
     U, inf, grads = _run(g["params"], g["omega"], int(g["n_steps"]))
-    np.testing.assert_allclose(U, g["U"], atol=1e-11)
-    np.testing.assert_allclose(inf, g["infidelity"], atol=1e-11)
+    assert_parity(U, g["U"], True)
+    assert_parity(inf, g["infidelity"], True, scale=1.0)
     np.testing.assert_allclose(grads, g["grads"], atol=2e-7, rtol=1e-6)

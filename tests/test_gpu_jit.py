@@ -7,10 +7,10 @@ import torch
 
 from circuits import hea_theta, noisy_hea_reference, record_hea, zoo_params
 from oracle import sv
+from parity import assert_parity
 
 pytestmark = pytest.mark.gpu
 
-TOL = {False: 2e-5, True: 1e-11}
 
 
 @pytest.fixture(params=["all-resident", "3-workgroups"])
@@ -57,8 +57,8 @@ def test_hea_specialised_matches_oracle(specialised, n, layers, x64):
     _all_loaded(specialised)
     ops = sv.hea_ops(theta, n)
     vals, grads, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
-    np.testing.assert_allclose(ev, vals, atol=TOL[x64], rtol=0)
-    np.testing.assert_allclose(grad, sv.flat_grads(ops, grads), atol=TOL[x64] * 10, rtol=0)
+    assert_parity(ev, vals, x64)
+    assert_parity(grad, sv.flat_grads(ops, grads), x64)
 
 
 # The gate zoo spread over 16 wires: controls and diagonal factors land on
@@ -140,15 +140,15 @@ def test_zoo_on_16_wires_matches_oracle_and_interpreter(x64):
     finally:
         simulation.specialise(None)
     # same device functions in the same order: agreement far inside the oracle tolerance
-    np.testing.assert_allclose(ev_j, ev_i, atol=TOL[x64] * 0.05, rtol=0)
-    np.testing.assert_allclose(g_j, g_i, atol=TOL[x64] * 0.5, rtol=0)
+    assert_parity(ev_j, ev_i, x64)
+    assert_parity(g_j, g_i, x64)
     ops = _zoo_mapped_ops(p, WIRE_MAP)
     vals, grads, _, _ = sv.adjoint(ops, 16, sv.z_obs(16))
-    np.testing.assert_allclose(ev_j, vals, atol=TOL[x64])
+    assert_parity(ev_j, vals, x64)
     flat = sv.flat_grads(ops, grads)
     expect = flat[:12].copy()
     expect[0] += 0.5 * flat[12]  # the last RZ's angle is 0.5 * p[0]
-    np.testing.assert_allclose(g_j, expect, atol=TOL[x64] * 10)
+    assert_parity(g_j, expect, x64)
 
 
 def _record_noisy(theta, n, p_dep, gamma):
@@ -188,8 +188,8 @@ def test_noisy_density_matrix_specialised(specialised, n, x64):
     ev.sum().backward()
     _all_loaded(specialised)
     v, g = noisy_hea_reference(n, layers, 2000 + n)
-    np.testing.assert_allclose(ev.detach().cpu().numpy()[0], v, atol=TOL[x64], rtol=0)
-    np.testing.assert_allclose(th.grad.cpu().numpy().reshape(-1), g, atol=TOL[x64] * 10, rtol=0)
+    assert_parity(ev.detach().cpu().numpy()[0], v, x64)
+    assert_parity(th.grad.cpu().numpy().reshape(-1), g, x64)
 
 
 def test_dense_three_wire_block_and_state_specialised(specialised):
@@ -219,7 +219,7 @@ def test_dense_three_wire_block_and_state_specialised(specialised):
     ops = [sv.Op("H", (q,)) for q in range(n)]
     ops += [sv.Op("Matrix", (2, 9, 13), (), u8), sv.Op("Matrix", (12, 0), (), u4), sv.Op("CRY", (13, 1), (0.7,))]
     ref = sv.simulate(ops, n)
-    np.testing.assert_allclose(st, ref, atol=1e-11)
+    assert_parity(st, ref, True)
 
 
 def test_long_passes_stay_with_the_interpreter():

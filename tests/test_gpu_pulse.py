@@ -9,6 +9,7 @@ import pytest
 import torch
 
 from oracle import pulse
+from parity import assert_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -38,8 +39,8 @@ def test_propagators_and_infidelity(n_steps):
     V = pulse.target_rx_half_pi()
     for s in range(5):
         ref = pulse.propagate(params[s], omega[s], pulse.J_C5, 50.0, n_steps)
-        np.testing.assert_allclose(U[s], ref, atol=1e-11)
-        np.testing.assert_allclose(inf[s], pulse.infidelity(ref, V), atol=1e-11)
+        assert_parity(U[s], ref, True)
+        assert_parity(inf[s], pulse.infidelity(ref, V), True, scale=1.0)
 
 
 def test_gradient_against_finite_differences():
@@ -68,7 +69,7 @@ def test_zero_drive_is_diagonal_phase():
     U, _, _ = _run(params, omega, 64, want_grad=False)
     z = np.array([1, -1])
     e = np.array([15.0 * z0 + 15.5 * z1 + pulse.J_C5 * z0 * z1 for z0 in z for z1 in z])
-    np.testing.assert_allclose(U[0], np.diag(np.exp(-1j * 50.0 * e)), atol=1e-12)
+    assert_parity(U[0], np.diag(np.exp(-1j * 50.0 * e)), True)
 
 
 def test_autograd_api_matches_oracle_and_kernel_gradient():
@@ -86,8 +87,8 @@ def test_autograd_api_matches_oracle_and_kernel_gradient():
     U, inf, g = P.propagate(pt, wt, V, n_steps=n_steps, coupling=pulse.J_C5)
     for s in range(4):
         ref = pulse.propagate(params[s], omega[s], pulse.J_C5, 50.0, n_steps)
-        np.testing.assert_allclose(U[s].cpu().numpy(), ref, atol=1e-11)
-        np.testing.assert_allclose(inf[s].item(), pulse.infidelity(ref, V), atol=1e-11)
+        assert_parity(U[s].cpu().numpy(), ref, True)
+        assert_parity(inf[s].item(), pulse.infidelity(ref, V), True, scale=1.0)
     w = torch.tensor([1.0, -2.0, 0.5, 3.0], dtype=torch.float64, device="cuda")
     f = P.infidelity(pt, wt, torch.tensor(V), n_steps=n_steps, coupling=pulse.J_C5)
     np.testing.assert_allclose(f.detach().cpu().numpy(), inf.cpu().numpy(), atol=1e-15, rtol=0)

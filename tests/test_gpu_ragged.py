@@ -7,10 +7,10 @@ import torch
 
 from circuits import record_hea
 from oracle import sv
+from parity import assert_parity
 
 pytestmark = pytest.mark.gpu
 
-TOL = {False: 2e-5, True: 1e-11}
 
 
 @pytest.mark.parametrize("specialised", [False, True])
@@ -49,8 +49,8 @@ def test_uneven_depth_batch(n, x64, specialised):
             ops = sv.hea_ops(thetas[g][b], n, features=feats[g][b])
             ct = cts[g][b].cpu().numpy()
             vals, og, _, _ = sv.adjoint(ops, n, sv.z_obs(n), ct=ct)
-            np.testing.assert_allclose(evs[g][b].cpu().numpy(), vals, atol=TOL[x64], rtol=0)
-            np.testing.assert_allclose(grads[g][b].cpu().numpy(), sv.flat_grads(ops, og), atol=TOL[x64] * 20, rtol=0)
+            assert_parity(evs[g][b].cpu().numpy(), vals, x64)
+            assert_parity(grads[g][b].cpu().numpy(), sv.flat_grads(ops, og), x64)
 
 
 def test_groups_reject_shared_plan():

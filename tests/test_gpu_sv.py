@@ -6,10 +6,10 @@ import torch
 
 from circuits import hea_theta, record_hea, record_zoo, zoo_ops, zoo_params
 from oracle import sv
+from parity import assert_parity
 
 pytestmark = pytest.mark.gpu
 
-TOL = {False: 2e-5, True: 1e-11}  # complex64 / complex128 against the c128 oracle
 
 
 def _script_expval(n, theta, x64, obs_wires=None):
@@ -32,8 +32,8 @@ def test_hea_expval_and_adjoint(n, layers, x64):
     ev, grad = _script_expval(n, theta, x64)
     ops = sv.hea_ops(theta, n)
     vals, grads, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
-    np.testing.assert_allclose(ev, vals, atol=TOL[x64], rtol=0)
-    np.testing.assert_allclose(grad, sv.flat_grads(ops, grads), atol=TOL[x64] * 10, rtol=0)
+    assert_parity(ev, vals, x64)
+    assert_parity(grad, sv.flat_grads(ops, grads), x64)
 
 
 @pytest.mark.parametrize("x64", [False, True])
@@ -46,8 +46,8 @@ def test_deep_circuit_split_by_slot_budget(x64):
     ev, grad = _script_expval(n, theta, x64)
     ops = sv.hea_ops(theta, n)
     vals, grads, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
-    np.testing.assert_allclose(ev, vals, atol=TOL[x64], rtol=0)
-    np.testing.assert_allclose(grad, sv.flat_grads(ops, grads), atol=TOL[x64] * 10, rtol=0)
+    assert_parity(ev, vals, x64)
+    assert_parity(grad, sv.flat_grads(ops, grads), x64)
 
 
 @pytest.mark.parametrize("x64", [False, True])
@@ -60,18 +60,18 @@ def test_gate_zoo_state_and_gradients(x64):
     ops = zoo_ops(p)
     ref = sv.simulate(ops, 5)
     st = J.Script(lambda: record_zoo(p), n_qubits=5).execute(type="state").cpu().numpy()
-    np.testing.assert_allclose(st, ref, atol=TOL[x64])
+    assert_parity(st, ref, x64)
     pt = torch.tensor(p, dtype=torch.float64, device="cuda", requires_grad=True)
     obs = [J.PauliZ(wires=q, record=False) for q in range(5)]
     ev = J.Script(record_zoo, n_qubits=5).execute(type="expval", obs=obs, args=(pt,))
     ev.sum().backward()
     vals, grads, _, _ = sv.adjoint(ops, 5, sv.z_obs(5))
-    np.testing.assert_allclose(ev.detach().cpu().numpy(), vals, atol=TOL[x64])
+    assert_parity(ev.detach().cpu().numpy(), vals, x64)
     flat = sv.flat_grads(ops, grads)
     # oracle parameters in tape order: p[0..11], then the last RZ's angle 0.5 * p[0]
     expect = flat[:12].copy()
     expect[0] += 0.5 * flat[12]
-    np.testing.assert_allclose(pt.grad.cpu().numpy(), expect, atol=TOL[x64] * 10)
+    assert_parity(pt.grad.cpu().numpy(), expect, x64)
 
 
 def test_batched_in_axes_matches_loop():
@@ -96,9 +96,9 @@ def test_batched_in_axes_matches_loop():
     for b in range(9):
         ops = sv.hea_ops(theta, n, features=x[b])
         vals, grads, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
-        np.testing.assert_allclose(ev[b].detach().cpu().numpy(), vals, atol=2e-5)
+        assert_parity(ev[b].detach().cpu().numpy(), vals, False)
         g += sv.flat_grads(ops, grads)[n:]
-    np.testing.assert_allclose(tht.grad.cpu().numpy().reshape(-1), g, atol=2e-4)
+    assert_parity(tht.grad.cpu().numpy().reshape(-1), g, False)
 
 
 def test_probs_state_density_outputs():
@@ -108,9 +108,9 @@ def test_probs_state_density_outputs():
     theta = hea_theta(n, 2, 5)
     ref = sv.simulate(sv.hea_ops(theta, n), n)
     s = J.Script(lambda: record_hea(theta, n), n_qubits=n)
-    np.testing.assert_allclose(s.execute(type="state").cpu().numpy(), ref, atol=2e-6)
-    np.testing.assert_allclose(s.execute(type="probs").cpu().numpy(), np.abs(ref) ** 2, atol=2e-6)
-    np.testing.assert_allclose(s.execute(type="density").cpu().numpy(), np.outer(ref, ref.conj()), atol=2e-6)
+    assert_parity(s.execute(type="state").cpu().numpy(), ref, False)
+    assert_parity(s.execute(type="probs").cpu().numpy(), np.abs(ref) ** 2, False)
+    assert_parity(s.execute(type="density").cpu().numpy(), np.outer(ref, ref.conj()), False)
 
 
 def test_initial_state_and_its_cotangent():
@@ -128,9 +128,9 @@ def test_initial_state_and_its_cotangent():
     ev.sum().backward()
     ops = sv.hea_ops(theta, n)
     vals, _, _, lam0 = sv.adjoint(ops, n, sv.z_obs(n), psi0=psi0)
-    np.testing.assert_allclose(ev.detach().cpu().numpy(), vals, atol=1e-11)
+    assert_parity(ev.detach().cpu().numpy(), vals, True)
     # PyTorch's cotangent is the conjugate of JAX's
-    np.testing.assert_allclose(p0.grad.cpu().numpy(), np.conj(lam0), atol=1e-11)
+    assert_parity(p0.grad.cpu().numpy(), np.conj(lam0), True)
 
 
 def test_lowered_entry_points_for_a_custom_vjp_bridge():
@@ -152,8 +152,8 @@ def test_lowered_entry_points_for_a_custom_vjp_bridge():
     g = S.adjoint_lowered(lowered, a, obs, ct).cpu().numpy()
     ops = sv.hea_ops(theta, n)
     vals, grads, _, _ = sv.adjoint(ops, n, sv.z_obs(n), ct=np.arange(1, n + 1, dtype=float))
-    np.testing.assert_allclose(ev, vals, atol=1e-11)
-    np.testing.assert_allclose(g, sv.flat_grads(ops, grads), atol=1e-10)
+    assert_parity(ev, vals, True)
+    assert_parity(g, sv.flat_grads(ops, grads), True)
 
 
 @pytest.mark.parametrize("x64", [False, True])
@@ -185,5 +185,5 @@ def test_general_observables_expval_and_gradient(x64):
     (ev * torch.tensor(ct, device="cuda", dtype=ev.dtype)).sum().backward()
     ops = sv.hea_ops(theta, n)
     vals, grads, _, _ = sv.adjoint(ops, n, ref_obs, ct=ct)
-    np.testing.assert_allclose(ev.detach().cpu().numpy(), vals, atol=TOL[x64] * 10)
-    np.testing.assert_allclose(th.grad.cpu().numpy().reshape(-1), sv.flat_grads(ops, grads), atol=TOL[x64] * 40)
+    assert_parity(ev.detach().cpu().numpy(), vals, x64)
+    assert_parity(th.grad.cpu().numpy().reshape(-1), sv.flat_grads(ops, grads), x64)

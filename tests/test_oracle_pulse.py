@@ -70,3 +70,62 @@ def test_infidelity_of_target_is_zero():
     V = pulse.target_rx_half_pi()
     assert abs(pulse.infidelity(V, V)) < 1e-15
     assert abs(pulse.infidelity(np.exp(0.3j) * V, V)) < 1e-15
+
+
+def _gauss_legendre(T, n=200):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * T * (x + 1), 0.5 * T * w
+
+
+def _qubit0_block(U, q1=0):
+    """The 2x2 block of a qubit-0 operator at qubit 1 = q1 (wire 0 is the
+    most significant index bit)."""
+    idx = [q1, 2 + q1]
+    return U[np.ix_(idx, idx)]
+
+
+def test_drive_axis_against_the_pulse_area_closed_form():
+    """beta = 0 and w0 = w1 = J = 0: H(t) = E(t) (cos phi X + sin phi Y) commutes
+    with itself, so U = exp(-i S (cos phi X + sin phi Y)) (x) I with S the pulse
+    area, by Gauss-Legendre (the reference's own check, tests/test_training.py:
+    131-157).  The off-diagonal entry -i sin(S) e^{-i phi} fixes the sign and
+    the axis of the in-phase drive."""
+    T, N = 50.0, 4000
+    t, w = _gauss_legendre(T)
+    for A, sigma, phi in ((0.7, 10.0, 1.1), (0.3, 6.0, -2.4), (1.0, 18.0, 0.0)):
+        E, _ = pulse.envelopes(t, A, sigma, 0.0, T / 2)
+        S = float(np.sum(w * E))
+        U = pulse.propagate((A, sigma, 0.0, phi), (0.0, 0.0), 0.0, T, N)
+        ref = math.cos(S) * np.eye(2) - 1j * math.sin(S) * (math.cos(phi) * sv.PX + math.sin(phi) * sv.PY)
+        for q1 in (0, 1):
+            # the midpoint rule's area differs from S by O(h^2) (~1e-7 here)
+            np.testing.assert_allclose(_qubit0_block(U, q1), ref, atol=1e-6)
+        np.testing.assert_allclose(U[0, 1], 0.0, atol=1e-15)  # qubit 1 untouched
+        assert abs(U[0, 2] - (-1j * math.sin(S) * np.exp(-1j * phi))) < 1e-6
+
+
+def test_quadrature_axis_against_the_second_order_magnus_term():
+    """phi = 0, w0 = w1 = J = 0, small A: H = E X + Q Y with Q = -beta dE/dt.
+    The area of Q vanishes, and the second Magnus term is
+    -1/2 int int_{t1>t2} [H(t1), H(t2)] = -i K Z with
+    K = int int_{t1>t2} (E1 Q2 - Q1 E2) = -2 beta int E^2 dt (integrating
+    Q = -beta E' by parts with E(0) = E(T) = 0).  So the Z component of U is
+    -i K sin(r)/r, r = |(S, K)|, up to O(A^3): its sign fixes the axis of the
+    quadrature (Y for phi = 0, i.e. phi + pi/2)."""
+    T, N, A, sigma = 50.0, 4000, 1e-3, 10.0
+    t, w = _gauss_legendre(T)
+    for beta in (0.8, -0.5):
+        E, _ = pulse.envelopes(t, A, sigma, beta, T / 2)
+        S = float(np.sum(w * E))
+        K = -2.0 * beta * float(np.sum(w * E * E))
+        r = math.hypot(S, K)
+        U = pulse.propagate((A, sigma, beta, 0.0), (0.0, 0.0), 0.0, T, N)
+        u = _qubit0_block(U)
+        zc = np.trace(sv.PZ @ u) / 2
+        xc = np.trace(sv.PX @ u) / 2
+        yc = np.trace(sv.PY @ u) / 2
+        assert abs(zc - (-1j * K * math.sin(r) / r)) < 1e-3 * abs(K)
+        assert abs(xc - (-1j * S * math.sin(r) / r)) < 1e-6 * abs(S)
+        assert abs(yc) < 1e-3 * abs(K)
+        # a quadrature on -Y instead would flip the sign of the Z component
+        assert (zc.imag > 0) == (beta > 0)
