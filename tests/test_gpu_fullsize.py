@@ -79,8 +79,14 @@ def test_full_size_gradient_complex128(cpu_reference):
     ev, g, norm = _gpu(theta, nat.JQ_JIT_SYNC, x64=True)
     print(f"  rel dev expvals {max_rel_err(ev, ev_c):.2e}, gradient {max_rel_err(g, g_c):.2e}, "
           f"norm - 1 {norm - 1:.1e}", flush=True)
-    assert_parity(ev, ev_c, True)
-    assert_parity(g, g_c, True)
+    # <Z_q> of a random 24-qubit state is a sum of 2^24 terms +-|psi_i|^2
+    # that cancel to ~1e-4: its rounding error scales with sum |psi_i|^2 = 1,
+    # not with the tiny result (seen on MI355X: 4.5e-15 absolute, 9e-12 of
+    # max |<Z_q>|).  Each value and gradient is therefore held to 1e-12 of the
+    # observable's norm ||Z_q|| = 1 (the gradient of sum_q <Z_q> is bounded by
+    # ||sum_q Z_q|| = 24, so this is the stricter of the two natural scales).
+    assert_parity(ev, ev_c, True, scale=1.0)
+    assert_parity(g, g_c, True, scale=1.0)
     assert abs(norm - 1.0) < 1e-12
 
 
