@@ -51,7 +51,7 @@ extern "C" {
 #define JQ_ERR_DEVICE 4      /* HIP runtime error */
 #define JQ_ERR_NONFINITE 5   /* non-finite or singular result */
 
-#define JQ_MAX_WIRES 5
+#define JQ_MAX_WIRES 6
 
 /* member kinds: one primitive factor of a recorded gate */
 #define JQ_M_ROT 0    /* controlled(cos(t/2) I - i sin(t/2) P_word, nctrl) */
@@ -66,7 +66,14 @@ extern "C" {
 /* op structure flags (GateStructure, operations.py:176-185) */
 #define JQ_S_DIAGONAL 1
 #define JQ_S_PERMUTATION 2
-#define JQ_S_NONUNITARY 4 /* channel superoperator: un-applied by its inverse */
+#define JQ_S_NONUNITARY 4 /* channel superoperator (not unitary)                */
+
+/* A non-unitary block whose condition number (the product of its ops'
+ * jq_op.cond) exceeds this bound is not un-applied by its inverse in the
+ * adjoint sweep: it runs in a pass of its own, whose backward reads the
+ * checkpointed input state instead.  Singular channels (cond = inf, e.g.
+ * BitFlip(0.5), AmplitudeDamping(1)) always take that path. */
+#define JQ_COND_MAX 4.0
 
 typedef struct jq_member {
   int32_t kind;   /* JQ_M_* */
@@ -83,9 +90,11 @@ typedef struct jq_op {
   int32_t wires[JQ_MAX_WIRES];
   int32_t structure;    /* JQ_S_* */
   int32_t controls;     /* GateStructure.controls */
-  int8_t perm[32];      /* GateStructure.permutation when JQ_S_PERMUTATION */
+  int8_t perm[64];      /* GateStructure.permutation when JQ_S_PERMUTATION */
   int32_t member_begin; /* members of this op: [member_begin, member_end) */
   int32_t member_end;
+  double cond;          /* JQ_S_NONUNITARY: 2-norm condition number of the op's
+                           matrix (inf if singular or unknown); ignored otherwise */
 } jq_op;
 
 /* observable kinds for the fused reductions */
@@ -184,6 +193,34 @@ int jq_sv_adjoint(jq_plan* plan, int32_t batch, const double* angles, const void
                   const void* psi0, int32_t psi0_batched, void* state, int32_t obs_kind, int32_t n_obs,
                   const int32_t* obs_wires, const double* obs_d, const double* ct, double* expvals,
                   double* grad_angles, void* grad_matrices, void* grad_psi0, void* stream);
+
+/* The adjoint in two calls, so that a caller that needs the forward's
+ * outputs before it knows the cotangent (an autograd forward / backward
+ * pair) runs the forward passes once:
+ *   jq_sv_forward_ckpt  runs jq_sv_forward and keeps every pass output in the
+ *                       plan; *token identifies that forward;
+ *   jq_sv_backward      runs the adjoint sweep over those checkpoints.  The
+ *                       caller passes the same batch, angles, matrices, psi0
+ *                       and the untouched `state` of that forward.  If any
+ *                       other call has used the plan since (the token no
+ *                       longer matches), the forward is run again first, so
+ *                       the result is always that of jq_sv_adjoint.
+ * The cotangent is either ct (observables, as jq_sv_adjoint) or, when
+ * lam_seed is not NULL, a caller-given cotangent of the final vector:
+ * lam_seed [batch][2^n] in the plan dtype, JAX's convention (for a real loss
+ * L, 2 dL/dpsi -- the conjugate of PyTorch's gradient).  lam_seed is used as
+ * the running adjoint vector and overwritten.
+ * Replaces the fwd/bwd pair of _adjoint_expval's custom_vjp
+ * (simulation.py:586-654), and the reverse-mode autodiff the reference runs
+ * through measure_state / measure_density for probs, state and density
+ * outputs (simulation.py:658-732). */
+int jq_sv_forward_ckpt(jq_plan* plan, int32_t batch, const double* angles, const void* matrices, const void* psi0,
+                       int32_t psi0_batched, void* state, int32_t obs_kind, int32_t n_obs, const int32_t* obs_wires,
+                       const double* obs_d, double* expvals, int64_t* token, void* stream);
+int jq_sv_backward(jq_plan* plan, int64_t token, int32_t batch, const double* angles, const void* matrices,
+                   const void* psi0, int32_t psi0_batched, void* state, int32_t obs_kind, int32_t n_obs,
+                   const int32_t* obs_wires, const double* obs_d, const double* ct, void* lam_seed,
+                   double* grad_angles, void* grad_matrices, void* grad_psi0, void* stream);
 
 /* One group of a ragged batch: `batch` circuits sharing `plan`'s structure. */
 typedef struct jq_group {

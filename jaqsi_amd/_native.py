@@ -28,7 +28,8 @@ JQ_M_ROT, JQ_M_PHASE, JQ_M_MATRIX = 0, 1, 2
 JQ_MF_CONJ, JQ_MF_SUPEROP, JQ_MF_GRAD = 1, 2, 4
 JQ_S_DIAGONAL, JQ_S_PERMUTATION, JQ_S_NONUNITARY = 1, 2, 4
 JQ_OBS_NONE, JQ_OBS_SV_DIAG, JQ_OBS_DM_DIAG, JQ_OBS_SV_PARITY = 0, 1, 2, 3
-MAX_WIRES = 5
+MAX_WIRES = 6
+JQ_COND_MAX = 4.0
 
 
 class Member(C.Structure):
@@ -49,9 +50,10 @@ class Op(C.Structure):
         ("wires", C.c_int32 * MAX_WIRES),
         ("structure", C.c_int32),
         ("controls", C.c_int32),
-        ("perm", C.c_int8 * 32),
+        ("perm", C.c_int8 * 64),
         ("member_begin", C.c_int32),
         ("member_end", C.c_int32),
+        ("cond", C.c_double),
     ]
 
 
@@ -118,6 +120,8 @@ EXPORTS = [
     "jq_plan_jit",
     "jq_sv_forward",
     "jq_sv_adjoint",
+    "jq_sv_forward_ckpt",
+    "jq_sv_backward",
     "jq_sv_adjoint_groups",
     "jq_sv_probs",
     "jq_sv_outer",
@@ -160,6 +164,10 @@ def lib():
         L.jq_plan_jit.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
         L.jq_sv_forward.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
         L.jq_sv_adjoint.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.jq_sv_forward_ckpt.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp, vp,
+                                         C.POINTER(C.c_int64), vp]
+        L.jq_sv_backward.argtypes = [vp, C.c_int64, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp,
+                                     vp, vp, vp, vp, vp, vp]
         L.jq_sv_adjoint_groups.argtypes = [C.c_int32, C.POINTER(Group), C.c_int32, C.c_int32, vp, vp, vp]
         L.jq_pulse_propagate.argtypes = [C.POINTER(PulseDesc), vp, vp, vp, vp, vp, vp]
         L.jq_sv_probs.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp]

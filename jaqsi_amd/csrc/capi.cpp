@@ -21,6 +21,7 @@
 #include "pulse.h"
 #include "program.h"
 #include "sv_kernels.h"
+#include "wide.h"
 
 namespace {
 thread_local std::string g_err;
@@ -58,10 +59,12 @@ struct JitFn {
   int nt = 0;            // threads per workgroup
   int64_t pool_off = 0;  // SvArgs::pool_off of the pass
   int resident = 0;      // workgroups of the device resident at once (persistent kernels)
+  const jq::WideGeom* wide = nullptr;  // a wide-block pass: run the wide-block kernels
 };
 
 hipError_t launch_pass(const JitFn& jf, int variant, int dtype, int T, const jq::SvArgs& a, int ntiles, int batch,
                        size_t smem, hipStream_t st) {
+  if (jf.wide != nullptr) return jq::launch_wide(variant & 1, dtype, a, *jf.wide, batch, st);
   if (jf.f == nullptr) return jq::launch_sv(variant, dtype, T, a, ntiles, batch, smem, st);
   jq::SvArgs arg = a;
   arg.pool_off = jf.pool_off;
@@ -178,9 +181,20 @@ struct jq_plan {
   DevBuf prog_bj;
   bool uploaded = false;
   int class_mask = 0;  // block classes of the prep / chain kernels (jq::block_class)
+  std::vector<int32_t> wg_blocks;  // blocks of the workgroup prep / chain classes
+  DevBuf wg_list;
+  std::vector<char> pass_wide;               // pass q is one wide block (program.h PF_WIDE)
+  std::vector<jq::WideGeom> wide_geom;       // its launch geometry
+  bool any_wide = false;
   JitState jit;
+  // Checkpoints of the last jq_sv_forward_ckpt: valid while no other call
+  // has run on the plan since (every call that touches the workspaces
+  // advances `generation`).
+  int64_t generation = 0;
+  int64_t ckpt_token = -1;
+  int ckpt_batch = 0;
   // workspaces
-  DevBuf pool, partials, bars, ev_part, lam, ckpt, status, obs, fstate, groups;
+  DevBuf pool, partials, bars, ev_part, lam, ckpt, obs, fstate, groups;
 };
 
 namespace {
@@ -383,6 +397,10 @@ bool jit_ready(jq_plan* p) {
 
 JitFn jit_fn(jq_plan* p, bool on, int dir, int q) {
   JitFn f;
+  if (p->pass_wide[q]) {
+    f.wide = &p->wide_geom[q];
+    return f;
+  }
   if (on) {
     f.f = p->jit.fn[p->jit.plan.kernel_of[dir][q]];
     f.nt = p->jit.nt[dir];
@@ -435,15 +453,26 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   p->n_params = desc->n_params;
   p->n_matrix_elems = desc->n_matrix_elems;
   p->checkpoints = desc->checkpoints;
+  // (a wide pass reads its input state from the checkpoints in the backward)
   // per-pass slot capacity in LDS (the planner keeps every pass within it)
   const int* W = p->hp.words.data();
+  p->pass_wide.assign(p->hp.n_passes, 0);
+  p->wide_geom.assign(p->hp.n_passes, jq::WideGeom());
   for (int q = 0; q < p->hp.n_passes; ++q) {
     const int* pass = W + W[jq::HDR_WORDS + q];
+    if (pass[1] & jq::PF_WIDE) {  // one substep, one gate record
+      const int* sub = W + pass[4 + T];
+      p->pass_wide[q] = 1;
+      p->any_wide = true;
+      jq::wide_geom(sub + 1 + T, desc->n_wires, p->hp.ntiles, pass[3], desc->dtype, p->wide_geom[q]);
+      continue;
+    }
     if (2 * lds_state + sizeof(jq::ObsTable) + (64 + 2 * (size_t)pass[2]) * 8 > 160 * 1024) {
       delete p;
       return fail(JQ_ERR_UNSUPPORTED, "too many gradient slots in one pass for LDS");
     }
   }
+  if (p->any_wide) p->checkpoints = 1;
   p->pass_full.assign(p->hp.n_passes, 0);
   bool any_full = false;
   for (int q = 0; q < p->hp.n_passes; ++q) {
@@ -499,8 +528,13 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   }
   p->n_slots = (int)smv.size();
   p->slot_words = sm;
-  for (size_t r = 0; r + jq::BW <= p->hp.block_table.size(); r += jq::BW)
-    p->class_mask |= 1 << jq::block_class(p->hp.block_table[r + jq::B_DIAGF] != 0, p->hp.block_table[r + jq::B_D]);
+  for (size_t r = 0; r + jq::BW <= p->hp.block_table.size(); r += jq::BW) {
+    const int cls = jq::block_class(p->hp.block_table[r + jq::B_DIAGF] != 0, p->hp.block_table[r + jq::B_D]);
+    if (cls >= jq::WG_FIRST_CLASS)
+      p->wg_blocks.push_back((int32_t)(r / jq::BW));
+    else
+      p->class_mask |= 1 << cls;
+  }
   int ndev = 0;
   if (jit_on && hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) {  // nothing to run them on otherwise
     p->jit.mode = JQ_JIT_ASYNC;
@@ -537,7 +571,7 @@ int jq_plan_destroy(jq_plan* p) {
   if (!p) return JQ_OK;
   if (p->jit.worker.joinable()) p->jit.worker.join();
   for (DevBuf* b : {&p->prog, &p->prog_b, &p->prog_bj, &p->block_table, &p->member_list, &p->member_table, &p->member_values, &p->slot_map,
-                    &p->pool, &p->partials, &p->bars, &p->ev_part, &p->lam, &p->ckpt, &p->status, &p->obs, &p->fstate, &p->groups})
+                    &p->pool, &p->partials, &p->bars, &p->ev_part, &p->lam, &p->ckpt, &p->obs, &p->fstate, &p->groups, &p->wg_list})
     b->release();
   delete p;
   return JQ_OK;
@@ -598,6 +632,7 @@ int ensure_uploaded(jq_plan* p) {
   if (e == hipSuccess) e = upload(p->member_table, p->hp.member_table);
   if (e == hipSuccess) e = upload(p->member_values, p->hp.member_values);
   if (e == hipSuccess) e = upload(p->slot_map, p->slot_words);
+  if (e == hipSuccess) e = upload(p->wg_list, p->wg_blocks);
   if (e != hipSuccess) return fail(JQ_ERR_DEVICE, std::string("plan upload: ") + hipGetErrorString(e));
   p->uploaded = true;
   return JQ_OK;
@@ -641,7 +676,6 @@ int fill_obs(jq_plan* p, jq::SvArgs& a, int n_wires, int obs_kind, int n_obs, co
 int run_prep(jq_plan* p, int batch, const double* angles, const void* matrices, hipStream_t st) {
   const size_t eb = elem_bytes(p->dtype);
   HIP_TRY(p->pool.ensure(std::max<size_t>(1, (size_t)p->hp.pool * batch) * eb));
-  HIP_TRY(p->status.ensure(sizeof(int)));
   jq::PrepArgs pa;
   pa.n_blocks = p->hp.n_dev_blocks;
   pa.batch = batch;
@@ -655,11 +689,11 @@ int run_prep(jq_plan* p, int batch, const double* angles, const void* matrices, 
   pa.n_matrix_elems = p->n_matrix_elems;
   pa.pool = p->pool.p;
   pa.pool_stride = p->hp.pool;
-  pa.status = (int*)p->status.p;
   pa.class_mask = p->class_mask;
+  pa.wg_blocks = (const int*)p->wg_list.p;
+  pa.n_wg = (int)p->wg_blocks.size();
   if (p->n_params > 0 && !angles) return fail(JQ_ERR_ARG, "angles required");
   if (p->n_matrix_elems > 0 && !matrices) return fail(JQ_ERR_ARG, "explicit matrices required");
-  HIP_TRY(jq::launch_zero(p->status.p, sizeof(int), st));
   HIP_TRY(jq::launch_prep(p->dtype, pa, st));
   return JQ_OK;
 }
@@ -683,7 +717,8 @@ size_t smem_for(jq_plan* p, int extra_reals, int tiles) {
 int max_slots(jq_plan* p) {
   int m = 0;
   const int* W = p->hp.words.data();
-  for (int q = 0; q < p->hp.n_passes; ++q) m = std::max(m, W[W[jq::HDR_WORDS + q] + 2]);
+  for (int q = 0; q < p->hp.n_passes; ++q)
+    if (!p->pass_wide[q]) m = std::max(m, W[W[jq::HDR_WORDS + q] + 2]);
   return m;
 }
 
@@ -725,9 +760,109 @@ int finish_grads(jq_plan* p, int batch, const double* angles, const void* matric
     c.grad_angles = grad_angles;
     c.grad_matrices = grad_matrices;
     c.class_mask = p->class_mask;
+    c.wg_blocks = (const int*)p->wg_list.p;
+    c.n_wg = (int)p->wg_blocks.size();
     if (grad_angles || grad_matrices) HIP_TRY(jq::launch_chain(c, st));
   }
   return JQ_OK;
+}
+
+// Output of pass q: the checkpoint slice when checkpoints are kept, else the
+// final vector (passes then run in place).
+void* pass_out(jq_plan* p, bool ck, void* state, size_t vbytes, int q) {
+  if (q == p->hp.n_passes - 1 || !ck) return state;
+  return (char*)p->ckpt.p + vbytes * (size_t)q;
+}
+
+// Forward passes of one call (prep, passes, expectation values).
+int run_forward(jq_plan* p, int batch, const double* angles, const void* matrices, const void* psi0,
+                int psi0_batched, void* state, int obs_kind, int n_obs, const int32_t* obs_wires, const double* obs_d,
+                double* expvals, bool ck, hipStream_t st) {
+  int rc = ensure_uploaded(p);
+  if (rc) return rc;
+  ++p->generation;  // the workspaces (pool, checkpoints) change from here on
+  rc = run_prep(p, batch, angles, matrices, st);
+  if (rc) return rc;
+  jq::SvArgs a = base_args(p);
+  rc = fill_obs(p, a, p->hp.n_wires, expvals ? obs_kind : JQ_OBS_NONE, n_obs, obs_wires, obs_d, st);
+  if (rc) return rc;
+  const int ntiles = p->hp.ntiles;
+  const int P = p->hp.n_passes;
+  if (a.obs_kind != JQ_OBS_NONE) {
+    HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * a.n_obs) * 8));
+    a.ev_part = (double*)p->ev_part.p;
+  }
+  const size_t vbytes = (size_t)a.vec_len * elem_bytes(p->dtype) * batch;
+  ck = ck && P > 1;
+  if (ck) HIP_TRY(p->ckpt.ensure(vbytes * (size_t)(P - 1)));
+  const bool jit = jit_ready(p);
+  for (int q = 0; q < P; ++q) {
+    a.pass = q;
+    a.src = q == 0 ? psi0 : pass_out(p, ck, state, vbytes, q - 1);
+    a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
+    a.dst = pass_out(p, ck, state, vbytes, q);
+    a.dst_stride = (int64_t)a.vec_len;
+    HIP_TRY(timed_launch(0, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st, false,
+                         jit_fn(p, jit, 0, q), q == 0, q == P - 1));
+  }
+  if (a.obs_kind != JQ_OBS_NONE && expvals)
+    HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
+  return JQ_OK;
+}
+
+// Backward sweep over the passes of the preceding run_forward (same ck).
+int run_backward(jq_plan* p, int batch, const double* angles, const void* matrices, const void* psi0,
+                 int psi0_batched, void* state, int obs_kind, int n_obs, const int32_t* obs_wires,
+                 const double* obs_d, const double* ct, void* lam_seed, double* grad_angles, void* grad_matrices,
+                 void* grad_psi0, bool ck, hipStream_t st) {
+  jq::SvArgs a = base_args(p);
+  if (lam_seed) {
+    a.lam_given = 1;
+  } else {
+    int rc = fill_obs(p, a, p->hp.n_wires, obs_kind, n_obs, obs_wires, obs_d, st);
+    if (rc) return rc;
+    a.ct = ct;
+  }
+  const int ntiles = p->hp.ntiles;
+  const int P = p->hp.n_passes;
+  const size_t vbytes = (size_t)a.vec_len * elem_bytes(p->dtype) * batch;
+  ck = ck && P > 1;
+  HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * std::max(1, a.n_obs)) * 8));
+  HIP_TRY(p->partials.ensure(std::max<size_t>(1, (size_t)batch * p->hp.partial_reals) * 8));
+  a.grad_part = (double*)p->partials.p;
+  a.part_stride = p->hp.partial_reals;
+  a.ev_part = (double*)p->ev_part.p;
+  // the running adjoint vector ends as the initial-state cotangent
+  void* lam = grad_psi0;
+  if (lam_seed && grad_psi0) {
+    HIP_TRY(hipMemcpyAsync(grad_psi0, lam_seed, vbytes, hipMemcpyDeviceToDevice, st));
+  } else if (lam_seed) {
+    lam = lam_seed;
+  } else if (!lam) {
+    HIP_TRY(p->lam.ensure(vbytes));
+    lam = p->lam.p;
+  }
+  a.lam = lam;
+  a.lam_stride = (int64_t)a.vec_len;
+  const bool jit = jit_ready(p);
+  const size_t sm = smem_for(p, 2 * max_slots(p), 2);
+  a.prog = (const int*)(jit ? p->prog_bj.p : p->prog_b.p);
+  for (int q = P - 1; q >= 0; --q) {
+    a.pass = q;
+    if (p->pass_wide[q]) {  // reads the pass's input state
+      a.src = q == 0 ? psi0 : pass_out(p, ck, state, vbytes, q - 1);
+      a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
+      a.dst = nullptr;
+    } else {
+      a.src = pass_out(p, ck, state, vbytes, q);
+      a.src_stride = (int64_t)a.vec_len;
+      a.dst = ck ? nullptr : state;  // without checkpoints psi is un-applied in place
+    }
+    a.dst_stride = (int64_t)a.vec_len;
+    HIP_TRY(timed_launch(1, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, sm, st, p->wide_bwd,
+                         jit_fn(p, jit, 1, q), q == P - 1, q == 0));
+  }
+  return finish_grads(p, batch, angles, matrices, a.grad_part, a.part_stride, grad_angles, grad_matrices, st);
 }
 }  // namespace
 
@@ -738,31 +873,8 @@ int jq_sv_forward(jq_plan* p, int32_t batch, const double* angles, const void* m
                   const double* obs_d, double* expvals, void* stream) {
   if (!p || !state || batch < 1) return fail(JQ_ERR_ARG, "bad arguments to jq_sv_forward");
   if (batch > 65535) return fail(JQ_ERR_ARG, "batch above 65535 per call");
-  hipStream_t st = (hipStream_t)stream;
-  int rc = ensure_uploaded(p);
-  if (rc) return rc;
-  rc = run_prep(p, batch, angles, matrices, st);
-  if (rc) return rc;
-  jq::SvArgs a = base_args(p);
-  rc = fill_obs(p, a, p->hp.n_wires, expvals ? obs_kind : JQ_OBS_NONE, n_obs, obs_wires, obs_d, st);
-  if (rc) return rc;
-  const int ntiles = p->hp.ntiles;
-  if (a.obs_kind != JQ_OBS_NONE) {
-    HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * a.n_obs) * 8));
-    a.ev_part = (double*)p->ev_part.p;
-  }
-  const bool jit = jit_ready(p);
-  for (int q = 0; q < p->hp.n_passes; ++q) {
-    a.pass = q;
-    a.src = q == 0 ? psi0 : state;
-    a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
-    a.dst = state;
-    a.dst_stride = (int64_t)a.vec_len;
-    HIP_TRY(timed_launch(0, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st, false,
-                         jit_fn(p, jit, 0, q), q == 0, q == p->hp.n_passes - 1));
-  }
-  if (a.obs_kind != JQ_OBS_NONE) HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
-  return JQ_OK;
+  return run_forward(p, batch, angles, matrices, psi0, psi0_batched, state, obs_kind, n_obs, obs_wires, obs_d, expvals,
+                     false, (hipStream_t)stream);
 }
 
 int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* matrices, const void* psi0,
@@ -773,63 +885,48 @@ int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* m
   if (batch > 65535) return fail(JQ_ERR_ARG, "batch above 65535 per call");
   if (obs_kind == JQ_OBS_NONE) return fail(JQ_ERR_ARG, "the adjoint needs observables");
   hipStream_t st = (hipStream_t)stream;
-  int rc = ensure_uploaded(p);
+  const bool ck = p->checkpoints != 0;
+  int rc = run_forward(p, batch, angles, matrices, psi0, psi0_batched, state, obs_kind, n_obs, obs_wires, obs_d,
+                       expvals, ck, st);
   if (rc) return rc;
-  rc = run_prep(p, batch, angles, matrices, st);
+  return run_backward(p, batch, angles, matrices, psi0, psi0_batched, state, obs_kind, n_obs, obs_wires, obs_d, ct,
+                      nullptr, grad_angles, grad_matrices, grad_psi0, ck, st);
+}
+
+int jq_sv_forward_ckpt(jq_plan* p, int32_t batch, const double* angles, const void* matrices, const void* psi0,
+                       int32_t psi0_batched, void* state, int32_t obs_kind, int32_t n_obs, const int32_t* obs_wires,
+                       const double* obs_d, double* expvals, int64_t* token, void* stream) {
+  if (!p || !state || batch < 1 || !token) return fail(JQ_ERR_ARG, "bad arguments to jq_sv_forward_ckpt");
+  if (batch > 65535) return fail(JQ_ERR_ARG, "batch above 65535 per call");
+  *token = -1;
+  int rc = run_forward(p, batch, angles, matrices, psi0, psi0_batched, state, obs_kind, n_obs, obs_wires, obs_d,
+                       expvals, true, (hipStream_t)stream);
   if (rc) return rc;
-  jq::SvArgs a = base_args(p);
-  rc = fill_obs(p, a, p->hp.n_wires, obs_kind, n_obs, obs_wires, obs_d, st);
-  if (rc) return rc;
-  a.ct = ct;
-  const int ntiles = p->hp.ntiles;
-  const int P = p->hp.n_passes;
-  const size_t eb = elem_bytes(p->dtype);
-  const size_t vbytes = (size_t)a.vec_len * eb * batch;
-  HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * std::max(1, a.n_obs)) * 8));
-  HIP_TRY(p->partials.ensure(std::max<size_t>(1, (size_t)batch * p->hp.partial_reals) * 8));
-  a.grad_part = (double*)p->partials.p;
-  a.part_stride = p->hp.partial_reals;
-  a.ev_part = (double*)p->ev_part.p;
-  void* lam = grad_psi0;
-  if (!lam) {
-    HIP_TRY(p->lam.ensure(vbytes));
-    lam = p->lam.p;
+  p->ckpt_token = p->generation;
+  p->ckpt_batch = batch;
+  *token = p->ckpt_token;
+  return JQ_OK;
+}
+
+int jq_sv_backward(jq_plan* p, int64_t token, int32_t batch, const double* angles, const void* matrices,
+                   const void* psi0, int32_t psi0_batched, void* state, int32_t obs_kind, int32_t n_obs,
+                   const int32_t* obs_wires, const double* obs_d, const double* ct, void* lam_seed,
+                   double* grad_angles, void* grad_matrices, void* grad_psi0, void* stream) {
+  if (!p || !state || batch < 1) return fail(JQ_ERR_ARG, "bad arguments to jq_sv_backward");
+  if (batch > 65535) return fail(JQ_ERR_ARG, "batch above 65535 per call");
+  if (!lam_seed && (obs_kind == JQ_OBS_NONE || !ct))
+    return fail(JQ_ERR_ARG, "jq_sv_backward needs observables and their cotangents, or lam_seed");
+  hipStream_t st = (hipStream_t)stream;
+  const bool valid = token >= 0 && token == p->ckpt_token && token == p->generation && batch == p->ckpt_batch;
+  if (!valid) {  // another call has used the plan since: run the forward again
+    int rc = run_forward(p, batch, angles, matrices, psi0, psi0_batched, state, obs_kind, n_obs, obs_wires, obs_d,
+                         nullptr, true, st);
+    if (rc) return rc;
+    p->ckpt_token = p->generation;
+    p->ckpt_batch = batch;
   }
-  a.lam = lam;
-  a.lam_stride = (int64_t)a.vec_len;
-  {
-    // forward: keep every pass output (checkpoints) when enabled
-    const bool ck = p->checkpoints != 0 && P > 1;
-    if (ck) HIP_TRY(p->ckpt.ensure(vbytes * (size_t)(P - 1)));
-    auto pass_out = [&](int q) -> void* {
-      if (q == P - 1 || !ck) return state;
-      return (char*)p->ckpt.p + vbytes * (size_t)q;
-    };
-    const bool jit = jit_ready(p);
-    for (int q = 0; q < P; ++q) {
-      a.pass = q;
-      a.src = q == 0 ? psi0 : pass_out(q - 1);
-      a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
-      a.dst = pass_out(q);
-      a.dst_stride = (int64_t)a.vec_len;
-      HIP_TRY(timed_launch(0, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st, false,
-                           jit_fn(p, jit, 0, q), q == 0, q == P - 1));
-    }
-    if (expvals) HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
-    // backward sweep
-    const size_t sm = smem_for(p, 2 * max_slots(p), 2);
-    a.prog = (const int*)(jit ? p->prog_bj.p : p->prog_b.p);
-    for (int q = P - 1; q >= 0; --q) {
-      a.pass = q;
-      a.src = pass_out(q);
-      a.src_stride = (int64_t)a.vec_len;
-      a.dst = ck ? nullptr : state;  // without checkpoints psi is un-applied in place
-      a.dst_stride = (int64_t)a.vec_len;
-      HIP_TRY(timed_launch(1, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, sm, st, p->wide_bwd,
-                           jit_fn(p, jit, 1, q), q == P - 1, q == 0));
-    }
-  }
-  return finish_grads(p, batch, angles, matrices, a.grad_part, a.part_stride, grad_angles, grad_matrices, st);
+  return run_backward(p, batch, angles, matrices, psi0, psi0_batched, state, obs_kind, n_obs, obs_wires, obs_d, ct,
+                      lam_seed, grad_angles, grad_matrices, grad_psi0, true, st);
 }
 
 int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_kind, int32_t n_obs,
@@ -859,8 +956,13 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
   // instead of one per pass step, but each pass runs its straight-line
   // kernel).  Otherwise all groups share the interpreter's launches below.
   bool all_jit = true;
-  for (int g = 0; g < n_groups && all_jit; ++g) all_jit = jit_ready(groups[g].plan);
-  if (all_jit) {
+  bool any_wide = false;
+  for (int g = 0; g < n_groups; ++g) {
+    all_jit = all_jit && jit_ready(groups[g].plan);
+    any_wide = any_wide || groups[g].plan->any_wide;
+  }
+  // (wide-block passes run their own kernels: such groups take this path too)
+  if (all_jit || any_wide) {
     // Groups share nothing (each plan owns its workspaces), so they run on
     // side streams forked from and joined back to the caller's stream with
     // events: small groups' launches overlap instead of leaving the device
@@ -911,6 +1013,7 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
     jq_plan* p = G.plan;
     rc = ensure_uploaded(p);
     if (rc) return rc;
+    ++p->generation;
     rc = run_prep(p, G.batch, G.angles, G.matrices, st);
     if (rc) return rc;
     const int P = p->hp.n_passes;

@@ -306,6 +306,11 @@ void jit_plan(const JitSpec& spec, JitPlan& out) {
   for (int dir = 0; dir < 2; ++dir) {
     const ProgView pv = view(dir == 0 ? *spec.words_f : *spec.words_b);
     for (int q = 0; q < pv.n_passes; ++q) {
+      if (pv.pass(q)[1] & PF_WIDE) {  // wide-block passes run their own kernels
+        out.kernel_of[dir].push_back(-1);
+        out.pool_off[dir].push_back(0);
+        continue;
+      }
       const int rebase = pass_rebase(pv, q);
       std::ostringstream o;
       Ends e;

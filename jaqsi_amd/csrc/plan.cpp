@@ -8,7 +8,9 @@
 // (simulation.py:243-335); here those runs are instead scheduled into tile
 // passes, where permutations and diagonals cost no HBM traffic of their own.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <limits>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -33,6 +35,8 @@ struct HB {  // host block
   bool grad = false;
   bool nonunitary = false;
   bool quat = false;  // MC1 block of the form [[a, -b*], [b, a*]] for every angle
+  bool wide = false;  // applied in a pass of its own (AK_WIDE)
+  double cond = 1.0;  // bound on the condition number: product over its non-unitary ops
   int mat = -1, inv = -1, bar = -1, bar_size = 0;
   std::vector<int> targets;  // wires that must sit in registers, in matrix order
   std::vector<int> ctrls;    // control wires
@@ -66,6 +70,14 @@ bool is_mcswap(const std::vector<int>& p) {  // swap of the last two wires under
 
 int gbit(int n, int wire) { return n - 1 - wire; }
 
+// Declared condition number of a non-unitary op; unknown (<= 0, NaN) or
+// singular counts as infinite.
+double op_cond(const jq_op& op) {
+  if (!(op.structure & JQ_S_NONUNITARY)) return 1.0;
+  const double c = op.cond;
+  return c >= 1.0 ? c : std::numeric_limits<double>::infinity();
+}
+
 }  // namespace
 
 void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgram& out, int slot_budget) {
@@ -77,7 +89,7 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
   for (int oi = 0; oi < d.n_ops; ++oi) {
     const jq_op& op = d.ops[oi];
     if (op.nwires < 1 || op.nwires > JQ_MAX_WIRES)
-      throw PlanError(JQ_ERR_UNSUPPORTED, "gates on more than 5 wires are not supported");
+      throw PlanError(JQ_ERR_UNSUPPORTED, "gates on more than 6 wires are not supported");
     std::vector<int> w(op.wires, op.wires + op.nwires);
     for (int x : w)
       if (x < 0 || x >= n) throw PlanError(JQ_ERR_ARG, "wire index out of range");
@@ -101,6 +113,7 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
       if (!np.empty()) s |= JQ_S_PERMUTATION;
       if ((b.structure | op.structure) & JQ_S_NONUNITARY) s |= JQ_S_NONUNITARY;
       b.structure = s;
+      b.cond *= op_cond(op);
       b.controls = std::min(b.controls, op.controls);
       for (int m = op.member_begin; m < op.member_end; ++m) b.members.push_back(m);
       continue;
@@ -110,6 +123,7 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
     b.structure = op.structure;
     b.controls = op.controls;
     b.perm = perm;
+    b.cond = op_cond(op);
     for (int m = op.member_begin; m < op.member_end; ++m) b.members.push_back(m);
     blocks.push_back(b);
     for (int x : w) last[x] = (int)blocks.size() - 1;
@@ -128,10 +142,15 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
       if (mm.param >= d.n_params) throw PlanError(JQ_ERR_ARG, "member parameter index out of range");
     }
     bool dense = false;
+    // A non-unitary block is un-applied by its inverse in the backward sweep
+    // only when it is well conditioned; otherwise it runs in a pass of its
+    // own, whose backward reads the checkpointed input instead.
+    const bool illcond = b.nonunitary && !(b.cond <= JQ_COND_MAX);
     if (b.structure & JQ_S_DIAGONAL) {
       b.kind = AK_DIAG;
       b.D = full;
       b.diagf = 1;
+      if (illcond || b.k > 5 || (b.grad && b.k > 2)) b.wide = true;
     } else if (b.structure & JQ_S_PERMUTATION) {
       if (is_identity(b.perm)) {
         b.kind = AK_NOP;
@@ -153,7 +172,7 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
     if (dense) {
       int c = std::max(0, std::min(b.controls, b.k - 1));
       int t = b.k - c;
-      if (t == 1) {
+      if (t == 1 && !illcond) {
         b.kind = AK_MC1;
         b.D = 2;
         // Products of Pauli rotations of the target (cos I - i sin P, P in
@@ -166,19 +185,30 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
           b.quat = b.quat && mm.kind == JQ_M_ROT && !(mm.flags & JQ_MF_SUPEROP) && mm.word >= 1 && mm.word <= 3 &&
                    mm.nctrl == b.k - 1;
         }
-      } else if (t == 2) {
+      } else if (t == 2 && !illcond) {
         b.kind = AK_MC2;
         b.D = 4;
-      } else if (c == 0 && b.k <= 3) {
+      } else if (c == 0 && b.k <= 3 && !b.grad && !illcond) {
         b.kind = AK_DENSEK;
         b.D = full;
-        if (b.grad) throw PlanError(JQ_ERR_UNSUPPORTED, "gradients of dense gates on 3 wires are not supported yet");
       } else {
-        throw PlanError(JQ_ERR_UNSUPPORTED, "dense gates with more than 2 target wires are not supported yet");
+        b.wide = true;
       }
-      b.offs = full - b.D;
-      b.targets.assign(b.wires.begin() + c, b.wires.end());
-      b.ctrls.assign(b.wires.begin(), b.wires.begin() + c);
+      if (!b.wide) {
+        b.offs = full - b.D;
+        b.targets.assign(b.wires.begin() + c, b.wires.end());
+        b.ctrls.assign(b.wires.begin(), b.wires.begin() + c);
+      }
+    }
+    if (b.wide) {
+      // the full 2^k matrix (controls folded in), or its diagonal
+      if (!b.diagf && b.k > WIDE_MAX_DENSE)
+        throw PlanError(JQ_ERR_UNSUPPORTED, "dense blocks on more than 5 wires are not supported");
+      b.kind = AK_WIDE;
+      b.D = full;
+      b.offs = 0;
+      b.targets.clear();
+      b.ctrls.clear();
     }
     if ((int)b.targets.size() > maxk) throw PlanError(JQ_ERR_UNSUPPORTED, "gate needs more register bits than the tile offers");
     if (b.kind == AK_MCX || b.kind == AK_MCSWAP || b.kind == AK_NOP) {
@@ -189,14 +219,11 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
     const int elems = b.diagf ? b.D : b.D * b.D;
     b.mat = pool;
     pool += elems;
-    if (b.nonunitary) {
+    if (b.nonunitary && !b.wide) {
       b.inv = pool;
       pool += elems;
     }
     if (b.grad) {
-      if (b.kind == AK_DENSEK) throw PlanError(JQ_ERR_UNSUPPORTED, "dense 3-wire gradient");
-      if (b.kind == AK_DIAG && b.k > 2)
-        throw PlanError(JQ_ERR_UNSUPPORTED, "gradients of diagonal gates on more than 2 wires are not supported yet");
       b.bar = bars;
       b.bar_size = elems;
       bars += elems;
@@ -244,8 +271,8 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
     rec[B_DIAGF] = b.diagf;
     rec[B_OFFS] = b.offs;
     out.block_table.insert(out.block_table.end(), rec, rec + BW);
-    if (b.D > 8 && !b.diagf) throw PlanError(JQ_ERR_UNSUPPORTED, "dense blocks wider than 3 wires are not supported yet");
-    if (b.D > 32) throw PlanError(JQ_ERR_UNSUPPORTED, "diagonal blocks wider than 5 wires are not supported");
+    if (!b.wide && b.D > 8 && !b.diagf) throw PlanError(JQ_ERR_ARG, "internal: dense tile block wider than 3 wires");
+    if (!b.wide && b.D > 32) throw PlanError(JQ_ERR_ARG, "internal: diagonal tile block wider than 5 wires");
   }
   out.n_dev_blocks = bi_dev;
   for (int m = 0; m < d.n_members; ++m) {
@@ -270,6 +297,7 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
   struct Pass {
     std::vector<int> tile;  // global bits ascending
     std::vector<int> blocks;
+    bool wide = false;      // one AK_WIDE block
   };
   std::vector<Pass> passes;
   auto low_bits = [&]() {
@@ -289,9 +317,23 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
     Pass P;
     std::vector<int> left;
     int pslots = 0;  // cotangent reals flushed by this pass
+    if (blocks[remaining[0]].wide) {  // a wide block runs alone
+      P.wide = true;
+      P.blocks.push_back(remaining[0]);
+      left.assign(remaining.begin() + 1, remaining.end());
+      for (int g = 0; g < T; ++g) P.tile.push_back(g);
+      passes.push_back(P);
+      remaining = left;
+      continue;
+    }
     for (int bi : remaining) {
       HB& b = blocks[bi];
-      bool blk = false;
+      bool blk = b.wide;  // deferred to a pass of its own
+      if (blk) {
+        for (int w : b.wires) blocked[gbit(n, w)] = 1;
+        left.push_back(bi);
+        continue;
+      }
       for (int w : b.wires)
         if (blocked[gbit(n, w)]) blk = true;
       const int reals = b.grad ? b.bar_size * 2 : 0;
@@ -320,7 +362,9 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
     passes.push_back(P);
     remaining = left;
   }
-  if (passes.empty()) {  // empty circuit: one pass with no gates (copies psi0)
+  if (passes.empty() || passes.back().wide) {
+    // empty circuit: one pass with no gates (copies psi0); after a final
+    // wide block, a gate-free tile pass that measures and seeds the adjoint
     Pass P;
     for (int g = 0; g < T; ++g) P.tile.push_back(g);
     passes.push_back(P);
@@ -368,6 +412,7 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
     int flags = 0;
     if (p == 0) flags |= PF_FIRST;
     if (p + 1 == passes.size()) flags |= PF_LAST;
+    if (P.wide) flags |= PF_WIDE;
     prog[rec + 1] = flags;
     std::vector<int> tloc(n + T + 1, -1);  // global bit -> tile-local bit
     for (int j = 0; j < T; ++j) {
@@ -376,8 +421,28 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
     }
     // substeps
     std::vector<std::vector<int32_t>> subwords;
-    std::vector<int> rem = P.blocks;
+    std::vector<int> rem = P.wide ? std::vector<int>() : P.blocks;
     int nsub = 0;
+    if (P.wide) {  // one substep (default layout) holding the wide block's record
+      HB& b = blocks[P.blocks[0]];
+      std::vector<int32_t> sw(1 + T, 0);
+      sw[0] = 1;
+      for (int i = 0; i < RB; ++i) sw[1 + i] = T - 1 - i;
+      for (int j = 0; j < T - RB; ++j) sw[1 + RB + j] = j;
+      int g[GW];
+      for (int& x : g) x = 0;
+      g[G_KIND] = AK_WIDE;
+      g[G_BLOCK] = dev_index[P.blocks[0]];
+      g[G_RA] = b.diagf;
+      g[G_MAT] = b.mat;
+      g[G_INV] = -1;
+      g[G_SLOT] = slot_of[P.blocks[0]];
+      g[G_K] = b.k;
+      for (int j = 0; j < b.k; ++j) g[G_DW0 + j / 4] |= gbit(n, b.wires[j]) << (8 * (j % 4));
+      sw.insert(sw.end(), g, g + GW);
+      subwords.push_back(sw);
+      nsub = 1;
+    }
     std::vector<int> prev_reg;
     while (!rem.empty()) {
       std::vector<int> reg;  // tile-local bits in registers, by register position

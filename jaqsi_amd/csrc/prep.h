@@ -17,14 +17,20 @@ struct PrepArgs {
   int64_t n_matrix_elems;
   void* pool;               // [batch][pool_stride] plan dtype
   int64_t pool_stride;
-  int* status;              // bit 0: singular non-unitary block
   int class_mask;           // block classes present (block_class), one kernel instance each
+  const int* wg_blocks;     // blocks of the workgroup classes (block_class >= 3)
+  int n_wg;
 };
 
-// Block classes of the prep / chain kernels, each compiled with matrices of
-// a fixed size: 0 diagonal <= 4, 1 diagonal <= 32, 2 dense 2x2, 3 dense 4x4,
-// 4 dense 8x8.
-__host__ __device__ inline int block_class(bool diagf, int D) { return diagf ? (D <= 4 ? 0 : 1) : (D == 2 ? 2 : D == 4 ? 3 : 4); }
+// Block classes of the prep / chain kernels: 0 diagonal <= 4, 1 diagonal
+// <= 32, 2 dense 2x2 (one thread per block, matrices in registers);
+// 3 dense 4x4, 4 dense 8x8, 5 dense 16x16 / 32x32 and diagonals of 64
+// (one workgroup per block, one matrix entry per thread, matrices in LDS).
+__host__ __device__ inline int block_class(bool diagf, int D) {
+  if (diagf) return D <= 4 ? 0 : D <= 32 ? 1 : 5;
+  return D == 2 ? 2 : D == 4 ? 3 : D == 8 ? 4 : 5;
+}
+constexpr int WG_FIRST_CLASS = 3;
 
 struct ReduceArgs {
   const int* slot_map;      // n_slots x {base, nslots, local, bar, size}, sorted by bar
@@ -52,6 +58,8 @@ struct ChainArgs {
   double* grad_angles;      // [batch][n_params], pre-zeroed
   void* grad_matrices;      // [batch][n_matrix_elems] complex128, pre-zeroed, may be null
   int class_mask;           // block classes present (block_class)
+  const int* wg_blocks;     // blocks of the workgroup classes (block_class >= 3)
+  int n_wg;
 };
 
 struct ExpvalArgs {

@@ -252,7 +252,7 @@ __device__ __forceinline__ void prep_block(const PrepArgs& a, int circ, const in
     } else {
       ok = invert_d(M, T, DM);
     }
-    if (!ok) atomicOr(a.status, 1);
+    (void)ok;  // the planner inverts only blocks of declared condition number <= JQ_COND_MAX
 #pragma unroll
     for (int e = 0; e < N; ++e)
       if (e < ne) pool[bt[B_INV] + e] = {(R)T[e].x, (R)T[e].y};
@@ -272,8 +272,105 @@ __global__ void prep_kernel(PrepArgs a) {
   const int* bt = a.block_table + b * BW;
   if (block_class(bt[B_DIAGF] != 0, bt[B_D]) != CLS) return;
   constexpr bool DIAG = CLS <= 1;
-  constexpr int DM = CLS == 0 ? 4 : CLS == 1 ? 32 : CLS == 2 ? 2 : CLS == 3 ? 4 : 8;
+  constexpr int DM = CLS == 0 ? 4 : CLS == 1 ? 32 : 2;
   prep_block<R, DIAG, DM>(a, circ, bt);
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup forms for the larger blocks (4x4 and up): one workgroup of one
+// wave per (block, circuit), one matrix entry per thread, matrices in LDS.
+// A thread per block serialised their FP64 products (a 4x4 density-matrix
+// block of nine members took ~1 ms per call that way).
+// ---------------------------------------------------------------------------
+constexpr int WG_NT = 64;
+constexpr int WG_MAXE = 32 * 32;  // entries of the largest dense matrix
+
+__device__ __forceinline__ void wg_identity(cd* M, int D, bool diagf) {
+  const int ne = diagf ? D : D * D;
+  for (int e = threadIdx.x; e < ne; e += WG_NT) M[e] = {diagf ? 1.0 : ((e / D) == (e % D) ? 1.0 : 0.0), 0.0};
+}
+
+__device__ __forceinline__ void wg_eff(const int* mt, double th, const cd* xm, int D, int offs, bool diagf, bool deriv,
+                                       cd* E) {
+  const int ne = diagf ? D : D * D;
+  for (int e = threadIdx.x; e < ne; e += WG_NT) {
+    const int r = diagf ? e : e / D, c = diagf ? e : e % D;
+    E[e] = member_entry(mt, th, xm, offs + r, offs + c, deriv);
+  }
+}
+
+// C = op(A) B, where op(A) = A or A^T (dense) or the elementwise product
+// (diagonal); C aliases neither.
+__device__ __forceinline__ void wg_mul(const cd* A, const cd* B, cd* C, int D, bool diagf, bool transA) {
+  if (diagf) {
+    for (int e = threadIdx.x; e < D; e += WG_NT) C[e] = cmul(A[e], B[e]);
+    return;
+  }
+  for (int e = threadIdx.x; e < D * D; e += WG_NT) {
+    const int r = e / D, c = e % D;
+    cd acc = {0.0, 0.0};
+    for (int k = 0; k < D; ++k) acc = cadd(acc, cmul(transA ? A[k * D + r] : A[r * D + k], B[k * D + c]));
+    C[e] = acc;
+  }
+}
+
+// C = A B^T (dense) or elementwise
+__device__ __forceinline__ void wg_mul_bt(const cd* A, const cd* B, cd* C, int D, bool diagf) {
+  if (diagf) {
+    for (int e = threadIdx.x; e < D; e += WG_NT) C[e] = cmul(A[e], B[e]);
+    return;
+  }
+  for (int e = threadIdx.x; e < D * D; e += WG_NT) {
+    const int r = e / D, c = e % D;
+    cd acc = {0.0, 0.0};
+    for (int k = 0; k < D; ++k) acc = cadd(acc, cmul(A[r * D + k], B[c * D + k]));
+    C[e] = acc;
+  }
+}
+
+__device__ __forceinline__ void wg_copy(const cd* A, cd* B, int ne) {
+  for (int e = threadIdx.x; e < ne; e += WG_NT) B[e] = A[e];
+}
+
+template <typename R>
+__global__ __launch_bounds__(WG_NT) void prep_wg_kernel(PrepArgs a) {
+  __shared__ cd M[WG_MAXE], E[WG_MAXE], T[WG_MAXE];
+  const int b = a.wg_blocks[blockIdx.x];
+  const int circ = blockIdx.y;
+  const int* bt = a.block_table + b * BW;
+  const int D = bt[B_D], offs = bt[B_OFFS];
+  const bool diagf = bt[B_DIAGF] != 0;
+  const int ne = diagf ? D : D * D;
+  const double* angles = a.angles + (int64_t)circ * a.n_params;
+  const cd* xm = reinterpret_cast<const cd*>(a.matrices) + (a.matrices ? (int64_t)circ * a.n_matrix_elems : 0);
+  wg_identity(M, D, diagf);
+  for (int mi = bt[B_MBEG]; mi < bt[B_MEND]; ++mi) {
+    const int m = a.member_list[mi];
+    const int* mt = a.member_table + m * MW;
+    const double th = member_theta(mt, a.member_values[m], angles);
+    __syncthreads();
+    wg_eff(mt, th, xm, D, offs, diagf, false, E);
+    __syncthreads();
+    wg_mul(E, M, T, D, diagf, false);
+    __syncthreads();
+    wg_copy(T, M, ne);
+  }
+  __syncthreads();
+  Out2<R>* pool = reinterpret_cast<Out2<R>*>(a.pool) + (int64_t)circ * a.pool_stride;
+  for (int e = threadIdx.x; e < ne; e += WG_NT) pool[bt[B_MAT] + e] = {(R)M[e].x, (R)M[e].y};
+  if (bt[B_INV] >= 0 && threadIdx.x == 0) {
+    // tile-pass non-unitary blocks (<= 8 rows), of declared condition <= JQ_COND_MAX
+    if (diagf) {
+      for (int r = 0; r < D; ++r) {
+        const double den = M[r].x * M[r].x + M[r].y * M[r].y;
+        pool[bt[B_INV] + r] = {(R)(M[r].x / den), (R)(-M[r].y / den)};
+      }
+    } else if (D <= 8) {
+      cd inv[64];
+      invert_d(M, inv, D);
+      for (int e = 0; e < D * D; ++e) pool[bt[B_INV] + e] = {(R)inv[e].x, (R)inv[e].y};
+    }
+  }
 }
 
 // sum over tiles of each cotangent slot -> bars (FP64 reals)
@@ -435,8 +532,80 @@ __global__ void chain_kernel(ChainArgs a) {
   const int* bt = a.block_table + b * BW;
   if (bt[B_BAR] < 0 || block_class(bt[B_DIAGF] != 0, bt[B_D]) != CLS) return;
   constexpr bool DIAG = CLS <= 1;
-  constexpr int DM = CLS == 0 ? 4 : CLS == 1 ? 32 : CLS == 2 ? 2 : CLS == 3 ? 4 : 8;
+  constexpr int DM = CLS == 0 ? 4 : CLS == 1 ? 32 : 2;
   chain_block<DIAG, DM>(a, circ, bt);
+}
+
+// Workgroup form of chain_block: W_j = L_j^T bar R_j^T with the prefix R_j
+// kept in LDS and the suffix L_j rebuilt per differentiated member.
+__global__ __launch_bounds__(WG_NT) void chain_wg_kernel(ChainArgs a) {
+  __shared__ cd Rp[WG_MAXE], Lm[WG_MAXE], E[WG_MAXE], T[WG_MAXE], B[WG_MAXE];
+  __shared__ double red[WG_NT / 64];
+  const int b = a.wg_blocks[blockIdx.x];
+  const int circ = blockIdx.y;
+  const int* bt = a.block_table + b * BW;
+  if (bt[B_BAR] < 0) return;
+  const int D = bt[B_D], offs = bt[B_OFFS];
+  const bool diagf = bt[B_DIAGF] != 0;
+  const int ne = diagf ? D : D * D;
+  const double* angles = a.angles + (int64_t)circ * a.n_params;
+  const cd* xm = reinterpret_cast<const cd*>(a.matrices) + (a.matrices ? (int64_t)circ * a.n_matrix_elems : 0);
+  const cd* bar = reinterpret_cast<const cd*>(a.bars) + (int64_t)circ * a.bars_stride + bt[B_BAR];
+  double* gang = a.grad_angles ? a.grad_angles + (int64_t)circ * a.n_params : nullptr;
+  cd* gmat = a.grad_matrices ? reinterpret_cast<cd*>(a.grad_matrices) + (int64_t)circ * a.n_matrix_elems : nullptr;
+  const int mb = bt[B_MBEG], me = bt[B_MEND];
+  wg_copy(bar, B, ne);
+  wg_identity(Rp, D, diagf);
+  for (int j = mb; j < me; ++j) {
+    const int m = a.member_list[j];
+    const int* mt = a.member_table + m * MW;
+    const double th = member_theta(mt, a.member_values[m], angles);
+    const bool want_theta = mt[M_PARAM] >= 0 && gang != nullptr;
+    const bool want_mat = mt[M_KIND] == JQ_M_MATRIX && (mt[M_FLAGS] & JQ_MF_GRAD) && gmat != nullptr;
+    __syncthreads();
+    if (want_theta || want_mat) {
+      wg_identity(Lm, D, diagf);
+      for (int q = j + 1; q < me; ++q) {
+        const int mq = a.member_list[q];
+        const int* mtq = a.member_table + mq * MW;
+        __syncthreads();
+        wg_eff(mtq, member_theta(mtq, a.member_values[mq], angles), xm, D, offs, diagf, false, E);
+        __syncthreads();
+        wg_mul(E, Lm, T, D, diagf, false);
+        __syncthreads();
+        wg_copy(T, Lm, ne);
+      }
+      __syncthreads();
+      wg_mul(Lm, B, T, D, diagf, true);  // L^T bar
+      __syncthreads();
+      wg_mul_bt(T, Rp, Lm, D, diagf);    // W = L^T bar R^T (into Lm)
+      __syncthreads();
+      if (want_theta) {
+        wg_eff(mt, th, xm, D, offs, diagf, true, E);
+        __syncthreads();
+        double g = 0;
+        for (int e = threadIdx.x; e < ne; e += WG_NT) g += Lm[e].x * E[e].x - Lm[e].y * E[e].y;
+        g = wave_sum_d(g);
+        if (threadIdx.x == 0) atomicAdd(&gang[mt[M_PARAM]], g);
+      }
+      if (want_mat) {
+        const int dim = 1 << mt[M_KB];  // explicit matrices are unstructured: D == dim, offs == 0
+        for (int e = threadIdx.x; e < ne; e += WG_NT) {
+          const int r = diagf ? e : e / D, c = diagf ? e : e % D;
+          cd* dst = &gmat[mt[M_MATRIX] + r * dim + c];
+          atomicAdd(&dst->x, Lm[e].x);
+          atomicAdd(&dst->y, Lm[e].y);
+        }
+      }
+      __syncthreads();
+    }
+    wg_eff(mt, th, xm, D, offs, diagf, false, E);
+    __syncthreads();
+    wg_mul(E, Rp, T, D, diagf, false);
+    __syncthreads();
+    wg_copy(T, Rp, ne);
+  }
+  (void)red;
 }
 
 // ---------------------------------------------------------------------------
@@ -570,8 +739,15 @@ hipError_t launch_prep(int dtype, const PrepArgs& a, hipStream_t st) {
     else                                                                              \
       hipLaunchKernelGGL((prep_kernel<double, C>), grid, dim3(tpb), 0, st, a);        \
   }
-  JQ_PREP(0) JQ_PREP(1) JQ_PREP(2) JQ_PREP(3) JQ_PREP(4)
+  JQ_PREP(0) JQ_PREP(1) JQ_PREP(2)
 #undef JQ_PREP
+  if (a.n_wg > 0) {
+    const dim3 g((unsigned)a.n_wg, (unsigned)a.batch);
+    if (dtype == 0)
+      hipLaunchKernelGGL(prep_wg_kernel<float>, g, dim3(WG_NT), 0, st, a);
+    else
+      hipLaunchKernelGGL(prep_wg_kernel<double>, g, dim3(WG_NT), 0, st, a);
+  }
   return hipGetLastError();
 }
 
@@ -617,8 +793,7 @@ hipError_t launch_chain(const ChainArgs& a, hipStream_t st) {
   if (a.class_mask & 1) hipLaunchKernelGGL(chain_kernel<0>, grid, dim3(64), 0, st, a);
   if (a.class_mask & 2) hipLaunchKernelGGL(chain_kernel<1>, grid, dim3(64), 0, st, a);
   if (a.class_mask & 4) hipLaunchKernelGGL(chain_kernel<2>, grid, dim3(64), 0, st, a);
-  if (a.class_mask & 8) hipLaunchKernelGGL(chain_kernel<3>, grid, dim3(64), 0, st, a);
-  if (a.class_mask & 16) hipLaunchKernelGGL(chain_kernel<4>, grid, dim3(64), 0, st, a);
+  if (a.n_wg > 0) hipLaunchKernelGGL(chain_wg_kernel, dim3((unsigned)a.n_wg, (unsigned)a.batch), dim3(WG_NT), 0, st, a);
   return hipGetLastError();
 }
 

@@ -32,6 +32,7 @@ constexpr int HDR_WORDS = 16;
 // pass flags
 constexpr int PF_FIRST = 1;  // first pass of the circuit (reads psi0 or |0>)
 constexpr int PF_LAST = 2;   // last pass (computes expvals; seeds lambda in the backward)
+constexpr int PF_WIDE = 4;   // a pass of one wide block (AK_WIDE), run by the wide-block kernels
 
 // apply kinds
 constexpr int AK_NOP = 0;
@@ -41,6 +42,16 @@ constexpr int AK_MCX = 3;     // X on reg bit ra under controls
 constexpr int AK_MCSWAP = 4;  // swap reg bits ra, rb under controls
 constexpr int AK_DIAG = 5;    // diagonal over k wires located anywhere
 constexpr int AK_DENSEK = 6;  // dense 2^k on reg bits k-1..0 (wire j <-> reg bit k-1-j)
+// A block applied in a pass of its own, by the wide-block kernels: dense on
+// up to 5 wires (controls folded into the 2^k matrix) or diagonal on up to
+// 6, with its cotangent reduced over the whole vector; its backward reads
+// the checkpointed input state, so it is never inverted.  Record words:
+// G_BLOCK, G_MAT, G_SLOT, G_K, G_RA = 1 for diagonal storage, and
+// G_DW0 / G_DW0+1 = global bit of wire j in byte j (wire 0 = most
+// significant bit of the matrix index).
+constexpr int AK_WIDE = 7;
+constexpr int WIDE_MAX_DENSE = 5;
+constexpr int WIDE_MAX_DIAG = 6;
 
 // gate record words
 constexpr int GW = 16;

@@ -1898,7 +1898,7 @@ __device__ __forceinline__ void backward_tile(const SvArgs& a, int pass_idx, int
     canonical_layout<RB, LNT>(tbits, L);
     const C2<R>* ps = reinterpret_cast<const C2<R>*>(a.src) + (int64_t)circ * a.src_stride;
     load_canon<R, RB>(pr, pi, ps, a.vec_len, L, base, a.n_wires >= T);
-    if (flags & PF_LAST) {
+    if ((flags & PF_LAST) && !a.lam_given) {
       seed_lambda<R, RB>(pr, pi, lr, li, a, obs, circ, L, base);
     } else {
       const C2<R>* ls = reinterpret_cast<const C2<R>*>(a.lam) + (int64_t)circ * a.lam_stride;
@@ -2074,7 +2074,7 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
   double* slot_buf = reinterpret_cast<double*>(obs + 1);
   cint* pass = P + P[HDR_WORDS + a.pass];
   cint* tbits = pass + 4;
-  const bool seed = (pass[1] & PF_LAST) != 0;
+  const bool seed = (pass[1] & PF_LAST) != 0 && !a.lam_given;
   const int nslots = pass[2];
   const int slot_base = pass[3];
   // work items w = circuit * ntiles + tile over all rows of the launch, so

@@ -65,6 +65,9 @@ struct SvArgs {
   int n_groups;
   int step;
   int dir;
+  // backward: the caller has put the cotangent of the final vector in
+  // `lam`; the last pass reads it instead of seeding it from observables
+  int lam_given;
 };
 
 // Observable table: global bit and diagonal (d0, d1) of each observable wire.

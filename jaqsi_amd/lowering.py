@@ -97,8 +97,11 @@ class _Builder:
         self.members.append(nat.Member(kind, flags, m.nctrl, word, param, mat, 0.0))
 
     def add_op(self, wires: Sequence[int], members: Sequence[Member], structure: int, controls: int, perm: tuple,
-               extra_flags: int = 0, superop_matrix: Optional[np.ndarray] = None) -> None:
+               extra_flags: int = 0, superop_matrix: Optional[np.ndarray] = None, cond: float = 1.0) -> None:
+        if len(wires) > nat.MAX_WIRES:
+            raise NotImplementedError(f"gates on more than {nat.MAX_WIRES} wires are not supported")
         op = nat.Op()
+        op.cond = cond
         op.nwires = len(wires)
         for j, w in enumerate(wires):
             op.wires[j] = int(w)
@@ -114,7 +117,11 @@ class _Builder:
                 self.add_member(m, extra_flags)
         op.member_end = len(self.members)
         self.ops.append(op)
-        self.key.append(("op", tuple(wires), structure, controls, perm))
+        # the planner routes ill-conditioned blocks (product of their channels'
+        # condition numbers above JQ_COND_MAX) to checkpointed passes, so a
+        # channel's condition number is part of the structure
+        kc = float(cond) if structure & nat.JQ_S_NONUNITARY else 1.0
+        self.key.append(("op", tuple(wires), structure, controls, perm, kc))
 
     def result(self) -> Lowered:
         return Lowered(tuple(self.key), self.n_wires, self.ops, self.members, self.angles, self.mats, self.n_mat,
@@ -145,7 +152,7 @@ def lower_mixed(tape: Sequence[Operation], n_qubits: int) -> Lowered:
         bra = [w + n for w in ket]
         if isinstance(op, KrausChannel):
             sup = op.superoperator()
-            b.add_op(ket + bra, [], nat.JQ_S_NONUNITARY, 0, (), superop_matrix=sup)
+            b.add_op(ket + bra, [], nat.JQ_S_NONUNITARY, 0, (), superop_matrix=sup, cond=channel_cond(sup))
             continue
         flags, controls, perm = _structure_flags(op)
         if len(ket) == 1 and not perm:
@@ -155,6 +162,18 @@ def lower_mixed(tape: Sequence[Operation], n_qubits: int) -> Lowered:
             b.add_op(ket, op.lower(), flags, controls, perm)
             b.add_op(bra, op.lower(), flags, controls, perm, extra_flags=nat.JQ_MF_CONJ)
     return b.result()
+
+
+def channel_cond(sup) -> float:
+    """2-norm condition number of a channel superoperator (inf if singular
+    or if it is a tensor that requires grad: its value may change between
+    calls, so it is never inverted)."""
+    if isinstance(sup, torch.Tensor):
+        if sup.requires_grad:
+            return float("inf")
+        sup = sup.detach().cpu().numpy()
+    s = np.linalg.svd(np.asarray(sup, dtype=np.complex128), compute_uv=False)
+    return float(s[0] / s[-1]) if s[-1] > 0 else float("inf")
 
 
 def angle_tensor(values: list, batch: Optional[int]) -> torch.Tensor:

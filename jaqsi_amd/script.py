@@ -98,26 +98,23 @@ class Script:
             with recording() as tape:
                 self.f(*a, **kwargs)
             box["tape"] = tape
-            vals = []
-            for op in tape:
-                for m in op.lower():
-                    if m.kind in ("rot", "phase"):
-                        vals.append(torch.as_tensor(m.param, dtype=torch.float64) if not isinstance(m.param, torch.Tensor) else m.param.to(torch.float64))
-            mats = []
-            for op in tape:
-                for m in op.lower():
-                    if m.kind == "matrix":
-                        mats.append(torch.as_tensor(np.asarray(m.matrix)) if not isinstance(m.matrix, torch.Tensor) else m.matrix)
-            return (torch.stack(vals) if vals else torch.zeros(0, dtype=torch.float64),
-                    tuple(mt.to(torch.complex128).reshape(-1) for mt in mats))
+            n_q = self._n_qubits or simulation.infer_n_qubits(tape, obs)
+            # the values come from the same lowering the simulation uses, so
+            # their order matches the plan for statevectors and density
+            # matrices alike (channels contribute their superoperators)
+            dens = simulation.has_noise(tape)
+            low = lowering.lower_mixed(tape, n_q) if dens else lowering.lower_pure(tape, n_q)
+            vals = [v.to(torch.float64) if isinstance(v, torch.Tensor) else torch.as_tensor(float(v), dtype=torch.float64)
+                    for v in low.angle_values]
+            mats = [(v if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v))).to(torch.complex128).reshape(-1)
+                    for _, v in low.matrix_values]
+            return (torch.stack(vals) if vals else torch.zeros(0, dtype=torch.float64), tuple(mats))
 
         angles, mats = torch.func.vmap(single, in_dims=tuple(in_axes))(*args)
         tape = box["tape"]
         n_qubits = self._n_qubits or simulation.infer_n_qubits(tape, obs)
         use_density = simulation.has_noise(tape)
         mat = torch.cat([m.reshape(B, -1) for m in mats], dim=1) if len(mats) else None
-        if use_density and mat is not None:
-            raise NotImplementedError("batched density-matrix circuits with explicit matrices are not supported yet")
         return simulation.simulate_and_measure(
             tape, n_qubits, type, obs, use_density, shots=shots, key=key, initial_state=initial_state,
             batch=B, values=(angles.reshape(B, -1), mat),

@@ -12,6 +12,7 @@ GPU or no built library is present, the calls raise.
 
 from __future__ import annotations
 
+import ctypes
 import threading
 from typing import List, Optional, Sequence, Tuple
 
@@ -116,176 +117,6 @@ def diag_observables(obs: Sequence[Operation]) -> Optional[List[Tuple[int, float
     return out
 
 
-# ---------------------------------------------------------------------------
-# general observables: Pauli decomposition + basis-rotated parity groups
-# ---------------------------------------------------------------------------
-_PAULI_1Q = {
-    "I": np.eye(2, dtype=complex),
-    "X": np.array([[0, 1], [1, 0]], dtype=complex),
-    "Y": np.array([[0, -1j], [1j, 0]], dtype=complex),
-    "Z": np.array([[1, 0], [0, -1]], dtype=complex),
-}
-
-
-def pauli_terms(m: np.ndarray, wires: Sequence[int], tol: float = 1e-12) -> List[Tuple[dict, float]]:
-    """Decompose a Hermitian matrix on `wires` (first wire = most significant
-    index bit) into real-weighted Pauli words: [({wire: letter}, coeff)]."""
-    import itertools
-
-    k = len(wires)
-    m = np.asarray(m, dtype=complex)
-    if m.shape != (2**k, 2**k):
-        raise ValueError("observable matrix does not match its wires")
-    if not np.allclose(m, m.conj().T, atol=1e-10):
-        raise NotImplementedError("gradients of non-Hermitian observables are not supported")
-    out = []
-    for letters in itertools.product("IXYZ", repeat=k):
-        P = np.array([[1.0 + 0j]])
-        for c in letters:
-            P = np.kron(P, _PAULI_1Q[c])
-        c = np.real(np.trace(P @ m)) / 2**k
-        if abs(c) > tol:
-            out.append(({w: l for w, l in zip(wires, letters) if l != "I"}, float(c)))
-    return out
-
-
-def pauli_groups(obs: Sequence[Operation]):
-    """Group the Pauli terms of all observables qubit-wise by measurement
-    basis.  Returns (groups, n_obs) with groups = [(basis {wire: X|Y|Z},
-    [(obs index, coeff, wire mask)])]; a term with no wires (identity) gets
-    mask 0 and contributes coeff * 1."""
-    groups: List[Tuple[dict, list]] = []
-    for oi, ob in enumerate(obs):
-        if len(ob.wires) > 6:
-            raise NotImplementedError("observables on more than 6 wires are not supported by the gradient path")
-        for word, c in pauli_terms(_obs_matrix_f64(ob), ob.wires):
-            mask = 0
-            for w in word:
-                mask |= 1 << w
-            for basis, terms in groups:
-                if all(basis.get(w, l) == l for w, l in word.items()):
-                    basis.update(word)
-                    terms.append((oi, c, mask))
-                    break
-            else:
-                groups.append((dict(word), [(oi, c, mask)]))
-    return groups
-
-
-def _basis_rotations(basis: dict) -> List[Operation]:
-    """Gates mapping each wire's measurement basis onto Z: X -> H, Y -> H S^dagger."""
-    from jaqsi_amd import gateset as G
-
-    ops: List[Operation] = []
-    for w, l in sorted(basis.items()):
-        if l == "X":
-            ops.append(G.H(wires=w, record=False))
-        elif l == "Y":
-            ops.append(G.RZ(-np.pi / 2, wires=w, record=False))  # S^dagger up to a global phase
-            ops.append(G.H(wires=w, record=False))
-    return ops
-
-
-class _ParityRunner:
-    """One adjoint runner per basis group; observables combine the groups'
-    Pauli-word expectation values with their coefficients."""
-
-    def __init__(self, tape, n_qubits, groups, n_obs, batch, psi0_batched, dtype, base_low, mats):
-        self.groups = []
-        P0, E0 = len(base_low.angle_values), base_low.n_matrix_elems
-        for basis, terms in groups:
-            low = lowering.lower_pure(list(tape) + _basis_rotations(basis), n_qubits)
-            plan = get_plan(low, dtype)
-            # the rotations only append: angles [0, P0) and matrix elements
-            # [0, E0) are the circuit's own, the rest are the rotation constants
-            if len(low.angle_values) < P0 or low.n_matrix_elems < E0:
-                raise RuntimeError("basis rotations changed the circuit's own parameter layout")
-            extra_angles = torch.tensor([float(a) for a in low.angle_values[P0:]], dtype=torch.float64)
-            extra_m = lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, None)
-            extra_m = extra_m.reshape(-1)[E0:] if extra_m is not None else None
-            if mats is not None:
-                gm = torch.cat([mats.reshape(batch, -1).detach().to(torch.complex128).cpu(),
-                                extra_m.reshape(1, -1).expand(batch, -1)], dim=1) if extra_m is not None and extra_m.numel() else mats.reshape(batch, -1)
-            else:
-                gm = extra_m.reshape(1, -1).expand(batch, -1) if extra_m is not None and extra_m.numel() else None
-            words = np.zeros((len(terms), 2), dtype=np.int32)
-            for t, (_, _, mask) in enumerate(terms):
-                words[t, 0] = np.int32(np.uint32(mask & 0xFFFFFFFF).astype(np.int32))
-                words[t, 1] = np.int32(np.uint32(mask >> 32).astype(np.int32))
-            d = np.tile(np.array([1.0, -1.0]), len(terms))
-            W = np.zeros((n_obs, len(terms)))  # expval_o = sum_t W[o, t] <P_t>
-            for t, (oi, c, _) in enumerate(terms):
-                W[oi, t] = c
-            self.groups.append((plan, gm, words, d, torch.tensor(W, dtype=torch.float64), len(terms), extra_angles))
-        self.P0 = P0
-        self.batch = batch
-        self.psi0_batched = psi0_batched
-        self.dtype = dtype
-        self.n_qubits = n_qubits
-        self.n_obs = n_obs
-
-    def _mats(self, mats, dev):
-        if mats is None:
-            return None
-        return mats.reshape(self.batch, -1).to(dev, torch.complex128).contiguous()
-
-    def _angles(self, angles, extra):
-        if angles.shape[1] != self.P0:
-            raise ValueError("angle vector does not match the circuit")
-        if extra.numel() == 0:
-            return angles
-        return torch.cat([angles, extra.to(angles.device).reshape(1, -1).expand(self.batch, -1)], dim=1).contiguous()
-
-    def forward(self, angles, psi0):
-        dev = angles.device
-        out = torch.zeros((self.batch, self.n_obs), dtype=torch.float64, device=dev)
-        for plan, mats, words, d, W, nt, extra in self.groups:
-            angles_g = self._angles(angles, extra)
-            state = torch.empty((self.batch, 1 << self.n_qubits), dtype=_cdtype_torch(self.dtype), device=dev)
-            ev = torch.empty((self.batch, nt), dtype=torch.float64, device=dev)
-            m_g = self._mats(mats, dev)
-            nat.check(nat.lib().jq_sv_forward(
-                plan.handle, self.batch, nat.ptr(angles_g), nat.ptr(m_g), nat.ptr(psi0),
-                int(self.psi0_batched), nat.ptr(state), nat.JQ_OBS_SV_PARITY, nt, nat.ptr(words), nat.ptr(d),
-                nat.ptr(ev), _stream()))
-            out += ev @ W.to(dev).T
-        return out
-
-    def adjoint(self, angles, psi0, ct):
-        dev = angles.device
-        grad = torch.zeros_like(angles)
-        for plan, mats, words, d, W, nt, extra in self.groups:
-            angles_g = self._angles(angles, extra)
-            m_g = self._mats(mats, dev)
-            ctp = (ct @ W.to(dev)).contiguous()  # cotangent of each Pauli word
-            state = torch.empty((self.batch, 1 << self.n_qubits), dtype=_cdtype_torch(self.dtype), device=dev)
-            ga = torch.empty_like(angles_g)
-            nat.check(nat.lib().jq_sv_adjoint(
-                plan.handle, self.batch, nat.ptr(angles_g), nat.ptr(m_g), nat.ptr(psi0),
-                int(self.psi0_batched), nat.ptr(state), nat.JQ_OBS_SV_PARITY, nt, nat.ptr(words), nat.ptr(d),
-                nat.ptr(ctp), None, nat.ptr(ga), None, None, _stream()))
-            grad += ga[:, : self.P0]
-        return grad
-
-
-class _ParityExpval(torch.autograd.Function):
-    """Expectation values of general observables, adjoint backward per basis group."""
-
-    @staticmethod
-    def forward(ctx, angles, psi0, runner):
-        ctx.runner = runner
-        ctx.save_for_backward(angles, psi0 if psi0 is not None else torch.empty(0))
-        ctx.has_psi0 = psi0 is not None
-        return runner.forward(angles, psi0)
-
-    @staticmethod
-    def backward(ctx, g):
-        angles, psi0 = ctx.saved_tensors
-        psi0 = psi0 if ctx.has_psi0 else None
-        ga = ctx.runner.adjoint(angles, psi0, g.to(torch.float64).contiguous())
-        return ga, None, None
-
-
 def _obs_arrays(diags):
     wires = np.array([w for w, _, _ in diags], dtype=np.int32)
     d = np.array([[a, b] for _, a, b in diags], dtype=np.float64).reshape(-1)
@@ -353,6 +184,37 @@ class _Runner:
             _stream()))
         return ev, state
 
+    def forward_ckpt(self, angles, mats, psi0):
+        """Forward keeping every pass output in the plan: (expvals or None,
+        final state, token for backward())."""
+        dev = angles.device
+        state = torch.empty((self.batch, self.vec_len()), dtype=_cdtype_torch(self.dtype), device=dev)
+        nobs = len(self.diags)
+        ev = torch.empty((self.batch, nobs), dtype=torch.float64, device=dev) if nobs else None
+        tok = ctypes.c_int64(-1)
+        nat.check(nat.lib().jq_sv_forward_ckpt(
+            self.plan.handle, self.batch, nat.ptr(angles), nat.ptr(mats), nat.ptr(psi0), int(self.psi0_batched),
+            nat.ptr(state), self.obs_kind if nobs else nat.JQ_OBS_NONE, nobs, nat.ptr(self.wires), nat.ptr(self.d),
+            nat.ptr(ev), ctypes.byref(tok), _stream()))
+        return ev, state, tok.value
+
+    def backward(self, token, angles, mats, psi0, state, ct=None, lam=None, want_psi0=False):
+        """Adjoint sweep over the checkpoints of forward_ckpt (the library
+        re-runs the forward if the plan has been used since).  The cotangent
+        is ``ct`` (per observable) or ``lam``, the JAX-convention cotangent of
+        the final vector (overwritten)."""
+        dev = angles.device
+        ga = torch.empty_like(angles)
+        gm = torch.empty_like(mats) if mats is not None else None
+        gp = torch.empty((self.batch, self.vec_len()), dtype=_cdtype_torch(self.dtype), device=dev) if want_psi0 else None
+        nobs = len(self.diags) if lam is None else 0
+        nat.check(nat.lib().jq_sv_backward(
+            self.plan.handle, token, self.batch, nat.ptr(angles), nat.ptr(mats), nat.ptr(psi0),
+            int(self.psi0_batched), nat.ptr(state), self.obs_kind if nobs else nat.JQ_OBS_NONE, nobs,
+            nat.ptr(self.wires), nat.ptr(self.d), nat.ptr(ct), nat.ptr(lam), nat.ptr(ga), nat.ptr(gm), nat.ptr(gp),
+            _stream()))
+        return ga, gm, gp
+
     def adjoint(self, angles, mats, psi0, ct, want_psi0):
         dev = angles.device
         state = torch.empty((self.batch, self.vec_len()), dtype=_cdtype_torch(self.dtype), device=dev)
@@ -367,35 +229,199 @@ class _Runner:
         return ev, ga, gm, gp
 
 
+def _save(ctx, angles, mats, psi0, extra=()):
+    ctx.save_for_backward(angles, mats if mats is not None else torch.empty(0),
+                          psi0 if psi0 is not None else torch.empty(0), *extra)
+    ctx.has_mats = mats is not None
+    ctx.has_psi0 = psi0 is not None
+
+
+def _saved(ctx):
+    t = ctx.saved_tensors
+    angles, mats, psi0 = t[0], t[1], t[2]
+    return angles, (mats if ctx.has_mats else None), (psi0 if ctx.has_psi0 else None), t[3:]
+
+
+def _to_torch_grads(r, psi0, gm, gp):
+    """JAX-convention cotangents -> PyTorch's convention (conjugate)."""
+    gm = gm.conj() if gm is not None else None
+    if gp is not None:
+        gp = gp.conj()
+        if not r.psi0_batched:
+            gp = gp.sum(0)
+        gp = gp.reshape(psi0.shape)
+    return gm, gp
+
+
 class _AdjointExpval(torch.autograd.Function):
-    """Expectation values whose backward is the native adjoint sweep."""
+    """Expectation values whose backward is the native adjoint sweep over the
+    checkpoints the forward left in the plan (the fwd/bwd residual pair of
+    _adjoint_expval's custom_vjp, simulation.py:586-654)."""
 
     @staticmethod
     def forward(ctx, angles, mats, psi0, runner):
-        ev, _ = runner.forward(angles, mats, psi0)
+        ev, state, token = runner.forward_ckpt(angles, mats, psi0)
         ctx.runner = runner
-        ctx.save_for_backward(angles, mats if mats is not None else torch.empty(0), psi0 if psi0 is not None else torch.empty(0))
-        ctx.has_mats = mats is not None
-        ctx.has_psi0 = psi0 is not None
+        ctx.token = token
+        _save(ctx, angles, mats, psi0, (state,))
         return ev
 
     @staticmethod
     def backward(ctx, g):
-        angles, mats, psi0 = ctx.saved_tensors
-        mats = mats if ctx.has_mats else None
-        psi0 = psi0 if ctx.has_psi0 else None
+        angles, mats, psi0, (state,) = _saved(ctx)
         r = ctx.runner
         ct = g.to(torch.float64).contiguous()
         want_psi0 = psi0 is not None and ctx.needs_input_grad[2]
-        _, ga, gm, gp = r.adjoint(angles, mats, psi0, ct, want_psi0)
-        # JAX-convention cotangents -> PyTorch's convention (conjugate)
-        gm = gm.conj() if gm is not None else None
-        if gp is not None:
-            gp = gp.conj()
-            if not r.psi0_batched:
-                gp = gp.sum(0)
-            gp = gp.reshape(psi0.shape)
+        ga, gm, gp = r.backward(ctx.token, angles, mats, psi0, state, ct=ct, want_psi0=want_psi0)
+        gm, gp = _to_torch_grads(r, psi0, gm, gp)
         return ga, gm, gp, None
+
+
+class _FinalState(torch.autograd.Function):
+    """The final vector (state, or rho as a 2n-wire vector), differentiable:
+    the backward runs the native adjoint seeded with the cotangent of the
+    vector -- the reverse-mode autodiff the reference applies to probs /
+    state / density outputs (simulation.py:658-732)."""
+
+    @staticmethod
+    def forward(ctx, angles, mats, psi0, runner):
+        _, state, token = runner.forward_ckpt(angles, mats, psi0)
+        ctx.runner = runner
+        ctx.token = token
+        _save(ctx, angles, mats, psi0, (state,))
+        return state
+
+    @staticmethod
+    def backward(ctx, g):
+        angles, mats, psi0, (state,) = _saved(ctx)
+        r = ctx.runner
+        # PyTorch's gradient of a real loss w.r.t. a complex vector is the
+        # conjugate of JAX's cotangent
+        # (a physical conjugate: g.conj() is a lazy view over unconjugated memory)
+        lam = torch.conj_physical(g).to(state.dtype).contiguous()
+        want_psi0 = psi0 is not None and ctx.needs_input_grad[2]
+        ga, gm, gp = r.backward(ctx.token, angles, mats, psi0, state, lam=lam, want_psi0=want_psi0)
+        gm, gp = _to_torch_grads(r, psi0, gm, gp)
+        return ga, gm, gp, None
+
+
+class _SvProbs(torch.autograd.Function):
+    """|psi_i|^2 (native), backward 2 g_i psi_i (simulation.py:748-750)."""
+
+    @staticmethod
+    def forward(ctx, state, n_qubits, dtype):
+        ctx.save_for_backward(state)
+        return _probs(state, n_qubits, False, state.shape[0], dtype)
+
+    @staticmethod
+    def backward(ctx, g):
+        (state,) = ctx.saved_tensors
+        return 2 * g.to(state.real.dtype) * state, None, None
+
+
+class _DmProbs(torch.autograd.Function):
+    """Re diag(rho) (native), backward into the diagonal (simulation.py:838-840)."""
+
+    @staticmethod
+    def forward(ctx, rho_vec, n_qubits, dtype):
+        ctx.dim = 1 << n_qubits
+        return _probs(rho_vec, n_qubits, True, rho_vec.shape[0], dtype)
+
+    @staticmethod
+    def backward(ctx, g):
+        dim = ctx.dim
+        out = torch.zeros((g.shape[0], dim * dim), dtype=torch.complex128 if g.dtype == torch.float64 else torch.complex64,
+                          device=g.device)
+        out[:, torch.arange(dim, device=g.device) * (dim + 1)] = g.to(out.dtype)
+        return out, None, None
+
+
+class _SvOuter(torch.autograd.Function):
+    """rho = psi psi^dagger (native), backward (G + G^dagger) psi (simulation.py:752-754)."""
+
+    @staticmethod
+    def forward(ctx, state, n_qubits, dtype):
+        ctx.save_for_backward(state)
+        dim = 1 << n_qubits
+        B = state.shape[0]
+        rho = torch.empty((B, dim, dim), dtype=state.dtype, device=state.device)
+        nat.check(nat.lib().jq_sv_outer(n_qubits, B, dtype, nat.ptr(state), nat.ptr(rho), _stream()))
+        return rho
+
+    @staticmethod
+    def backward(ctx, G):
+        (state,) = ctx.saved_tensors
+        G = G.to(state.dtype)
+        H = G + G.conj().transpose(-1, -2)
+        return torch.matmul(H, state.unsqueeze(-1)).squeeze(-1), None, None
+
+
+def _lift_apply(m, wires, vec, n):
+    """O (on `wires`) applied to each row of vec (B, 2^n)."""
+    from jaqsi_amd.operations import _contract
+
+    B = vec.shape[0]
+    t = vec.reshape((B,) + (2,) * n)
+    o = torch.as_tensor(m, dtype=vec.dtype, device=vec.device)
+    return _contract(t, o, [w + 1 for w in wires]).reshape(B, -1)
+
+
+class _SvExpvalMatrix(torch.autograd.Function):
+    """Re <psi|O|psi> for one k-wire observable (native jq_expval_matrix),
+    backward 2 g O psi (simulation.py:806-809)."""
+
+    @staticmethod
+    def forward(ctx, state, n_qubits, dtype, wires, m):
+        B = state.shape[0]
+        col = torch.zeros(B, dtype=torch.float64, device=state.device)
+        w = np.array(wires, dtype=np.int32)
+        mt = torch.as_tensor(m).to(state.device)
+        nat.check(nat.lib().jq_expval_matrix(n_qubits, B, dtype, 0, nat.ptr(state), len(wires), nat.ptr(w),
+                                             nat.ptr(mt), nat.ptr(col), _stream()))
+        ctx.save_for_backward(state)
+        ctx.n, ctx.wires, ctx.m = n_qubits, list(wires), m
+        return col
+
+    @staticmethod
+    def backward(ctx, g):
+        (state,) = ctx.saved_tensors
+        # d Re<psi|O|psi> / d psi (PyTorch's convention) = (O + O^dagger) psi
+        m = np.asarray(ctx.m, dtype=complex)
+        h = m + m.conj().T
+        return g.to(state.real.dtype)[:, None] * _lift_apply(h, ctx.wires, state, ctx.n), None, None, None, None
+
+
+class _DmExpvalMatrix(torch.autograd.Function):
+    """Re Tr(O rho) for one k-wire observable (native jq_expval_matrix),
+    backward g * O lifted to every wire (simulation.py:850-852)."""
+
+    @staticmethod
+    def forward(ctx, rho_vec, n_qubits, dtype, wires, m):
+        B = rho_vec.shape[0]
+        col = torch.zeros(B, dtype=torch.float64, device=rho_vec.device)
+        w = np.array(wires, dtype=np.int32)
+        mt = torch.as_tensor(m).to(rho_vec.device)
+        nat.check(nat.lib().jq_expval_matrix(n_qubits, B, dtype, 1, nat.ptr(rho_vec), len(wires), nat.ptr(w),
+                                             nat.ptr(mt), nat.ptr(col), _stream()))
+        ctx.n = n_qubits
+        ctx.wires = list(wires)
+        ctx.m = m
+        ctx.cdt = rho_vec.dtype
+        return col
+
+    @staticmethod
+    def backward(ctx, g):
+        from jaqsi_amd.operations import _contract
+
+        n, dim = ctx.n, 1 << ctx.n
+        eye = torch.eye(dim, dtype=torch.complex128, device=g.device).reshape((2,) * (2 * n))
+        # O lifted to all wires: O applied to the ket axes of the identity
+        lifted = _contract(eye, torch.as_tensor(ctx.m, dtype=torch.complex128, device=g.device), ctx.wires)
+        # d Re Tr(O rho) / d rho (PyTorch's convention): conj(O^T) = O^dagger-transpose;
+        # for a Hermitian O this is O itself
+        grad = lifted.reshape(dim, dim).transpose(0, 1).conj()
+        out = g.to(torch.complex128)[:, None] * grad.reshape(1, -1)
+        return out.to(ctx.cdt), None, None, None, None
 
 
 # ---------------------------------------------------------------------------
@@ -413,7 +439,8 @@ def _prepare(tape, n_qubits, use_density, batch, initial_state, dtype, values=No
         angles = angles.reshape(1, -1)
         mats = mats.reshape(1, -1) if mats is not None else None
     angles = angles.to(dev, torch.float64).contiguous()
-    mats = mats.to(dev, torch.complex128).contiguous() if mats is not None else None
+    # (resolve_conj: native code reads memory, not torch's lazy conjugate views)
+    mats = mats.to(dev, torch.complex128).resolve_conj().contiguous() if mats is not None else None
     psi0 = None
     psi0_batched = False
     if initial_state is not None:
@@ -425,7 +452,7 @@ def _prepare(tape, n_qubits, use_density, batch, initial_state, dtype, values=No
             psi0 = (p[:, :, None] * p.conj()[:, None, :]).reshape(p.shape[0], -1)
         if psi0.dim() == 2 and psi0.shape[0] > 1:
             psi0_batched = True
-        psi0 = psi0.reshape(psi0.shape[0] if psi0_batched else 1, -1).contiguous()
+        psi0 = psi0.reshape(psi0.shape[0] if psi0_batched else 1, -1).resolve_conj().contiguous()
     plan = get_plan(low, dtype)
     return low, plan, angles, mats, psi0, psi0_batched
 
@@ -468,26 +495,22 @@ def simulate_and_measure(
     def unbatch(x):
         return x if batch is not None else x[0]
 
+    needs_grad = torch.is_grad_enabled() and any(t is not None and t.requires_grad for t in (angles, mats, psi0))
     if type == "expval" and shots is None:
         diags = diag_observables(obs)
         if diags is not None and len(diags) > 0:
             runner = _Runner(plan, B, use_density, diags, n_qubits, psi0_batched, dtype)
             ev = _AdjointExpval.apply(angles, mats, psi0, runner)
             return unbatch(ev.to(rdt))
-        # general observables: gradients through Pauli-word parity groups
-        if any(t is not None and t.requires_grad for t in (angles, mats, psi0)) and torch.is_grad_enabled():
-            if use_density:
-                raise NotImplementedError(
-                    "gradients of non-diagonal observables of density matrices are not supported yet"
-                )
-            if (mats is not None and mats.requires_grad) or (psi0 is not None and psi0.requires_grad):
-                raise NotImplementedError(
-                    "gradients of non-diagonal observables w.r.t. explicit matrices or initial states are not supported yet"
-                )
-            groups = pauli_groups(obs)
-            runner = _ParityRunner(tape, n_qubits, groups, len(obs), B, psi0_batched, dtype, low, mats)
-            ev = _ParityExpval.apply(angles, psi0, runner)
-            return unbatch(ev.to(rdt))
+        if needs_grad:
+            # general observables: one forward, then one adjoint sweep seeded
+            # with the cotangent of the final vector
+            runner = _Runner(plan, B, use_density, [], n_qubits, psi0_batched, dtype)
+            vec = _FinalState.apply(angles, mats, psi0, runner)
+            fn = _DmExpvalMatrix if use_density else _SvExpvalMatrix
+            cols = [fn.apply(vec, n_qubits, dtype, list(ob.wires), _obs_matrix_f64(ob)) for ob in obs]
+            out = torch.stack(cols, dim=-1) if cols else torch.zeros((B, 0), dtype=torch.float64, device=angles.device)
+            return unbatch(out.to(rdt))
         state = _forward_state(plan, B, angles, mats, psi0, psi0_batched, dtype)
         out = torch.zeros((B, len(obs)), dtype=torch.float64, device=angles.device)
         for i, ob in enumerate(obs):
@@ -499,10 +522,18 @@ def simulate_and_measure(
             out[:, i] = col
         return unbatch(out.to(rdt))
 
-    if any(t is not None and t.requires_grad for t in (angles, mats, psi0)) and torch.is_grad_enabled():
-        raise NotImplementedError(
-            f"gradients of {type!r} outputs are not supported by the GPU path; use type='expval'"
-        )
+    if needs_grad and shots is None:
+        runner = _Runner(plan, B, use_density, [], n_qubits, psi0_batched, dtype)
+        vec = _FinalState.apply(angles, mats, psi0, runner)
+        dim = 1 << n_qubits
+        if type == "state":
+            return unbatch(vec)
+        if type == "probs":
+            fn = _DmProbs if use_density else _SvProbs
+            return unbatch(fn.apply(vec, n_qubits, dtype))
+        if use_density:
+            return unbatch(vec.reshape(B, dim, dim))
+        return unbatch(_SvOuter.apply(vec, n_qubits, dtype))
     state = _forward_state(plan, B, angles, mats, psi0, psi0_batched, dtype)
     dim = 1 << n_qubits
     if shots is not None and type in ("probs", "expval"):

@@ -16,7 +16,8 @@ import numpy as np
 
 HDR = 16
 GW, BW, MW = 16, 12, 8
-AK_NOP, AK_MC1, AK_MC2, AK_MCX, AK_MCSWAP, AK_DIAG, AK_DENSEK = range(7)
+AK_NOP, AK_MC1, AK_MC2, AK_MCX, AK_MCSWAP, AK_DIAG, AK_DENSEK, AK_WIDE = range(8)
+PF_WIDE = 4
 PAULI = [
     np.eye(2, dtype=complex),
     np.array([[0, 1], [1, 0]], dtype=complex),
@@ -213,6 +214,43 @@ class Program:
             x[idx[q]] = y[q]
         return x
 
+    def _wide_index(self, g):
+        """[rest, c] amplitude indices of a wide block: rest over the
+        indices with every wire bit zero, c over the 2^k wire values (wire j
+        = bit k-1-j of c)."""
+        n, k = self.n, int(g[11])
+        gb = [(int(g[12 + j // 4]) >> (8 * (j % 4))) & 0xFF for j in range(k)]
+        i = np.arange(1 << n, dtype=np.int64)
+        mask = sum(1 << b for b in gb)
+        rest = i[(i & mask) == 0]
+        cbits = np.zeros(1 << k, dtype=np.int64)
+        for c in range(1 << k):
+            for j in range(k):
+                if (c >> (k - 1 - j)) & 1:
+                    cbits[c] |= 1 << gb[j]
+        return rest[:, None] | cbits[None, :]
+
+    def _wide_forward(self, psi, g, pool):
+        idx = self._wide_index(g)
+        M = pool[int(g[1])][0]
+        X = psi[idx]
+        out = psi.copy()
+        out[idx] = X * M[None, :] if int(g[2]) else X @ M.T
+        return out
+
+    def _wide_backward(self, psi_in, lam, g, pool, partial, p):
+        idx = self._wide_index(g)
+        M = pool[int(g[1])][0]
+        X, L = psi_in[idx], lam[idx]
+        if int(g[10]) >= 0:
+            vals = (L * X).sum(0) if int(g[2]) else (L.T @ X).reshape(-1)
+            flat = np.stack([vals.real, vals.imag], axis=-1).reshape(-1)
+            key = (p, int(g[10]))
+            partial[key] = partial.get(key, 0) + flat
+        out = lam.copy()
+        out[idx] = L * M[None, :] if int(g[2]) else L @ M
+        return out
+
     def _pool(self, angles, xm):
         pool = {}
         for b in range(len(self.bt)):
@@ -232,6 +270,9 @@ class Program:
             psi[:] = psi0
         pool = self._pool(angles, xm)
         for p, flags, nslots, sbase, tbits, subs in self.passes():
+            if flags & PF_WIDE:
+                psi = self._wide_forward(psi, subs[0][2][0], pool)
+                continue
             out = psi.copy()
             for t, base in self.tiles(tbits):
                 gl = self.gidx(tbits, base)
@@ -256,7 +297,12 @@ class Program:
         else:
             psi[:] = psi0
         ck = []
+        psi_init = psi.copy()
         for p, flags, nslots, sbase, tbits, subs in self.passes():
+            if flags & PF_WIDE:
+                psi = self._wide_forward(psi, subs[0][2][0], pool)
+                ck.append(psi)
+                continue
             out = psi.copy()
             for t, base in self.tiles(tbits):
                 gl = self.gidx(tbits, base)
@@ -273,6 +319,9 @@ class Program:
         lam = lam_seed_fn(final)
         partial = {}
         for p, flags, nslots, sbase, tbits, subs in reversed(list(self.passes(backward=True))):
+            if flags & PF_WIDE:  # reads the pass's checkpointed input
+                lam = self._wide_backward(ck[p - 1] if p > 0 else psi_init, lam, subs[0][2][0], pool, partial, p)
+                continue
             src = ck[p]
             newlam = lam.copy()
             for t, base in self.tiles(tbits):

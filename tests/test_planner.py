@@ -168,12 +168,16 @@ def test_planner_errors_map_to_exceptions():
     import jaqsi_amd as J
     from jaqsi_amd import _native as nat, lowering
 
-    tape = _record(lambda: J.PauliRot(0.3, "XYZX", wires=[0, 1, 2, 3]))
-    low = lowering.lower_pure(tape, 4)
+    # dense blocks run on up to 5 wires (a pass of their own beyond the tile's 3)
+    tape = _record(lambda: J.PauliRot(0.3, "XYZXYZ", wires=[0, 1, 2, 3, 4, 5]))
+    low = lowering.lower_pure(tape, 6)
     with pytest.raises(NotImplementedError):
-        nat.Plan(4, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0)
+        nat.Plan(6, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0)
     with pytest.raises(ValueError):
         nat.Plan(2, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0)
+    tape = _record(lambda: J.PauliRot(0.3, "XYZXYZX", wires=list(range(7))))
+    with pytest.raises(NotImplementedError):
+        lowering.lower_pure(tape, 7)
 
 
 def test_deep_circuit_passes_stay_within_the_slot_budget():
@@ -197,3 +201,125 @@ def test_deep_circuit_passes_stay_within_the_slot_budget():
     _, _, grads, _ = prog.adjoint(angles, lambda f: 2 * zs * np.conj(f), xm)
     _, og, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
     np.testing.assert_allclose(grads, sv.flat_grads(ops, og), atol=1e-10)
+
+
+def _wide_circuit(p):
+    """Gates the tile passes cannot hold with a gradient: a dense 3-wire
+    rotation, a diagonal 3-wire rotation and a dense 4-wire rotation, between
+    single-qubit layers (5 wires)."""
+    import jaqsi_amd as J
+
+    for q in range(5):
+        J.RY(p[q], wires=q)
+    J.PauliRot(p[5], "XYZ", wires=[0, 2, 4])
+    J.PauliRot(p[6], "ZZZ", wires=[3, 1, 2])
+    J.CX(wires=[1, 3])
+    J.PauliRot(p[7], "XXYZ", wires=[4, 0, 1, 3])
+    for q in range(5):
+        J.RX(p[8 + q], wires=q)
+
+
+def _wide_ops(p):
+    O = sv.Op
+    ops = [O("RY", (q,), (p[q],)) for q in range(5)]
+    ops += [O("PauliRot", (0, 2, 4), (p[5],), "XYZ"), O("PauliRot", (3, 1, 2), (p[6],), "ZZZ"),
+            O("CX", (1, 3)), O("PauliRot", (4, 0, 1, 3), (p[7],), "XXYZ")]
+    ops += [O("RX", (q,), (p[8 + q],)) for q in range(5)]
+    return ops
+
+
+@pytest.mark.parametrize("dtype", [0, 1])
+def test_wide_blocks_run_in_passes_of_their_own(dtype):
+    """Dense blocks of 3+ target wires with a gradient, 4-wire dense blocks and
+    3-wire diagonals with a gradient are scheduled as wide-block passes
+    (program flag PF_WIDE); their adjoint reads the checkpointed input state.
+    Forward and gradients match the oracle."""
+    from plan_interp import PF_WIDE, Program
+
+    p = np.random.default_rng(31).uniform(0, 2 * np.pi, 13)
+    plan, angles, xm = _plan(_record(lambda: _wide_circuit(p)), 5, dtype)
+    prog = Program(plan)
+    flags = [f for _, f, _, _, _, _ in prog.passes()]
+    assert sum(1 for f in flags if f & PF_WIDE) == 3
+    ops = _wide_ops(p)
+    np.testing.assert_allclose(prog.forward(angles, xm), sv.simulate(ops, 5), atol=1e-12)
+    zs = sv.z_signs(5).sum(0)
+    _, _, grads, _ = prog.adjoint(angles, lambda f: 2 * zs * np.conj(f), xm)
+    _, og, _, _ = sv.adjoint(ops, 5, sv.z_obs(5))
+    np.testing.assert_allclose(grads, sv.flat_grads(ops, og), atol=1e-12)
+
+
+def _noisy_strong(t):
+    """Channels whose superoperators are singular or badly conditioned
+    between rotations on 2 qubits."""
+    import jaqsi_amd as J
+    from jaqsi_amd import noise
+
+    J.RX(t[0], wires=0)
+    noise.BitFlip(0.5, wires=0)
+    J.RY(t[1], wires=0)
+    J.RY(t[2], wires=1)
+    noise.DepolarizingChannel(0.7, wires=1)
+    J.CX(wires=[0, 1])
+    noise.AmplitudeDamping(0.999, wires=0)
+    J.RX(t[3], wires=0)
+    noise.PhaseFlip(0.5, wires=1)
+    J.RY(t[4], wires=1)
+    noise.AmplitudeDamping(0.99, wires=1)
+    J.RZ(t[5], wires=1)
+    J.RX(t[6], wires=1)
+    J.CX(wires=[1, 0])
+    noise.PhaseDamping(0.3, wires=0)
+    J.RY(t[7], wires=0)
+
+
+def _noisy_strong_items(t):
+    from oracle import dm
+
+    O = sv.Op
+    return [O("RX", (0,), (t[0],)), ("kraus", dm.kraus("BitFlip", 0.5), (0,)), O("RY", (0,), (t[1],)),
+            O("RY", (1,), (t[2],)), ("kraus", dm.kraus("DepolarizingChannel", 0.7), (1,)), O("CX", (0, 1)),
+            ("kraus", dm.kraus("AmplitudeDamping", 0.999), (0,)), O("RX", (0,), (t[3],)),
+            ("kraus", dm.kraus("PhaseFlip", 0.5), (1,)), O("RY", (1,), (t[4],)),
+            ("kraus", dm.kraus("AmplitudeDamping", 0.99), (1,)), O("RZ", (1,), (t[5],)), O("RX", (1,), (t[6],)),
+            O("CX", (1, 0)), ("kraus", dm.kraus("PhaseDamping", 0.3), (0,)), O("RY", (0,), (t[7],))]
+
+
+@pytest.mark.parametrize("dtype", [0, 1])
+def test_singular_and_ill_conditioned_channels_are_never_inverted(dtype):
+    """BitFlip(0.5) and PhaseFlip(0.5) have singular superoperators,
+    DepolarizingChannel(0.7) and AmplitudeDamping(0.99 / 0.999) condition
+    numbers of 15 to 1000: the blocks holding them run as wide-block passes
+    (no inverse in the pool), PhaseDamping(0.3) (condition 1.2) stays in a
+    tile pass and is un-applied by its inverse.  The adjoint gradient of
+    <Z_0> + <Z_1> matches the exact parameter-shift rule of the density oracle."""
+    from jaqsi_amd import _native as nat, lowering
+    from oracle import dm
+    from plan_interp import PF_WIDE, Program
+
+    n = 2
+    t = np.random.default_rng(5).uniform(0, 2 * np.pi, 8)
+    low = lowering.lower_mixed(_record(lambda: _noisy_strong(t)), n)
+    conds = sorted(op.cond for op in low.ops if op.structure & nat.JQ_S_NONUNITARY)
+    assert conds[0] < 1.5 and np.isinf(conds[-1])
+    plan = nat.Plan(2 * n, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, dtype)
+    angles = np.array([float(a) for a in low.angle_values])
+    xm = lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, None).numpy()
+    prog = Program(plan)
+    flags = [f for _, f, _, _, _, _ in prog.passes()]
+    assert sum(1 for f in flags if f & PF_WIDE) >= 4
+    bt = prog.bt
+    wide_blocks = {int(g[1]) for _, f, _, _, _, subs in prog.passes() if f & PF_WIDE for g in subs[0][2]}
+    assert all(bt[b][6] < 0 for b in wide_blocks)  # no inverse for a wide block
+    assert any(bt[b][6] >= 0 for b in range(len(bt)) if b not in wide_blocks)  # PhaseDamping(0.3)
+    rho = prog.forward(angles, xm).reshape(4, 4)
+    items = _noisy_strong_items(t)
+    np.testing.assert_allclose(rho, dm.simulate(items, n), atol=1e-12)
+    dim = 1 << n
+    zsum = sv.z_signs(n).sum(0)
+    seed = np.zeros(dim * dim, dtype=complex)
+    seed[np.arange(dim) * (dim + 1)] = zsum
+    _, _, grads, _ = prog.adjoint(angles, lambda f: seed, xm)
+    v, g_ref = dm.value_and_shift_grad(items, n, [(sv.PZ, (0,)), (sv.PZ, (1,))], np.array([1.0, 1.0]))
+    assert np.all(np.isfinite(grads))
+    np.testing.assert_allclose(grads, g_ref, atol=1e-12)
