@@ -288,6 +288,35 @@ typedef struct jq_pulse_desc {
 int jq_pulse_propagate(const jq_pulse_desc* desc, const double* params, const double* target,
                        double* unitaries, double* infidelity, double* grad, void* stream);
 
+/* Batched propagators of time-dependent Hamiltonians, in FP64:
+ *   U_g = exp(-i A_{g,M-1}) ... exp(-i A_{g,1}) exp(-i A_{g,0}),
+ *   A_{g,m} = sum_k coef[g][m][k] H_{g,k}   (H_k Hermitian, d = 2 or 4)
+ * Every fixed-step scheme of the reference's Evolution is such a product
+ * with host-side coefficients: Magnus-2 (one factor per step, coef = h f_k
+ * at the midpoint), CFM4 (two factors per step, Gauss-point combinations)
+ * and the single-term closed form exp(-i F H) (one factor, F the pulse
+ * area).  Replaces Evolution._build_magnus_evolve_solver /
+ * _build_closed_form_evolve_solver (evolution.py:194-263, 353-404) and the
+ * batched resolve_pending solve (evolution.py:730-773).
+ *   coef       [G][M][K] float64 (device)
+ *   hmats      complex128, [G][K][d][d] if hmats_batched else [K][d][d] (device)
+ *   unitaries  [G][d][d] complex128 (device, output)
+ * jq_evolve_vjp: the gradient of a real loss L w.r.t. coef, given
+ *   cot        [G][d][d] complex128: dL/dRe U + i dL/dIm U (PyTorch's convention)
+ *   grad_coef  [G][M][K] float64 (output). */
+typedef struct jq_evolve_desc {
+  int32_t n_gates;
+  int32_t dim;
+  int32_t n_terms;
+  int32_t n_factors;
+  int32_t hmats_batched;
+  int32_t reserved;
+} jq_evolve_desc;
+
+int jq_evolve(const jq_evolve_desc* desc, const double* coef, const double* hmats, double* unitaries, void* stream);
+int jq_evolve_vjp(const jq_evolve_desc* desc, const double* coef, const double* hmats, const double* cot,
+                  double* grad_coef, void* stream);
+
 /* Kernel timing for measurement: when enabled, each run of consecutive
  * tile-pass launches of one kind (the forward passes of a call, its backward
  * passes) is bracketed by a pair of HIP events on its stream; jq_timing_read

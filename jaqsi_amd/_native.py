@@ -102,6 +102,17 @@ class PulseDesc(C.Structure):
 
 JQ_ENV_DRAG_GAUSSIAN = 1
 
+
+class EvolveDesc(C.Structure):
+    _fields_ = [
+        ("n_gates", C.c_int32),
+        ("dim", C.c_int32),
+        ("n_terms", C.c_int32),
+        ("n_factors", C.c_int32),
+        ("hmats_batched", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
 # plan-specialised kernels (jq_plan_jit)
 JQ_JIT_OFF, JQ_JIT_ASYNC, JQ_JIT_SYNC, JQ_JIT_QUERY = 0, 1, 2, -1
 JQ_JIT_NONE, JQ_JIT_COMPILING, JQ_JIT_COMPILED, JQ_JIT_LOADED, JQ_JIT_FAILED = 0, 1, 2, 3, -1
@@ -128,6 +139,8 @@ EXPORTS = [
     "jq_dm_diag",
     "jq_expval_matrix",
     "jq_pulse_propagate",
+    "jq_evolve",
+    "jq_evolve_vjp",
     "jq_timing_enable",
     "jq_timing_read",
     "jq_last_error",
@@ -170,6 +183,8 @@ def lib():
                                      vp, vp, vp, vp, vp, vp]
         L.jq_sv_adjoint_groups.argtypes = [C.c_int32, C.POINTER(Group), C.c_int32, C.c_int32, vp, vp, vp]
         L.jq_pulse_propagate.argtypes = [C.POINTER(PulseDesc), vp, vp, vp, vp, vp, vp]
+        L.jq_evolve.argtypes = [C.POINTER(EvolveDesc), vp, vp, vp, vp]
+        L.jq_evolve_vjp.argtypes = [C.POINTER(EvolveDesc), vp, vp, vp, vp, vp]
         L.jq_sv_probs.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp]
         L.jq_sv_outer.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp]
         L.jq_dm_diag.argtypes = [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp]

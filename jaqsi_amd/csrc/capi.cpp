@@ -22,6 +22,7 @@
 #include "program.h"
 #include "sv_kernels.h"
 #include "wide.h"
+#include "evolve.h"
 
 namespace {
 thread_local std::string g_err;
@@ -1090,6 +1091,49 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
                       nullptr, st);
     if (rc) return rc;
   }
+  return JQ_OK;
+}
+
+namespace {
+int evolve_args(const jq_evolve_desc* d, const double* coef, const double* hmats, jq::EvolveArgs& a) {
+  if (!d || !coef || !hmats) return fail(JQ_ERR_ARG, "null argument to jq_evolve");
+  if (d->dim != 2 && d->dim != 4)
+    return fail(JQ_ERR_UNSUPPORTED, "propagators of one- and two-qubit Hamiltonians (dim 2 or 4) are supported");
+  if (d->n_gates < 0 || d->n_terms < 1 || d->n_factors < 1) return fail(JQ_ERR_ARG, "bad jq_evolve_desc");
+  if (d->n_gates > 2147483647 / 64) return fail(JQ_ERR_ARG, "too many gates in one call");
+  a.n_gates = d->n_gates;
+  a.dim = d->dim;
+  a.n_terms = d->n_terms;
+  a.n_factors = d->n_factors;
+  a.hmats_batched = d->hmats_batched != 0;
+  a.coef = coef;
+  a.hmats = hmats;
+  a.unitaries = nullptr;
+  a.cot = nullptr;
+  a.grad_coef = nullptr;
+  return JQ_OK;
+}
+}  // namespace
+
+int jq_evolve(const jq_evolve_desc* d, const double* coef, const double* hmats, double* unitaries, void* stream) {
+  jq::EvolveArgs a;
+  int rc = evolve_args(d, coef, hmats, a);
+  if (rc) return rc;
+  if (!unitaries) return fail(JQ_ERR_ARG, "null unitaries");
+  a.unitaries = unitaries;
+  HIP_TRY(jq::launch_evolve(a, 0, (hipStream_t)stream));
+  return JQ_OK;
+}
+
+int jq_evolve_vjp(const jq_evolve_desc* d, const double* coef, const double* hmats, const double* cot,
+                  double* grad_coef, void* stream) {
+  jq::EvolveArgs a;
+  int rc = evolve_args(d, coef, hmats, a);
+  if (rc) return rc;
+  if (!cot || !grad_coef) return fail(JQ_ERR_ARG, "null cotangent or gradient");
+  a.cot = cot;
+  a.grad_coef = grad_coef;
+  HIP_TRY(jq::launch_evolve(a, 1, (hipStream_t)stream));
   return JQ_OK;
 }
 

@@ -1,25 +1,27 @@
 """``Gates.<Name>(...)``: the recording entry point (mirrors jaqsi/gates.py).
 
-Routes a call to ``UnitaryGates`` (gates.py:88-176), filtering keyword
-arguments the same way.  Pulse-level gates (``pulse=True``, the reference's
-``PulseGates``, jaqsi/pulses.py:1370-1665) are not lowered by this backend:
-such calls raise ``NotImplementedError``.  Batched pulse propagators of the
-two-qubit drive Hamiltonian are available from ``jaqsi_amd.pulse``.
+Routes a call to ``UnitaryGates`` or, with ``pulse=True``, to
+``PulseGates`` (gates.py:88-176), filtering keyword arguments the same way
+and checking ``pulse_params`` as the reference does.  Under
+``pulse_manager_context`` the model's pulse parameters scale each gate's
+calibrated parameters element-wise (gates.py:142-176).
 """
 
 from __future__ import annotations
 
 import logging
+import numbers
 from contextlib import contextmanager
 from typing import Callable, List, Union
 
+import numpy as np
+import torch
+
 from jaqsi_amd.gateset import Barrier as BarrierOp
+from jaqsi_amd.pulses import PulseEnvelope, PulseGates, PulseInformation, PulseParamManager, PulseParams  # noqa: F401
 from jaqsi_amd.unitary import UnitaryGates
 
 log = logging.getLogger(__name__)
-
-_PULSE_GATES = ("pulse-level gates (pulse=True) are not lowered by this backend; batched two-qubit pulse "
-                "propagators with gradients are available from jaqsi_amd.pulse")
 
 
 class _GateHandler:
@@ -58,14 +60,45 @@ class Gates(metaclass=GatesMeta):
         pulse = kwargs.pop("pulse", False)
         if not isinstance(pulse, bool):
             raise TypeError(f"'pulse' must be a bool, got {type(pulse).__name__}.")
-        if pulse:
-            raise NotImplementedError(_PULSE_GATES)
         allowed = ["w", "wires", "phi", "theta", "omega", "noise_params", "random_key"]
-        backend = UnitaryGates
+        if pulse:
+            backend = PulseGates
+            allowed += ["pulse_params"]
+        else:
+            backend = UnitaryGates
         extra = kwargs.keys() - allowed
         if extra:
             log.debug(f"Unsupported keyword arguments: {list(extra)}")
         kwargs = {k: v for k, v in kwargs.items() if k in allowed}
+        pulse_params = kwargs.get("pulse_params")
+        mgr = getattr(cls, "_pulse_mgr", None)
+        if pulse_params is not None:
+            if isinstance(pulse_params, PulseParams):
+                kwargs["pulse_params"] = pulse_params.params
+                flat = list(np.ravel(np.asarray(pulse_params.params)))
+            elif isinstance(pulse_params, torch.Tensor):
+                flat = list(pulse_params.reshape(-1))
+            elif isinstance(pulse_params, (list, tuple, np.ndarray)):
+                flat = list(np.ravel(np.asarray(pulse_params, dtype=object)))
+            else:
+                raise TypeError(f"Unsupported pulse_params type: {type(pulse_params)}")
+            if not all(isinstance(x, (numbers.Real, torch.Tensor)) for x in flat):
+                raise TypeError(f"All elements in pulse_params must be int or float, got {pulse_params}, "
+                                f"type {type(pulse_params)}. ")
+            if not isinstance(mgr, PulseParamManager):
+                n_params = PulseInformation.gate_by_name(gate_name).size
+                if len(flat) != n_params:
+                    raise ValueError(f"Gate '{gate_name}' expects {n_params} pulse parameters, got {len(flat)}")
+            if isinstance(pulse_params, (list, tuple)) and any(isinstance(x, torch.Tensor) for x in flat):
+                kwargs["pulse_params"] = torch.stack([torch.as_tensor(x, dtype=torch.float64) for x in flat])
+        if pulse and isinstance(mgr, PulseParamManager):
+            n_params = PulseInformation.gate_by_name(gate_name).size
+            scalers = mgr.get(n_params)
+            base = PulseInformation.gate_by_name(gate_name).params
+            if isinstance(scalers, torch.Tensor):
+                kwargs["pulse_params"] = torch.as_tensor(base, dtype=torch.float64, device=scalers.device) * scalers
+            else:
+                kwargs["pulse_params"] = np.asarray(base) * np.asarray(scalers)
         gate = getattr(backend, gate_name, None)
         if gate is None:
             raise AttributeError(f"'{backend.__name__}' object has no attribute '{gate_name}'")
@@ -74,8 +107,13 @@ class Gates(metaclass=GatesMeta):
     @classmethod
     @contextmanager
     def pulse_manager_context(cls, pulse_params):
-        raise NotImplementedError(_PULSE_GATES)
-        yield  # pragma: no cover  (keeps this a generator for contextmanager)
+        """Scale every pulse gate recorded inside by consecutive slices of
+        ``pulse_params`` (gates.py:178-186)."""
+        cls._pulse_mgr = PulseParamManager(pulse_params)
+        try:
+            yield
+        finally:
+            cls._pulse_mgr = None
 
     @classmethod
     def parse_gates(cls, gates, set_of_gates=None):

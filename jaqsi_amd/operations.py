@@ -284,3 +284,18 @@ class Hermitian(Operation):
 
         H = self._np_matrix if self._np_matrix is not None else self.matrix
         return ParametrizedHamiltonian(terms=[(coeff_fn, H, self.wires)])
+
+    def evolve(self, name: Optional[str] = None, **odeint_kwargs) -> Callable:
+        """Gate factory ``(t, wires=0) -> Operation`` of exp(-i t H)
+        (operations.py:704-719)."""
+        from jaqsi_amd.evolution import Evolution
+
+        return Evolution.evolve(self, name=name, **odeint_kwargs)
+
+
+def __getattr__(name):  # ParametrizedHamiltonian lives with its solver
+    if name == "ParametrizedHamiltonian":
+        from jaqsi_amd.evolution import ParametrizedHamiltonian
+
+        return ParametrizedHamiltonian
+    raise AttributeError(name)

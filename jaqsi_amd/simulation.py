@@ -483,9 +483,9 @@ def simulate_and_measure(
             "Measurement type 'state' is not defined for mixed (noisy) circuits. Use 'density' instead."
         )
     tape = [op for op in tape if not isinstance(op, Barrier)]
-    if any(getattr(op, "_pending", False) for op in tape):
-        from jaqsi_amd.evolution import resolve_pending
+    from jaqsi_amd.evolution import PendingEvolution, resolve_pending
 
+    if any(isinstance(op, PendingEvolution) for op in tape):
         resolve_pending(tape)
     dtype = _dtype_code()
     low, plan, angles, mats, psi0, psi0_batched = _prepare(tape, n_qubits, use_density, batch, initial_state, dtype, values)

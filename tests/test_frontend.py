@@ -21,17 +21,44 @@ def test_recording_and_wire_validation():
         J.Gates.RX(0.1, wires=0, pulse="yes")
 
 
-def test_pulse_gates_raise_and_propagator_validates():
-    """Pulse-level circuit gates are not lowered: a clear NotImplementedError.
-    The batched propagator API checks its arguments before any GPU work."""
+def test_pulse_gates_record_pending_evolutions_and_validate():
+    """Gates.<G>(pulse=True) records pending evolutions of the pulse
+    decomposition (gates.py:88-176, pulses.py:1370-1665); pulse_params are
+    checked for type and length; the pulse manager scales the calibrated
+    parameters; the batched propagator API checks its arguments."""
     import jaqsi_amd as J
     from jaqsi_amd import pulse
+    from jaqsi_amd.evolution import PendingEvolution
+    from jaqsi_amd.pulses import PulseInformation
+    from jaqsi_amd.tape import recording
 
-    with pytest.raises(NotImplementedError, match="jaqsi_amd.pulse"):
+    with recording() as tape:
         J.Gates.RX(0.1, wires=0, pulse=True)
-    with pytest.raises(NotImplementedError):
-        with J.Gates.pulse_manager_context([1.0]):
-            pass
+        J.Gates.CX(wires=[0, 1], pulse=True)
+    assert all(isinstance(op, PendingEvolution) for op in tape)
+    # RX; CX = H(target) CZ H(target), H = RZ(pi) RY(pi/2) + phase correction
+    assert [op.name for op in tape] == ["RX", "RZ", "RY", "H", "CZ", "RZ", "RY", "H"]
+    assert PulseInformation.H.size == 4 and PulseInformation.CX.size == 9
+    assert [len(x) for x in PulseInformation.CX.split_params(np.arange(9.0))] == [4, 1, 4]
+    with pytest.raises(ValueError, match="expects 3 pulse parameters"):
+        J.Gates.RX(0.1, wires=0, pulse=True, pulse_params=[1.0, 2.0])
+    with pytest.raises(TypeError):
+        J.Gates.RX(0.1, wires=0, pulse=True, pulse_params="abc")
+    with pytest.raises(TypeError, match="'pulse' must be a bool"):
+        J.Gates.RX(0.1, wires=0, pulse=1)
+    with recording() as tape:
+        with J.Gates.pulse_manager_context(np.array([2.0, 1.0, 1.0, 0.5])):
+            J.Gates.RX(0.3, wires=0, pulse=True)
+            J.Gates.RZ(0.3, wires=0, pulse=True)
+    base = np.asarray(PulseInformation.RX.params)
+    np.testing.assert_allclose(np.asarray(tape[0]._params[0])[:3], base * [2.0, 1.0, 1.0])
+    np.testing.assert_allclose(np.asarray(tape[1]._params[0])[0], 0.5 * 0.5)
+    with PulseInformation.preserve_state():
+        PulseInformation.set_envelope("drag", rwa=True)
+        with recording() as tape:
+            J.Gates.RX(0.3, wires=0, pulse=True)
+        assert len(tape[0]._fns) == 2  # DRAG drives the second quadrature
+    assert PulseInformation.get_envelope() == "gaussian"
     p, w = torch.zeros(3, 4, dtype=torch.float64), torch.zeros(3, 2, dtype=torch.float64)
     with pytest.raises(ValueError, match="CUDA"):
         pulse.propagate(p, w, np.eye(4))
@@ -41,6 +68,32 @@ def test_pulse_gates_raise_and_propagator_validates():
         pulse.propagate(p[:, :3], w, np.eye(4))
     with pytest.raises(ValueError, match="4x4"):
         pulse._target_host(np.eye(2))
+
+
+def test_parametrized_hamiltonian_algebra():
+    """coeff * Hermitian builds one term, + concatenates, - negates; terms
+    must share wires and shapes (operations.py:722-863)."""
+    import jaqsi_amd as J
+
+    X = np.array([[0, 1], [1, 0]])
+    Zm = np.array([[1, 0], [0, -1]])
+    f = lambda p, t: p * t  # noqa: E731
+    g = lambda p, t: p  # noqa: E731
+    H = f * J.Hermitian(X, wires=0, record=False) + g * J.Hermitian(Zm, wires=0, record=False)
+    assert H.n_terms == 2 and H.wires == [0]
+    Hn = -H
+    assert Hn.coeff_fns[0](2.0, 3.0) == -6.0
+    assert (H - H).n_terms == 4
+    with pytest.raises(ValueError, match="same wires"):
+        f * J.Hermitian(X, wires=0, record=False) + g * J.Hermitian(X, wires=1, record=False)
+    with pytest.raises(ValueError, match="same shape"):
+        f * J.Hermitian(X, wires=0, record=False) + g * J.Hermitian(np.eye(4), wires=0, record=False)
+    with pytest.raises(TypeError):
+        3.0 * J.Hermitian(X, wires=0, record=False)
+    with pytest.raises(ValueError, match="parameter set"):
+        H.evolve()([1.0], 1.0)
+    with pytest.raises(ValueError, match="Unknown solver"):
+        H.evolve(solver="rk4")
 
 
 def test_noise_params_attach_channels():
