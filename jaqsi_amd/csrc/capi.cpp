@@ -123,8 +123,16 @@ int fail(int code, const std::string& msg) {
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
-  hipError_t ensure(size_t want) {
+  // Grow to `want` bytes.  A captured HIP graph keeps the workspace
+  // addresses it saw, so a workspace never grows while `st` is capturing:
+  // the call fails instead (one eager call at the largest size first).
+  hipError_t ensure(size_t want, hipStream_t st) {
     if (want <= bytes) return hipSuccess;
+    if (st != nullptr) {
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+        return hipErrorStreamCaptureUnsupported;
+    }
     if (p) hipFree(p);
     p = nullptr;
     bytes = 0;
@@ -142,7 +150,7 @@ struct DevBuf {
 template <typename T>
 hipError_t upload(DevBuf& b, const std::vector<T>& v) {
   size_t n = std::max<size_t>(1, v.size()) * sizeof(T);
-  hipError_t e = b.ensure(n);
+  hipError_t e = b.ensure(n, nullptr);
   if (e != hipSuccess) return e;
   if (!v.empty()) return hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
   return hipSuccess;
@@ -640,7 +648,7 @@ int ensure_uploaded(jq_plan* p) {
 }
 
 int fill_obs(jq_plan* p, jq::SvArgs& a, int n_wires, int obs_kind, int n_obs, const int32_t* obs_wires,
-             const double* obs_d, hipStream_t st) {
+             const double* obs_d, hipStream_t st, void* table = nullptr) {
   a.obs_kind = obs_kind;
   a.n_obs = 0;
   a.obs = nullptr;
@@ -667,16 +675,23 @@ int fill_obs(jq_plan* p, jq::SvArgs& a, int n_wires, int obs_kind, int n_obs, co
     t.d[2 * o] = obs_d[2 * o];
     t.d[2 * o + 1] = obs_d[2 * o + 1];
   }
-  HIP_TRY(p->obs.ensure(sizeof(jq::ObsTable)));
-  HIP_TRY(jq::launch_obs_upload(t, p->obs.p, st));
-  a.obs = p->obs.p;
+  if (!table) {
+    HIP_TRY(p->obs.ensure(sizeof(jq::ObsTable), st));
+    table = p->obs.p;
+  }
+  HIP_TRY(jq::launch_obs_upload(t, table, st));
+  a.obs = table;
   a.n_obs = n_obs;
   return JQ_OK;
 }
 
-int run_prep(jq_plan* p, int batch, const double* angles, const void* matrices, hipStream_t st) {
+int run_prep(jq_plan* p, int batch, const double* angles, const void* matrices, hipStream_t st,
+             void* pool = nullptr) {
   const size_t eb = elem_bytes(p->dtype);
-  HIP_TRY(p->pool.ensure(std::max<size_t>(1, (size_t)p->hp.pool * batch) * eb));
+  if (!pool) {
+    HIP_TRY(p->pool.ensure(std::max<size_t>(1, (size_t)p->hp.pool * batch) * eb, st));
+    pool = p->pool.p;
+  }
   jq::PrepArgs pa;
   pa.n_blocks = p->hp.n_dev_blocks;
   pa.batch = batch;
@@ -688,7 +703,7 @@ int run_prep(jq_plan* p, int batch, const double* angles, const void* matrices, 
   pa.n_params = p->n_params;
   pa.matrices = matrices;
   pa.n_matrix_elems = p->n_matrix_elems;
-  pa.pool = p->pool.p;
+  pa.pool = pool;
   pa.pool_stride = p->hp.pool;
   pa.class_mask = p->class_mask;
   pa.wg_blocks = (const int*)p->wg_list.p;
@@ -734,7 +749,7 @@ int finish_grads(jq_plan* p, int batch, const double* angles, const void* matric
   if (grad_matrices && p->n_matrix_elems > 0)
     HIP_TRY(jq::launch_zero(grad_matrices, (size_t)batch * p->n_matrix_elems * 16, st));
   if (nreals > 0 && (grad_angles || grad_matrices)) {
-    HIP_TRY(p->bars.ensure((size_t)batch * nreals * 8));
+    HIP_TRY(p->bars.ensure((size_t)batch * nreals * 8, st));
     jq::ReduceArgs r;
     r.slot_map = (const int*)p->slot_map.p;
     r.n_slots = p->n_slots;
@@ -790,12 +805,12 @@ int run_forward(jq_plan* p, int batch, const double* angles, const void* matrice
   const int ntiles = p->hp.ntiles;
   const int P = p->hp.n_passes;
   if (a.obs_kind != JQ_OBS_NONE) {
-    HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * a.n_obs) * 8));
+    HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * a.n_obs) * 8, st));
     a.ev_part = (double*)p->ev_part.p;
   }
   const size_t vbytes = (size_t)a.vec_len * elem_bytes(p->dtype) * batch;
   ck = ck && P > 1;
-  if (ck) HIP_TRY(p->ckpt.ensure(vbytes * (size_t)(P - 1)));
+  if (ck) HIP_TRY(p->ckpt.ensure(vbytes * (size_t)(P - 1), st));
   const bool jit = jit_ready(p);
   for (int q = 0; q < P; ++q) {
     a.pass = q;
@@ -828,8 +843,8 @@ int run_backward(jq_plan* p, int batch, const double* angles, const void* matric
   const int P = p->hp.n_passes;
   const size_t vbytes = (size_t)a.vec_len * elem_bytes(p->dtype) * batch;
   ck = ck && P > 1;
-  HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * std::max(1, a.n_obs)) * 8));
-  HIP_TRY(p->partials.ensure(std::max<size_t>(1, (size_t)batch * p->hp.partial_reals) * 8));
+  HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * std::max(1, a.n_obs)) * 8, st));
+  HIP_TRY(p->partials.ensure(std::max<size_t>(1, (size_t)batch * p->hp.partial_reals) * 8, st));
   a.grad_part = (double*)p->partials.p;
   a.part_stride = p->hp.partial_reals;
   a.ev_part = (double*)p->ev_part.p;
@@ -840,7 +855,7 @@ int run_backward(jq_plan* p, int batch, const double* angles, const void* matric
   } else if (lam_seed) {
     lam = lam_seed;
   } else if (!lam) {
-    HIP_TRY(p->lam.ensure(vbytes));
+    HIP_TRY(p->lam.ensure(vbytes, st));
     lam = p->lam.p;
   }
   a.lam = lam;
@@ -1019,10 +1034,10 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
     if (rc) return rc;
     const int P = p->hp.n_passes;
     const size_t vbytes = (size_t)a.vec_len * eb * G.batch;
-    HIP_TRY(p->ckpt.ensure(std::max<size_t>(1, vbytes * (size_t)std::max(0, P - 1))));
-    HIP_TRY(p->lam.ensure(vbytes));
-    HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)G.batch * ntiles * std::max(1, n_obs)) * 8));
-    HIP_TRY(p->partials.ensure(std::max<size_t>(1, (size_t)G.batch * p->hp.partial_reals) * 8));
+    HIP_TRY(p->ckpt.ensure(std::max<size_t>(1, vbytes * (size_t)std::max(0, P - 1)), st));
+    HIP_TRY(p->lam.ensure(vbytes, st));
+    HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)G.batch * ntiles * std::max(1, n_obs)) * 8, st));
+    HIP_TRY(p->partials.ensure(std::max<size_t>(1, (size_t)G.batch * p->hp.partial_reals) * 8, st));
     jq::SvGroup& t = table[k];
     std::memset(&t, 0, sizeof(t));
     t.prog_f = (const int*)p->prog.p;
@@ -1045,7 +1060,7 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
     maxP = std::max(maxP, P);
     smem_b = std::max(smem_b, smem_for(p, 2 * max_slots(p), 2));
   }
-  HIP_TRY(p0->groups.ensure(sizeof(jq::SvGroup) * n_groups));
+  HIP_TRY(p0->groups.ensure(sizeof(jq::SvGroup) * n_groups, st));
   HIP_TRY(jq::launch_bytes_upload(table.data(), sizeof(jq::SvGroup) * n_groups, p0->groups.p, st));
   a.groups = (const jq::SvGroup*)p0->groups.p;
   a.n_groups = n_groups;
