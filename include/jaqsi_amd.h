@@ -222,6 +222,21 @@ int jq_sv_backward(jq_plan* plan, int64_t token, int32_t batch, const double* an
                    const int32_t* obs_wires, const double* obs_d, const double* ct, void* lam_seed,
                    double* grad_angles, void* grad_matrices, void* grad_psi0, void* stream);
 
+/* jq_sv_forward without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The
+ * plan's block matrices, expectation partials and observable table live in
+ * a fixed per-device scratch array of JQ_NOALLOC_SCRATCH_BYTES; a call whose
+ * workspace does not fit fails with JQ_ERR_ARG, as does a plan whose
+ * program is not yet resident on the device (run any other entry point on
+ * it once first).  Every call on a device shares that device's array:
+ * calls that overlap on one GPU (other streams, other threads) race.  The
+ * outputs equal jq_sv_forward's. */
+#define JQ_NOALLOC_SCRATCH_BYTES (32u << 20)
+int jq_sv_forward_noalloc(jq_plan* plan, int32_t batch, const double* angles, const void* matrices, const void* psi0,
+                          int32_t psi0_batched, void* state, int32_t obs_kind, int32_t n_obs,
+                          const int32_t* obs_wires, const double* obs_d, double* expvals, void* stream);
+
 /* One group of a ragged batch: `batch` circuits sharing `plan`'s structure. */
 typedef struct jq_group {
   jq_plan* plan;

@@ -30,6 +30,7 @@ JQ_S_DIAGONAL, JQ_S_PERMUTATION, JQ_S_NONUNITARY = 1, 2, 4
 JQ_OBS_NONE, JQ_OBS_SV_DIAG, JQ_OBS_DM_DIAG, JQ_OBS_SV_PARITY = 0, 1, 2, 3
 MAX_WIRES = 6
 JQ_COND_MAX = 4.0
+JQ_NOALLOC_SCRATCH_BYTES = 32 << 20
 
 
 class Member(C.Structure):
@@ -131,6 +132,7 @@ EXPORTS = [
     "jq_plan_jit",
     "jq_sv_forward",
     "jq_sv_adjoint",
+    "jq_sv_forward_noalloc",
     "jq_sv_forward_ckpt",
     "jq_sv_backward",
     "jq_sv_adjoint_groups",
@@ -177,6 +179,8 @@ def lib():
         L.jq_plan_jit.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
         L.jq_sv_forward.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
         L.jq_sv_adjoint.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.jq_sv_forward_noalloc.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp,
+                                            vp, vp]
         L.jq_sv_forward_ckpt.argtypes = [vp, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp, vp,
                                          C.POINTER(C.c_int64), vp]
         L.jq_sv_backward.argtypes = [vp, C.c_int64, C.c_int32, vp, vp, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, vp,

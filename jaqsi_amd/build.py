@@ -21,7 +21,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libjaqsi_amd.so")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["plan.cpp", "capi.cpp", "jit.cpp", "sv_f32.hip", "sv_f64.hip", "prep.hip", "pulse.hip", "wide.hip", "evolve.hip"]
+SOURCES = ["plan.cpp", "capi.cpp", "jit.cpp", "sv_f32.hip", "sv_f64.hip", "prep.hip", "pulse.hip", "wide.hip", "evolve.hip", "noalloc.hip"]
 
 # Headers the run-time compiler needs for plan-specialised kernels (jit.cpp):
 # embedded in the library, so it does not depend on the source tree.

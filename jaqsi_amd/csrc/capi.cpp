@@ -23,6 +23,7 @@
 #include "sv_kernels.h"
 #include "wide.h"
 #include "evolve.h"
+#include "noalloc.h"
 
 namespace {
 thread_local std::string g_err;
@@ -891,6 +892,55 @@ int jq_sv_forward(jq_plan* p, int32_t batch, const double* angles, const void* m
   if (batch > 65535) return fail(JQ_ERR_ARG, "batch above 65535 per call");
   return run_forward(p, batch, angles, matrices, psi0, psi0_batched, state, obs_kind, n_obs, obs_wires, obs_d, expvals,
                      false, (hipStream_t)stream);
+}
+
+int jq_sv_forward_noalloc(jq_plan* p, int32_t batch, const double* angles, const void* matrices, const void* psi0,
+                          int32_t psi0_batched, void* state, int32_t obs_kind, int32_t n_obs,
+                          const int32_t* obs_wires, const double* obs_d, double* expvals, void* stream) {
+  if (!p || !state || batch < 1) return fail(JQ_ERR_ARG, "bad arguments to jq_sv_forward_noalloc");
+  if (batch > 65535) return fail(JQ_ERR_ARG, "batch above 65535 per call");
+  if (!p->uploaded)
+    return fail(JQ_ERR_ARG, "jq_sv_forward_noalloc: the plan's program is not resident on the device yet "
+                            "(run any other entry point on the plan once first)");
+  hipStream_t st = (hipStream_t)stream;
+  // scratch layout: observable table | block matrices | expectation partials
+  const size_t eb = elem_bytes(p->dtype);
+  const int ntiles = p->hp.ntiles;
+  const size_t off_pool = 4096;  // >= sizeof(jq::ObsTable), 256-aligned
+  const size_t pool_bytes = ((std::max<size_t>(1, (size_t)p->hp.pool * batch) * eb + 255) / 256) * 256;
+  const size_t off_ev = off_pool + pool_bytes;
+  const size_t ev_bytes = expvals ? (size_t)batch * ntiles * std::max(1, n_obs) * 8 : 0;
+  static_assert(sizeof(jq::ObsTable) <= 4096, "observable table exceeds its scratch slot");
+  if (off_ev + ev_bytes > (size_t)JQ_NOALLOC_SCRATCH_BYTES)
+    return fail(JQ_ERR_ARG, "jq_sv_forward_noalloc: the call needs " + std::to_string(off_ev + ev_bytes) +
+                                " bytes of scratch, above JQ_NOALLOC_SCRATCH_BYTES (" +
+                                std::to_string((size_t)JQ_NOALLOC_SCRATCH_BYTES) + "); use jq_sv_forward");
+  void* scr = nullptr;
+  HIP_TRY(jq::noalloc_scratch(&scr));
+  char* base = (char*)scr;
+  int rc = run_prep(p, batch, angles, matrices, st, base + off_pool);
+  if (rc) return rc;
+  ++p->generation;  // the plan's own checkpoints no longer match its pool
+  jq::SvArgs a = base_args(p);
+  a.mats = base + off_pool;
+  rc = fill_obs(p, a, p->hp.n_wires, expvals ? obs_kind : JQ_OBS_NONE, n_obs, obs_wires, obs_d, st, base);
+  if (rc) return rc;
+  if (a.obs_kind != JQ_OBS_NONE) a.ev_part = (double*)(base + off_ev);
+  // plan-specialised kernels only when already loaded (loading allocates)
+  const bool jit = p->jit.mode != JQ_JIT_OFF && p->jit.phase.load(std::memory_order_acquire) == JQ_JIT_LOADED;
+  const int P = p->hp.n_passes;
+  for (int q = 0; q < P; ++q) {
+    a.pass = q;
+    a.src = q == 0 ? psi0 : state;
+    a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
+    a.dst = state;
+    a.dst_stride = (int64_t)a.vec_len;
+    const int variant = 0 + (p->pass_full[q] ? 2 : 0);
+    HIP_TRY(launch_pass(jit_fn(p, jit, 0, q), variant, p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st));
+  }
+  if (a.obs_kind != JQ_OBS_NONE && expvals)
+    HIP_TRY(jq::launch_reduce_ev(a.ev_part, expvals, ntiles, a.n_obs, batch, st));
+  return JQ_OK;
 }
 
 int jq_sv_adjoint(jq_plan* p, int32_t batch, const double* angles, const void* matrices, const void* psi0,
