@@ -52,6 +52,30 @@ def specialise(plan):
 HBM_PEAK_GBS = 8000.0
 
 
+def median_runs(fn, runs=3, warmup=1):
+    """The CPU-baseline protocol: `warmup` untimed runs, then the median
+    wall time of `runs` timed ones (seconds) and the individual times."""
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), [round(t, 4) for t in ts]
+
+
+def host_cpu():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def record_hea(theta, n, features=None):
     from jaqsi_amd import Gates
 
@@ -298,10 +322,14 @@ def cpu_baseline_c3(res, budget_s=15.0):
             list(ex.map(lambda b: cpu.hea_grad(res["thetas"][b], res["feats"][b], 1), range(done, hi)))
             done = hi
     dt = time.perf_counter() - t0
+    # 1 thread: the first 32 circuits, one after the other (1 warm-up circuit)
+    one, _ = median_runs(lambda: [cpu.hea_grad(res["thetas"][b], res["feats"][b], 1) for b in range(32)], 1, 0)
     return {
         "value": round(done / dt, 2),
         "unit": "circuits/s",
         "cores": cores,
+        "cpu": host_cpu(),
+        "threads_1": {"value": round(32 / one, 2), "unit": "circuits/s", "sample": "circuits 0..31, sequential"},
         "kind": "port",
         "sample": f"the first {done} circuits of the batch, {cores} at a time (one per core; forward + adjoint "
         "each), C++ restatement of the reference plan",
@@ -477,49 +505,210 @@ def bench_c5(steps, warmup, n_sets=8192, n_steps=2000, seed=1005):
             "unit": "TFLOP/s",
             "frac": round(ach / FP64_PEAK_TFLOPS, 4),
             "traffic": None,
-            "flop_model": "1536 FP64 FLOP per step per set (4x4 complex products: 512 forward + 1024 backward)",
+            "flop_model": "a floor, not executed FLOPs: 1536 FP64 FLOP per step per set, the 4x4 complex products "
+            "(512 forward + 1024 backward) any product scheme needs; the kernel forms the step in closed form (two "
+            "SU(2) blocks with sincos / exp / expm1), so it is bound by transcendental and dependent-FMA latency",
             "kernel_ms": round(kms, 4),
         },
         "sample_infidelity": float(inf[0].cpu()),
     }
 
 
-def cpu_baseline_c5(budget_s=10.0):
-    from oracle import pulse
+def cpu_baseline_c5(n_sample=2048):
+    """The C++/OpenMP port of the reference's Magnus-2 solve (one 4x4 expm per
+    step) with the infidelity and its adjoint gradient w.r.t. (A, sigma,
+    beta, phi): the same work as the GPU leg, on a bounded sample of sets."""
+    from oracle import cpu, pulse
 
     params, omega = pulse.workload(8192, 1005)
-    t0 = time.perf_counter()
-    done = 0
-    while time.perf_counter() - t0 < budget_s and done < 8192:
-        pulse.propagate(params[done], omega[done], pulse.J_C5, pulse.T_C5, pulse.N_C5)
-        done += 1
-    dt = time.perf_counter() - t0
+    V = pulse.target_rx_half_pi()
+    cores = len(os.sched_getaffinity(0))
+    P, W = params[:n_sample], omega[:n_sample]
+    dt, ts = median_runs(lambda: cpu.pulse_grad(P, W, pulse.N_C5, pulse.T_C5, pulse.J_C5, V, cores), 3, 1)
+    one, _ = median_runs(lambda: cpu.pulse_grad(P[:64], W[:64], pulse.N_C5, pulse.T_C5, pulse.J_C5, V, 1), 1, 0)
     return {
-        "value": round(done / dt, 3),
-        "unit": "parameter sets/s (forward propagator only)",
-        "cores": 1,
+        "value": round(n_sample / dt, 2),
+        "unit": "parameter sets/s (propagator + infidelity + 4-parameter gradient)",
+        "cores": cores,
+        "cpu": host_cpu(),
         "kind": "port",
-        "sample": f"the first {done} sets, forward Magnus-2 product only (SciPy expm per step, the reference step); "
-        "no gradient",
-        "seconds": round(dt, 2),
+        "sample": f"the first {n_sample} of the 8192 sets, 2000 Magnus-2 steps each (FP64 expm per step, adjoint "
+        "gradient through the step exponentials' Frechet derivatives); 1 warm-up, median of 3",
+        "seconds": round(dt, 3),
+        "runs_s": ts,
+        "threads_1": {"value": round(64 / one, 2), "unit": "parameter sets/s", "sample": "sets 0..63, 1 thread"},
     }
+
+
+def cpu_baseline_c4(theta):
+    """The C++/OpenMP port of the reference's density-matrix plan (fused
+    superoperator sweeps, CX pairs as gathers), FORWARD ONLY: the reference
+    differentiates it by autodiff over taped rho's, which the port does not
+    restate."""
+    from oracle import cpu
+
+    cores = len(os.sched_getaffinity(0))
+    dt, ts = median_runs(lambda: cpu.dm_forward(theta, 0.01, 0.005, cores), 3, 1)
+    return {
+        "value": round(1.0 / dt, 4),
+        "unit": "forward evaluations/s (no gradient)",
+        "cores": cores,
+        "cpu": host_cpu(),
+        "kind": "port",
+        "sample": "one forward of the full 12-qubit noisy workload (rho 4096^2 complex64, 288 superoperator sweeps); "
+        "1 warm-up, median of 3; forward only",
+        "seconds": round(dt, 3),
+        "runs_s": ts,
+    }
+
+
+def bench_c1(steps=50, warmup=5, n=10, layers=5, seed=1001):
+    """The reference's CPU-runnable case: 10 qubits, 5 layers, 150 angles,
+    10 Z expvals, batch 1; latency of one forward + adjoint gradient (ct =
+    ones), through Script.execute + backward and through the C-ABI."""
+    import torch
+
+    import jaqsi_amd as J
+    from jaqsi_amd import _native as nat
+
+    rng = np.random.default_rng(seed)
+    theta = rng.uniform(0, 2 * np.pi, (layers, n, 3))
+    J.enable_x64(False)
+    th = torch.tensor(theta, dtype=torch.float64, device="cuda", requires_grad=True)
+    obs = [J.PauliZ(wires=q, record=False) for q in range(n)]
+    script = J.Script(lambda t: record_hea_t(t, n), n_qubits=n)
+
+    def script_step():
+        th.grad = None
+        ev = script.execute(type="expval", obs=obs, args=(th,))
+        ev.sum().backward()
+        return ev
+
+    for _ in range(warmup):
+        script_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        script_step()
+    torch.cuda.synchronize()
+    script_ms = (time.perf_counter() - t0) / steps * 1e3
+    plan, mats = build_plan(n, layers)
+    plan.jit(nat.JQ_JIT_OFF)
+    dev = torch.device("cuda", 0)
+    mats = mats.reshape(1, -1).to(dev) if mats is not None else None
+    angles = torch.tensor(theta.reshape(1, -1), dtype=torch.float64, device=dev)
+    ct = torch.ones((1, n), dtype=torch.float64, device=dev)
+    state = torch.empty((1, 1 << n), dtype=torch.complex64, device=dev)
+    ev = torch.empty((1, n), dtype=torch.float64, device=dev)
+    ga = torch.empty_like(angles)
+    wires = np.arange(n, dtype=np.int32)
+    d = np.tile(np.array([1.0, -1.0]), n)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def capi_step():
+        nat.check(nat.lib().jq_sv_adjoint(plan.handle, 1, nat.ptr(angles), nat.ptr(mats), None, 0, nat.ptr(state),
+                                          nat.JQ_OBS_SV_DIAG, n, nat.ptr(wires), nat.ptr(d), nat.ptr(ct), nat.ptr(ev),
+                                          nat.ptr(ga), None, None, stream))
+
+    for _ in range(warmup):
+        capi_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        capi_step()
+    torch.cuda.synchronize()
+    capi_ms = (time.perf_counter() - t0) / steps * 1e3
+    out = {
+        "metric": "forward + adjoint latency (10 qubits, 5 layers, 150 angles, batch 1)",
+        "value": round(script_ms, 4),
+        "unit": "ms per gradient (Script.execute + backward)",
+        "higher_is_better": False,
+        "capi_ms": round(capi_ms, 4),
+        "steps": steps,
+        "dtype": "c64",
+        "config": {"workload": "statevector 10 qubits, HEA 5 layers (RX,RY,RZ per wire + CX ring), 150 angles "
+                   "uniform[0,2pi) seed 1001, 10 Z expvals, gradient of their sum; batch 1"},
+    }
+    from oracle import cpu
+
+    cores = len(os.sched_getaffinity(0))
+    one, ts1 = median_runs(lambda: cpu.hea_grad(theta, None, 1), runs=5, warmup=1)
+    allc, tsa = median_runs(lambda: cpu.hea_grad(theta, None, cores), runs=5, warmup=1)
+    out["cpu_baseline"] = {
+        "value": round(one * 1e3, 4),
+        "unit": "ms per gradient, 1 thread",
+        "cores": 1,
+        "cpu": host_cpu(),
+        "kind": "port",
+        "sample": "the full C1 workload: C++ restatement of the reference plan, forward + adjoint; 1 warm-up, "
+        "median of 5",
+        "all_cores": {"value": round(allc * 1e3, 4), "unit": "ms per gradient", "cores": cores, "runs_s": tsa},
+        "runs_s": ts1,
+    }
+    return out
+
+
+def record_hea_t(t, n):
+    from jaqsi_amd import Gates
+
+    for layer in range(t.shape[0]):
+        for q in range(n):
+            Gates.RX(t[layer, q, 0], wires=q)
+            Gates.RY(t[layer, q, 1], wires=q)
+            Gates.RZ(t[layer, q, 2], wires=q)
+        for q in range(n):
+            Gates.CX(wires=[q, (q + 1) % n])
+
+
+def script_path_c2(theta, steps=10, warmup=2):
+    """The headline step as a user runs it: Script.execute(...) on the
+    resident angles, then .sum().backward() (the backward consumes the
+    checkpoints the forward left in the plan)."""
+    import torch
+
+    import jaqsi_amd as J
+    from jaqsi_amd import _native as nat
+    from jaqsi_amd import simulation
+
+    n = theta.shape[1]
+    J.enable_x64(False)
+    simulation.specialise(nat.JQ_JIT_SYNC if USE_JIT else nat.JQ_JIT_OFF)
+    th = torch.tensor(theta, dtype=torch.float64, device="cuda", requires_grad=True)
+    obs = [J.PauliZ(wires=q, record=False) for q in range(n)]
+    script = J.Script(lambda t: record_hea_t(t, n), n_qubits=n)
+
+    def step():
+        th.grad = None
+        script.execute(type="expval", obs=obs, args=(th,)).sum().backward()
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    simulation.specialise(None)
+    return (time.perf_counter() - t0) / steps * 1e3
 
 
 def cpu_baseline_c2(theta):
     from oracle import cpu
 
     cores = len(os.sched_getaffinity(0))
-    t0 = time.perf_counter()
-    cpu.hea_grad(theta, None, cores)
-    dt = time.perf_counter() - t0
+    dt, ts = median_runs(lambda: cpu.hea_grad(theta, None, cores), runs=3, warmup=1)
     return {
         "value": round(1.0 / dt, 5),
         "unit": "gradient evaluations/s",
         "cores": cores,
         "kind": "port",
-        "sample": "one forward + adjoint gradient of the full 24-qubit, 8-layer workload (complex64), "
-        "C++/OpenMP restatement of the reference plan (same-wire fusion, composed CX-ring gather)",
+        "cpu": host_cpu(),
+        "sample": "forward + adjoint gradient of the full 24-qubit, 8-layer workload (complex64), C++/OpenMP "
+        "restatement of the reference plan (same-wire fusion, composed CX-ring gather); 1 warm-up, median of 3",
         "seconds": round(dt, 2),
+        "runs_s": ts,
+        "threads_1": "not timed by default: one single-thread gradient takes minutes at 24 qubits (the C1 and C3 "
+        "legs carry 1-thread figures of the same code)",
     }
 
 
@@ -698,7 +887,7 @@ def main():
     ap.add_argument("--no-naive", action="store_true", help="skip the plain-PyTorch GPU comparison of the headline")
     ap.add_argument(
         "--legs",
-        default="c2,c3,c4,c5",
+        default="c1,c2,c3,c4,c5",
         help="comma-separated workloads to run (profiling runs select one); the headline line needs c2",
     )
     ap.add_argument("--interpreter", action="store_true",
@@ -777,6 +966,14 @@ def main():
         "roofline": roofline_c2(res),
     }
     print(f"c2 leg done: {value:.3f} gradient evaluations/s", file=sys.stderr, flush=True)
+    sp_ms = script_path_c2(res["theta"])
+    line["script_path"] = {
+        "ms_per_step": round(sp_ms, 4),
+        "vs_capi_step": round(sp_ms / ms, 4),
+        "note": "Script.execute(type='expval', ...) then .sum().backward() on the same workload: one forward "
+        "(checkpoints kept in the plan) and one backward sweep, plus the host recording and lowering",
+    }
+    print("c2 script path done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_c2(res["theta"])
         print("c2 cpu baseline done", file=sys.stderr, flush=True)
@@ -818,8 +1015,15 @@ def main():
         line["c3"] = c3
         print("c3 leg done", file=sys.stderr, flush=True)
     if not args.no_c4 and "c4" in legs:
-        line["c4"] = bench_c4(3, 1)
+        c4 = bench_c4(3, 1)
+        if not args.no_cpu:
+            th4 = np.random.default_rng(1004).uniform(0, 2 * np.pi, (6, 12, 3))
+            c4["cpu_baseline"] = cpu_baseline_c4(th4)
+        line["c4"] = c4
         print("c4 leg done", file=sys.stderr, flush=True)
+    if "c1" in legs:
+        line["c1"] = bench_c1()
+        print("c1 leg done", file=sys.stderr, flush=True)
     if not args.no_c5 and "c5" in legs:
         c5 = bench_c5(3, 1)
         if not args.no_cpu:
