@@ -39,7 +39,10 @@ class GatesMeta(type):
     def __getattr__(cls, gate_name):
         if gate_name.startswith("__"):
             raise AttributeError(gate_name)
-        return _GateHandler(cls, gate_name)
+        h = cls._handlers.get(gate_name)
+        if h is None:
+            h = cls._handlers[gate_name] = _GateHandler(cls, gate_name)
+        return h
 
 
 def _no_op(*args, **kwargs):
@@ -50,13 +53,26 @@ def Barrier(wires: Union[int, List[int]], *args, **kwargs):
     return BarrierOp(wires)
 
 
+_PLAIN_KWARGS = frozenset(("wires",))
+
+
 class Gates(metaclass=GatesMeta):
     """Entry point for applying gates inside a circuit function."""
+
+    _handlers: dict = {}
+    _pulse_mgr = None
 
     @classmethod
     def _inner_getattr(cls, gate_name, *args, **kwargs):
         if gate_name == "Barrier":
             return Barrier(*args, **kwargs)
+        if kwargs.keys() <= _PLAIN_KWARGS:
+            # the common case (a unitary gate given only wires) skips the
+            # keyword filtering and pulse handling below
+            gate = getattr(UnitaryGates, gate_name, None)
+            if gate is None:
+                raise AttributeError(f"'UnitaryGates' object has no attribute '{gate_name}'")
+            return gate(*args, **kwargs)
         pulse = kwargs.pop("pulse", False)
         if not isinstance(pulse, bool):
             raise TypeError(f"'pulse' must be a bool, got {type(pulse).__name__}.")
@@ -71,7 +87,7 @@ class Gates(metaclass=GatesMeta):
             log.debug(f"Unsupported keyword arguments: {list(extra)}")
         kwargs = {k: v for k, v in kwargs.items() if k in allowed}
         pulse_params = kwargs.get("pulse_params")
-        mgr = getattr(cls, "_pulse_mgr", None)
+        mgr = cls._pulse_mgr
         if pulse_params is not None:
             if isinstance(pulse_params, PulseParams):
                 kwargs["pulse_params"] = pulse_params.params

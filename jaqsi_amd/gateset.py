@@ -203,7 +203,7 @@ def _rot_matrix(theta, pauli: np.ndarray) -> torch.Tensor:
     c = torch.cos(t / 2)[..., None, None]
     s = torch.sin(t / 2)[..., None, None]
     eye = torch.eye(d, dtype=torch.complex128)
-    p = torch.as_tensor(pauli)
+    p = _as_tensor(pauli)
     return c * eye - 1j * s * p
 
 
@@ -224,11 +224,11 @@ def _make_rotation_gate(pauli_class, name: str):
         _num_wires = 1
         is_unitary = True
         _param_names = ("theta",)
+        _structure = GateStructure(diagonal=pauli_class is PauliZ)
 
         def __init__(self, theta, wires: Union[int, List[int]] = 0, **kwargs) -> None:
             self.theta = theta
             super().__init__(wires=wires, **kwargs)
-            self._structure = GateStructure(diagonal=pauli_class is PauliZ)
 
         @property
         def matrix(self):
@@ -472,3 +472,16 @@ def build_parity_observable(qubit_group: List[int]) -> Hermitian:
     obs = Hermitian(matrix=mat, wires=qubit_group, record=False)
     obs._pauli_label = "Z" * len(qubit_group)
     return obs
+
+
+def _freeze_class_matrices() -> None:
+    """Class-level gate matrices are shared constants: read-only, so no
+    caller can change a gate for everyone (and device copies of them can be
+    cached by identity)."""
+    for obj in list(globals().values()):
+        m = getattr(obj, "_matrix", None) if isinstance(obj, type) else None
+        if isinstance(m, np.ndarray):
+            m.setflags(write=False)
+
+
+_freeze_class_matrices()

@@ -25,7 +25,7 @@ import torch
 from jaqsi_amd import _native as nat
 from jaqsi_amd.gateset import Barrier
 from jaqsi_amd.noise import KrausChannel
-from jaqsi_amd.operations import Member, Operation
+from jaqsi_amd.operations import Member, Operation, _as_tensor
 
 _CODES = {"I": 0, "X": 1, "Y": 2, "Z": 3}
 
@@ -37,15 +37,53 @@ def word_code(word: str) -> int:
     return w
 
 
-class Lowered(NamedTuple):
-    key: tuple                # structure key (for plan caching)
-    n_wires: int
-    ops: list                 # nat.Op
-    members: list             # nat.Member
-    angle_values: list        # one entry per angle slot: float / tensor (0-d or (B,))
-    matrix_values: list       # (offset, value) explicit matrices (numpy / tensor)
-    n_matrix_elems: int
-    grad_matrices: list       # indices into matrix_values that require grad
+class Lowered:
+    """A lowered tape: the structure key (for plan caching), the C-ABI op and
+    member records (built on first use: a cached plan never needs them), the
+    angle values (float / tensor, 0-d or (B,)) and the explicit matrices
+    (offset, value)."""
+
+    __slots__ = ("key", "n_wires", "_ops_raw", "_members_raw", "_ops", "_members", "angle_values", "matrix_values",
+                 "n_matrix_elems", "grad_matrices")
+
+    def __init__(self, key, n_wires, ops_raw, members_raw, angle_values, matrix_values, n_matrix_elems,
+                 grad_matrices):
+        self.key = key
+        self.n_wires = n_wires
+        self._ops_raw = ops_raw
+        self._members_raw = members_raw
+        self._ops = None
+        self._members = None
+        self.angle_values = angle_values
+        self.matrix_values = matrix_values
+        self.n_matrix_elems = n_matrix_elems
+        self.grad_matrices = grad_matrices
+
+    @property
+    def ops(self) -> list:
+        if self._ops is None:
+            out = []
+            for wires, structure, controls, perm, mb, me, cond in self._ops_raw:
+                op = nat.Op()
+                op.cond = cond
+                op.nwires = len(wires)
+                for j, w in enumerate(wires):
+                    op.wires[j] = int(w)
+                op.structure = structure
+                op.controls = controls
+                for j, p in enumerate(perm):
+                    op.perm[j] = int(p)
+                op.member_begin = mb
+                op.member_end = me
+                out.append(op)
+            self._ops = out
+        return self._ops
+
+    @property
+    def members(self) -> list:
+        if self._members is None:
+            self._members = [nat.Member(*m) for m in self._members_raw]
+        return self._members
 
 
 def _structure_flags(op: Operation) -> Tuple[int, int, tuple]:
@@ -94,29 +132,19 @@ class _Builder:
                 self.grad_mats.append(len(self.mats))
             self.mats.append((mat, val))
             self.key.append((kind, shape, m.nctrl, flags))
-        self.members.append(nat.Member(kind, flags, m.nctrl, word, param, mat, 0.0))
+        self.members.append((kind, flags, m.nctrl, word, param, mat, 0.0))
 
     def add_op(self, wires: Sequence[int], members: Sequence[Member], structure: int, controls: int, perm: tuple,
                extra_flags: int = 0, superop_matrix: Optional[np.ndarray] = None, cond: float = 1.0) -> None:
         if len(wires) > nat.MAX_WIRES:
             raise NotImplementedError(f"gates on more than {nat.MAX_WIRES} wires are not supported")
-        op = nat.Op()
-        op.cond = cond
-        op.nwires = len(wires)
-        for j, w in enumerate(wires):
-            op.wires[j] = int(w)
-        op.structure = structure
-        op.controls = controls
-        for j, p in enumerate(perm):
-            op.perm[j] = int(p)
-        op.member_begin = len(self.members)
+        mb = len(self.members)
         if superop_matrix is not None:
             self.add_member(Member("matrix", matrix=superop_matrix))
         else:
             for m in members:
                 self.add_member(m, extra_flags)
-        op.member_end = len(self.members)
-        self.ops.append(op)
+        self.ops.append((tuple(wires), structure, controls, perm, mb, len(self.members), cond))
         # the planner routes ill-conditioned blocks (product of their channels'
         # condition numbers above JQ_COND_MAX) to checkpointed passes, so a
         # channel's condition number is part of the structure
@@ -176,11 +204,38 @@ def channel_cond(sup) -> float:
     return float(s[0] / s[-1]) if s[-1] > 0 else float("inf")
 
 
+def _gather_views(values: list) -> Optional[torch.Tensor]:
+    """When every angle is a 0-d view of one contiguous float64 tensor (a
+    circuit indexing its parameter tensor, ``theta[l, q, k]``), the angle
+    vector is ONE gather from that tensor: one autograd node instead of a
+    select per angle (whose backward would launch a kernel per angle)."""
+    if not values:
+        return None
+    v0 = values[0]
+    if not isinstance(v0, torch.Tensor):
+        return None
+    base = v0._base
+    if base is None or base.dtype != torch.float64 or not base.is_contiguous():
+        return None
+    offs = []
+    b0 = base.storage_offset()
+    for v in values:
+        if not isinstance(v, torch.Tensor) or v._base is not base or v.dim() != 0:
+            return None
+        offs.append(v.storage_offset() - b0)
+    idx = torch.as_tensor(offs, dtype=torch.int64).to(base.device, non_blocking=True)
+    return base.reshape(-1).index_select(0, idx)
+
+
 def angle_tensor(values: list, batch: Optional[int]) -> torch.Tensor:
     """Stack the angle values into (P,) or (B, P) float64, keeping autograd.
 
     Constant angles (Python numbers, host tensors) join the device of the
     first device tensor among the values, so a tape may mix both."""
+    if batch is None:
+        g = _gather_views(values)
+        if g is not None:
+            return g
     dev = next((v.device for v in values if isinstance(v, torch.Tensor) and v.device.type != "cpu"), None)
     cols = []
     for v in values:
@@ -198,13 +253,37 @@ def angle_tensor(values: list, batch: Optional[int]) -> torch.Tensor:
     return torch.stack(cols, dim=-1)
 
 
+_const_pools: dict = {}
+_CONST_POOLS_MAX = 64
+
+
+def constant_pool(mats: list, device) -> Optional[torch.Tensor]:
+    """The packed pool (1, E) on `device` when every explicit matrix is a
+    host constant held by a long-lived object (a gate class's matrix): kept
+    on the device across calls, so a recorded CX ring does not cross PCIe on
+    every execute.  None when any matrix is a tensor or a fresh array."""
+    if not mats or any(isinstance(v, torch.Tensor) for _, v in mats):
+        return None
+    key = (str(device), tuple((off, id(v)) for off, v in mats))
+    hit = _const_pools.get(key)
+    if hit is not None and all(a is b for (_, a), b in zip(mats, hit[0])):
+        return hit[1]
+    if not all(getattr(v, "flags", None) is not None and not v.flags.writeable for _, v in mats):
+        return None  # only read-only (class-level) arrays are safe to cache by identity
+    pool = matrix_tensor(mats, 0, None).reshape(1, -1).to(device)
+    if len(_const_pools) >= _CONST_POOLS_MAX:
+        _const_pools.pop(next(iter(_const_pools)))
+    _const_pools[key] = ([v for _, v in mats], pool)
+    return pool
+
+
 def matrix_tensor(mats: list, n_elems: int, batch: Optional[int]) -> Optional[torch.Tensor]:
     """Pack explicit matrices into (E,) or (B, E) complex128, keeping autograd."""
     if not mats:
         return None
     parts = []
     for off, v in mats:
-        t = v if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v))
+        t = _as_tensor(v)
         t = t.to(torch.complex128)
         if batch is not None:
             t = t.expand(batch, *t.shape[-2:]) if t.dim() == 2 else t

@@ -74,7 +74,10 @@ class Member(NamedTuple):
 def _as_tensor(x, dtype=None):
     if isinstance(x, torch.Tensor):
         return x if dtype is None else x.to(dtype)
-    return torch.as_tensor(np.asarray(x), dtype=dtype)
+    a = np.asarray(x)
+    if not a.flags.writeable:  # shared read-only constants (class gate matrices)
+        a = a.copy()
+    return torch.as_tensor(a, dtype=dtype)
 
 
 def _embed_matrix(mat: torch.Tensor, op_wires: list, all_wires: list) -> torch.Tensor:
@@ -122,13 +125,12 @@ class Operation:
         name: Optional[str] = None,
     ) -> None:
         self.name = name or self.__class__.__name__
-        self.wires = list(wires) if isinstance(wires, (list, tuple)) else [wires]
-        if self._num_wires is not None and len(self.wires) != self._num_wires:
-            raise ValueError(
-                f"{self.name} expects {self._num_wires} wire(s), got {len(self.wires)}: {self.wires}"
-            )
-        if len(self.wires) != len(set(self.wires)):
-            raise ValueError(f"{self.name} received duplicate wires: {self.wires}")
+        ws = list(wires) if isinstance(wires, (list, tuple)) else [wires]
+        self._wires = ws
+        if self._num_wires is not None and len(ws) != self._num_wires:
+            raise ValueError(f"{self.name} expects {self._num_wires} wire(s), got {len(ws)}: {ws}")
+        if len(ws) > 1 and len(ws) != len(set(ws)):
+            raise ValueError(f"{self.name} received duplicate wires: {ws}")
         if matrix is not None:
             self._matrix = matrix
             self._structure = GateStructure()

@@ -19,7 +19,7 @@ import numpy as np
 import torch
 
 from jaqsi_amd import lowering, simulation
-from jaqsi_amd.operations import Operation
+from jaqsi_amd.operations import Operation, _as_tensor
 from jaqsi_amd.tape import recording
 
 
@@ -106,7 +106,7 @@ class Script:
             low = lowering.lower_mixed(tape, n_q) if dens else lowering.lower_pure(tape, n_q)
             vals = [v.to(torch.float64) if isinstance(v, torch.Tensor) else torch.as_tensor(float(v), dtype=torch.float64)
                     for v in low.angle_values]
-            mats = [(v if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v))).to(torch.complex128).reshape(-1)
+            mats = [_as_tensor(v).to(torch.complex128).reshape(-1)
                     for _, v in low.matrix_values]
             return (torch.stack(vals) if vals else torch.zeros(0, dtype=torch.float64), tuple(mats))
 

@@ -23,7 +23,7 @@ from jaqsi_amd import _native as nat
 from jaqsi_amd import lowering
 from jaqsi_amd.gateset import Barrier
 from jaqsi_amd.noise import KrausChannel
-from jaqsi_amd.operations import Operation, x64_enabled
+from jaqsi_amd.operations import Operation, _as_tensor, x64_enabled
 
 _plans: dict = {}
 _plans_lock = threading.Lock()
@@ -362,7 +362,7 @@ def _lift_apply(m, wires, vec, n):
 
     B = vec.shape[0]
     t = vec.reshape((B,) + (2,) * n)
-    o = torch.as_tensor(m, dtype=vec.dtype, device=vec.device)
+    o = _as_tensor(m, vec.dtype).to(vec.device)
     return _contract(t, o, [w + 1 for w in wires]).reshape(B, -1)
 
 
@@ -375,7 +375,7 @@ class _SvExpvalMatrix(torch.autograd.Function):
         B = state.shape[0]
         col = torch.zeros(B, dtype=torch.float64, device=state.device)
         w = np.array(wires, dtype=np.int32)
-        mt = torch.as_tensor(m).to(state.device)
+        mt = _as_tensor(m).to(state.device)
         nat.check(nat.lib().jq_expval_matrix(n_qubits, B, dtype, 0, nat.ptr(state), len(wires), nat.ptr(w),
                                              nat.ptr(mt), nat.ptr(col), _stream()))
         ctx.save_for_backward(state)
@@ -400,7 +400,7 @@ class _DmExpvalMatrix(torch.autograd.Function):
         B = rho_vec.shape[0]
         col = torch.zeros(B, dtype=torch.float64, device=rho_vec.device)
         w = np.array(wires, dtype=np.int32)
-        mt = torch.as_tensor(m).to(rho_vec.device)
+        mt = _as_tensor(m).to(rho_vec.device)
         nat.check(nat.lib().jq_expval_matrix(n_qubits, B, dtype, 1, nat.ptr(rho_vec), len(wires), nat.ptr(w),
                                              nat.ptr(mt), nat.ptr(col), _stream()))
         ctx.n = n_qubits
@@ -416,7 +416,7 @@ class _DmExpvalMatrix(torch.autograd.Function):
         n, dim = ctx.n, 1 << ctx.n
         eye = torch.eye(dim, dtype=torch.complex128, device=g.device).reshape((2,) * (2 * n))
         # O lifted to all wires: O applied to the ket axes of the identity
-        lifted = _contract(eye, torch.as_tensor(ctx.m, dtype=torch.complex128, device=g.device), ctx.wires)
+        lifted = _contract(eye, _as_tensor(ctx.m, torch.complex128).to(g.device), ctx.wires)
         # d Re Tr(O rho) / d rho (PyTorch's convention): conj(O^T) = O^dagger-transpose;
         # for a Hermitian O this is O itself
         grad = lifted.reshape(dim, dim).transpose(0, 1).conj()
@@ -432,7 +432,9 @@ def _prepare(tape, n_qubits, use_density, batch, initial_state, dtype, values=No
     dev = device()
     if values is None:
         angles = lowering.angle_tensor(low.angle_values, batch)
-        mats = lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, batch)
+        mats = lowering.constant_pool(low.matrix_values, dev) if batch is None else None
+        if mats is None:
+            mats = lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, batch)
     else:
         angles, mats = values
     if batch is None:
@@ -514,7 +516,7 @@ def simulate_and_measure(
         state = _forward_state(plan, B, angles, mats, psi0, psi0_batched, dtype)
         out = torch.zeros((B, len(obs)), dtype=torch.float64, device=angles.device)
         for i, ob in enumerate(obs):
-            m = torch.as_tensor(_obs_matrix(ob)).to(angles.device)
+            m = _as_tensor(_obs_matrix(ob)).to(angles.device)
             col = torch.zeros(B, dtype=torch.float64, device=angles.device)
             w = np.array(ob.wires, dtype=np.int32)
             nat.check(nat.lib().jq_expval_matrix(n_qubits, B, dtype, int(use_density), nat.ptr(state), len(ob.wires),
