@@ -164,6 +164,19 @@ def bench_c2(steps, warmup, n=24, layers=8):
     }
 
 
+def stored_traffic(leg, kernel):
+    """HBM bytes per launch of `kernel` from the stored FETCH_SIZE /
+    WRITE_SIZE calibration (profiles/traffic.json; its commands are in
+    profiles/traffic_*.txt): a measurement of the committed kernels, not of
+    this run, so it is labelled as stored."""
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(tf):
+        return None
+    with open(tf) as f:
+        rec = json.load(f).get(leg, {}).get(kernel)
+    return rec["total"] if rec else None
+
+
 def roofline_c2(res):
     n = res["n"]
     sweep = 2 * 8 * (1 << n)  # read + write of one complex64 vector
@@ -172,11 +185,7 @@ def roofline_c2(res):
     tot, cnt = res["kernels"][dom]
     avg_s = tot / 1e3 / max(cnt, 1)
     achieved = bytes_per[dom] / avg_s / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_c2.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get(dom)
+    traffic = stored_traffic("c2", dom)
     others = {k: {"avg_us": round(v[0] / max(v[1], 1) * 1e3, 2), "launches": v[1]} for k, v in res["kernels"].items()}
     return {
         "bound": "hbm",
@@ -300,6 +309,9 @@ def roofline_c3(res, steps):
         "unit": "GB/s",
         "frac": round(ach / HBM_PEAK_GBS, 4),
         "traffic": None,
+        "traffic_per_launch": {k: stored_traffic("c3", k) for k in kernel_names()},
+        "traffic_note": "launches cover groups of different sizes, so one per-launch byte figure is not comparable "
+        "with the per-circuit algorithmic bytes; traffic_per_launch is the stored FETCH_SIZE/WRITE_SIZE average",
         "algorithmic_bytes_note": "per circuit pass 2*8*2^14 B forward, 3*8*2^14 B backward, summed over all "
         "circuit passes of the timed steps, divided by their wall time",
         "kernels": out,
@@ -426,7 +438,8 @@ def bench_c4(steps, warmup, n=12, layers=6, seed=1004):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(kern[dom]["GB/s"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": stored_traffic("c4", dom),
+            "algorithmic_bytes_per_launch": per[dom],
             "kernels": kern,
         },
         "expval": float(ev.cpu()[0, 0]),
@@ -434,6 +447,8 @@ def bench_c4(steps, warmup, n=12, layers=6, seed=1004):
 
 
 FP64_PEAK_TFLOPS = 78.6
+# 64 x (5.892e7 ADD + 1.079e8 MUL + 2 x 1.661e8 FMA) F64 VALU instructions per launch at 8192 sets x 2000 steps
+C5_EXECUTED_FP64_FLOP = 64 * (5.892e7 + 1.079e8 + 2 * 1.661e8)
 
 
 def c5_inputs(n_sets, seed):
@@ -483,7 +498,7 @@ def bench_c5(steps, warmup, n_sets=8192, n_steps=2000, seed=1005):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     kms = e0.elapsed_time(e1) / steps
-    flops = 1536.0 * n_steps * n_sets  # 4x4 complex product floor: 512 forward + 1024 backward per step
+    flops = C5_EXECUTED_FP64_FLOP * (n_sets / 8192) * (n_steps / 2000)
     ach = flops / (kms / 1e3) / 1e12
     return {
         "metric": "pulse-propagator gradient evaluations/s (8192 sets, 2 qubits, 2000 Magnus-2 steps, FP64)",
@@ -505,9 +520,9 @@ def bench_c5(steps, warmup, n_sets=8192, n_steps=2000, seed=1005):
             "unit": "TFLOP/s",
             "frac": round(ach / FP64_PEAK_TFLOPS, 4),
             "traffic": None,
-            "flop_model": "a floor, not executed FLOPs: 1536 FP64 FLOP per step per set, the 4x4 complex products "
-            "(512 forward + 1024 backward) any product scheme needs; the kernel forms the step in closed form (two "
-            "SU(2) blocks with sincos / exp / expm1), so it is bound by transcendental and dependent-FMA latency",
+            "flop_count": "executed FP64 FLOP per launch: 64 lanes x (ADD + MUL + 2 FMA) F64 VALU instructions "
+            "counted by SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 at 8192 sets x 2000 steps (profiles/counters_c5_pulse_kernel.txt), "
+            "scaled linearly to this launch; F64 transcendentals (SQ_INSTS_VALU_TRANS_F64) are not counted",
             "kernel_ms": round(kms, 4),
         },
         "sample_infidelity": float(inf[0].cpu()),
