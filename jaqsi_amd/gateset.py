@@ -148,25 +148,31 @@ class DiagonalQubitUnitary(Operation):
         self._structure = GateStructure(diagonal=True)
 
     def decompose(self) -> List[Operation]:
-        """Walsh-Hadamard split of a real diagonal generator into Z-string
-        rotations (gateset.py:270-318)."""
+        """Z-string rotations whose product is exp(-i scale diag(generator))
+        up to a global phase (gateset.py:270-318).
+
+        A real diagonal H = diag(h) on k wires is sum_S alpha_S Z_S over the
+        subsets S of wires, alpha_S = 2^-k sum_j h_j (-1)^(popcount(j & S)):
+        the Walsh-Hadamard transform of h, done in place one wire axis at a
+        time with the butterfly (u, v) -> (u + v, u - v).  The Z strings
+        commute, so exp(-i x H) = prod_S PauliRot(2 x alpha_S, Z_S); the empty
+        string is the global phase and is dropped, as are zero weights."""
         if self._generator is None:
             return super().decompose()
         k = len(self.wires)
-        dim = 2**k
-        marks = np.asarray(self._generator, dtype=float).reshape(-1)
-        signs = np.array([[1 - 2 * ((j >> (k - 1 - i)) & 1) for j in range(dim)] for i in range(k)])
+        h = np.array(self._generator, dtype=float).reshape((2,) * k)
+        for ax in range(k):
+            u, v = np.take(h, 0, axis=ax), np.take(h, 1, axis=ax)
+            h = np.stack((u + v, u - v), axis=ax)
+        alpha = h.reshape(-1) / 2**k
         ops: List[Operation] = []
-        for mask in range(1, dim):
-            chi = np.ones(dim)
-            for i in range(k):
-                if (mask >> i) & 1:
-                    chi = chi * signs[i]
-            alpha = float(marks @ chi) / dim
-            if abs(alpha) < 1e-12:
+        for idx in range(1, 2**k):
+            if abs(alpha[idx]) < 1e-12:
                 continue
-            word = "".join("Z" if (mask >> i) & 1 else "I" for i in range(k))
-            ops.append(PauliRot(2.0 * alpha * self._scale, word, wires=self.wires, record=False))
+            # bit (k-1-i) of idx marks wire position i (the first wire is the
+            # most significant bit of the diagonal index)
+            word = "".join("Z" if (idx >> (k - 1 - i)) & 1 else "I" for i in range(k))
+            ops.append(PauliRot(2.0 * float(alpha[idx]) * self._scale, word, wires=self.wires, record=False))
         return ops
 
 

@@ -149,54 +149,66 @@ class PhaseDamping(KrausChannel):
         ]
 
 
+def kraus_from_superoperator(S: np.ndarray, tol: float = 1e-14) -> List[np.ndarray]:
+    """Kraus operators of a completely positive map given by its
+    superoperator S on row-major vec(rho) (S[(k,l),(i,j)] is the weight of
+    rho_ij in rho'_kl).
+
+    Reshuffling S to C[(k,i),(l,j)] = S[(k,l),(i,j)] gives the Choi matrix
+    sum_m vec(K_m) vec(K_m)^dagger, vec(K)[(k,i)] = K_ki; its eigenvectors
+    with positive eigenvalues, scaled by their square roots, are a minimal
+    Kraus set (Choi 1975)."""
+    d = int(round(np.sqrt(S.shape[0])))
+    C = S.reshape(d, d, d, d).transpose(0, 2, 1, 3).reshape(d * d, d * d)
+    lam, vec = np.linalg.eigh((C + C.conj().T) / 2)
+    keep = lam > tol * max(1.0, float(lam.max()))
+    return [np.sqrt(l) * vec[:, m].reshape(d, d) for m, l in zip(np.flatnonzero(keep), lam[keep])]
+
+
 class ThermalRelaxationError(KrausChannel):
-    """T1/T2 relaxation (noise.py:319-431)."""
+    """Relaxation toward the thermal state and dephasing over a gate of
+    length ``tg`` (noise.py:319-431).
+
+    With the relaxed fraction a = 1 - exp(-tg / t1), the excited-state
+    population ``pe`` of the thermal state and the coherence decay
+    c = exp(-tg / t2), the channel maps rho to
+
+        rho'_11 = (1 - a) rho_11 + a pe,   rho'_00 = 1 - rho'_11,
+        rho'_01 = c rho_01,                rho'_10 = c rho_10,
+
+    which is completely positive for t2 <= 2 t1.  The Kraus operators are
+    read off the map's Choi matrix (at most four; a Kraus set is fixed only up
+    to a unitary mixing, the channel itself is unique)."""
 
     _num_wires = 1
     _param_names = ("pe", "t1", "t2", "tg")
 
     def __init__(self, pe: float, t1: float, t2: float, tg: float, wires: Union[int, List[int]] = 0) -> None:
         _check_unit_interval(pe, "pe")
-        if t1 <= 0:
-            raise ValueError("t1 must be > 0.")
-        if t2 <= 0:
-            raise ValueError("t2 must be > 0.")
+        if not t1 > 0 or not t2 > 0:
+            raise ValueError(f"relaxation times must be positive (t1={t1}, t2={t2}).")
         if t2 > 2 * t1:
-            raise ValueError("t2 must be <= 2·t1.")
+            raise ValueError(f"t2={t2} exceeds 2*t1={2 * t1}: the channel would not be completely positive.")
         if tg < 0:
-            raise ValueError("tg must be >= 0.")
+            raise ValueError(f"gate time tg={tg} must be non-negative.")
         self.pe, self.t1, self.t2, self.tg = pe, t1, t2, tg
         super().__init__(wires=wires)
 
-    def kraus_matrices(self):
+    def relaxation_superoperator(self) -> np.ndarray:
+        """The map above as a 4x4 matrix on row-major vec(rho)."""
         pe, t1, t2, tg = (float(v) for v in (self.pe, self.t1, self.t2, self.tg))
-        eT1 = np.exp(-tg / t1)
-        p_reset = 1.0 - eT1
-        eT2 = np.exp(-tg / t2)
-        if t2 <= t1:
-            pz = (1.0 - p_reset) * (1.0 - eT2 / eT1) / 2.0
-            pr0 = (1.0 - pe) * p_reset
-            pr1 = pe * p_reset
-            pid = 1.0 - pz - pr0 - pr1
-            return [
-                np.sqrt(pid) * np.eye(2, dtype=np.complex128),
-                np.sqrt(pz) * np.array([[1, 0], [0, -1]], dtype=np.complex128),
-                np.sqrt(pr0) * np.array([[1, 0], [0, 0]], dtype=np.complex128),
-                np.sqrt(pr0) * np.array([[0, 1], [0, 0]], dtype=np.complex128),
-                np.sqrt(pr1) * np.array([[0, 0], [1, 0]], dtype=np.complex128),
-                np.sqrt(pr1) * np.array([[0, 0], [0, 1]], dtype=np.complex128),
-            ]
-        choi = np.array(
-            [
-                [1 - pe * p_reset, 0, 0, eT2],
-                [0, pe * p_reset, 0, 0],
-                [0, 0, (1 - pe) * p_reset, 0],
-                [eT2, 0, 0, 1 - (1 - pe) * p_reset],
-            ],
-            dtype=np.complex128,
-        )
-        w, v = np.linalg.eigh(choi)
-        return [np.sqrt(abs(w[i])) * v[:, i].reshape(2, 2, order="F") for i in range(4)]
+        a = -np.expm1(-tg / t1)
+        c = np.exp(-tg / t2)
+        S = np.zeros((4, 4), dtype=np.complex128)
+        S[3, 0] = a * pe            # rho_00 -> rho'_11
+        S[3, 3] = 1.0 - a * (1.0 - pe)
+        S[0, 0] = 1.0 - S[3, 0]     # trace preservation fixes the first row
+        S[0, 3] = 1.0 - S[3, 3]
+        S[1, 1] = S[2, 2] = c
+        return S
+
+    def kraus_matrices(self):
+        return kraus_from_superoperator(self.relaxation_superoperator())
 
 
 class QubitChannel(KrausChannel):

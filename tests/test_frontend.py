@@ -107,6 +107,91 @@ def test_noise_params_attach_channels():
     assert kinds[1] is BitFlip and kinds[2] is DepolarizingChannel and kinds[-1] is QubitChannel
 
 
+def _rx_rho(theta):
+    psi = np.array([np.cos(theta / 2), -1j * np.sin(theta / 2)])
+    return np.outer(psi, psi.conj())
+
+
+@pytest.mark.parametrize("pe,t1,t2,tg", [(0.0, 1e-4, 5e-5, 1e-6), (0.0, 1.0, 0.3, 0.8), (0.25, 1.0, 0.8, 0.6),
+                                         (0.4, 1.0, 1.7, 0.9), (0.1, 2.0, 4.0, 3.0), (0.3, 1.0, 1.0, 0.0)])
+def test_thermal_relaxation_matches_t1_t2_decay(pe, t1, t2, tg):
+    """The reference's tests/test_jaqsi.py:689-711 (pe = 0, t2 < t1) and the
+    general case: populations relax toward (1 - pe, pe) with exp(-tg/t1),
+    coherences decay with exp(-tg/t2), in both regimes (t2 <= t1 and
+    t1 < t2 <= 2 t1); the state after RX(1) is used as input."""
+    from jaqsi_amd import noise
+
+    rho = _rx_rho(1.0)
+    ch = noise.ThermalRelaxationError(pe, t1, t2, tg, wires=0)
+    ks = ch.kraus_matrices()
+    assert len(ks) <= 4
+    assert np.allclose(sum(k.conj().T @ k for k in ks), np.eye(2), atol=1e-14)
+    out = sum(k @ rho @ k.conj().T for k in ks)
+    e1, e2 = np.exp(-tg / t1), np.exp(-tg / t2)
+    p1 = e1 * rho[1, 1] + (1 - e1) * pe
+    ref = np.array([[1 - p1, e2 * rho[0, 1]], [e2 * rho[1, 0], p1]])
+    np.testing.assert_allclose(out, ref, atol=1e-14)
+    # the channel fixes the thermal state
+    th = np.diag([1 - pe, pe]).astype(complex)
+    np.testing.assert_allclose(sum(k @ th @ k.conj().T for k in ks), th, atol=1e-14)
+
+
+def test_thermal_relaxation_rejects_non_physical_times():
+    from jaqsi_amd import noise
+
+    for args in [(0.1, 0.0, 1.0, 0.1), (0.1, 1.0, -1.0, 0.1), (0.1, 1.0, 2.5, 0.1), (0.1, 1.0, 1.0, -0.1),
+                 (1.2, 1.0, 1.0, 0.1)]:
+        with pytest.raises(ValueError):
+            noise.ThermalRelaxationError(*args)
+
+
+def test_kraus_from_superoperator_round_trip():
+    """Random CPTP maps (random Stinespring isometries) on 1 and 2 qubits:
+    the Kraus set read off the Choi matrix rebuilds the same superoperator
+    and is trace preserving."""
+    from jaqsi_amd.noise import kraus_from_superoperator
+
+    rng = np.random.default_rng(7)
+    for d, r in ((2, 3), (4, 5), (2, 1)):
+        A = rng.normal(size=(d * r, d)) + 1j * rng.normal(size=(d * r, d))
+        V, _ = np.linalg.qr(A)  # isometry C^d -> C^(d r)
+        ks = [V[i * d:(i + 1) * d] for i in range(r)]
+        S = sum(np.kron(k, k.conj()) for k in ks)
+        ks2 = kraus_from_superoperator(S)
+        assert len(ks2) == r
+        np.testing.assert_allclose(sum(np.kron(k, k.conj()) for k in ks2), S, atol=1e-13)
+        np.testing.assert_allclose(sum(k.conj().T @ k for k in ks2), np.eye(d), atol=1e-13)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_diagonal_decompose_recomposes_the_diagonal(k):
+    """The reference's tests/test_jaqsi.py:540-565: the product of the
+    PauliRot factors equals diag(exp(-i x g)) times the dropped global phase
+    exp(i x mean(g)) (the identity term), checked with the PauliRot matrices
+    themselves; zero-weight strings are omitted."""
+    import jaqsi_amd as J
+    from jaqsi_amd.gateset import DiagonalQubitUnitary
+
+    J.enable_x64(True)
+    try:
+        rng = np.random.default_rng(k)
+        x = 0.83
+        for g in (rng.normal(size=2**k), np.arange(2**k, dtype=float) ** 2):
+            d = np.exp(-1j * g * x)
+            op = DiagonalQubitUnitary(torch.tensor(d), wires=list(range(k)), generator=g, scale=x, record=False)
+            U = np.eye(2**k, dtype=complex)
+            for f in op.decompose():
+                U = f.matrix.numpy() @ U
+            np.testing.assert_allclose(U, np.diag(d) * np.exp(1j * x * g.mean()), atol=1e-12)
+        # a generator that is one Z string has exactly one factor
+        z = np.array([1.0, -1.0] * 2 ** (k - 1))
+        facs = DiagonalQubitUnitary(torch.tensor(np.exp(-1j * z)), wires=list(range(k)), generator=z, scale=1.0,
+                                    record=False).decompose()
+        assert len(facs) == 1 and facs[0].pauli_word == "I" * (k - 1) + "Z"
+    finally:
+        J.enable_x64(False)
+
+
 def test_kraus_channels_are_trace_preserving():
     from jaqsi_amd import noise
 
