@@ -570,29 +570,81 @@ def _probs(state, n_qubits, density, B, dtype):
     return out
 
 
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _mix64(z: np.ndarray) -> np.ndarray:
+    """The splitmix64 finaliser (Steele, Lea & Flood, OOPSLA 2014): a
+    bijective 64-bit mix, used here as a counter-based generator."""
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def shot_seed(key) -> np.uint64:
+    """64-bit seed from a shot key: None (the reference's default key 0), an
+    integer, or an array of key words (a JAX PRNG key array arrives as its
+    uint32 words through np.asarray) folded in order."""
+    if key is None:
+        words = np.zeros(1, dtype=np.uint64)
+    elif isinstance(key, torch.Tensor):
+        words = key.detach().cpu().numpy().astype(np.int64).view(np.uint64).ravel()
+    else:
+        words = np.asarray(key).astype(np.int64).view(np.uint64).ravel()
+    s = np.uint64(0)
+    with np.errstate(over="ignore"):
+        for w in words:
+            s = _mix64(np.asarray(s ^ w + _GOLDEN))[()]
+    return np.uint64(s)
+
+
+def shot_uniforms(key, batch: int, shots: int) -> np.ndarray:
+    """(batch, shots) uniforms in [0, 1): entry (b, s) is the mix of a
+    per-row seed and the shot counter, so row b depends only on the key and
+    b (the reference splits its key per batch element, script.py:628-629)."""
+    seed = shot_seed(key)
+    with np.errstate(over="ignore"):
+        rows = _mix64(seed ^ _mix64(np.arange(batch, dtype=np.uint64) * _GOLDEN + _GOLDEN))
+        ctr = (np.arange(shots, dtype=np.uint64) + np.uint64(1)) * _GOLDEN
+        z = _mix64(rows[:, None] + ctr[None, :])
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
 def sample_shots(probs, n_qubits, type, obs, shots, key=None):
     """Shot sampling from exact probabilities (simulation.py:860-917).
 
-    The reference draws with jax.random.choice; here torch.multinomial with a
-    generator seeded from *key*, so samples differ (statistical parity only).
-    """
-    g = torch.Generator(device=probs.device)
-    g.manual_seed(int(key) if key is not None else 0)
-    p = probs.to(torch.float64)
-    p = p / p.sum(-1, keepdim=True)
-    samples = torch.multinomial(p, shots, replacement=True, generator=g)
+    Each row of *probs* ((dim,) or (B, dim)) is sampled *shots* times by
+    inverse-CDF lookup of counter-based uniforms (``shot_uniforms``); the
+    reference draws with jax.random.choice, so individual samples differ and
+    parity is statistical.  Returns estimated probabilities, or expectation
+    values of the observables' diagonals under them (exact for diagonal
+    observables, as in the reference)."""
+    if type not in ("probs", "expval"):
+        raise ValueError(f"Shot simulation is only supported for 'probs' and 'expval', got {type!r}.")
+    if int(shots) <= 0:
+        raise ValueError(f"shots must be a positive integer, got {shots!r}")
+    p = probs.detach().to(torch.float64)
+    squeeze = p.dim() == 1
+    p = p.reshape(-1, p.shape[-1])
+    B, dim = p.shape
+    cdf = torch.cumsum(p.clamp_min(0.0), dim=-1)
+    cdf = cdf / cdf[:, -1:]
+    u = torch.as_tensor(shot_uniforms(key, B, int(shots)), device=p.device)
+    samples = torch.searchsorted(cdf, u, right=True).clamp_(max=dim - 1)
     counts = torch.zeros_like(p)
-    counts.scatter_add_(1, samples, torch.ones_like(samples, dtype=p.dtype))
+    counts.scatter_add_(1, samples, torch.ones_like(u))
     est = counts / shots
     if type == "probs":
-        return est.to(probs.dtype)
-    if type == "expval":
+        out = est
+    else:
         cols = []
         for ob in obs:
             diag = torch.as_tensor(np.real(_lifted_diag(_obs_matrix(ob), ob.wires, n_qubits))).to(p.device)
             cols.append(est @ diag)
-        return torch.stack(cols, dim=-1).to(probs.dtype)
-    raise ValueError(f"Shot simulation is only supported for 'probs' and 'expval', got {type!r}.")
+        out = torch.stack(cols, dim=-1)
+    out = out[0] if squeeze else out
+    return out.to(probs.dtype if probs.dtype.is_floating_point else torch.float64)
 
 
 def _lifted_diag(m, wires, n):
