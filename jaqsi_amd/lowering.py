@@ -290,4 +290,7 @@ def matrix_tensor(mats: list, n_elems: int, batch: Optional[int]) -> Optional[to
             parts.append(t.reshape(batch, -1))
         else:
             parts.append(t.reshape(-1))
-    return torch.cat(parts, dim=-1)
+    # constants come from numpy (host), matrices built from device tensors
+    # stay on the device: gather everything where the device tensors are
+    dev = next((p.device for p in parts if p.device.type != "cpu"), parts[0].device)
+    return torch.cat([p.to(dev) for p in parts], dim=-1)

@@ -93,6 +93,7 @@ class Script:
         if initial_state is not None and np.ndim(initial_state) == 2 and np.shape(initial_state)[0] != B:
             raise ValueError("2D initial_state must have one state per batch sample")
         box = {}
+        dev = next((a.device for a, ax in zip(args, in_axes) if ax is not None), torch.device("cpu"))
 
         def single(*a):
             with recording() as tape:
@@ -104,9 +105,10 @@ class Script:
             # matrices alike (channels contribute their superoperators)
             dens = simulation.has_noise(tape)
             low = lowering.lower_mixed(tape, n_q) if dens else lowering.lower_pure(tape, n_q)
-            vals = [v.to(torch.float64) if isinstance(v, torch.Tensor) else torch.as_tensor(float(v), dtype=torch.float64)
-                    for v in low.angle_values]
-            mats = [_as_tensor(v).to(torch.complex128).reshape(-1)
+            # constants join the batched values on the arguments' device
+            vals = [v.to(dev, torch.float64) if isinstance(v, torch.Tensor)
+                    else torch.as_tensor(float(v), dtype=torch.float64, device=dev) for v in low.angle_values]
+            mats = [_as_tensor(v).to(dev, torch.complex128).reshape(-1)
                     for _, v in low.matrix_values]
             return (torch.stack(vals) if vals else torch.zeros(0, dtype=torch.float64), tuple(mats))
 

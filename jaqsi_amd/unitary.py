@@ -24,21 +24,26 @@ class UnitaryGates:
     @staticmethod
     def NQubitDepolarizingChannel(p: float, wires: List[int]) -> noise.QubitChannel:
         """n-qubit depolarizing channel (unitary.py:21-79)."""
-        if not (0.0 <= p <= 1.0):
+        if not (0.0 <= float(p.detach() if isinstance(p, torch.Tensor) else p) <= 1.0):
             raise ValueError(f"Probability p must be between 0 and 1, got {p}")
         n = len(wires)
         if n < 2:
             raise ValueError(f"Number of qubits must be >= 2, got {n}")
         paulis = [np.eye(2), gateset.PauliX._matrix, gateset.PauliY._matrix, gateset.PauliZ._matrix]
         dim = 2**n
-        ops = [np.sqrt(1 - p * (4**n - 1) / (4**n)) * np.eye(dim, dtype=np.complex128)]
+        # a p that requires grad keeps the Kraus operators in torch, so the
+        # channel's superoperator carries the gradient back to p
+        grad = isinstance(p, torch.Tensor) and p.requires_grad
+        sqrt = (lambda x: torch.sqrt(x).to(torch.complex128)) if grad else np.sqrt
+        mat = (lambda m: torch.as_tensor(m, dtype=torch.complex128).to(p.device)) if grad else (lambda m: m)
+        ops = [sqrt(1 - p * (4**n - 1) / (4**n)) * mat(np.eye(dim, dtype=np.complex128))]
         for i, idx in enumerate(itertools.product(range(4), repeat=n)):
             if i == 0:
                 continue
             P = np.eye(1)
             for j in idx:
                 P = np.kron(P, paulis[j])
-            ops.append(np.sqrt(p / (4**n)) * P)
+            ops.append(sqrt(p / (4**n)) * mat(P))
         return noise.QubitChannel(ops, wires=wires)
 
     @staticmethod

@@ -4,7 +4,8 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 
 Restates:
 
-* Kraus sets of the channels                 -- jaqsi/noise.py:123-465
+* Kraus sets of the channels                 -- jaqsi/noise.py:123-465,
+                                                jaqsi/unitary.py:21-79
 * rho -> U rho U^dagger, rho -> sum K rho K^dagger
                                              -- jaqsi/simulation.py:348-381, 443-473
 * Tr(O rho) and diag(rho)                    -- jaqsi/simulation.py:814-857
@@ -14,7 +15,10 @@ row (ket) and column (bra) index, as in the statevector engine.
 
 How it is pinned (tests/test_oracle_dm.py): trace preservation of every
 Kraus set; depolarizing and amplitude damping against their closed-form
-actions on the Bloch vector / excited population; noiseless circuits against
+actions on the Bloch vector / excited population; thermal relaxation against
+the T1 / T2 decay of populations and coherences (the reference's
+tests/test_jaqsi.py:689-711); the multi-qubit depolarizing channel against
+rho -> (1 - q) rho + q I / 2^k with q = p (its Pauli twirl); noiseless circuits against
 |psi><psi| from the statevector oracle; Hermiticity, unit trace and positive
 semidefiniteness after noisy circuits; and gradients of Tr(Z rho) against the
 exact parameter-shift rule (exact for exp(-i theta P / 2) gates inside any
@@ -56,6 +60,34 @@ def kraus(name: str, *params: float) -> List[np.ndarray]:
     if name == "PhaseDamping":  # noise.py:280-316
         (g,) = params
         return [np.array([[1, 0], [0, math.sqrt(1 - g)]], dtype=C), np.array([[0, 0], [0, math.sqrt(g)]], dtype=C)]
+    if name == "ThermalRelaxationError":  # noise.py:319-431, regime t2 <= t1
+        # Markovian mixture: identity, a Z flip with the dephasing that
+        # relaxation does not already cause, and resets to |0> / |1> with the
+        # relaxed fraction split by the thermal population pe
+        pe, t1, t2, tg = params
+        if t2 > t1:
+            raise ValueError("the oracle restates the t2 <= t1 regime only")
+        relaxed = 1.0 - math.exp(-tg / t1)
+        flip = 0.5 * (math.exp(-tg / t1) - math.exp(-tg / t2))
+        r0, r1 = (1.0 - pe) * relaxed, pe * relaxed
+        keep = 1.0 - flip - relaxed
+        P0 = np.array([[1, 0], [0, 0]], dtype=C)
+        P1 = np.array([[0, 0], [0, 1]], dtype=C)
+        L0 = np.array([[0, 1], [0, 0]], dtype=C)  # |0><1|
+        L1 = np.array([[0, 0], [1, 0]], dtype=C)  # |1><0|
+        return [math.sqrt(keep) * I2, math.sqrt(flip) * Z, math.sqrt(r0) * P0, math.sqrt(r0) * L0,
+                math.sqrt(r1) * L1, math.sqrt(r1) * P1]
+    if name == "MultiQubitDepolarizing":  # unitary.py:21-79
+        # every non-identity Pauli string on k wires with weight p / 4^k
+        p, k = params
+        ks = []
+        for word in np.ndindex(*(4,) * int(k)):
+            P = np.eye(1, dtype=C)
+            for c in word:
+                P = np.kron(P, (I2, X, Y, Z)[c])
+            w = 1.0 - p * (4**k - 1) / 4**k if not any(word) else p / 4**k
+            ks.append(math.sqrt(w) * P)
+        return ks
     raise ValueError(name)
 
 

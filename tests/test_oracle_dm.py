@@ -39,6 +39,38 @@ def test_channel_closed_forms():
     np.testing.assert_allclose(b, [b0[0], 0.6 * b0[1], 0.6 * b0[2]], atol=1e-14)
 
 
+@pytest.mark.parametrize("pe,t1,t2,tg", [(0.0, 1e-4, 5e-5, 1e-6), (0.3, 1.0, 0.6, 0.7), (0.1, 2.0, 2.0, 1.5)])
+def test_thermal_relaxation_decay(pe, t1, t2, tg):
+    """Populations relax to (1 - pe, pe) with exp(-tg/t1) and coherences
+    decay with exp(-tg/t2) (the reference's tests/test_jaqsi.py:689-711)."""
+    ks = dm.kraus("ThermalRelaxationError", pe, t1, t2, tg)
+    assert np.allclose(sum(k.conj().T @ k for k in ks), np.eye(2), atol=1e-14)
+    psi = sv.simulate([sv.Op("RX", (0,), (1.0,)), sv.Op("RZ", (0,), (0.3,))], 1)
+    rho = np.outer(psi, psi.conj())
+    out = dm.apply_channel(rho, ks, (0,), 1)
+    e1, e2 = math.exp(-tg / t1), math.exp(-tg / t2)
+    p1 = e1 * rho[1, 1] + (1 - e1) * pe
+    np.testing.assert_allclose(out, [[1 - p1, e2 * rho[0, 1]], [e2 * rho[1, 0], p1]], atol=1e-14)
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_multi_qubit_depolarizing_is_a_mix_with_the_maximally_mixed_state(k):
+    """Averaging rho over all 4^k Pauli strings gives I / 2^k, so the channel
+    is rho -> (1 - p) rho + p I / 2^k."""
+    p = 0.37
+    n = k + 1
+    ops = [sv.Op("RY", (q,), (0.3 + q,)) for q in range(n)] + [sv.Op("CX", (0, 1))]
+    psi = sv.simulate(ops, n)
+    rho = np.outer(psi, psi.conj())
+    wires = tuple(range(1, k + 1))
+    out = dm.apply_channel(rho, dm.kraus("MultiQubitDepolarizing", p, k), wires, n)
+    # partial trace over the channel's wires, then the mixed state on them
+    r = rho.reshape(2, 2**k, 2, 2**k)
+    red = np.einsum("aibi->ab", r)
+    want = (1 - p) * rho + p * np.kron(red, np.eye(2**k) / 2**k)
+    np.testing.assert_allclose(out, want, atol=1e-14)
+
+
 def test_noiseless_matches_statevector():
     n = 4
     theta = RNG.uniform(0, 2 * np.pi, (2, n, 3))
