@@ -662,11 +662,11 @@ def _lifted_diag(m, wires, n):
 # lowered structure is built once, angles and cotangents cross as tensors.
 # ---------------------------------------------------------------------------
 def lower_for_jax(tape: List[Operation], n_qubits: int, use_density: bool):
-    """Lower a recorded tape; returns (lowered structure, float64 angle vector)."""
-    tape = [op for op in tape if not isinstance(op, Barrier)]
-    low = lowering.lower_mixed(tape, n_qubits) if use_density else lowering.lower_pure(tape, n_qubits)
-    angles = np.array([float(a) for a in low.angle_values], dtype=np.float64)
-    return (low, n_qubits, use_density), angles
+    """(lowered structure, angle parameters in plan-slot order, unconverted);
+    see ``jaqsi_amd.bridge``."""
+    from jaqsi_amd.bridge import lower_for_jax as _lower
+
+    return _lower(tape, n_qubits, use_density)
 
 
 def _runner_for(lowered, obs, batch=1):
