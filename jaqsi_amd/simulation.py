@@ -14,7 +14,8 @@ from __future__ import annotations
 
 import ctypes
 import threading
-from typing import List, Optional, Sequence, Tuple
+from contextlib import contextmanager
+from typing import List, NamedTuple, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -491,6 +492,9 @@ def simulate_and_measure(
         resolve_pending(tape)
     dtype = _dtype_code()
     low, plan, angles, mats, psi0, psi0_batched = _prepare(tape, n_qubits, use_density, batch, initial_state, dtype, values)
+    runs = getattr(_capture, "runs", None)
+    if runs is not None:
+        runs.append(CapturedRun(plan, n_qubits, use_density, batch, dtype, angles, mats, psi0, psi0_batched))
     B = angles.shape[0]
     rdt = torch.float64 if dtype else torch.float32
 
@@ -661,6 +665,69 @@ def _lifted_diag(m, wires, n):
 # thin entry points for a custom_vjp bridge on the reference side: the
 # lowered structure is built once, angles and cotangents cross as tensors.
 # ---------------------------------------------------------------------------
+# ---------------------------------------------------------------------------
+# Jacobians of the final vector w.r.t. the recorded angles (quantum Fisher
+# information, math.py:331-383)
+# ---------------------------------------------------------------------------
+class CapturedRun(NamedTuple):
+    plan: object
+    n_qubits: int
+    use_density: bool
+    batch: Optional[int]
+    dtype: int
+    angles: torch.Tensor
+    mats: Optional[torch.Tensor]
+    psi0: Optional[torch.Tensor]
+    psi0_batched: bool
+
+
+_capture = threading.local()
+
+
+@contextmanager
+def capture_runs():
+    """Collect the lowered inputs (plan, angle vector with its autograd
+    history, matrices, initial state) of every simulation run inside the
+    block."""
+    prev = getattr(_capture, "runs", None)
+    _capture.runs = []
+    try:
+        yield _capture.runs
+    finally:
+        _capture.runs = prev
+
+
+def angle_jacobian(run: CapturedRun) -> Tuple[torch.Tensor, torch.Tensor]:
+    """(final vector (L,), J (L, P)): J[k, j] = d vec_k / d angle_j of one
+    captured circuit, complex128.
+
+    One batched forward of 2L copies of the circuit, then one adjoint sweep
+    whose copy k is seeded with the cotangent of Re vec_k (copies 0..L-1) or
+    Im vec_k (copies L..2L-1): the per-copy angle gradients are the real and
+    imaginary parts of J's rows."""
+    if run.batch is not None:
+        raise ValueError("the Jacobian is taken of one circuit, not a batch")
+    L = 1 << run.plan.n_wires
+    B = 2 * L
+    dev = run.angles.device
+    a = run.angles.detach().reshape(1, -1).expand(B, -1).contiguous()
+    m = run.mats.detach().reshape(1, -1).expand(B, -1).contiguous() if run.mats is not None else None
+    p0 = run.psi0.detach() if run.psi0 is not None else None
+    if p0 is not None and run.psi0_batched:
+        raise ValueError("the Jacobian is taken of one circuit, not a batch")
+    runner = _Runner(run.plan, B, run.use_density, [], run.n_qubits, False, run.dtype)
+    _, state, tok = runner.forward_ckpt(a, m, p0)
+    final = state[0].to(torch.complex128, copy=True)  # the sweep below runs the state buffer back to the input
+    # native cotangents are conj(torch's): e_k seeds Re vec_k, -i e_k seeds Im vec_k
+    lam = torch.zeros((B, L), dtype=_cdtype_torch(run.dtype), device=dev)
+    idx = torch.arange(L, device=dev)
+    lam[idx, idx] = 1.0
+    lam[idx + L, idx] = -1j
+    ga, _, _ = runner.backward(tok, a, m, p0, state, lam=lam)
+    J = torch.complex(ga[:L], ga[L:])
+    return final, J
+
+
 def lower_for_jax(tape: List[Operation], n_qubits: int, use_density: bool):
     """(lowered structure, angle parameters in plan-slot order, unconverted);
     see ``jaqsi_amd.bridge``."""
