@@ -41,6 +41,7 @@ import torch
 
 from jaqsi_amd import _native as nat
 from jaqsi_amd.operations import Member, Operation, x64_enabled
+from jaqsi_amd.tape import require_per_sample_recording
 
 __all__ = ["Evolution", "ParametrizedHamiltonian", "PendingEvolution", "resolve_pending"]
 
@@ -349,9 +350,13 @@ class Evolution:
             else:
                 t0, t1 = 0.0, T
             return PendingEvolution(ph.wires, name, fns, H, params, t0, t1, solver,
-                                    base_key + (tuple(np.shape(_host(p)) for p in params),))
+                                    base_key + (tuple(_shape(p) for p in params),))
 
         return _apply
+
+
+def _shape(x) -> tuple:
+    return tuple(x.shape) if isinstance(x, torch.Tensor) else np.shape(x)
 
 
 def _host(x):
@@ -366,6 +371,7 @@ class PendingEvolution(Operation):
     is_unitary = True
 
     def __init__(self, wires, name, fns, H, params, t0, t1, solver, group_key) -> None:
+        require_per_sample_recording()
         super().__init__(wires=wires, name=name)
         self._fns = fns
         self._H = H
@@ -414,7 +420,7 @@ def _solve(ops: List[PendingEvolution], dev, grad: bool, n_steps: int) -> torch.
     with ctx:
         coefs = []
         for op in ops:
-            params = tuple(p if isinstance(p, torch.Tensor) and p.device == dev else _to_dev(p, dev) for p in op._params)
+            params = tuple(_to_dev(p, dev) for p in op._params)  # float64 on the device, whatever came in
             t0, t1 = _to_dev(op._t0, dev).reshape(()), _to_dev(op._t1, dev).reshape(())
             if s["method"] == "closed":
                 coefs.append(_area(op._fns[0], params[0], t0, t1, n_steps).reshape(1, 1))

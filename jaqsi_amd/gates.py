@@ -19,6 +19,7 @@ import torch
 
 from jaqsi_amd.gateset import Barrier as BarrierOp
 from jaqsi_amd.pulses import PulseEnvelope, PulseGates, PulseInformation, PulseParamManager, PulseParams  # noqa: F401
+from jaqsi_amd.tape import require_per_sample_recording
 from jaqsi_amd.unitary import UnitaryGates
 
 log = logging.getLogger(__name__)
@@ -76,6 +77,8 @@ class Gates(metaclass=GatesMeta):
         pulse = kwargs.pop("pulse", False)
         if not isinstance(pulse, bool):
             raise TypeError(f"'pulse' must be a bool, got {type(pulse).__name__}.")
+        if pulse:
+            require_per_sample_recording()
         allowed = ["w", "wires", "phi", "theta", "omega", "noise_params", "random_key"]
         if pulse:
             backend = PulseGates

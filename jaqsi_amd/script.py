@@ -19,8 +19,9 @@ import numpy as np
 import torch
 
 from jaqsi_amd import lowering, simulation
+from jaqsi_amd.gateset import Barrier
 from jaqsi_amd.operations import Operation, _as_tensor
-from jaqsi_amd.tape import recording
+from jaqsi_amd.tape import RecordPerSample, recording, vectorized_recording
 
 
 def _to_tensor(a):
@@ -80,6 +81,39 @@ class Script:
             tape, n_qubits, type, obs, use_density, shots=shots, key=key, initial_state=initial_state
         )
 
+    def _execute_per_sample(self, type, obs, args, kwargs, in_axes, B, shots, key, initial_state):
+        """Batched execution of circuits with pulse gates: the circuit is
+        recorded once per sample (the propagators cannot be solved inside
+        vmap), every pending evolution of all samples is solved together
+        (one propagator launch per solver group, identical gates once), and
+        the samples run as one batched simulation.  The samples must share
+        one gate structure."""
+        from jaqsi_amd.evolution import resolve_pending
+
+        tapes = []
+        for b in range(B):
+            ab = [a.select(ax, b) if ax is not None else a for a, ax in zip(args, in_axes)]
+            with recording() as tape:
+                self.f(*ab, **kwargs)
+            tapes.append([op for op in tape if not isinstance(op, Barrier)])
+        resolve_pending([op for t in tapes for op in t])
+        n_qubits = self._n_qubits or simulation.infer_n_qubits(tapes[0], obs)
+        dens = simulation.has_noise(tapes[0])
+        lower = lowering.lower_mixed if dens else lowering.lower_pure
+        lows = [lower(t, n_qubits) for t in tapes]
+        if any(low.key != lows[0].key for low in lows[1:]):
+            raise ValueError("the samples of a batched execution must share one gate structure")
+        dev = simulation.device()
+        angles = torch.stack([lowering.angle_tensor(low.angle_values, None).to(dev) for low in lows])
+        mats = None
+        if lows[0].n_matrix_elems:
+            mats = torch.stack([lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, None).to(dev)
+                                for low in lows])
+        return simulation.simulate_and_measure(
+            tapes[0], n_qubits, type, obs, dens, shots=shots, key=key, initial_state=initial_state,
+            batch=B, values=(angles.reshape(B, -1), mats),
+        )
+
     def _execute_batched(self, type, obs, args, kwargs, in_axes, shots, key, initial_state):
         if not isinstance(in_axes, (tuple, list)):
             in_axes = (in_axes,) * len(args)
@@ -96,7 +130,7 @@ class Script:
         dev = next((a.device for a, ax in zip(args, in_axes) if ax is not None), torch.device("cpu"))
 
         def single(*a):
-            with recording() as tape:
+            with recording() as tape, vectorized_recording():
                 self.f(*a, **kwargs)
             box["tape"] = tape
             n_q = self._n_qubits or simulation.infer_n_qubits(tape, obs)
@@ -112,7 +146,10 @@ class Script:
                     for _, v in low.matrix_values]
             return (torch.stack(vals) if vals else torch.zeros(0, dtype=torch.float64), tuple(mats))
 
-        angles, mats = torch.func.vmap(single, in_dims=tuple(in_axes))(*args)
+        try:
+            angles, mats = torch.func.vmap(single, in_dims=tuple(in_axes))(*args)
+        except RecordPerSample:
+            return self._execute_per_sample(type, obs, args, kwargs, in_axes, B, shots, key, initial_state)
         tape = box["tape"]
         n_qubits = self._n_qubits or simulation.infer_n_qubits(tape, obs)
         use_density = simulation.has_noise(tape)

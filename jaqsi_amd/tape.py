@@ -21,6 +21,29 @@ def _tape_stack() -> List[list]:
     return _local.stack
 
 
+class RecordPerSample(Exception):
+    """Raised while a circuit is recorded once for a whole batch (under
+    vmap) when it reaches a gate that must be recorded per sample (a pulse
+    gate, whose propagator is solved by a native launch)."""
+
+
+@contextmanager
+def vectorized_recording() -> Iterator[None]:
+    """Mark the recording inside as one recording for a whole batch."""
+    prev = getattr(_local, "vectorized", False)
+    _local.vectorized = True
+    try:
+        yield
+    finally:
+        _local.vectorized = prev
+
+
+def require_per_sample_recording() -> None:
+    """Called by gates that cannot be recorded once for a batch."""
+    if getattr(_local, "vectorized", False):
+        raise RecordPerSample()
+
+
 def active_tape() -> Optional[list]:
     """The innermost recording tape of this thread, or None (tape.py:24-34)."""
     stack = _tape_stack()

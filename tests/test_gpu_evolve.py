@@ -150,7 +150,7 @@ def _adam(loss_fn, x0, lr, steps):
         mh = m / (1 - 0.9**k)
         vh = v / (1 - 0.999**k)
         x = x - lr * mh / (torch.sqrt(vh) + 1e-8)
-        losses.append(float(L))
+        losses.append(float(L.detach()))
     losses.append(float(loss_fn(x)))
     return np.array(losses), x.cpu().numpy()
 
