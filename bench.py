@@ -92,7 +92,7 @@ def record_hea(theta, n, features=None):
 
 
 def build_plan(n, layers, features=False, tile_bits=0):
-    from jaqsi_amd import lowering
+    from jaqsi_amd import autotune, lowering
     from jaqsi_amd import _native as nat
     from jaqsi_amd.tape import recording
 
@@ -100,8 +100,11 @@ def build_plan(n, layers, features=False, tile_bits=0):
     with recording() as tape:
         record_hea(theta, n, np.zeros(n) if features else None)
     low = lowering.lower_pure(tape, n)
-    plan = nat.Plan(n, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0, tile_bits, 1)
+    knobs = autotune.config_for(low, 0, 1, jit=USE_JIT)
+    plan = nat.Plan(n, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0, tile_bits, 1,
+                    knobs["low_bits"], knobs["variant_flags"])
     mats = lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, None)
+    plan.tuning = autotune.report(low, 0, 1)
     return plan, mats
 
 
@@ -161,6 +164,7 @@ def bench_c2(steps, warmup, n=24, layers=8):
         "grads": ga.cpu().numpy()[0],
         "n": n,
         "compile_s": compile_s,
+        "tuning": plan.tuning,
     }
 
 
@@ -220,7 +224,14 @@ def bench_c3(steps, warmup, B=4096, n=14, seed=1003):
     uniq = sorted(set(int(d) for d in depths))
     members = {d: [b for b in range(B) if depths[b] == d] for d in uniq}
     tapes = [record(record_hea, np.zeros((d, n, 3)), n, np.zeros(n)) for d in uniq]
-    rb = RaggedBatch(tapes, [len(members[d]) for d in uniq], n, [(q, 1.0, -1.0) for q in range(n)], dtype=0)
+    # one set of tuning knobs for all groups: the deepest group's choice
+    from jaqsi_amd import autotune, lowering
+
+    rep = lowering.lower_pure(tapes[-1], n)
+    knobs = autotune.config_for(rep, 0, len(members[uniq[-1]]), jit=USE_JIT)
+    tuning = autotune.report(rep, 0, len(members[uniq[-1]]))
+    rb = RaggedBatch(tapes, [len(members[d]) for d in uniq], n, [(q, 1.0, -1.0) for q in range(n)], dtype=0,
+                     knobs=knobs)
     t_c = time.perf_counter()
     rb.specialise(nat.JQ_JIT_SYNC if USE_JIT else nat.JQ_JIT_OFF)
     compile_s = round(time.perf_counter() - t_c, 2) if USE_JIT else 0.0
@@ -277,6 +288,7 @@ def bench_c3(steps, warmup, B=4096, n=14, seed=1003):
         "kernels": kt,
         "circuit_passes": circuit_passes,
         "groups": len(uniq),
+        "tuning": tuning,
         "depths": depths,
         "feats": feats,
         "thetas": thetas,
@@ -388,6 +400,9 @@ def bench_c4(steps, warmup, n=12, layers=6, seed=1004):
     with recording() as tape:
         record_noisy_hea(theta, n, 0.01, 0.005)
     low, plan, angles, mats, psi0, psi0_b = S._prepare(tape, n, True, None, None, 0)
+    from jaqsi_amd import autotune
+
+    tuning = autotune.report(low, 0, 1)
     compile_s = specialise(plan)
     runner = S._Runner(plan, 1, True, [(0, 1.0, -1.0)], n, False, 0)
     ct = torch.ones((1, 1), dtype=torch.float64, device=angles.device)
@@ -430,6 +445,7 @@ def bench_c4(steps, warmup, n=12, layers=6, seed=1004):
             "blocks": info["n_blocks"],
             "pass_kernels": "plan-specialised" if USE_JIT else "interpreter",
             "specialise_compile_s": compile_s,
+            "autotune": tuning,
         },
         "roofline": {
             "bound": "hbm",
@@ -905,6 +921,8 @@ def main():
         default="c1,c2,c3,c4,c5",
         help="comma-separated workloads to run (profiling runs select one); the headline line needs c2",
     )
+    ap.add_argument("--autotune", default="lookup", choices=["off", "lookup", "tune"],
+                    help="plan knobs: built-in (off), the tuned tables (lookup) or tune now on a table miss (tune)")
     ap.add_argument("--interpreter", action="store_true",
                     help="run the program-interpreter pass kernels instead of the plan-specialised ones")
     ap.add_argument(
@@ -917,6 +935,9 @@ def main():
     args = ap.parse_args()
     global USE_JIT
     USE_JIT = not args.interpreter
+    from jaqsi_amd import autotune
+
+    autotune.set_mode(args.autotune)
     if args.gpus != 1:
         print(
             "bench.py measures one GPU: every run in this build uses a single MI355X; "
@@ -977,6 +998,7 @@ def main():
             "tile_bits": res["info"]["tile_bits"],
             "pass_kernels": "plan-specialised" if USE_JIT else "interpreter",
             "specialise_compile_s": res["compile_s"],
+            "autotune": res["tuning"],
         },
         "roofline": roofline_c2(res),
     }
@@ -1011,6 +1033,7 @@ def main():
                 "(RX,RY,RZ per wire + CX ring), x ~ U[-pi,pi), angles ~ U[0,2pi), seed 1003; "
                 "14 Z expvals + adjoint gradient of sum_q <Z_q> per circuit",
                 "groups": r3["groups"],
+                "autotune": r3["tuning"],
                 "circuit_passes": r3["circuit_passes"],
                 "pass_kernels": "plan-specialised (one launch sequence per group)" if USE_JIT
                 else "interpreter (one launch per pass step over all groups)",

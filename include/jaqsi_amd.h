@@ -119,7 +119,19 @@ typedef struct jq_plan_desc {
   int32_t dtype;          /* 0 = complex64, 1 = complex128 */
   int32_t tile_bits;      /* 0 = choose; else tile size 2^tile_bits (<= n_wires) */
   int32_t checkpoints;    /* 1 = keep every pass output for the backward sweep */
+  /* Tuning knobs, 0 = the built-in choice (jaqsi_amd.autotune times the
+   * candidates per circuit structure and batch and caches the winner):
+   *   low_bits       lowest global bits every tile holds, 3..6 (contiguous
+   *                  runs of 2^low_bits amplitudes per load; fewer leave more
+   *                  tile bits to the gates and can cut passes)
+   *   variant_flags  JQ_TUNE_* bits selecting kernel variants */
+  int32_t low_bits;
+  int32_t variant_flags;
 } jq_plan_desc;
+
+#define JQ_TUNE_NO_PERSIST_BWD 1 /* backward: one tile per workgroup, no persistent loop with prefetch */
+#define JQ_TUNE_NO_DMA_FWD 2     /* complex64 forward: loads through registers, not direct to LDS */
+#define JQ_TUNE_NARROW_BWD 4     /* complex64 plan-specialised backward: 8 amplitudes x 1024 threads */
 
 typedef struct jq_plan jq_plan;
 

@@ -30,6 +30,9 @@ JQ_S_DIAGONAL, JQ_S_PERMUTATION, JQ_S_NONUNITARY = 1, 2, 4
 JQ_OBS_NONE, JQ_OBS_SV_DIAG, JQ_OBS_DM_DIAG, JQ_OBS_SV_PARITY = 0, 1, 2, 3
 MAX_WIRES = 6
 JQ_COND_MAX = 4.0
+JQ_TUNE_NO_PERSIST_BWD = 1
+JQ_TUNE_NO_DMA_FWD = 2
+JQ_TUNE_NARROW_BWD = 4
 JQ_NOALLOC_SCRATCH_BYTES = 32 << 20
 
 
@@ -70,6 +73,8 @@ class PlanDesc(C.Structure):
         ("dtype", C.c_int32),
         ("tile_bits", C.c_int32),
         ("checkpoints", C.c_int32),
+        ("low_bits", C.c_int32),
+        ("variant_flags", C.c_int32),
     ]
 
 
@@ -226,12 +231,15 @@ def ptr(t) -> Optional[int]:
 class Plan:
     """Owner of a native jq_plan (one fused, scheduled circuit structure)."""
 
-    def __init__(self, n_wires, ops, members, n_params, n_matrix_elems, dtype, tile_bits=0, checkpoints=1):
+    def __init__(self, n_wires, ops, members, n_params, n_matrix_elems, dtype, tile_bits=0, checkpoints=1,
+                 low_bits=0, variant_flags=0):
         L = lib()
         self._ops = (Op * max(1, len(ops)))(*ops)
         self._members = (Member * max(1, len(members)))(*members)
         d = PlanDesc(n_wires, len(ops), self._ops, len(members), self._members, n_params, n_matrix_elems, dtype,
-                     tile_bits, checkpoints)
+                     tile_bits, checkpoints, low_bits, variant_flags)
+        self.low_bits = low_bits
+        self.variant_flags = variant_flags
         h = C.c_void_p()
         check(L.jq_plan_create(C.byref(d), C.byref(h)))
         self.handle = h

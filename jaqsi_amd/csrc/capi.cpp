@@ -187,6 +187,7 @@ struct jq_plan {
   std::vector<int32_t> slot_words;
   std::vector<char> pass_full;  // pass holds a 4x4 register block or a dense 3-wire block
   bool wide_bwd = false;        // backward program built for 16 amplitudes x 512 threads (complex64)
+  int variant_flags = 0;        // JQ_TUNE_* kernel variants
   std::vector<int32_t> words_bj;  // backward program of the plan-specialised kernels
   DevBuf prog_bj;
   bool uploaded = false;
@@ -259,6 +260,8 @@ void jit_build(jq_plan* p) {
   spec.words_f = &p->hp.words;
   spec.words_b = &p->words_bj;
   spec.dtype = p->dtype;
+  spec.persist_bwd = !(p->variant_flags & JQ_TUNE_NO_PERSIST_BWD);
+  spec.dma = !(p->variant_flags & JQ_TUNE_NO_DMA_FWD);
   p->jit.dma = spec.dma && p->dtype == 0;
   p->jit.persist[0] = false;
   p->jit.persist[1] = spec.persist_bwd;
@@ -459,6 +462,11 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
     delete p;
     return fail(JQ_ERR_ARG, e.what());
   }
+  if (desc->variant_flags & ~(JQ_TUNE_NO_PERSIST_BWD | JQ_TUNE_NO_DMA_FWD | JQ_TUNE_NARROW_BWD)) {
+    delete p;
+    return fail(JQ_ERR_ARG, "unknown variant_flags bits");
+  }
+  p->variant_flags = desc->variant_flags;
   p->dtype = desc->dtype;
   p->n_params = desc->n_params;
   p->n_matrix_elems = desc->n_matrix_elems;
@@ -523,7 +531,9 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
       delete p;
       return fail(JQ_ERR_ARG, "internal: wide backward program changed the pass structure");
     }
-    p->words_bj = wide.words_b;
+    // (JQ_TUNE_NARROW_BWD keeps the 8-amplitude program for the specialised
+    // backward too, unless 4x4 / dense 3-wire blocks need the wide one)
+    if (!(p->variant_flags & JQ_TUNE_NARROW_BWD) || any_full) p->words_bj = wide.words_b;
     if (any_full) {
       p->hp.words_b = wide.words_b;
       p->wide_bwd = true;

@@ -240,7 +240,9 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
   // gates and cut passes (the 24-qubit HEA: 24 passes with 7 fixed bits, 21
   // with 5); measured on MI355X, 5 is the best trade: 4 and 3 cost more in
   // access efficiency than they save in passes.
-  const int lowfix = std::min(n, std::min(T, T >= 13 ? 5 : 6));
+  const int lowdef = T >= 13 ? 5 : 6;
+  if (d.low_bits != 0 && (d.low_bits < 3 || d.low_bits > 6)) throw PlanError(JQ_ERR_ARG, "low_bits must be 0 or 3..6");
+  const int lowfix = std::min(n, std::min(T, d.low_bits ? d.low_bits : lowdef));
   // Global bits >= n are phantom (n < T): their amplitudes do not exist.
 
   out = HostProgram();

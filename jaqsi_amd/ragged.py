@@ -27,12 +27,15 @@ class RaggedBatch:
     single-wire diagonals given as (wire, d0, d1) (PauliZ: (q, 1, -1))."""
 
     def __init__(self, tapes: Sequence[list], batch: Sequence[int], n_qubits: int,
-                 diags: Sequence[Tuple[int, float, float]], dtype: Optional[int] = None):
+                 diags: Sequence[Tuple[int, float, float]], dtype: Optional[int] = None,
+                 knobs: Optional[dict] = None):
         self.n_qubits = n_qubits
         self.dtype = _dtype_code() if dtype is None else dtype
         self.batch = list(batch)
         self.lowered = [lowering.lower_pure(t, n_qubits) for t in tapes]
-        self.plans = [get_plan(low, self.dtype) for low in self.lowered]
+        # one set of tuning knobs for every group (given, or the autotuner's
+        # cached choice for each group's own structure)
+        self.plans = [get_plan(low, self.dtype, batch=B, knobs=knobs) for low, B in zip(self.lowered, self.batch)]
         self.n_params = [len(low.angle_values) for low in self.lowered]
         self.diags = list(diags)
         self.wires = np.array([d[0] for d in diags], dtype=np.int32)

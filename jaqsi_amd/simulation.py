@@ -81,13 +81,21 @@ def _cdtype_torch(code: int) -> torch.dtype:
     return torch.complex128 if code else torch.complex64
 
 
-def get_plan(low: lowering.Lowered, dtype: int, tile_bits: int = 0, checkpoints: int = 1) -> nat.Plan:
-    key = (low.key, dtype, tile_bits, checkpoints)
+def get_plan(low: lowering.Lowered, dtype: int, tile_bits: int = 0, checkpoints: int = 1, batch: int = 1,
+             density: bool = False, knobs: Optional[dict] = None) -> nat.Plan:
+    """The cached plan of a lowered structure; its tuning knobs come from the
+    autotuner's cache for (structure, dtype, batch) unless given."""
+    if knobs is None:
+        from jaqsi_amd import autotune
+
+        knobs = autotune.config_for(low, dtype, batch, density)
+    lb, vf = int(knobs.get("low_bits", 0)), int(knobs.get("variant_flags", 0))
+    key = (low.key, dtype, tile_bits, checkpoints, lb, vf)
     with _plans_lock:
         p = _plans.get(key)
         if p is None:
             p = nat.Plan(low.n_wires, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, dtype,
-                         tile_bits, checkpoints)
+                         tile_bits, checkpoints, lb, vf)
             if _jit_mode is not None:
                 p.jit(_jit_mode)
             if len(_plans) >= _PLAN_CACHE_MAX:
@@ -456,7 +464,7 @@ def _prepare(tape, n_qubits, use_density, batch, initial_state, dtype, values=No
         if psi0.dim() == 2 and psi0.shape[0] > 1:
             psi0_batched = True
         psi0 = psi0.reshape(psi0.shape[0] if psi0_batched else 1, -1).resolve_conj().contiguous()
-    plan = get_plan(low, dtype)
+    plan = get_plan(low, dtype, batch=angles.shape[0], density=use_density)
     return low, plan, angles, mats, psi0, psi0_batched
 
 

@@ -323,3 +323,39 @@ def test_singular_and_ill_conditioned_channels_are_never_inverted(dtype):
     v, g_ref = dm.value_and_shift_grad(items, n, [(sv.PZ, (0,)), (sv.PZ, (1,))], np.array([1.0, 1.0]))
     assert np.all(np.isfinite(grads))
     np.testing.assert_allclose(grads, g_ref, atol=1e-12)
+
+
+@pytest.mark.parametrize("low_bits,dtype", [(3, 0), (4, 0), (6, 0), (4, 1), (5, 1)])
+def test_low_bits_knob_keeps_results(low_bits, dtype):
+    """The autotuner's low_bits knob changes the tiles (the lowest low_bits
+    global bits in every tile) and the pass count, not the results."""
+    from circuits import record_hea
+    from jaqsi_amd import _native as nat, lowering
+    from plan_interp import Program
+
+    n, layers = 16, 2
+    theta = np.random.default_rng(9).uniform(0, 2 * np.pi, (layers, n, 3))
+    low = lowering.lower_pure(_record(lambda: record_hea(theta, n)), n)
+    plan = nat.Plan(n, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, dtype, low_bits=low_bits)
+    w = plan.program()
+    T = int(w[2])
+    for q in range(plan.info()["n_passes"]):
+        rec = int(w[16 + q])
+        assert [int(x) for x in w[rec + 4: rec + 4 + low_bits]] == list(range(low_bits))
+    prog = Program(plan)
+    angles = np.array([float(a) for a in low.angle_values])
+    ops = sv.hea_ops(theta, n)
+    np.testing.assert_allclose(prog.forward(angles, None), sv.simulate(ops, n), atol=1e-12)
+    zs = sv.z_signs(n).sum(0)
+    _, _, grads, _ = prog.adjoint(angles, lambda f: 2 * zs * np.conj(f), None)
+    _, og, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
+    np.testing.assert_allclose(grads, sv.flat_grads(ops, og), atol=1e-12)
+
+
+def test_tuning_knobs_are_validated():
+    from jaqsi_amd import _native as nat, lowering
+
+    low = lowering.lower_pure(_record(lambda: __import__("jaqsi_amd").Gates.RX(0.1, wires=0)), 2)
+    for kw in ({"low_bits": 2}, {"low_bits": 7}, {"variant_flags": 8}):
+        with pytest.raises(ValueError):
+            nat.Plan(2, low.ops, low.members, 1, low.n_matrix_elems, 0, **kw)
