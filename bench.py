@@ -268,7 +268,7 @@ def bench_c3(steps, warmup, B=4096, n=14, seed=1003):
     torch.cuda.current_stream().wait_stream(side)
     torch.cuda.synchronize()
     graph = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(graph):
+    with torch.cuda.graph(graph, stream=side):  # the stream whose workspaces the warm-up sized
         gevs, ggrads = rb.adjoint(angles, cts)
     graph.replay()
     torch.cuda.synchronize()

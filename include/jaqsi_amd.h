@@ -136,7 +136,10 @@ typedef struct jq_plan_desc {
 typedef struct jq_plan jq_plan;
 
 /* Lower, fuse and schedule one circuit structure; upload its program.
- * Replaces simulation._compile/_compile_mixed (simulation.py:338-381). */
+ * Replaces simulation._compile/_compile_mixed (simulation.py:338-381).
+ * A plan keeps one set of workspaces (block matrices, checkpoints, partials)
+ * per stream it has run on, so calls on different streams may overlap; the
+ * sets live until jq_plan_destroy. */
 int jq_plan_create(const jq_plan_desc* desc, jq_plan** out);
 int jq_plan_destroy(jq_plan* plan);
 

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <atomic>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <unordered_map>
 #include <cstdio>
@@ -176,6 +177,17 @@ struct JitState {
   int nt[2] = {0, 0};
 };
 
+// Buffers a call works in, and the token of the checkpoints the last
+// jq_sv_forward_ckpt on this stream left: valid while no other call has run
+// on the plan on this stream since (every call that touches the workspaces
+// advances `generation`).
+struct Workspace {
+  DevBuf pool, partials, bars, ev_part, lam, ckpt, obs, fstate, groups;
+  int64_t generation = 0;
+  int64_t ckpt_token = -1;
+  int ckpt_batch = 0;
+};
+
 struct jq_plan {
   jq::HostProgram hp;
   int dtype = 0;
@@ -198,15 +210,20 @@ struct jq_plan {
   std::vector<jq::WideGeom> wide_geom;       // its launch geometry
   bool any_wide = false;
   JitState jit;
-  // Checkpoints of the last jq_sv_forward_ckpt: valid while no other call
-  // has run on the plan since (every call that touches the workspaces
-  // advances `generation`).
-  int64_t generation = 0;
-  int64_t ckpt_token = -1;
-  int ckpt_batch = 0;
-  // workspaces
-  DevBuf pool, partials, bars, ev_part, lam, ckpt, obs, fstate, groups;
+  // Workspaces, one set per stream the plan runs on, so calls on different
+  // streams never share buffers (a plan is re-entrant per stream; calls on
+  // one stream are ordered by the stream itself).  Created on a stream's
+  // first call, freed with the plan.
+  std::mutex ws_mu;
+  std::map<hipStream_t, std::unique_ptr<Workspace>> ws_map;
 };
+
+Workspace& WS(jq_plan* p, hipStream_t st) {
+  std::lock_guard<std::mutex> lock(p->ws_mu);
+  std::unique_ptr<Workspace>& w = p->ws_map[st];
+  if (!w) w.reset(new Workspace());
+  return *w;
+}
 
 namespace {
 
@@ -591,8 +608,13 @@ int jq_plan_destroy(jq_plan* p) {
   if (!p) return JQ_OK;
   if (p->jit.worker.joinable()) p->jit.worker.join();
   for (DevBuf* b : {&p->prog, &p->prog_b, &p->prog_bj, &p->block_table, &p->member_list, &p->member_table, &p->member_values, &p->slot_map,
-                    &p->pool, &p->partials, &p->bars, &p->ev_part, &p->lam, &p->ckpt, &p->obs, &p->fstate, &p->groups, &p->wg_list})
+                    &p->wg_list})
     b->release();
+  for (auto& kv : p->ws_map) {
+    Workspace& w = *kv.second;
+    for (DevBuf* b : {&w.pool, &w.partials, &w.bars, &w.ev_part, &w.lam, &w.ckpt, &w.obs, &w.fstate, &w.groups})
+      b->release();
+  }
   delete p;
   return JQ_OK;
 }
@@ -687,8 +709,8 @@ int fill_obs(jq_plan* p, jq::SvArgs& a, int n_wires, int obs_kind, int n_obs, co
     t.d[2 * o + 1] = obs_d[2 * o + 1];
   }
   if (!table) {
-    HIP_TRY(p->obs.ensure(sizeof(jq::ObsTable), st));
-    table = p->obs.p;
+    HIP_TRY(WS(p, st).obs.ensure(sizeof(jq::ObsTable), st));
+    table = WS(p, st).obs.p;
   }
   HIP_TRY(jq::launch_obs_upload(t, table, st));
   a.obs = table;
@@ -700,8 +722,8 @@ int run_prep(jq_plan* p, int batch, const double* angles, const void* matrices, 
              void* pool = nullptr) {
   const size_t eb = elem_bytes(p->dtype);
   if (!pool) {
-    HIP_TRY(p->pool.ensure(std::max<size_t>(1, (size_t)p->hp.pool * batch) * eb, st));
-    pool = p->pool.p;
+    HIP_TRY(WS(p, st).pool.ensure(std::max<size_t>(1, (size_t)p->hp.pool * batch) * eb, st));
+    pool = WS(p, st).pool.p;
   }
   jq::PrepArgs pa;
   pa.n_blocks = p->hp.n_dev_blocks;
@@ -725,13 +747,13 @@ int run_prep(jq_plan* p, int batch, const double* angles, const void* matrices, 
   return JQ_OK;
 }
 
-jq::SvArgs base_args(jq_plan* p) {
+jq::SvArgs base_args(jq_plan* p, hipStream_t st) {
   jq::SvArgs a;
   std::memset(&a, 0, sizeof(a));
   a.prog = (const int*)p->prog.p;
   a.n_wires = p->hp.n_wires;
   a.vec_len = 1ull << p->hp.n_wires;
-  a.mats = p->pool.p;
+  a.mats = WS(p, st).pool.p;
   a.pool_stride = p->hp.pool;
   return a;
 }
@@ -760,14 +782,14 @@ int finish_grads(jq_plan* p, int batch, const double* angles, const void* matric
   if (grad_matrices && p->n_matrix_elems > 0)
     HIP_TRY(jq::launch_zero(grad_matrices, (size_t)batch * p->n_matrix_elems * 16, st));
   if (nreals > 0 && (grad_angles || grad_matrices)) {
-    HIP_TRY(p->bars.ensure((size_t)batch * nreals * 8, st));
+    HIP_TRY(WS(p, st).bars.ensure((size_t)batch * nreals * 8, st));
     jq::ReduceArgs r;
     r.slot_map = (const int*)p->slot_map.p;
     r.n_slots = p->n_slots;
     r.partials = grad_part;
     r.part_stride = part_stride;
     r.ntiles = ntiles;
-    r.bars = (double*)p->bars.p;
+    r.bars = (double*)WS(p, st).bars.p;
     r.n_reals = nreals;
     r.batch = batch;
     HIP_TRY(jq::launch_reduce_grad(r, st));
@@ -782,7 +804,7 @@ int finish_grads(jq_plan* p, int batch, const double* angles, const void* matric
     c.n_params = p->n_params;
     c.matrices = matrices;
     c.n_matrix_elems = p->n_matrix_elems;
-    c.bars = (const double*)p->bars.p;
+    c.bars = (const double*)WS(p, st).bars.p;
     c.bars_stride = p->hp.bars;
     c.grad_angles = grad_angles;
     c.grad_matrices = grad_matrices;
@@ -796,9 +818,9 @@ int finish_grads(jq_plan* p, int batch, const double* angles, const void* matric
 
 // Output of pass q: the checkpoint slice when checkpoints are kept, else the
 // final vector (passes then run in place).
-void* pass_out(jq_plan* p, bool ck, void* state, size_t vbytes, int q) {
+void* pass_out(jq_plan* p, bool ck, void* state, size_t vbytes, int q, hipStream_t st) {
   if (q == p->hp.n_passes - 1 || !ck) return state;
-  return (char*)p->ckpt.p + vbytes * (size_t)q;
+  return (char*)WS(p, st).ckpt.p + vbytes * (size_t)q;
 }
 
 // Forward passes of one call (prep, passes, expectation values).
@@ -807,27 +829,27 @@ int run_forward(jq_plan* p, int batch, const double* angles, const void* matrice
                 double* expvals, bool ck, hipStream_t st) {
   int rc = ensure_uploaded(p);
   if (rc) return rc;
-  ++p->generation;  // the workspaces (pool, checkpoints) change from here on
+  ++WS(p, st).generation;  // the workspaces (pool, checkpoints) change from here on
   rc = run_prep(p, batch, angles, matrices, st);
   if (rc) return rc;
-  jq::SvArgs a = base_args(p);
+  jq::SvArgs a = base_args(p, st);
   rc = fill_obs(p, a, p->hp.n_wires, expvals ? obs_kind : JQ_OBS_NONE, n_obs, obs_wires, obs_d, st);
   if (rc) return rc;
   const int ntiles = p->hp.ntiles;
   const int P = p->hp.n_passes;
   if (a.obs_kind != JQ_OBS_NONE) {
-    HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * a.n_obs) * 8, st));
-    a.ev_part = (double*)p->ev_part.p;
+    HIP_TRY(WS(p, st).ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * a.n_obs) * 8, st));
+    a.ev_part = (double*)WS(p, st).ev_part.p;
   }
   const size_t vbytes = (size_t)a.vec_len * elem_bytes(p->dtype) * batch;
   ck = ck && P > 1;
-  if (ck) HIP_TRY(p->ckpt.ensure(vbytes * (size_t)(P - 1), st));
+  if (ck) HIP_TRY(WS(p, st).ckpt.ensure(vbytes * (size_t)(P - 1), st));
   const bool jit = jit_ready(p);
   for (int q = 0; q < P; ++q) {
     a.pass = q;
-    a.src = q == 0 ? psi0 : pass_out(p, ck, state, vbytes, q - 1);
+    a.src = q == 0 ? psi0 : pass_out(p, ck, state, vbytes, q - 1, st);
     a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
-    a.dst = pass_out(p, ck, state, vbytes, q);
+    a.dst = pass_out(p, ck, state, vbytes, q, st);
     a.dst_stride = (int64_t)a.vec_len;
     HIP_TRY(timed_launch(0, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, smem_for(p, 0), st, false,
                          jit_fn(p, jit, 0, q), q == 0, q == P - 1));
@@ -842,7 +864,7 @@ int run_backward(jq_plan* p, int batch, const double* angles, const void* matric
                  int psi0_batched, void* state, int obs_kind, int n_obs, const int32_t* obs_wires,
                  const double* obs_d, const double* ct, void* lam_seed, double* grad_angles, void* grad_matrices,
                  void* grad_psi0, bool ck, hipStream_t st) {
-  jq::SvArgs a = base_args(p);
+  jq::SvArgs a = base_args(p, st);
   if (lam_seed) {
     a.lam_given = 1;
   } else {
@@ -854,11 +876,11 @@ int run_backward(jq_plan* p, int batch, const double* angles, const void* matric
   const int P = p->hp.n_passes;
   const size_t vbytes = (size_t)a.vec_len * elem_bytes(p->dtype) * batch;
   ck = ck && P > 1;
-  HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * std::max(1, a.n_obs)) * 8, st));
-  HIP_TRY(p->partials.ensure(std::max<size_t>(1, (size_t)batch * p->hp.partial_reals) * 8, st));
-  a.grad_part = (double*)p->partials.p;
+  HIP_TRY(WS(p, st).ev_part.ensure(std::max<size_t>(1, (size_t)batch * ntiles * std::max(1, a.n_obs)) * 8, st));
+  HIP_TRY(WS(p, st).partials.ensure(std::max<size_t>(1, (size_t)batch * p->hp.partial_reals) * 8, st));
+  a.grad_part = (double*)WS(p, st).partials.p;
   a.part_stride = p->hp.partial_reals;
-  a.ev_part = (double*)p->ev_part.p;
+  a.ev_part = (double*)WS(p, st).ev_part.p;
   // the running adjoint vector ends as the initial-state cotangent
   void* lam = grad_psi0;
   if (lam_seed && grad_psi0) {
@@ -866,8 +888,8 @@ int run_backward(jq_plan* p, int batch, const double* angles, const void* matric
   } else if (lam_seed) {
     lam = lam_seed;
   } else if (!lam) {
-    HIP_TRY(p->lam.ensure(vbytes, st));
-    lam = p->lam.p;
+    HIP_TRY(WS(p, st).lam.ensure(vbytes, st));
+    lam = WS(p, st).lam.p;
   }
   a.lam = lam;
   a.lam_stride = (int64_t)a.vec_len;
@@ -877,11 +899,11 @@ int run_backward(jq_plan* p, int batch, const double* angles, const void* matric
   for (int q = P - 1; q >= 0; --q) {
     a.pass = q;
     if (p->pass_wide[q]) {  // reads the pass's input state
-      a.src = q == 0 ? psi0 : pass_out(p, ck, state, vbytes, q - 1);
+      a.src = q == 0 ? psi0 : pass_out(p, ck, state, vbytes, q - 1, st);
       a.src_stride = q == 0 ? (psi0_batched ? (int64_t)a.vec_len : 0) : (int64_t)a.vec_len;
       a.dst = nullptr;
     } else {
-      a.src = pass_out(p, ck, state, vbytes, q);
+      a.src = pass_out(p, ck, state, vbytes, q, st);
       a.src_stride = (int64_t)a.vec_len;
       a.dst = ck ? nullptr : state;  // without checkpoints psi is un-applied in place
     }
@@ -930,8 +952,8 @@ int jq_sv_forward_noalloc(jq_plan* p, int32_t batch, const double* angles, const
   char* base = (char*)scr;
   int rc = run_prep(p, batch, angles, matrices, st, base + off_pool);
   if (rc) return rc;
-  ++p->generation;  // the plan's own checkpoints no longer match its pool
-  jq::SvArgs a = base_args(p);
+  ++WS(p, st).generation;  // the plan's own checkpoints no longer match its pool
+  jq::SvArgs a = base_args(p, st);
   a.mats = base + off_pool;
   rc = fill_obs(p, a, p->hp.n_wires, expvals ? obs_kind : JQ_OBS_NONE, n_obs, obs_wires, obs_d, st, base);
   if (rc) return rc;
@@ -975,12 +997,14 @@ int jq_sv_forward_ckpt(jq_plan* p, int32_t batch, const double* angles, const vo
   if (!p || !state || batch < 1 || !token) return fail(JQ_ERR_ARG, "bad arguments to jq_sv_forward_ckpt");
   if (batch > 65535) return fail(JQ_ERR_ARG, "batch above 65535 per call");
   *token = -1;
+  hipStream_t st = (hipStream_t)stream;
   int rc = run_forward(p, batch, angles, matrices, psi0, psi0_batched, state, obs_kind, n_obs, obs_wires, obs_d,
-                       expvals, true, (hipStream_t)stream);
+                       expvals, true, st);
   if (rc) return rc;
-  p->ckpt_token = p->generation;
-  p->ckpt_batch = batch;
-  *token = p->ckpt_token;
+  Workspace& w = WS(p, st);
+  w.ckpt_token = w.generation;
+  w.ckpt_batch = batch;
+  *token = w.ckpt_token;
   return JQ_OK;
 }
 
@@ -993,13 +1017,14 @@ int jq_sv_backward(jq_plan* p, int64_t token, int32_t batch, const double* angle
   if (!lam_seed && (obs_kind == JQ_OBS_NONE || !ct))
     return fail(JQ_ERR_ARG, "jq_sv_backward needs observables and their cotangents, or lam_seed");
   hipStream_t st = (hipStream_t)stream;
-  const bool valid = token >= 0 && token == p->ckpt_token && token == p->generation && batch == p->ckpt_batch;
-  if (!valid) {  // another call has used the plan since: run the forward again
+  Workspace& w = WS(p, st);
+  const bool valid = token >= 0 && token == w.ckpt_token && token == w.generation && batch == w.ckpt_batch;
+  if (!valid) {  // another call has used the plan on this stream since: run the forward again
     int rc = run_forward(p, batch, angles, matrices, psi0, psi0_batched, state, obs_kind, n_obs, obs_wires, obs_d,
                          nullptr, true, st);
     if (rc) return rc;
-    p->ckpt_token = p->generation;
-    p->ckpt_batch = batch;
+    w.ckpt_token = w.generation;
+    w.ckpt_batch = batch;
   }
   return run_backward(p, batch, angles, matrices, psi0, psi0_batched, state, obs_kind, n_obs, obs_wires, obs_d, ct,
                       lam_seed, grad_angles, grad_matrices, grad_psi0, true, st);
@@ -1075,7 +1100,7 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
   // groups with more passes first: the circuits still active at a step are a prefix
   std::stable_sort(order.begin(), order.end(),
                    [&](int x, int y) { return groups[x].plan->hp.n_passes > groups[y].plan->hp.n_passes; });
-  jq::SvArgs a = base_args(p0);
+  jq::SvArgs a = base_args(p0, st);
   int rc = fill_obs(p0, a, p0->hp.n_wires, obs_kind, n_obs, obs_wires, obs_d, st);
   if (rc) return rc;
   const size_t eb = elem_bytes(p0->dtype);
@@ -1089,28 +1114,28 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
     jq_plan* p = G.plan;
     rc = ensure_uploaded(p);
     if (rc) return rc;
-    ++p->generation;
+    ++WS(p, st).generation;
     rc = run_prep(p, G.batch, G.angles, G.matrices, st);
     if (rc) return rc;
     const int P = p->hp.n_passes;
     const size_t vbytes = (size_t)a.vec_len * eb * G.batch;
-    HIP_TRY(p->ckpt.ensure(std::max<size_t>(1, vbytes * (size_t)std::max(0, P - 1)), st));
-    HIP_TRY(p->lam.ensure(vbytes, st));
-    HIP_TRY(p->ev_part.ensure(std::max<size_t>(1, (size_t)G.batch * ntiles * std::max(1, n_obs)) * 8, st));
-    HIP_TRY(p->partials.ensure(std::max<size_t>(1, (size_t)G.batch * p->hp.partial_reals) * 8, st));
+    HIP_TRY(WS(p, st).ckpt.ensure(std::max<size_t>(1, vbytes * (size_t)std::max(0, P - 1)), st));
+    HIP_TRY(WS(p, st).lam.ensure(vbytes, st));
+    HIP_TRY(WS(p, st).ev_part.ensure(std::max<size_t>(1, (size_t)G.batch * ntiles * std::max(1, n_obs)) * 8, st));
+    HIP_TRY(WS(p, st).partials.ensure(std::max<size_t>(1, (size_t)G.batch * p->hp.partial_reals) * 8, st));
     jq::SvGroup& t = table[k];
     std::memset(&t, 0, sizeof(t));
     t.prog_f = (const int*)p->prog.p;
     t.prog_b = (const int*)p->prog_b.p;
-    t.mats = p->pool.p;
+    t.mats = WS(p, st).pool.p;
     t.pool_stride = p->hp.pool;
     t.psi0 = G.psi0;
     t.psi0_stride = G.psi0_batched ? (int64_t)a.vec_len : 0;
-    t.ckpt = p->ckpt.p;
+    t.ckpt = WS(p, st).ckpt.p;
     t.state = G.state;
-    t.lam = p->lam.p;
-    t.ev_part = (double*)p->ev_part.p;
-    t.grad_part = (double*)p->partials.p;
+    t.lam = WS(p, st).lam.p;
+    t.ev_part = (double*)WS(p, st).ev_part.p;
+    t.grad_part = (double*)WS(p, st).partials.p;
     t.part_stride = p->hp.partial_reals;
     t.ct = G.ct;
     t.n_passes = P;
@@ -1120,9 +1145,9 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
     maxP = std::max(maxP, P);
     smem_b = std::max(smem_b, smem_for(p, 2 * max_slots(p), 2));
   }
-  HIP_TRY(p0->groups.ensure(sizeof(jq::SvGroup) * n_groups, st));
-  HIP_TRY(jq::launch_bytes_upload(table.data(), sizeof(jq::SvGroup) * n_groups, p0->groups.p, st));
-  a.groups = (const jq::SvGroup*)p0->groups.p;
+  HIP_TRY(WS(p0, st).groups.ensure(sizeof(jq::SvGroup) * n_groups, st));
+  HIP_TRY(jq::launch_bytes_upload(table.data(), sizeof(jq::SvGroup) * n_groups, WS(p0, st).groups.p, st));
+  a.groups = (const jq::SvGroup*)WS(p0, st).groups.p;
   a.n_groups = n_groups;
   // rows active at step s: groups k with n_passes > s (a prefix of the order)
   auto active_rows = [&](int s, bool& full, int dir) -> int {
@@ -1149,7 +1174,7 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
   for (int k = 0; k < n_groups; ++k) {
     const jq_group& G = groups[order[k]];
     jq_plan* p = G.plan;
-    if (G.expvals) HIP_TRY(jq::launch_reduce_ev((double*)p->ev_part.p, G.expvals, ntiles, n_obs, G.batch, st));
+    if (G.expvals) HIP_TRY(jq::launch_reduce_ev((double*)WS(p, st).ev_part.p, G.expvals, ntiles, n_obs, G.batch, st));
   }
   for (int s = 0; s < maxP; ++s) {
     bool full;
@@ -1162,7 +1187,7 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
   for (int k = 0; k < n_groups; ++k) {
     const jq_group& G = groups[order[k]];
     jq_plan* p = G.plan;
-    rc = finish_grads(p, G.batch, G.angles, G.matrices, (double*)p->partials.p, p->hp.partial_reals, G.grad_angles,
+    rc = finish_grads(p, G.batch, G.angles, G.matrices, (double*)WS(p, st).partials.p, p->hp.partial_reals, G.grad_angles,
                       nullptr, st);
     if (rc) return rc;
   }

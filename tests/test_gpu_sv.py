@@ -146,7 +146,7 @@ def test_lowered_entry_points_for_a_custom_vjp_bridge():
         record_hea(theta, n)
     obs = [J.PauliZ(wires=q, record=False) for q in range(n)]
     lowered, params = S.lower_for_jax(tape, n, False)
-    a = torch.tensor([float(p) for p in params], device="cuda")
+    a = torch.tensor([float(p) for p in params], dtype=torch.float64, device="cuda")
     ev = S.run_lowered(lowered, a, obs).cpu().numpy()
     ct = torch.arange(1, n + 1, dtype=torch.float64, device="cuda")
     g = S.adjoint_lowered(lowered, a, obs, ct).cpu().numpy()

@@ -65,10 +65,10 @@ def test_sv_adjoint_replays_from_a_graph(specialised):
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
-        call(side.cuda_stream)  # workspaces are sized before the capture
+        call(side.cuda_stream)  # this stream's workspaces are sized before the capture
     torch.cuda.current_stream().wait_stream(side)
     torch.cuda.synchronize()
-    with torch.cuda.graph(g):
+    with torch.cuda.graph(g, stream=side):  # the stream whose workspaces the warm-up sized
         call(torch.cuda.current_stream().cuda_stream)
     for rep in range(2):
         for k, th in enumerate(thetas):
@@ -115,7 +115,7 @@ def test_ragged_groups_replay_from_a_graph(specialised):
     torch.cuda.current_stream().wait_stream(side)
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
+    with torch.cuda.graph(g, stream=side):  # the stream whose workspaces the warm-up sized
         evs, grads = rb.adjoint(angles, cts)
     for rep in range(2):
         for k, s in enumerate(sets):
