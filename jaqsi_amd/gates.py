@@ -25,14 +25,21 @@ from jaqsi_amd.unitary import UnitaryGates
 log = logging.getLogger(__name__)
 
 
+_PLAIN_KWARGS = frozenset(("wires",))
+
+
 class _GateHandler:
     """What ``Gates.<name>`` returns (gates.py:22-35)."""
 
     def __init__(self, gates, gate_name: str):
         self._gates = gates
         self.__name__ = gate_name
+        # the unitary gate of this name, called directly when only wires are given
+        self._plain = None if gate_name == "Barrier" else getattr(UnitaryGates, gate_name, None)
 
     def __call__(self, *args, **kwargs):
+        if self._plain is not None and kwargs.keys() <= _PLAIN_KWARGS:
+            return self._plain(*args, **kwargs)
         return self._gates._inner_getattr(self.__name__, *args, **kwargs)
 
 
@@ -52,9 +59,6 @@ def _no_op(*args, **kwargs):
 
 def Barrier(wires: Union[int, List[int]], *args, **kwargs):
     return BarrierOp(wires)
-
-
-_PLAIN_KWARGS = frozenset(("wires",))
 
 
 class Gates(metaclass=GatesMeta):

@@ -156,15 +156,65 @@ class _Builder:
                        self.grad_mats)
 
 
-def lower_pure(tape: Sequence[Operation], n_qubits: int) -> Lowered:
-    """Statevector lowering: one jq_op per recorded gate."""
+# Gate classes whose lowering depends only on the class, the wires and the
+# Pauli word / control count: their op and member records are built once per
+# such signature and replayed with the gate's parameters (``lower_pure``).
+_STATIC_LOWERING = frozenset((
+    "PauliX", "PauliY", "PauliZ", "H", "S", "SWAP", "RX", "RY", "RZ", "CX", "CY", "CZ", "CCX", "CSWAP",
+    "ControlledPhaseShift", "Rot", "PauliRot", "RXX", "RYY", "RZZ", "RZX", "ControlledPauliRot", "CRX", "CRY", "CRZ",
+))
+_fragments: dict = {}
+
+
+def _fragment(op: Operation):
+    """(key entries, op record, member records, matrices) of one gate with
+    parameter slots and matrix offsets relative to the gate, or None when
+    the gate is not of a static class (or carries its own matrix)."""
+    cls = type(op)
+    if cls.__name__ not in _STATIC_LOWERING or cls.__module__ != "jaqsi_amd.gateset" or getattr(op, "_explicit", False):
+        return None
+    sig = (cls, tuple(op._wires), getattr(op, "pauli_word", None), getattr(op, "n_controls", None))
+    frag = _fragments.get(sig)
+    if frag is None:
+        b = _Builder(0)
+        flags, controls, perm = _structure_flags(op)
+        b.add_op(op.wires, op.lower(), flags, controls, perm)
+        if b.grad_mats:
+            return None
+        ids = {id(v): i for i, v in enumerate(b.angles)}
+        slots = [ids.get(id(v)) for v in op.parameters]
+        if slots != list(range(len(b.angles))):
+            return None  # the members do not take the parameters in order
+        frag = (tuple(b.key), b.ops[0], tuple(b.members), tuple(b.mats), b.n_mat)
+        if len(_fragments) > 4096:
+            _fragments.clear()
+        _fragments[sig] = frag
+    return frag
+
+
+def lower_pure(tape: Sequence[Operation], n_qubits: int, fast: bool = True) -> Lowered:
+    """Statevector lowering: one jq_op per recorded gate (gates of static
+    classes replay their cached records)."""
     b = _Builder(n_qubits)
     b.key.append(("sv", n_qubits))
     for op in tape:
         if isinstance(op, Barrier):
             continue
-        flags, controls, perm = _structure_flags(op)
-        b.add_op(op.wires, op.lower(), flags, controls, perm)
+        frag = _fragment(op) if fast else None
+        if frag is None:
+            flags, controls, perm = _structure_flags(op)
+            b.add_op(op.wires, op.lower(), flags, controls, perm)
+            continue
+        key, rec, members, mats, n_mat = frag
+        p0, m0, mb = len(b.angles), b.n_mat, len(b.members)
+        b.angles.extend(op.parameters)
+        for off, val in mats:
+            b.mats.append((m0 + off, val))
+        b.n_mat += n_mat
+        for k, f, nc, w, prm, mat, v in members:
+            b.members.append((k, f, nc, w, p0 + prm if prm >= 0 else -1, m0 + mat if mat >= 0 else -1, v))
+        b.ops.append((rec[0], rec[1], rec[2], rec[3], mb, mb + len(members), rec[6]))
+        b.key.extend(key)
     return b.result()
 
 

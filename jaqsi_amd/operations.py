@@ -19,6 +19,7 @@ from typing import Callable, List, NamedTuple, Optional, Tuple, Union
 import numpy as np
 import torch
 
+from jaqsi_amd.tape import _local as _tape_local
 from jaqsi_amd.tape import active_tape, recording  # noqa: F401  (re-export)
 
 _X64 = os.environ.get("JAQSI_AMD_X64", "0") not in ("0", "", "false", "False")
@@ -136,9 +137,9 @@ class Operation:
             self._structure = GateStructure()
             self._explicit = True
         if record:
-            tape = active_tape()
-            if tape is not None:
-                tape.append(self)
+            stack = getattr(_tape_local, "stack", None)
+            if stack:
+                stack[-1].append(self)
 
     # -- introspection -----------------------------------------------------
     @property

@@ -102,9 +102,14 @@ class UnitaryGates:
 
 
 def _rotation(cls_name: str):
+    cls = getattr(gateset, cls_name)
+
     def gate(w, wires, noise_params=None, random_key=None):
+        if noise_params is None:
+            cls(w, wires=wires)
+            return
         w, random_key = UnitaryGates.GateError(w, noise_params, random_key)
-        getattr(gateset, cls_name)(w, wires=wires)
+        cls(w, wires=wires)
         UnitaryGates.Noise(wires, noise_params)
 
     gate.__name__ = cls_name
@@ -112,9 +117,12 @@ def _rotation(cls_name: str):
 
 
 def _fixed(cls_name: str):
+    cls = getattr(gateset, cls_name)
+
     def gate(wires, noise_params=None, random_key=None):
-        getattr(gateset, cls_name)(wires=wires)
-        UnitaryGates.Noise(wires, noise_params)
+        cls(wires=wires)
+        if noise_params is not None:
+            UnitaryGates.Noise(wires, noise_params)
 
     gate.__name__ = cls_name
     return staticmethod(gate)

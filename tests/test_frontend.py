@@ -291,3 +291,27 @@ def test_naive_batched_comparator_matches_the_oracle():
         vals, grads, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
         np.testing.assert_allclose(f[b].item(), vals.sum(), atol=1e-6)
         np.testing.assert_allclose(th.grad[b].numpy().reshape(-1), sv.flat_grads(ops, grads)[n:], atol=1e-6)
+
+
+def test_cached_gate_lowering_matches_the_full_lowering():
+    """Gates of static classes replay cached records: the plan key, op and
+    member records, angle slots and matrices equal a full lowering (gate zoo
+    plus a tensor-parameterised ansatz with explicit-matrix gates)."""
+    import jaqsi_amd as J
+    from circuits import record_zoo, zoo_params
+    from jaqsi_amd import lowering
+    from jaqsi_amd.tape import recording
+
+    t = torch.randn(2, 4, 3, dtype=torch.float64, requires_grad=True)
+    for fn, n in ((lambda: record_zoo(zoo_params()), 5),
+                  (lambda: [J.Gates.RX(t[l, q, 0], wires=q) for l in range(2) for q in range(4)]
+                   + [J.Operation(wires=[0, 2], matrix=np.eye(4, dtype=complex)), J.Gates.CRY(t[0, 1, 2], wires=[3, 1])], 4)):
+        for _ in range(2):  # second pass hits the fragment cache
+            with recording() as tape:
+                fn()
+            fast, slow = lowering.lower_pure(tape, n), lowering.lower_pure(tape, n, fast=False)
+            assert fast.key == slow.key
+            assert fast._ops_raw == slow._ops_raw and fast._members_raw == slow._members_raw
+            assert all(a is b for a, b in zip(fast.angle_values, slow.angle_values))
+            assert len(fast.angle_values) == len(slow.angle_values)
+            assert [(o, id(v)) for o, v in fast.matrix_values] == [(o, id(v)) for o, v in slow.matrix_values]
