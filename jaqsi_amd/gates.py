@@ -50,6 +50,9 @@ class GatesMeta(type):
         h = cls._handlers.get(gate_name)
         if h is None:
             h = cls._handlers[gate_name] = _GateHandler(cls, gate_name)
+            # later lookups find the handler as a class attribute instead of
+            # failing the normal lookup first (which costs more than the gate)
+            type.__setattr__(cls, gate_name, h)
         return h
 
 

@@ -37,6 +37,27 @@ def word_code(word: str) -> int:
     return w
 
 
+class StructKey(tuple):
+    """The structure key of a lowered tape: a tuple that computes its hash
+    once.  Plan and tuning caches look it up on every execute; a repeated
+    circuit hands them the same key object, so a lookup neither re-hashes
+    nor compares the (thousands of) entries.  repr and equality are the
+    tuple's, so stored tuning records keep matching."""
+
+    def __hash__(self):
+        try:
+            return self._h
+        except AttributeError:
+            self._h = h = tuple.__hash__(self)
+            return h
+
+    def __eq__(self, other):
+        return self is other or tuple.__eq__(self, other)
+
+    def __ne__(self, other):
+        return not self.__eq__(other)
+
+
 class Lowered:
     """A lowered tape: the structure key (for plan caching), the C-ABI op and
     member records (built on first use: a cached plan never needs them), the
@@ -152,7 +173,7 @@ class _Builder:
         self.key.append(("op", tuple(wires), structure, controls, perm, kc))
 
     def result(self) -> Lowered:
-        return Lowered(tuple(self.key), self.n_wires, self.ops, self.members, self.angles, self.mats, self.n_mat,
+        return Lowered(StructKey(self.key), self.n_wires, self.ops, self.members, self.angles, self.mats, self.n_mat,
                        self.grad_mats)
 
 
@@ -192,19 +213,53 @@ def _fragment(op: Operation):
     return frag
 
 
+# Gate classes whose records depend on nothing but the class and the wires
+# (PauliRot / ControlledPauliRot also depend on their word and controls).
+_TAPE_CLASSES: Optional[frozenset] = None
+# (n_qubits, tape signature) -> records of a tape built only from those
+# classes; a repeated circuit then skips the per-gate lowering.
+_tapes: dict = {}
+_TAPES_MAX = 64
+
+
+def _tape_classes() -> frozenset:
+    global _TAPE_CLASSES
+    if _TAPE_CLASSES is None:
+        from jaqsi_amd import gateset
+
+        _TAPE_CLASSES = frozenset(getattr(gateset, c) for c in _STATIC_LOWERING
+                                  if c not in ("PauliRot", "ControlledPauliRot"))
+    return _TAPE_CLASSES
+
+
 def lower_pure(tape: Sequence[Operation], n_qubits: int, fast: bool = True) -> Lowered:
     """Statevector lowering: one jq_op per recorded gate (gates of static
-    classes replay their cached records)."""
+    classes replay their cached records; a tape of such gates seen before
+    reuses all of its records and only collects the new parameters)."""
+    sig = None
+    if fast:
+        sig = (n_qubits, tuple([(op.__class__, op._explicit, *op._wires) for op in tape]))
+        hit = _tapes.get(sig)
+        if hit is not None:
+            key, ops_raw, members_raw, mats, n_mat = hit
+            angles = [getattr(op, nm) for op in tape for nm in op._param_names]
+            return Lowered(key, n_qubits, ops_raw, members_raw, angles, mats, n_mat, [])
     b = _Builder(n_qubits)
     b.key.append(("sv", n_qubits))
+    classes = _tape_classes()
+    cacheable = sig is not None
     for op in tape:
         if isinstance(op, Barrier):
+            cacheable = False
             continue
         frag = _fragment(op) if fast else None
         if frag is None:
+            cacheable = False
             flags, controls, perm = _structure_flags(op)
             b.add_op(op.wires, op.lower(), flags, controls, perm)
             continue
+        if op.__class__ not in classes:
+            cacheable = False
         key, rec, members, mats, n_mat = frag
         p0, m0, mb = len(b.angles), b.n_mat, len(b.members)
         b.angles.extend(op.parameters)
@@ -215,7 +270,12 @@ def lower_pure(tape: Sequence[Operation], n_qubits: int, fast: bool = True) -> L
             b.members.append((k, f, nc, w, p0 + prm if prm >= 0 else -1, m0 + mat if mat >= 0 else -1, v))
         b.ops.append((rec[0], rec[1], rec[2], rec[3], mb, mb + len(members), rec[6]))
         b.key.extend(key)
-    return b.result()
+    low = b.result()
+    if cacheable and not b.grad_mats:
+        if len(_tapes) >= _TAPES_MAX:
+            _tapes.pop(next(iter(_tapes)))
+        _tapes[sig] = (low.key, low._ops_raw, low._members_raw, low.matrix_values, low.n_matrix_elems)
+    return low
 
 
 def lower_mixed(tape: Sequence[Operation], n_qubits: int) -> Lowered:

@@ -115,6 +115,7 @@ class Operation:
     is_clifford = False
     is_unitary = False
     _matrix = None
+    _explicit = False  # True on instances built from a given matrix
     _num_wires: Optional[int] = None
     _param_names: Tuple[str, ...] = ()
 

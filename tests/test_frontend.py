@@ -315,3 +315,44 @@ def test_cached_gate_lowering_matches_the_full_lowering():
             assert all(a is b for a, b in zip(fast.angle_values, slow.angle_values))
             assert len(fast.angle_values) == len(slow.angle_values)
             assert [(o, id(v)) for o, v in fast.matrix_values] == [(o, id(v)) for o, v in slow.matrix_values]
+
+
+def test_repeated_tape_reuses_records_with_new_parameters():
+    """A tape of class-and-wires gates seen before is served from the tape
+    cache: same key object (so plan lookups hash it once), same records, the
+    new tape's parameters; a different structure misses it, and a gate built
+    from an explicit matrix is lowered in full."""
+    import jaqsi_amd as J
+    from jaqsi_amd import lowering
+    from jaqsi_amd.gateset import RX
+    from jaqsi_amd.tape import recording
+
+    n = 4
+
+    def tape_of(t, swap=False, explicit=False):
+        with recording() as tape:
+            for q in range(n):
+                J.Gates.RY(t[q], wires=q)
+            for q in range(n):
+                J.Gates.CX(wires=[q, (q + 1) % n] if not swap else [(q + 1) % n, q])
+            if explicit:
+                RX(0.0, wires=0, matrix=np.eye(2, dtype=complex))
+            else:
+                RX(0.0, wires=0)
+        return tape
+
+    t1, t2 = torch.randn(n, dtype=torch.float64), torch.randn(n, dtype=torch.float64)
+    a = lowering.lower_pure(tape_of(t1), n)
+    tape2 = tape_of(t2)
+    b = lowering.lower_pure(tape2, n)
+    assert b.key is a.key and b._ops_raw is a._ops_raw and b._members_raw is a._members_raw
+    assert all(x is op.theta for x, op in zip(b.angle_values, tape2[:n] + tape2[-1:]))
+    full = lowering.lower_pure(tape_of(t2), n, fast=False)
+    assert b.key == full.key and hash(b.key) == hash(tuple(full.key))
+    assert b._ops_raw == full._ops_raw and b._members_raw == full._members_raw
+    assert [float(v) for v in b.angle_values] == [float(v) for v in full.angle_values]
+    c = lowering.lower_pure(tape_of(t1, swap=True), n)
+    assert c.key != a.key
+    te = tape_of(t1, explicit=True)
+    e, ef = lowering.lower_pure(te, n), lowering.lower_pure(te, n, fast=False)
+    assert e.key == ef.key and e._ops_raw == ef._ops_raw and e._members_raw == ef._members_raw
