@@ -101,8 +101,8 @@ def build_plan(n, layers, features=False, tile_bits=0):
         record_hea(theta, n, np.zeros(n) if features else None)
     low = lowering.lower_pure(tape, n)
     knobs = autotune.config_for(low, 0, 1, jit=USE_JIT)
-    plan = nat.Plan(n, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0, tile_bits, 1,
-                    knobs["low_bits"], knobs["variant_flags"])
+    plan = nat.Plan(n, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0,
+                    tile_bits or knobs.get("tile_bits", 0), 1, knobs["low_bits"], knobs["variant_flags"])
     mats = lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, None)
     plan.tuning = autotune.report(low, 0, 1)
     return plan, mats

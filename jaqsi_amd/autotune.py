@@ -1,10 +1,12 @@
-"""Autotuner of the plan knobs: the low global bits every tile holds and the
-kernel variants (``low_bits`` / ``variant_flags`` of ``jq_plan_desc``).
+"""Autotuner of the plan knobs: the low global bits every tile holds, the
+tile size and the kernel variants (``low_bits`` / ``tile_bits`` /
+``variant_flags`` of ``jq_plan_desc``).
 
 For one circuit structure, dtype and batch it times a forward + adjoint
 step of each candidate plan on the GPU, by coordinate descent (first
-``low_bits`` with the built-in variants, then each variant flag on the best
-``low_bits``), and keeps the fastest.  Choices are cached per GPU
+``low_bits`` with the built-in variants, then, for complex64 circuits of 14
+or more wires, 14-bit tiles on the best ``low_bits``, then each variant
+flag on the best of those), and keeps the fastest.  Choices are cached per GPU
 architecture, dtype, batch bucket (the next power of two) and structure
 hash:
 
@@ -34,7 +36,7 @@ import torch
 
 from jaqsi_amd import _native as nat
 
-DEFAULT = {"low_bits": 0, "variant_flags": 0}
+DEFAULT = {"low_bits": 0, "variant_flags": 0, "tile_bits": 0}
 _LOW_CANDIDATES = {0: (5, 4, 6, 3), 1: (6, 5, 4)}  # complex64 / complex128, built-in first
 _FLAG_CANDIDATES = {0: (nat.JQ_TUNE_NO_PERSIST_BWD, nat.JQ_TUNE_NO_DMA_FWD, nat.JQ_TUNE_NARROW_BWD),
                     1: (nat.JQ_TUNE_NO_PERSIST_BWD,)}
@@ -171,16 +173,17 @@ def tune(low, dtype: int, batch: int, density: bool = False, jit: Optional[bool]
     use_jit = (low.n_wires >= 20) if jit is None else jit
     tried: List[dict] = []
 
-    def measure(low_bits: int, flags: int) -> float:
-        plan = nat.Plan(low.n_wires, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, dtype, 0, 1,
-                        low_bits, flags)
-        if use_jit:
+    def measure(low_bits: int, flags: int, tile_bits: int = 0) -> float:
+        plan = nat.Plan(low.n_wires, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, dtype,
+                        tile_bits, 1, low_bits, flags)
+        if use_jit or tile_bits == 14:
             plan.jit(nat.JQ_JIT_SYNC)
         ms = _time_step(plan, batch, len(low.angle_values), mats, density, reps, rounds)
-        tried.append({"low_bits": low_bits, "variant_flags": flags, "ms": round(ms, 4),
+        tried.append({"low_bits": low_bits, "tile_bits": tile_bits, "variant_flags": flags, "ms": round(ms, 4),
                       "passes": plan.info()["n_passes"]})
         if verbose:
-            print(f"  low_bits={low_bits} flags={flags}: {ms:.3f} ms ({tried[-1]['passes']} passes)", flush=True)
+            print(f"  low_bits={low_bits} tile_bits={tile_bits} flags={flags}: {ms:.3f} ms "
+                  f"({tried[-1]['passes']} passes)", flush=True)
         return ms
 
     cands = _LOW_CANDIDATES[dtype]
@@ -190,12 +193,25 @@ def tune(low, dtype: int, batch: int, density: bool = False, jit: Optional[bool]
         ms = measure(lb, 0)
         if ms < best_ms:
             best_low, best_ms = lb, ms
+    # 14-bit tiles (complex64): one 1024-thread workgroup per 2^14 amplitudes
+    # (plan-specialised kernels only; wide-pass and dense 3-wire blocks keep
+    # 13-bit tiles, and plan creation refuses them)
+    best_tile = 0
+    if dtype == 0 and low.n_wires >= 14:
+        try:
+            ms = measure(best_low, 0, 14)
+        except (NotImplementedError, ValueError, RuntimeError):
+            ms = float("inf")
+        if ms < best_ms:
+            best_tile, best_ms = 14, ms
     best_flags = 0
-    for fl in _FLAG_CANDIDATES[dtype]:
-        ms = measure(best_low, best_flags | fl)
+    # (14-bit plans always run the non-persistent, 32-amplitude backward)
+    flags = _FLAG_CANDIDATES[dtype] if best_tile != 14 else (nat.JQ_TUNE_NO_DMA_FWD,)
+    for fl in flags:
+        ms = measure(best_low, best_flags | fl, best_tile)
         if ms < best_ms:
             best_flags, best_ms = best_flags | fl, ms
-    rec = {"low_bits": best_low, "variant_flags": best_flags, "ms": round(best_ms, 4),
+    rec = {"low_bits": best_low, "tile_bits": best_tile, "variant_flags": best_flags, "ms": round(best_ms, 4),
            "default_ms": round(default_ms, 4), "batch": batch, "jit": use_jit, "tried": tried}
     _store(entry_key(arch(), low.key, dtype, batch), rec)
     _memo.clear()
@@ -215,7 +231,8 @@ def config_for(low, dtype: int, batch: int, density: bool = False, jit: Optional
     if rec is None and _mode == "tune":
         rec = tune(low, dtype, batch, density, jit=jit, verbose=bool(os.environ.get("JAQSI_AMD_AUTOTUNE_VERBOSE")))
     out = dict(DEFAULT) if rec is None else {"low_bits": int(rec["low_bits"]),
-                                               "variant_flags": int(rec["variant_flags"])}
+                                               "variant_flags": int(rec["variant_flags"]),
+                                               "tile_bits": int(rec.get("tile_bits", 0))}
     if len(_memo) > 256:
         _memo.clear()
     _memo[mk] = out

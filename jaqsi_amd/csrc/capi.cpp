@@ -174,6 +174,7 @@ struct JitState {
   bool persist[2] = {false, true};        // persistent tile loops (forward, backward)
   bool dma = false;                       // forward streams tiles into LDS (two tiles of LDS)
   bool declined = false;                  // passes too long to specialise: the interpreter runs the plan
+  bool required = false;                  // no prebuilt kernel serves the plan (14-bit tiles)
   int nt[2] = {0, 0};
 };
 
@@ -254,6 +255,9 @@ JitCache& jit_cache() {
 // with the gate count and so does the compile time; longer passes (deep
 // circuits on few wires) stay with the interpreter.
 constexpr int kJitMaxPassGates = 256;
+// Blocks per pass of a 14-bit-tile plan, so that its straight-line code
+// stays within reach of the instruction cache (JQ_T14_PASS_BLOCKS overrides).
+constexpr int kT14PassBlocks = 160;
 
 void jit_build(jq_plan* p) {
   {
@@ -277,7 +281,7 @@ void jit_build(jq_plan* p) {
   spec.words_f = &p->hp.words;
   spec.words_b = &p->words_bj;
   spec.dtype = p->dtype;
-  spec.persist_bwd = !(p->variant_flags & JQ_TUNE_NO_PERSIST_BWD);
+  spec.persist_bwd = !(p->variant_flags & JQ_TUNE_NO_PERSIST_BWD) && !p->jit.required;
   spec.dma = !(p->variant_flags & JQ_TUNE_NO_DMA_FWD);
   p->jit.dma = spec.dma && p->dtype == 0;
   p->jit.persist[0] = false;
@@ -285,27 +289,32 @@ void jit_build(jq_plan* p) {
   jq::jit_plan(spec, p->jit.plan);
   const int nk = (int)p->jit.plan.kernels.size();
   JitCache& C = jit_cache();
-  std::lock_guard<std::mutex> lock(C.mu);
   std::vector<std::string> keys(nk);
   std::vector<int> todo;
   std::vector<std::string> names;
   p->jit.kref.assign(nk, {-1, std::string()});
-  for (int id = 0; id < nk; ++id) {
-    keys[id] = jq::jit_key(spec, p->jit.plan, id);
-    auto it = C.kernels.find(keys[id]);
-    if (it != C.kernels.end()) {
-      p->jit.kref[id] = it->second;
-    } else {
-      todo.push_back(id);
-      names.push_back(jq::jit_kernel_name(p->jit.plan.kernels[id].dir, C.next++));
+  {
+    std::lock_guard<std::mutex> lock(C.mu);
+    for (int id = 0; id < nk; ++id) {
+      keys[id] = jq::jit_key(spec, p->jit.plan, id);
+      auto it = C.kernels.find(keys[id]);
+      if (it != C.kernels.end()) {
+        p->jit.kref[id] = it->second;
+      } else {
+        todo.push_back(id);
+        names.push_back(jq::jit_kernel_name(p->jit.plan.kernels[id].dir, C.next++));
+      }
     }
   }
   bool ok = true;
   std::string log;
   if (!todo.empty()) {
+    // compiled outside the cache lock (other plans may look kernels up
+    // meanwhile); a kernel two plans compile at once is compiled twice, and
+    // each keeps its own copy
     const std::string src = jq::jit_source(spec, p->jit.plan, todo, names);
     if (const char* dump = std::getenv("JQ_JIT_DUMP")) {  // debugging aid: keep the generated source
-      if (FILE* fh = std::fopen((std::string(dump) + "." + std::to_string(C.code.size()) + ".hip").c_str(), "w")) {
+      if (FILE* fh = std::fopen((std::string(dump) + "." + names[0] + ".hip").c_str(), "w")) {
         std::fwrite(src.data(), 1, src.size(), fh);
         std::fclose(fh);
       }
@@ -313,10 +322,11 @@ void jit_build(jq_plan* p) {
     std::vector<char> code;
     ok = jq::jit_compile(src, code, log);
     if (ok) {
+      std::lock_guard<std::mutex> lock(C.mu);
       const int ci = (int)C.code.size();
       C.code.push_back(std::move(code));
       for (size_t j = 0; j < todo.size(); ++j) {
-        C.kernels[keys[todo[j]]] = {ci, names[j]};
+        C.kernels.emplace(keys[todo[j]], std::make_pair(ci, names[j]));
         p->jit.kref[todo[j]] = {ci, names[j]};
       }
     }
@@ -419,6 +429,11 @@ hipError_t side_streams(SideStreams** out) {
 // Whether this call's passes run the specialised kernels.
 bool jit_ready(jq_plan* p) {
   if (p->jit.mode == JQ_JIT_OFF) return false;
+  if (p->jit.required) {  // no prebuilt kernel can stand in: wait for the compile
+    int expect = JQ_JIT_NONE;
+    if (p->jit.phase.compare_exchange_strong(expect, JQ_JIT_COMPILING)) jit_build(p);
+    if (p->jit.worker.joinable()) p->jit.worker.join();
+  }
   const int ph = p->jit.phase.load(std::memory_order_acquire);
   if (ph == JQ_JIT_LOADED) return true;
   if (ph == JQ_JIT_COMPILED) return jit_load(p);
@@ -449,6 +464,8 @@ const char* jq_last_error(void) { return g_err.c_str(); }
 const char* jq_version(void) { return "jaqsi_amd 0.1 (gfx950)"; }
 
 static int elem_bytes(int dtype) { return dtype == 0 ? 8 : 16; }
+// state regions a backward workgroup keeps in LDS (the kernels' lds_regions)
+static size_t lds_regions(int dtype, int T) { return 2 * ((size_t)elem_bytes(dtype) << T) <= 128 * 1024 ? 2 : 1; }
 
 int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   if (!desc || !out) return fail(JQ_ERR_ARG, "null argument");
@@ -462,16 +479,33 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   const int RB = 4;
   const int RBb = 3;
   if (T == 0) T = desc->dtype == 0 ? 13 : 12;
-  const bool okT = desc->dtype == 0 ? (T == 13) : (T == 12);
-  if (!okT) return fail(JQ_ERR_ARG, "tile_bits must be 13 (complex64) or 12 (complex128)");
-  // LDS per backward workgroup: two state tiles, the observable table, 64
-  // expectation reals and two buffers of the pass's cotangent slots (the
-  // persistent backward alternates them from tile to tile).
+  const bool okT = desc->dtype == 0 ? (T == 13 || T == 14) : (T == 12);
+  if (!okT) return fail(JQ_ERR_ARG, "tile_bits must be 13 or 14 (complex64) or 12 (complex128)");
+  // A 14-bit tile (complex64) is 2^14 amplitudes, 16 per thread of a
+  // 1024-thread workgroup: a 14-qubit circuit is one tile, so each pass
+  // reads and writes its vector once however many of its wires the gates
+  // touch.  Only plan-specialised kernels run such plans (non-persistent
+  // backward: 1024 threads leave 128 VGPRs, no room for a prefetch set).
+  const bool t14 = T == 14;
+  if (t14 && desc->n_wires < 14) return fail(JQ_ERR_ARG, "tile_bits 14 needs at least 14 wires");
+  // LDS per backward workgroup: the state region(s) (psi and lambda side by
+  // side, or one region they take in turn for 14-bit tiles), the observable
+  // table, 64 expectation reals and two buffers of the pass's cotangent
+  // slots (the persistent backward alternates them from tile to tile).
   const size_t lds_state = (size_t)elem_bytes(desc->dtype) << T;
-  const int slot_budget = ((int)((160 * 1024 - 2 * lds_state - sizeof(jq::ObsTable)) / 8) - 64) / 2;
+  const size_t regions = lds_regions(desc->dtype, T);
+  const int slot_budget = ((int)((160 * 1024 - regions * lds_state - sizeof(jq::ObsTable)) / 8) - 64) / 2;
+  // a pass's specialised code grows with its gates: 14-bit passes, which
+  // could otherwise take a whole circuit, close at a block budget
+  int max_blocks = 0;
+  if (t14) {
+    max_blocks = kT14PassBlocks;
+    if (const char* e = std::getenv("JQ_T14_PASS_BLOCKS")) max_blocks = std::max(1, std::atoi(e));
+    max_blocks = std::min(max_blocks, kJitMaxPassGates);
+  }
   jq_plan* p = new jq_plan();
   try {
-    jq::build_program(*desc, T, RB, RBb, p->hp, slot_budget);
+    jq::build_program(*desc, T, RB, RBb, p->hp, slot_budget, max_blocks);
   } catch (const jq::PlanError& e) {
     delete p;
     return fail(e.code, e.what());
@@ -502,7 +536,7 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
       jq::wide_geom(sub + 1 + T, desc->n_wires, p->hp.ntiles, pass[3], desc->dtype, p->wide_geom[q]);
       continue;
     }
-    if (2 * lds_state + sizeof(jq::ObsTable) + (64 + 2 * (size_t)pass[2]) * 8 > 160 * 1024) {
+    if (regions * lds_state + sizeof(jq::ObsTable) + (64 + 2 * (size_t)pass[2]) * 8 > 160 * 1024) {
       delete p;
       return fail(JQ_ERR_UNSUPPORTED, "too many gradient slots in one pass for LDS");
     }
@@ -538,8 +572,15 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   p->words_bj = p->hp.words_b;
   if (desc->dtype == 0) {
     jq::HostProgram wide;
+    // 14-bit tiles: 32 amplitudes x 512 threads (256-VGPR budget) unless
+    // JQ_T14_BWD_RB=4 asks for 16 x 1024
+    int rbw = 4;
+    if (t14) {
+      rbw = 5;
+      if (const char* e = std::getenv("JQ_T14_BWD_RB")) rbw = std::atoi(e) == 4 ? 4 : 5;
+    }
     try {
-      jq::build_program(*desc, T, RB, 4, wide, slot_budget);
+      jq::build_program(*desc, T, RB, rbw, wide, slot_budget, max_blocks);
     } catch (const std::exception& e) {
       delete p;
       return fail(JQ_ERR_ARG, e.what());
@@ -550,11 +591,26 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
     }
     // (JQ_TUNE_NARROW_BWD keeps the 8-amplitude program for the specialised
     // backward too, unless 4x4 / dense 3-wire blocks need the wide one)
-    if (!(p->variant_flags & JQ_TUNE_NARROW_BWD) || any_full) p->words_bj = wide.words_b;
-    if (any_full) {
+    if (!(p->variant_flags & JQ_TUNE_NARROW_BWD) || any_full || t14) p->words_bj = wide.words_b;
+    if (any_full || t14) {
       p->hp.words_b = wide.words_b;
       p->wide_bwd = true;
     }
+  }
+  if (t14) {
+    bool dense3 = false;
+    for (size_t r = 0; r + jq::BW <= p->hp.block_table.size(); r += jq::BW)
+      dense3 = dense3 || p->hp.block_table[r + jq::B_KIND] == jq::AK_DENSEK;
+    if (p->any_wide || dense3) {
+      delete p;
+      return fail(JQ_ERR_UNSUPPORTED, "tile_bits 14: wide-pass and dense 3-wire blocks need 13-bit tiles");
+    }
+    // compiled at its first use or jq_plan_jit call, not on a worker thread
+    // now: groups of a ragged batch then compile in turn and reuse the
+    // kernels of the passes they share
+    p->jit.required = true;
+    jit_on = false;
+    p->jit.mode = JQ_JIT_ASYNC;
   }
   std::vector<int32_t> sm;
   std::vector<jq::SlotMap> smv = p->hp.slot_map;
@@ -760,6 +816,7 @@ jq::SvArgs base_args(jq_plan* p, hipStream_t st) {
 
 size_t smem_for(jq_plan* p, int extra_reals, int tiles) {
   // state tile(s) | observable table (64 ints + 128 doubles) | 64 expval + slot reals
+  tiles = std::min<int>(tiles, (int)lds_regions(p->dtype, p->hp.T));
   return (size_t)tiles * ((size_t)elem_bytes(p->dtype) << p->hp.T) + sizeof(jq::ObsTable) + (size_t)(64 + extra_reals) * 8;
 }
 
@@ -845,6 +902,9 @@ int run_forward(jq_plan* p, int batch, const double* angles, const void* matrice
   ck = ck && P > 1;
   if (ck) HIP_TRY(WS(p, st).ckpt.ensure(vbytes * (size_t)(P - 1), st));
   const bool jit = jit_ready(p);
+  if (!jit && p->jit.required)
+    return fail(JQ_ERR_UNSUPPORTED, "tile_bits 14 plans run only on plan-specialised kernels: " +
+                                        (p->jit.log.empty() ? std::string("specialisation is off") : p->jit.log));
   for (int q = 0; q < P; ++q) {
     a.pass = q;
     a.src = q == 0 ? psi0 : pass_out(p, ck, state, vbytes, q - 1, st);
@@ -894,6 +954,9 @@ int run_backward(jq_plan* p, int batch, const double* angles, const void* matric
   a.lam = lam;
   a.lam_stride = (int64_t)a.vec_len;
   const bool jit = jit_ready(p);
+  if (!jit && p->jit.required)
+    return fail(JQ_ERR_UNSUPPORTED, "tile_bits 14 plans run only on plan-specialised kernels: " +
+                                        (p->jit.log.empty() ? std::string("specialisation is off") : p->jit.log));
   const size_t sm = smem_for(p, 2 * max_slots(p), 2);
   a.prog = (const int*)(jit ? p->prog_bj.p : p->prog_b.p);
   for (int q = P - 1; q >= 0; --q) {
@@ -960,6 +1023,9 @@ int jq_sv_forward_noalloc(jq_plan* p, int32_t batch, const double* angles, const
   if (a.obs_kind != JQ_OBS_NONE) a.ev_part = (double*)(base + off_ev);
   // plan-specialised kernels only when already loaded (loading allocates)
   const bool jit = p->jit.mode != JQ_JIT_OFF && p->jit.phase.load(std::memory_order_acquire) == JQ_JIT_LOADED;
+  if (!jit && p->jit.required)
+    return fail(JQ_ERR_UNSUPPORTED, "tile_bits 14 plans run only on plan-specialised kernels: " +
+                                        (p->jit.log.empty() ? std::string("specialisation is off") : p->jit.log));
   const int P = p->hp.n_passes;
   for (int q = 0; q < P; ++q) {
     a.pass = q;
@@ -1042,9 +1108,8 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
   for (int g = 0; g < n_groups; ++g) {
     const jq_group& G = groups[g];
     if (!G.plan || G.batch < 1 || !G.ct || !G.state) return fail(JQ_ERR_ARG, "bad group");
-    if (G.plan->hp.n_wires != p0->hp.n_wires || G.plan->dtype != p0->dtype || G.plan->hp.T != p0->hp.T ||
-        G.plan->wide_bwd != p0->wide_bwd)
-      return fail(JQ_ERR_ARG, "groups must share width, dtype, tile size and backward register tile");
+    if (G.plan->hp.n_wires != p0->hp.n_wires || G.plan->dtype != p0->dtype)
+      return fail(JQ_ERR_ARG, "groups must share width and dtype");
     for (int h = 0; h < g; ++h)
       if (groups[h].plan == G.plan)
         return fail(JQ_ERR_ARG, "two groups share a plan: merge circuits of one structure into one group");
@@ -1062,6 +1127,15 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
     all_jit = all_jit && jit_ready(groups[g].plan);
     any_wide = any_wide || groups[g].plan->any_wide;
   }
+  for (int g = 0; g < n_groups; ++g)
+    if (groups[g].plan->jit.required && !all_jit)
+      return fail(JQ_ERR_UNSUPPORTED, "tile_bits 14 groups need every group's plan-specialised kernels");
+  // (groups that run their own specialised adjoints may differ in tile size;
+  // the shared launches below need one program geometry)
+  if (!(all_jit || any_wide))
+    for (int g = 0; g < n_groups; ++g)
+      if (groups[g].plan->hp.T != p0->hp.T || groups[g].plan->wide_bwd != p0->wide_bwd)
+        return fail(JQ_ERR_ARG, "groups on the shared launches must share tile size and backward register tile");
   // (wide-block passes run their own kernels: such groups take this path too)
   if (all_jit || any_wide) {
     // Groups share nothing (each plan owns its workspaces), so they run on

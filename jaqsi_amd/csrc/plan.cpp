@@ -80,7 +80,8 @@ double op_cond(const jq_op& op) {
 
 }  // namespace
 
-void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgram& out, int slot_budget) {
+void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgram& out, int slot_budget,
+                   int max_blocks) {
   const int n = d.n_wires;
   if (n < 1 || n > 40) throw PlanError(JQ_ERR_ARG, "n_wires must be in [1, 40]");
   // ---- fusion (simulation.py:214-240) ---------------------------------
@@ -330,6 +331,10 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
     }
     for (int bi : remaining) {
       HB& b = blocks[bi];
+      if (max_blocks > 0 && (int)P.blocks.size() >= max_blocks) {  // the pass is full: the rest waits, in order
+        left.push_back(bi);
+        continue;
+      }
       bool blk = b.wide;  // deferred to a pass of its own
       if (blk) {
         for (int w : b.wires) blocked[gbit(n, w)] = 1;

@@ -38,6 +38,9 @@ struct HostProgram {
 
 // slot_budget: most cotangent reals one pass may flush (they live in LDS
 // next to the tiles); a pass closes early rather than exceed it.
-void build_program(const jq_plan_desc& d, int T, int RB, int RBb, HostProgram& out, int slot_budget);
+// max_blocks > 0 caps the blocks of a tile pass (a pass's straight-line
+// specialised code grows with its gate count).
+void build_program(const jq_plan_desc& d, int T, int RB, int RBb, HostProgram& out, int slot_budget,
+                   int max_blocks = 0);
 
 }  // namespace jq

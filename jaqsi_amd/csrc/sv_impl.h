@@ -34,6 +34,16 @@ struct C2 {
 template <typename R>
 constexpr int kTileBits = sizeof(R) == 4 ? 13 : 12;
 
+// LDS state regions of a tile of TB bits: two (psi and lambda side by side)
+// while both fit in 128 KiB, otherwise one (2^14 complex64 amplitudes fill
+// 128 KiB), through which the backward exchanges psi and lambda in turn.
+template <typename R, int TB>
+constexpr int kRegions = (2 * (sizeof(C2<R>) << TB) <= 128 * 1024) ? 2 : 1;
+template <typename R>
+__device__ __forceinline__ int lds_regions(int T) {
+  return (2 * (sizeof(C2<R>) << T) <= 128 * 1024) ? 2 : 1;
+}
+
 // Program words, block matrices and observable tables are read-only for the
 // whole launch: reading them through the constant address space lets every
 // wave-uniform access compile to a scalar load (s_load into SGPRs) instead of
@@ -1124,7 +1134,7 @@ __device__ __forceinline__ void store_canon(const R* re, const R* im, C2<R>* dst
 // writes go to the other region (psi and lambda in the backward).
 // The two halves of an exchange: every thread stores its registers under
 // layout `from`, and (after a barrier) loads them back under layout `to`.
-template <typename R, int RB>
+template <typename R, int RB, int TB = kTileBits<R>>
 __device__ __forceinline__ void relayout_put(const R* re, const R* im, C2<R>* lds, const Layout& from, int region) {
   constexpr int NR = 1 << RB;
   constexpr uint32_t ES = sizeof(R);
@@ -1134,7 +1144,7 @@ __device__ __forceinline__ void relayout_put(const R* re, const R* im, C2<R>* ld
   // 32-bit LDS addresses.  The tile starts the dynamic LDS block, which is
   // at LDS offset 0 (the kernels declare no static LDS), so the thread term
   // can carry the base and the register term is XORed in.
-  constexpr uint32_t IM = ES << kTileBits<R>;  // byte distance of the imaginary plane
+  constexpr uint32_t IM = ES << TB;  // byte distance of the imaginary plane
   const uint32_t lb = (uint32_t)(uintptr_t)(JQ_LDS char*)(char*)lds + (uint32_t)region * 2 * IM;
   const uint32_t sf = lb + lds_swz(from.toff) * ES;
 #pragma unroll
@@ -1149,14 +1159,14 @@ __device__ __forceinline__ void relayout_put(const R* re, const R* im, C2<R>* ld
   }
 }
 
-template <typename R, int RB>
+template <typename R, int RB, int TB = kTileBits<R>>
 __device__ __forceinline__ void relayout_get(R* re, R* im, C2<R>* lds, const Layout& to, int region) {
   constexpr int NR = 1 << RB;
   constexpr uint32_t ES = sizeof(R);
   uint32_t wt[RB];
 #pragma unroll
   for (int b = 0; b < RB; ++b) wt[b] = __builtin_amdgcn_readfirstlane(lds_swz(1u << to.rpos[b]) * ES);
-  constexpr uint32_t IM = ES << kTileBits<R>;
+  constexpr uint32_t IM = ES << TB;
   const uint32_t lb = (uint32_t)(uintptr_t)(JQ_LDS char*)(char*)lds + (uint32_t)region * 2 * IM;
   const uint32_t st = lb + lds_swz(to.toff) * ES;
 #pragma unroll
@@ -1171,26 +1181,37 @@ __device__ __forceinline__ void relayout_get(R* re, R* im, C2<R>* lds, const Lay
   }
 }
 
-template <typename R, int RB>
+template <typename R, int RB, int TB = kTileBits<R>>
 __device__ __forceinline__ void relayout(R* re, R* im, C2<R>* lds, const Layout& from, const Layout& to,
                                          int region = 0, bool tail_sync = true) {
-  relayout_put<R, RB>(re, im, lds, from, region);
+  relayout_put<R, RB, TB>(re, im, lds, from, region);
   __syncthreads();
-  relayout_get<R, RB>(re, im, lds, to, region);
+  relayout_get<R, RB, TB>(re, im, lds, to, region);
   if (tail_sync) __syncthreads();
 }
 
 // psi and lambda of the backward together (regions 0 and 1): both stores,
 // one barrier, both loads, and (tail_sync) one barrier before the regions
 // may be written again.
-template <typename R, int RB>
+// With one LDS region (large tiles) psi goes through it first, then lambda.
+template <typename R, int RB, int TB = kTileBits<R>>
 __device__ __forceinline__ void relayout_pair(R* pr, R* pi, R* lr, R* li, C2<R>* lds, const Layout& from,
                                               const Layout& to, bool tail_sync = true) {
-  relayout_put<R, RB>(pr, pi, lds, from, 0);
-  relayout_put<R, RB>(lr, li, lds, from, 1);
-  __syncthreads();
-  relayout_get<R, RB>(pr, pi, lds, to, 0);
-  relayout_get<R, RB>(lr, li, lds, to, 1);
+  if constexpr (kRegions<R, TB> == 1) {
+    relayout_put<R, RB, TB>(pr, pi, lds, from, 0);
+    __syncthreads();
+    relayout_get<R, RB, TB>(pr, pi, lds, to, 0);
+    __syncthreads();
+    relayout_put<R, RB, TB>(lr, li, lds, from, 0);
+    __syncthreads();
+    relayout_get<R, RB, TB>(lr, li, lds, to, 0);
+  } else {
+    relayout_put<R, RB, TB>(pr, pi, lds, from, 0);
+    relayout_put<R, RB, TB>(lr, li, lds, from, 1);
+    __syncthreads();
+    relayout_get<R, RB, TB>(pr, pi, lds, to, 0);
+    relayout_get<R, RB, TB>(lr, li, lds, to, 1);
+  }
   if (tail_sync) __syncthreads();
 }
 
@@ -1436,7 +1457,7 @@ template <typename R, int RB, int LNT, int... W>
 __device__ __forceinline__ void jit_relayout(R* re, R* im, C2<R>* lds, Layout& L) {
   Layout nl;
   layout_lit<RB, LNT, W...>(L.tbits, nl);
-  relayout<R, RB>(re, im, lds, L, nl);
+  relayout<R, RB, RB + LNT>(re, im, lds, L, nl);
   L = nl;
 }
 
@@ -1444,7 +1465,7 @@ template <typename R, int RB, int LNT, int... W>
 __device__ __forceinline__ void jit_relayout2(R* pr, R* pi, R* lr, R* li, C2<R>* lds, Layout& L) {
   Layout nl;
   layout_lit<RB, LNT, W...>(L.tbits, nl);
-  relayout_pair<R, RB>(pr, pi, lr, li, lds, L, nl);
+  relayout_pair<R, RB, RB + LNT>(pr, pi, lr, li, lds, L, nl);
   L = nl;
 }
 
@@ -1455,7 +1476,7 @@ template <typename R, int RB, int LNT, int... W>
 __device__ __forceinline__ void jit_relayout_nt(R* re, R* im, C2<R>* lds, Layout& L) {
   Layout nl;
   layout_lit<RB, LNT, W...>(L.tbits, nl);
-  relayout<R, RB>(re, im, lds, L, nl, 0, false);
+  relayout<R, RB, RB + LNT>(re, im, lds, L, nl, 0, false);
   L = nl;
 }
 
@@ -1463,7 +1484,7 @@ template <typename R, int RB, int LNT, int... W>
 __device__ __forceinline__ void jit_relayout2_nt(R* pr, R* pi, R* lr, R* li, C2<R>* lds, Layout& L) {
   Layout nl;
   layout_lit<RB, LNT, W...>(L.tbits, nl);
-  relayout_pair<R, RB>(pr, pi, lr, li, lds, L, nl, false);
+  relayout_pair<R, RB, RB + LNT>(pr, pi, lr, li, lds, L, nl, false);
   L = nl;
 }
 
@@ -1475,7 +1496,7 @@ __device__ __forceinline__ void jit_relayout_split(R* re, R* im, C2<R>* lds, Lay
   Layout nl;
   layout_lit<RB, LNT, W...>(L.tbits, nl);
   if (LEAD) __syncthreads();
-  relayout<R, RB>(re, im, lds, L, nl, 0, false);
+  relayout<R, RB, RB + LNT>(re, im, lds, L, nl, 0, false);
   L = nl;
 }
 
@@ -1490,7 +1511,7 @@ __device__ __forceinline__ void jit_relayout2_split(R* pr, R* pi, R* lr, R* li, 
   Layout nl;
   layout_lit<RB, LNT, W...>(L.tbits, nl);
   if (LEAD) __syncthreads();
-  relayout_pair<R, RB>(pr, pi, lr, li, lds, L, nl, false);
+  relayout_pair<R, RB, RB + LNT>(pr, pi, lr, li, lds, L, nl, false);
   L = nl;
 }
 
@@ -1751,7 +1772,7 @@ __device__ __forceinline__ void to_canonical(R* re, R* im, cint* tbits, Layout& 
   bool same = true;
 #pragma unroll
   for (int b = 0; b < RB; ++b) same = same && (cl.rpos[b] == L.rpos[b]);
-  if (!same) relayout<R, RB>(re, im, lds, L, cl);
+  if (!same) relayout<R, RB, RB + LNT>(re, im, lds, L, cl);
   L = cl;  // also when no exchange was needed: substep layouts carry no gthr
 }
 
@@ -1874,7 +1895,7 @@ __device__ __forceinline__ void to_canonical2(R* pr, R* pi, R* lr, R* li, cint* 
 #pragma unroll
   for (int b = 0; b < RB; ++b) same = same && (cl.rpos[b] == L.rpos[b]);
   if (!same) {
-    relayout_pair<R, RB>(pr, pi, lr, li, lds, L, cl);
+    relayout_pair<R, RB, RB + LNT>(pr, pi, lr, li, lds, L, cl);
   }
   L = cl;  // also when no exchange was needed: substep layouts carry no gthr
 }
@@ -1916,7 +1937,7 @@ __device__ __forceinline__ void sv_backward_impl(const SvArgs& a0) {
   const SvArgs a = resolve_args<R>(a0, circ);
   const int T = ((cint*)a.prog)[2];
   C2<R>* lds = reinterpret_cast<C2<R>*>(smem);  // psi and lambda tiles (relayout2)
-  ObsLds* obs = reinterpret_cast<ObsLds*>(smem + (2 * sizeof(C2<R>) << T));
+  ObsLds* obs = reinterpret_cast<ObsLds*>(smem + ((size_t)lds_regions<R>(T) * sizeof(C2<R>) << T));
   double* slots = reinterpret_cast<double*>(obs + 1);
   const int tile = blockIdx.x;
   cint* pass = (cint*)a.prog + ((cint*)a.prog)[HDR_WORDS + a.pass];
@@ -2067,7 +2088,7 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
   const int ntiles = P[8];
   const int tlog = __builtin_ctz((unsigned)ntiles);
   C2<R>* lds = reinterpret_cast<C2<R>*>(smem);  // psi and lambda tiles (relayout2)
-  ObsLds* obs = reinterpret_cast<ObsLds*>(smem + (2 * sizeof(C2<R>) << T));
+  ObsLds* obs = reinterpret_cast<ObsLds*>(smem + ((size_t)lds_regions<R>(T) * sizeof(C2<R>) << T));
   // two cotangent slot buffers, alternating from item to item: a buffer is
   // read (and cleared) right after its item's flushes and next written two
   // items later, so no barrier is needed to clear it

@@ -51,7 +51,10 @@ class RaggedBatch:
         """Plan-specialised kernels for every group's plan (nat.JQ_JIT_*).
         Once all groups have them loaded, ``adjoint`` runs each group through
         them; until then all groups share the interpreter's launches.
-        Returns the compile phase of each plan."""
+        Returns the compile phase of each plan.  Plans compile one after
+        the other, so a group reuses the kernels of passes it shares with
+        the groups before it (the run-time compiler does not overlap
+        compiles)."""
         return [p.jit(mode) for p in self.plans]
 
     def angle_shapes(self) -> List[Tuple[int, int]]:

@@ -90,6 +90,7 @@ def get_plan(low: lowering.Lowered, dtype: int, tile_bits: int = 0, checkpoints:
 
         knobs = autotune.config_for(low, dtype, batch, density)
     lb, vf = int(knobs.get("low_bits", 0)), int(knobs.get("variant_flags", 0))
+    tile_bits = int(knobs.get("tile_bits", tile_bits))
     key = (low.key, dtype, tile_bits, checkpoints, lb, vf)
     with _plans_lock:
         p = _plans.get(key)

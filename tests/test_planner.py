@@ -359,3 +359,54 @@ def test_tuning_knobs_are_validated():
     for kw in ({"low_bits": 2}, {"low_bits": 7}, {"variant_flags": 8}):
         with pytest.raises(ValueError):
             nat.Plan(2, low.ops, low.members, 1, low.n_matrix_elems, 0, **kw)
+
+
+@pytest.mark.parametrize("layers", [3, 7])
+def test_fourteen_bit_tile_programs_match_oracle(layers):
+    """14-bit tiles (complex64): a 14-qubit circuit is one tile held by a
+    1024-thread forward / 512-thread backward workgroup; its programs (passes
+    capped at a block budget, 32-amplitude backward layouts) reproduce the
+    oracle's state and adjoint gradients, in fewer passes than 13-bit tiles."""
+    from circuits import record_hea
+    from jaqsi_amd import _native as nat, lowering
+    from plan_interp import Program
+
+    n = 14
+    rng = np.random.default_rng(40 + layers)
+    theta = rng.uniform(0, 2 * np.pi, (layers, n, 3))
+    x = rng.uniform(-np.pi, np.pi, n)
+    low = lowering.lower_pure(_record(lambda: record_hea(theta, n, features=x)), n)
+    angles = np.array([float(a) for a in low.angle_values])
+    ops = sv.hea_ops(theta, n, features=x)
+    zs = sv.z_signs(n).sum(0)
+    _, og, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
+    passes = {}
+    for tb in (13, 14):
+        plan = nat.Plan(n, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0, tb)
+        prog = Program(plan, quat=True)
+        assert prog.T == tb
+        np.testing.assert_allclose(prog.forward(angles, None), sv.simulate(ops, n), atol=1e-12)
+        _, _, grads, _ = prog.adjoint(angles, lambda f: 2 * zs * np.conj(f), None)
+        np.testing.assert_allclose(grads, sv.flat_grads(ops, og), atol=1e-10)
+        passes[tb] = plan.info()["n_passes"]
+    assert passes[14] < passes[13]
+
+
+def test_fourteen_bit_tiles_refuse_what_they_cannot_run():
+    """14-bit tiles need at least 14 wires and no wide-pass or dense 3-wire
+    blocks (those keep 13-bit tiles); complex128 keeps 12-bit tiles."""
+    import jaqsi_amd as J
+    from jaqsi_amd import _native as nat, lowering
+
+    def plan(fn, n, tb, dtype=0):
+        low = lowering.lower_pure(_record(fn), n)
+        return nat.Plan(n, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, dtype, tb)
+
+    with pytest.raises(ValueError, match="at least 14 wires"):
+        plan(lambda: J.Gates.RX(0.1, wires=0), 12, 14)
+    with pytest.raises(ValueError, match="tile_bits"):
+        plan(lambda: J.Gates.RX(0.1, wires=0), 14, 14, dtype=1)
+    u8 = np.linalg.qr(np.random.default_rng(1).normal(size=(8, 8)) + 1j * np.random.default_rng(2).normal(size=(8, 8)))[0]
+    with pytest.raises(NotImplementedError, match="13-bit tiles"):
+        plan(lambda: J.Operation(wires=[0, 5, 9], matrix=u8), 14, 14)
+    assert plan(lambda: J.Gates.RX(0.1, wires=13), 14, 14).info()["tile_bits"] == 14
