@@ -280,6 +280,7 @@ def bench_c3(steps, warmup, B=4096, n=14, seed=1003):
     diff = max(max(float((a - b).abs().max()) for a, b in zip(gevs, evs)),
                max(float((a - b).abs().max()) for a, b in zip(ggrads, grads)))
     circuit_passes = sum(len(members[d]) * p.info()["n_passes"] for d, p in zip(uniq, rb.plans))
+    circuit_blocks = sum(len(members[d]) * p.info()["n_blocks"] for d, p in zip(uniq, rb.plans))
     return {
         "compile_s": compile_s,
         "seconds": g1 - g0,
@@ -287,6 +288,8 @@ def bench_c3(steps, warmup, B=4096, n=14, seed=1003):
         "graph_vs_eager_max_abs_diff": diff,
         "kernels": kt,
         "circuit_passes": circuit_passes,
+        "circuit_blocks": circuit_blocks,
+        "plans": rb.plans,
         "groups": len(uniq),
         "tuning": tuning,
         "depths": depths,
@@ -299,33 +302,42 @@ def bench_c3(steps, warmup, B=4096, n=14, seed=1003):
 
 
 def roofline_c3(res, steps):
-    """Groups run concurrently on side streams, so launch durations overlap:
-    `achieved` is the algorithmic bytes of every pass launch of the timed
-    steps over their wall time (graph replays; the per-kernel entries keep the
-    summed, overlapping event times of the eager steps for reference)."""
+    """C3 is bound by FP32 VALU issue, not HBM: with 14-bit tiles a 14-qubit
+    circuit is one tile and most circuits take one or two passes.  `achieved`
+    is the algorithmic FP32 FLOPs of the fused 2x2 blocks (per amplitude and
+    block 8 FMA forward -- U psi -- and 24 backward -- U^-1 psi, the cotangent
+    lambda psi, U^T lambda -- at 2 FLOP per FMA) of every circuit of the timed
+    steps over their wall time (graph replays), against the FP32 VALU peak
+    without packed instructions (the build keeps packed FP32 off: one FMA per
+    lane and cycle, 78.6 TFLOP/s; 157.3 with packed issue).  The HBM figure
+    (algorithmic bytes of every pass launch over the same wall time) is kept
+    beside it; launches of concurrent groups overlap, so the per-kernel
+    entries are summed event times of the eager steps."""
     n = res["n"]
+    flops = 2 * (8 + 24) * (1 << n) * res["circuit_blocks"] * steps
+    ach = flops / res["seconds"] / 1e12
     sweep = 2 * 8 * (1 << n)
     per = dict(zip(kernel_names(), (sweep, sweep * 3 // 2)))
     out = {}
     total_bytes = 0
     for k, (tot, cnt) in res["kernels"].items():
-        bytes_ = per[k] * res["circuit_passes"] * steps
-        total_bytes += bytes_
+        total_bytes += per[k] * res["circuit_passes"] * steps
         out[k] = {"ms_total_summed": round(tot, 3), "launches": cnt}
-    ach = round(total_bytes / res["seconds"] / 1e9, 1)
+    hbm = total_bytes / res["seconds"] / 1e9
     return {
-        "bound": "hbm",
+        "bound": "valu-fp32",
         "kernel": "all pass kernels (" + ", ".join(kernel_names()) + ")",
-        "achieved": ach,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(ach / HBM_PEAK_GBS, 4),
+        "achieved": round(ach, 2),
+        "peak": VALU_F32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(ach / VALU_F32_PEAK_TFLOPS, 4),
+        "peak_packed_fp32": 157.3,
         "traffic": None,
-        "traffic_per_launch": {k: stored_traffic("c3", k) for k in kernel_names()},
-        "traffic_note": "launches cover groups of different sizes, so one per-launch byte figure is not comparable "
-        "with the per-circuit algorithmic bytes; traffic_per_launch is the stored FETCH_SIZE/WRITE_SIZE average",
-        "algorithmic_bytes_note": "per circuit pass 2*8*2^14 B forward, 3*8*2^14 B backward, summed over all "
-        "circuit passes of the timed steps, divided by their wall time",
+        "flop_note": "per circuit: fused 2x2 blocks x 2^14 amplitudes x (8 + 24) FMA x 2 FLOP, summed over the "
+        "4096 circuits of each timed step",
+        "hbm": {"achieved": round(hbm, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm / HBM_PEAK_GBS, 4),
+                "traffic_per_launch": {k: stored_traffic("c3", k) for k in kernel_names()},
+                "bytes_note": "per circuit pass 2*8*2^14 B forward, 3*8*2^14 B backward"},
         "kernels": out,
     }
 
@@ -463,6 +475,9 @@ def bench_c4(steps, warmup, n=12, layers=6, seed=1004):
 
 
 FP64_PEAK_TFLOPS = 78.6
+# FP32 vector issue without packed instructions (one FMA per lane and cycle):
+# the build keeps packed FP32 off, so this is the VALU ceiling of the pass kernels
+VALU_F32_PEAK_TFLOPS = 78.6
 # 64 x (5.892e7 ADD + 1.079e8 MUL + 2 x 1.661e8 FMA) F64 VALU instructions per launch at 8192 sets x 2000 steps
 C5_EXECUTED_FP64_FLOP = 64 * (5.892e7 + 1.079e8 + 2 * 1.661e8)
 
@@ -1035,6 +1050,8 @@ def main():
                 "groups": r3["groups"],
                 "autotune": r3["tuning"],
                 "circuit_passes": r3["circuit_passes"],
+                "circuit_blocks": r3["circuit_blocks"],
+                "tile_bits": sorted({p.info()["tile_bits"] for p in r3["plans"]}),
                 "pass_kernels": "plan-specialised (one launch sequence per group)" if USE_JIT
                 else "interpreter (one launch per pass step over all groups)",
                 "specialise_compile_s": r3["compile_s"],

@@ -333,8 +333,20 @@ def _gather_views(values: list) -> Optional[torch.Tensor]:
         if not isinstance(v, torch.Tensor) or v._base is not base or v.dim() != 0:
             return None
         offs.append(v.storage_offset() - b0)
-    idx = torch.as_tensor(offs, dtype=torch.int64).to(base.device, non_blocking=True)
+    # the index vector is cached on the device: copying it from pageable
+    # host memory on every call would wait for the stream to drain
+    key = (base.device, tuple(offs))
+    idx = _gather_index.get(key)
+    if idx is None:
+        idx = torch.as_tensor(offs, dtype=torch.int64).to(base.device)
+        if len(_gather_index) >= _GATHER_INDEX_MAX:
+            _gather_index.pop(next(iter(_gather_index)))
+        _gather_index[key] = idx
     return base.reshape(-1).index_select(0, idx)
+
+
+_gather_index: dict = {}
+_GATHER_INDEX_MAX = 64
 
 
 def angle_tensor(values: list, batch: Optional[int]) -> torch.Tensor:

@@ -116,15 +116,34 @@ def diag_observables(obs: Sequence[Operation]) -> Optional[List[Tuple[int, float
     """
     out = []
     for ob in obs:
-        m = ob.__class__._matrix
-        if m is None or len(ob.wires) != 1 or getattr(ob, "_explicit", False):
+        if len(ob.wires) != 1 or ob._explicit:
             return None
-        m = np.asarray(m)
-        if m.shape != (2, 2) or not np.allclose(m - np.diag(np.diag(m)), 0):
+        d = _class_diag(ob.__class__)
+        if d is None:
             return None
-        d = np.real(np.diag(m))
-        out.append((int(ob.wires[0]), float(d[0]), float(d[1])))
+        out.append((int(ob.wires[0]), d[0], d[1]))
     return out
+
+
+_class_diags: dict = {}
+
+
+def _class_diag(cls) -> Optional[Tuple[float, float]]:
+    """(d0, d1) of a class whose matrix is a diagonal 2x2, else None
+    (decided once per class)."""
+    try:
+        return _class_diags[cls]
+    except KeyError:
+        pass
+    m = cls._matrix
+    d = None
+    if m is not None:
+        m = np.asarray(m)
+        if m.shape == (2, 2) and np.allclose(m - np.diag(np.diag(m)), 0):
+            dd = np.real(np.diag(m))
+            d = (float(dd[0]), float(dd[1]))
+    _class_diags[cls] = d
+    return d
 
 
 def _obs_arrays(diags):
