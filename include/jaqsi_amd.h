@@ -132,6 +132,9 @@ typedef struct jq_plan_desc {
 #define JQ_TUNE_NO_PERSIST_BWD 1 /* backward: one tile per workgroup, no persistent loop with prefetch */
 #define JQ_TUNE_NO_DMA_FWD 2     /* complex64 forward: loads through registers, not direct to LDS */
 #define JQ_TUNE_NARROW_BWD 4     /* complex64 plan-specialised backward: 8 amplitudes x 1024 threads */
+#define JQ_TUNE_BWD16 8          /* complex64 plan-specialised backward: 16 amplitudes per thread (13-bit tiles:
+                                    512 threads, persistent) instead of 32 (13-bit tiles: 256 threads, two
+                                    workgroups per CU) */
 
 typedef struct jq_plan jq_plan;
 
@@ -151,8 +154,10 @@ int jq_plan_info(const jq_plan* plan, int32_t* n_passes, int32_t* n_substeps, in
 int jq_plan_program(const jq_plan* plan, int32_t* out, int64_t capacity, int64_t* size);
 
 /* Host copies of the plan's tables: which = 0 block table, 1 member list,
- * 2 member table, 3 cotangent slot map.  No reference counterpart: these
- * let tests execute the scheduled program on the CPU. */
+ * 2 member table, 3 cotangent slot map, 4 backward program of the prebuilt
+ * kernels, 5 backward program of the plan-specialised kernels.  No
+ * reference counterpart: these let tests execute the scheduled program on
+ * the CPU. */
 int jq_plan_tables(const jq_plan* plan, int32_t which, int32_t* out, int64_t capacity, int64_t* size);
 
 /* Plan-specialised pass kernels.  The prebuilt kernels interpret the

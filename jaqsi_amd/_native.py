@@ -33,6 +33,7 @@ JQ_COND_MAX = 4.0
 JQ_TUNE_NO_PERSIST_BWD = 1
 JQ_TUNE_NO_DMA_FWD = 2
 JQ_TUNE_NARROW_BWD = 4
+JQ_TUNE_BWD16 = 8
 JQ_NOALLOC_SCRATCH_BYTES = 32 << 20
 
 

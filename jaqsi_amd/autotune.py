@@ -38,7 +38,7 @@ from jaqsi_amd import _native as nat
 
 DEFAULT = {"low_bits": 0, "variant_flags": 0, "tile_bits": 0}
 _LOW_CANDIDATES = {0: (5, 4, 6, 3), 1: (6, 5, 4)}  # complex64 / complex128, built-in first
-_FLAG_CANDIDATES = {0: (nat.JQ_TUNE_NO_PERSIST_BWD, nat.JQ_TUNE_NO_DMA_FWD, nat.JQ_TUNE_NARROW_BWD),
+_FLAG_CANDIDATES = {0: (nat.JQ_TUNE_BWD16, nat.JQ_TUNE_NO_PERSIST_BWD, nat.JQ_TUNE_NO_DMA_FWD, nat.JQ_TUNE_NARROW_BWD),
                     1: (nat.JQ_TUNE_NO_PERSIST_BWD,)}
 _mode = os.environ.get("JAQSI_AMD_AUTOTUNE", "lookup")
 _lock = threading.Lock()
@@ -206,7 +206,7 @@ def tune(low, dtype: int, batch: int, density: bool = False, jit: Optional[bool]
             best_tile, best_ms = 14, ms
     best_flags = 0
     # (14-bit plans always run the non-persistent, 32-amplitude backward)
-    flags = _FLAG_CANDIDATES[dtype] if best_tile != 14 else (nat.JQ_TUNE_NO_DMA_FWD,)
+    flags = _FLAG_CANDIDATES[dtype] if best_tile != 14 else (nat.JQ_TUNE_BWD16, nat.JQ_TUNE_NO_DMA_FWD)
     for fl in flags:
         ms = measure(best_low, best_flags | fl, best_tile)
         if ms < best_ms:

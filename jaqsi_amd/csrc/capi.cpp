@@ -203,6 +203,7 @@ struct jq_plan {
   int variant_flags = 0;        // JQ_TUNE_* kernel variants
   std::vector<int32_t> words_bj;  // backward program of the plan-specialised kernels
   DevBuf prog_bj;
+  int regions_b = 2, regions_bj = 2;  // LDS state regions of the interpreter's / specialised backward
   bool uploaded = false;
   int class_mask = 0;  // block classes of the prep / chain kernels (jq::block_class)
   std::vector<int32_t> wg_blocks;  // blocks of the workgroup prep / chain classes
@@ -228,7 +229,10 @@ Workspace& WS(jq_plan* p, hipStream_t st) {
 
 namespace {
 
-size_t smem_for(jq_plan* p, int extra_reals, int tiles = 1);
+// LDS bytes of a pass workgroup: `regions` state regions (forward 1;
+// backward the program's lds_regions), the observable table, 64 expectation
+// reals and `extra_reals` cotangent slot reals
+size_t smem_for(jq_plan* p, int extra_reals, int regions = 1);
 int max_slots(jq_plan* p);
 
 // Process-wide cache of compiled pass kernels.  A kernel serves every plan
@@ -281,7 +285,10 @@ void jit_build(jq_plan* p) {
   spec.words_f = &p->hp.words;
   spec.words_b = &p->words_bj;
   spec.dtype = p->dtype;
-  spec.persist_bwd = !(p->variant_flags & JQ_TUNE_NO_PERSIST_BWD) && !p->jit.required;
+  // persistent backward (with its next-tile prefetch registers) only for
+  // 512-thread or larger workgroups
+  spec.persist_bwd = !(p->variant_flags & JQ_TUNE_NO_PERSIST_BWD) && !p->jit.required &&
+                     (p->words_bj[2] - p->words_bj[3]) >= 9;
   spec.dma = !(p->variant_flags & JQ_TUNE_NO_DMA_FWD);
   p->jit.dma = spec.dma && p->dtype == 0;
   p->jit.persist[0] = false;
@@ -377,7 +384,7 @@ bool jit_load(jq_plan* p) {
   p->jit.resident.assign(p->jit.fn.size(), cus);
   for (size_t id = 0; id < p->jit.fn.size(); ++id) {
     const int dir = p->jit.plan.kernels[id].dir;
-    const size_t sm = dir == 0 ? smem_for(p, 0) : smem_for(p, 2 * max_slots(p), 2);
+    const size_t sm = dir == 0 ? smem_for(p, 0) : smem_for(p, 2 * max_slots(p), p->regions_bj);
     int nb = 0;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, p->jit.fn[id], p->jit.nt[dir], sm) == hipSuccess &&
         nb > 0)
@@ -465,7 +472,9 @@ const char* jq_version(void) { return "jaqsi_amd 0.1 (gfx950)"; }
 
 static int elem_bytes(int dtype) { return dtype == 0 ? 8 : 16; }
 // state regions a backward workgroup keeps in LDS (the kernels' lds_regions)
-static size_t lds_regions(int dtype, int T) { return 2 * ((size_t)elem_bytes(dtype) << T) <= 128 * 1024 ? 2 : 1; }
+static size_t lds_regions(int dtype, int T, int RB) {
+  return 2 * ((size_t)elem_bytes(dtype) << T) <= 128 * 1024 && T - RB >= 9 ? 2 : 1;
+}
 
 int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   if (!desc || !out) return fail(JQ_ERR_ARG, "null argument");
@@ -493,7 +502,9 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   // table, 64 expectation reals and two buffers of the pass's cotangent
   // slots (the persistent backward alternates them from tile to tile).
   const size_t lds_state = (size_t)elem_bytes(desc->dtype) << T;
-  const size_t regions = lds_regions(desc->dtype, T);
+  // (the slot budget holds for every backward program of the plan: 13-bit
+  // tiles keep the interpreter's two regions)
+  const size_t regions = t14 ? 1 : 2;
   const int slot_budget = ((int)((160 * 1024 - regions * lds_state - sizeof(jq::ObsTable)) / 8) - 64) / 2;
   // a pass's specialised code grows with its gates: 14-bit passes, which
   // could otherwise take a whole circuit, close at a block budget
@@ -513,7 +524,7 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
     delete p;
     return fail(JQ_ERR_ARG, e.what());
   }
-  if (desc->variant_flags & ~(JQ_TUNE_NO_PERSIST_BWD | JQ_TUNE_NO_DMA_FWD | JQ_TUNE_NARROW_BWD)) {
+  if (desc->variant_flags & ~(JQ_TUNE_NO_PERSIST_BWD | JQ_TUNE_NO_DMA_FWD | JQ_TUNE_NARROW_BWD | JQ_TUNE_BWD16)) {
     delete p;
     return fail(JQ_ERR_ARG, "unknown variant_flags bits");
   }
@@ -571,32 +582,42 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   // 1024-thread register cap.
   p->words_bj = p->hp.words_b;
   if (desc->dtype == 0) {
-    jq::HostProgram wide;
-    // 14-bit tiles: 32 amplitudes x 512 threads (256-VGPR budget) unless
-    // JQ_T14_BWD_RB=4 asks for 16 x 1024
-    int rbw = 4;
-    if (t14) {
-      rbw = 5;
-      if (const char* e = std::getenv("JQ_T14_BWD_RB")) rbw = std::atoi(e) == 4 ? 4 : 5;
-    }
+    // The specialised backward's register tile: 32 amplitudes per thread
+    // (13-bit tiles: 256 threads, two workgroups per CU, each exchanging psi
+    // and lambda in turn through one LDS region; 14-bit tiles: 512 threads),
+    // or with JQ_TUNE_BWD16 16 amplitudes (13-bit tiles: 512 threads, one
+    // persistent workgroup per CU with next-tile prefetch; 14-bit tiles: 1024
+    // threads).  13-bit plans with 4x4 / dense 3-wire blocks keep 16
+    // amplitudes, the program the prebuilt kernels run for them.
+    const bool bwd16 = (p->variant_flags & JQ_TUNE_BWD16) != 0;
+    const int rbw = bwd16 || (!t14 && any_full) ? 4 : 5;
+    jq::HostProgram wide, wide5;
     try {
-      jq::build_program(*desc, T, RB, rbw, wide, slot_budget, max_blocks);
+      jq::build_program(*desc, T, RB, 4, wide, slot_budget, max_blocks);
+      if (rbw == 5) jq::build_program(*desc, T, RB, 5, wide5, slot_budget, max_blocks);
     } catch (const std::exception& e) {
       delete p;
       return fail(JQ_ERR_ARG, e.what());
     }
-    if (wide.n_passes != p->hp.n_passes) {
+    if (wide.n_passes != p->hp.n_passes || (rbw == 5 && wide5.n_passes != p->hp.n_passes)) {
       delete p;
-      return fail(JQ_ERR_ARG, "internal: wide backward program changed the pass structure");
+      return fail(JQ_ERR_ARG, "internal: a backward program changed the pass structure");
     }
     // (JQ_TUNE_NARROW_BWD keeps the 8-amplitude program for the specialised
-    // backward too, unless 4x4 / dense 3-wire blocks need the wide one)
-    if (!(p->variant_flags & JQ_TUNE_NARROW_BWD) || any_full || t14) p->words_bj = wide.words_b;
+    // backward too, unless 4x4 / dense 3-wire blocks or 14-bit tiles rule it out)
+    if ((p->variant_flags & JQ_TUNE_NARROW_BWD) && !any_full && !t14)
+      p->words_bj = p->hp.words_b;
+    else
+      p->words_bj = rbw == 5 ? wide5.words_b : wide.words_b;
     if (any_full || t14) {
-      p->hp.words_b = wide.words_b;
+      // (14-bit plans never run the prebuilt kernels: their program is the
+      // specialised one, which is what the CPU program interpreter then checks)
+      p->hp.words_b = t14 ? p->words_bj : wide.words_b;
       p->wide_bwd = true;
     }
   }
+  p->regions_b = (int)lds_regions(desc->dtype, T, p->hp.words_b[3]);
+  p->regions_bj = (int)lds_regions(desc->dtype, T, p->words_bj[3]);
   if (t14) {
     bool dense3 = false;
     for (size_t r = 0; r + jq::BW <= p->hp.block_table.size(); r += jq::BW)
@@ -705,6 +726,7 @@ int jq_plan_tables(const jq_plan* p, int32_t which, int32_t* out, int64_t capaci
     case 2: v = &p->hp.member_table; break;
     case 3: v = &p->slot_words; break;
     case 4: v = &p->hp.words_b; break;
+    case 5: v = &p->words_bj; break;
     default: return fail(JQ_ERR_ARG, "unknown table");
   }
   if (size) *size = (int64_t)v->size();
@@ -814,10 +836,9 @@ jq::SvArgs base_args(jq_plan* p, hipStream_t st) {
   return a;
 }
 
-size_t smem_for(jq_plan* p, int extra_reals, int tiles) {
-  // state tile(s) | observable table (64 ints + 128 doubles) | 64 expval + slot reals
-  tiles = std::min<int>(tiles, (int)lds_regions(p->dtype, p->hp.T));
-  return (size_t)tiles * ((size_t)elem_bytes(p->dtype) << p->hp.T) + sizeof(jq::ObsTable) + (size_t)(64 + extra_reals) * 8;
+size_t smem_for(jq_plan* p, int extra_reals, int regions) {
+  // state region(s) | observable table (64 ints + 128 doubles) | 64 expval + slot reals
+  return (size_t)regions * ((size_t)elem_bytes(p->dtype) << p->hp.T) + sizeof(jq::ObsTable) + (size_t)(64 + extra_reals) * 8;
 }
 
 int max_slots(jq_plan* p) {
@@ -957,7 +978,7 @@ int run_backward(jq_plan* p, int batch, const double* angles, const void* matric
   if (!jit && p->jit.required)
     return fail(JQ_ERR_UNSUPPORTED, "tile_bits 14 plans run only on plan-specialised kernels: " +
                                         (p->jit.log.empty() ? std::string("specialisation is off") : p->jit.log));
-  const size_t sm = smem_for(p, 2 * max_slots(p), 2);
+  const size_t sm = smem_for(p, 2 * max_slots(p), jit ? p->regions_bj : p->regions_b);
   a.prog = (const int*)(jit ? p->prog_bj.p : p->prog_b.p);
   for (int q = P - 1; q >= 0; --q) {
     a.pass = q;
@@ -1217,7 +1238,7 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
     t.batch = G.batch;
     row_begin[k + 1] = row_begin[k] + G.batch;
     maxP = std::max(maxP, P);
-    smem_b = std::max(smem_b, smem_for(p, 2 * max_slots(p), 2));
+    smem_b = std::max(smem_b, smem_for(p, 2 * max_slots(p), p->regions_b));
   }
   HIP_TRY(WS(p0, st).groups.ensure(sizeof(jq::SvGroup) * n_groups, st));
   HIP_TRY(jq::launch_bytes_upload(table.data(), sizeof(jq::SvGroup) * n_groups, WS(p0, st).groups.p, st));

@@ -434,7 +434,11 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
       const char* sig =
           "(cint*, int, R* pr, R* pi, R* lr, R* li, Layout& L, uint64_t base, C2<R>* lds, cmat<R>* pool, "
           "double* slots) {\n    constexpr int LNT = __builtin_ctz(NT);\n";
-      const int load = spec.persist_bwd ? k.load : 0, store = spec.persist_bwd ? k.store : 0;
+      // (the persistent loop and the one-tile-per-workgroup kernel of
+      // narrower workgroups both read and write under the body's end layouts;
+      // the 1024-thread one-tile kernel goes through the canonical layout)
+      const bool ends = spec.persist_bwd || nt < 1024;
+      const int load = ends ? k.load : 0, store = ends ? k.store : 0;
       // persistent loop: every exchange after the first of a body opens with
       // the barrier that the previous one would have closed with
       const bool split = spec.persist_bwd;
@@ -461,9 +465,10 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
           << "  static __device__ __forceinline__ void bwd0" << sig << bbody0
           << "    (void)lds; (void)base; (void)pool; (void)slots;\n  }\n";
       o << "};\n"
-        << "extern \"C\" __global__ __launch_bounds__(" << nt << ") void " << kname << "(SvArgs a) {\n  "
-        << (spec.persist_bwd ? "sv_backward_loop<" : "sv_backward_impl<") << rt << ", " << pv.RB << ", " << nt
-        << (spec.persist_bwd ? "" : ", false") << ", JB" << id << ">(a);\n}\n\n";
+        << "extern \"C\" __global__ __launch_bounds__(" << nt << (nt < 512 ? ", 2" : "") << ") void " << kname
+        << "(SvArgs a) {\n  "
+        << (spec.persist_bwd ? "sv_backward_loop<" : ends ? "sv_backward_direct<" : "sv_backward_impl<") << rt << ", "
+        << pv.RB << ", " << nt << (spec.persist_bwd || ends ? "" : ", false") << ", JB" << id << ">(a);\n}\n\n";
     }
   }
   o << "}  // namespace jq\n";

@@ -34,14 +34,17 @@ struct C2 {
 template <typename R>
 constexpr int kTileBits = sizeof(R) == 4 ? 13 : 12;
 
-// LDS state regions of a tile of TB bits: two (psi and lambda side by side)
-// while both fit in 128 KiB, otherwise one (2^14 complex64 amplitudes fill
-// 128 KiB), through which the backward exchanges psi and lambda in turn.
-template <typename R, int TB>
-constexpr int kRegions = (2 * (sizeof(C2<R>) << TB) <= 128 * 1024) ? 2 : 1;
-template <typename R>
+// LDS state regions of a backward workgroup on a tile of TB bits with 2^RB
+// amplitudes per thread: two (psi and lambda side by side) while both fit in
+// 128 KiB and the workgroup has 512 or more threads, otherwise one, through
+// which psi and lambda are exchanged in turn: 2^14 complex64 amplitudes fill
+// 128 KiB, and workgroups of fewer threads (32-amplitude register tiles on
+// 13-bit tiles) share the CU with a second workgroup.
+template <typename R, int TB, int RB>
+constexpr int kRegions = (2 * (sizeof(C2<R>) << TB) <= 128 * 1024 && TB - RB >= 9) ? 2 : 1;
+template <typename R, int RB>
 __device__ __forceinline__ int lds_regions(int T) {
-  return (2 * (sizeof(C2<R>) << T) <= 128 * 1024) ? 2 : 1;
+  return (2 * (sizeof(C2<R>) << T) <= 128 * 1024 && T - RB >= 9) ? 2 : 1;
 }
 
 // Program words, block matrices and observable tables are read-only for the
@@ -1197,7 +1200,7 @@ __device__ __forceinline__ void relayout(R* re, R* im, C2<R>* lds, const Layout&
 template <typename R, int RB, int TB = kTileBits<R>>
 __device__ __forceinline__ void relayout_pair(R* pr, R* pi, R* lr, R* li, C2<R>* lds, const Layout& from,
                                               const Layout& to, bool tail_sync = true) {
-  if constexpr (kRegions<R, TB> == 1) {
+  if constexpr (kRegions<R, TB, RB> == 1) {
     relayout_put<R, RB, TB>(pr, pi, lds, from, 0);
     __syncthreads();
     relayout_get<R, RB, TB>(pr, pi, lds, to, 0);
@@ -1937,7 +1940,7 @@ __device__ __forceinline__ void sv_backward_impl(const SvArgs& a0) {
   const SvArgs a = resolve_args<R>(a0, circ);
   const int T = ((cint*)a.prog)[2];
   C2<R>* lds = reinterpret_cast<C2<R>*>(smem);  // psi and lambda tiles (relayout2)
-  ObsLds* obs = reinterpret_cast<ObsLds*>(smem + ((size_t)lds_regions<R>(T) * sizeof(C2<R>) << T));
+  ObsLds* obs = reinterpret_cast<ObsLds*>(smem + ((size_t)lds_regions<R, RB>(T) * sizeof(C2<R>) << T));
   double* slots = reinterpret_cast<double*>(obs + 1);
   const int tile = blockIdx.x;
   cint* pass = (cint*)a.prog + ((cint*)a.prog)[HDR_WORDS + a.pass];
@@ -2088,7 +2091,7 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
   const int ntiles = P[8];
   const int tlog = __builtin_ctz((unsigned)ntiles);
   C2<R>* lds = reinterpret_cast<C2<R>*>(smem);  // psi and lambda tiles (relayout2)
-  ObsLds* obs = reinterpret_cast<ObsLds*>(smem + ((size_t)lds_regions<R>(T) * sizeof(C2<R>) << T));
+  ObsLds* obs = reinterpret_cast<ObsLds*>(smem + ((size_t)lds_regions<R, RB>(T) * sizeof(C2<R>) << T));
   // two cotangent slot buffers, alternating from item to item: a buffer is
   // read (and cleared) right after its item's flushes and next written two
   // items later, so no barrier is needed to clear it
@@ -2192,6 +2195,71 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
     w = next;
     par ^= 1;
   }
+}
+
+// Backward of one tile per workgroup for plan-specialised bodies (workgroups
+// of fewer than 512 threads, or 14-bit tiles): no next-tile prefetch, so the
+// state registers fit the budget of two workgroups per CU; psi and lambda
+// are read under the first substep's layout and lambda (and psi) written
+// under the last one when the body allows it (Body::kLoad / kStore), as the
+// persistent loop does.
+template <typename R, int RB, int NT, typename Body>
+__device__ __forceinline__ void sv_backward_direct(const SvArgs& a0) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NR = 1 << RB;
+  constexpr int LNT = __builtin_ctz(NT);
+  int circ;
+  const SvArgs a = resolve_args<R>(a0, circ);
+  cint* P = (cint*)a.prog;
+  const int T = P[2];
+  C2<R>* lds = reinterpret_cast<C2<R>*>(smem);
+  ObsLds* obs = reinterpret_cast<ObsLds*>(smem + ((size_t)lds_regions<R, RB>(T) * sizeof(C2<R>) << T));
+  double* slots = reinterpret_cast<double*>(obs + 1);
+  cint* pass = P + P[HDR_WORDS + a.pass];
+  cint* tbits = pass + 4;
+  const bool seed = (pass[1] & PF_LAST) != 0 && !a.lam_given;
+  const int nslots = pass[2];
+  const int slot_base = pass[3];
+  const int tile = blockIdx.x;
+  for (int e = threadIdx.x; e < nslots; e += NT) slots[e] = 0;
+  load_obs(a, obs);  // (ends with a barrier)
+  const bool full = Body::kFull;
+  const uint64_t base = tile_base(tile, tbits, T, a.n_wires);
+  const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + (int64_t)circ * a.src_stride;
+  C2<R>* lam = reinterpret_cast<C2<R>*>(a.lam) + (int64_t)circ * a.lam_stride;
+  R pr[NR], pi[NR], lr[NR], li[NR];
+  Layout L;
+  if constexpr (Body::kLoad != 0) {
+    Body::template first_layout<RB, LNT>(tbits, L);
+    load_direct<R, RB, Body::kLoad == 1>(pr, pi, src, L, base);
+    if (!seed) load_direct<R, RB, Body::kLoad == 1>(lr, li, lam, L, base);
+  } else {
+    canonical_layout<RB, LNT>(tbits, L);
+    load_canon<R, RB>(pr, pi, src, a.vec_len, L, base, full);
+    if (!seed) load_canon<R, RB>(lr, li, lam, a.vec_len, L, base, full);
+  }
+  cmat<R>* pool = (cmat<R>*)a.mats + (int64_t)circ * a.pool_stride + a.pool_off;
+  if (seed) seed_lambda<R, RB>(pr, pi, lr, li, a, obs, circ, L, base);
+  if constexpr (Body::kLoad != 0)
+    Body::template bwd0<R, RB, NT, false>(P, a.pass, pr, pi, lr, li, L, base, lds, pool, slots);
+  else
+    Body::template bwd<R, RB, NT, false>(P, a.pass, pr, pi, lr, li, L, base, lds, pool, slots);
+  C2<R>* pout = a.dst != nullptr ? reinterpret_cast<C2<R>*>(a.dst) + (int64_t)circ * a.dst_stride : nullptr;
+  if constexpr (Body::kStore != 0) {
+    Body::template last_layout<RB, LNT>(tbits, L);
+    store_direct<R, RB, Body::kStore == 1>(lr, li, lam, L, base);
+    if (pout != nullptr) store_direct<R, RB, Body::kStore == 1>(pr, pi, pout, L, base);
+  } else if (pout != nullptr) {
+    to_canonical2<R, RB, NT>(pr, pi, lr, li, tbits, L, lds);
+    store_canon<R, RB>(lr, li, lam, a.vec_len, L, base, full);
+    store_canon<R, RB>(pr, pi, pout, a.vec_len, L, base, full);
+  } else {
+    to_canonical<R, RB, NT>(lr, li, tbits, L, lds);
+    store_canon<R, RB>(lr, li, lam, a.vec_len, L, base, full);
+  }
+  __syncthreads();  // every flush has landed
+  double* gp = a.grad_part + (int64_t)circ * a.part_stride + slot_base + tile;
+  for (int e = threadIdx.x; e < nslots; e += NT) gp[(int64_t)e * gridDim.x] = slots[e];
 }
 
 #ifndef __HIPCC_RTC__

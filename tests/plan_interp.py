@@ -65,13 +65,14 @@ def member_full(mt, theta, xm, deriv=False):
 
 
 class Program:
-    def __init__(self, plan, quat=False):
+    def __init__(self, plan, quat=False, backward_table=4):
         # quat: reduce the cotangent of MC1 gates flagged in word 15 to
         # S = bar00 + conj(bar11) (bar00 slot) and D = bar10 - conj(bar01)
         # (bar10 slot), as the plan-specialised backward does
         self.quat = quat
         self.w = plan.program()
-        self.wb = plan.table(4)
+        # backward program: 4 the prebuilt kernels', 5 the plan-specialised kernels'
+        self.wb = plan.table(backward_table)
         self.bt = plan.table(0).reshape(-1, BW)
         self.ml = plan.table(1)
         self.mt = plan.table(2).reshape(-1, MW)

@@ -356,7 +356,7 @@ def test_tuning_knobs_are_validated():
     from jaqsi_amd import _native as nat, lowering
 
     low = lowering.lower_pure(_record(lambda: __import__("jaqsi_amd").Gates.RX(0.1, wires=0)), 2)
-    for kw in ({"low_bits": 2}, {"low_bits": 7}, {"variant_flags": 8}):
+    for kw in ({"low_bits": 2}, {"low_bits": 7}, {"variant_flags": 16}):
         with pytest.raises(ValueError):
             nat.Plan(2, low.ops, low.members, 1, low.n_matrix_elems, 0, **kw)
 
@@ -410,3 +410,26 @@ def test_fourteen_bit_tiles_refuse_what_they_cannot_run():
     with pytest.raises(NotImplementedError, match="13-bit tiles"):
         plan(lambda: J.Operation(wires=[0, 5, 9], matrix=u8), 14, 14)
     assert plan(lambda: J.Gates.RX(0.1, wires=13), 14, 14).info()["tile_bits"] == 14
+
+
+@pytest.mark.parametrize("bwd_rb", [4, 5])
+def test_specialised_backward_program_matches_oracle(bwd_rb):
+    """The backward program the plan-specialised kernels run (16 amplitudes x
+    512 threads, or 32 x 256 threads sharing the CU two workgroups at a time)
+    gives the oracle's adjoint gradients on the CPU interpreter."""
+    from circuits import record_hea
+    from jaqsi_amd import _native as nat, lowering
+    from plan_interp import Program
+
+    n, layers = 16, 3
+    theta = np.random.default_rng(16).uniform(0, 2 * np.pi, (layers, n, 3))
+    low = lowering.lower_pure(_record(lambda: record_hea(theta, n)), n)
+    flags = nat.JQ_TUNE_BWD16 if bwd_rb == 4 else 0
+    plan = nat.Plan(n, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0, variant_flags=flags)
+    assert int(plan.table(5)[3]) == bwd_rb
+    angles = np.array([float(a) for a in low.angle_values])
+    ops = sv.hea_ops(theta, n)
+    zs = sv.z_signs(n).sum(0)
+    _, og, _, _ = sv.adjoint(ops, n, sv.z_obs(n))
+    _, _, grads, _ = Program(plan, quat=True, backward_table=5).adjoint(angles, lambda f: 2 * zs * np.conj(f), None)
+    np.testing.assert_allclose(grads, sv.flat_grads(ops, og), atol=1e-10)
