@@ -587,10 +587,10 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
     // and lambda in turn through one LDS region; 14-bit tiles: 512 threads),
     // or with JQ_TUNE_BWD16 16 amplitudes (13-bit tiles: 512 threads, one
     // persistent workgroup per CU with next-tile prefetch; 14-bit tiles: 1024
-    // threads).  13-bit plans with 4x4 / dense 3-wire blocks keep 16
-    // amplitudes, the program the prebuilt kernels run for them.
+    // threads).  (The prebuilt kernels keep their own 16-amplitude program
+    // for 13-bit plans with 4x4 / dense 3-wire blocks.)
     const bool bwd16 = (p->variant_flags & JQ_TUNE_BWD16) != 0;
-    const int rbw = bwd16 || (!t14 && any_full) ? 4 : 5;
+    const int rbw = bwd16 ? 4 : 5;
     jq::HostProgram wide, wide5;
     try {
       jq::build_program(*desc, T, RB, 4, wide, slot_budget, max_blocks);
