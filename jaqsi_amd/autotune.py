@@ -41,6 +41,7 @@ _LOW_CANDIDATES = {0: (5, 4, 6, 3), 1: (6, 5, 4)}  # complex64 / complex128, bui
 _FLAG_CANDIDATES = {0: (nat.JQ_TUNE_BWD16, nat.JQ_TUNE_NO_PERSIST_BWD, nat.JQ_TUNE_NO_DMA_FWD, nat.JQ_TUNE_NARROW_BWD),
                     1: (nat.JQ_TUNE_NO_PERSIST_BWD,)}
 _mode = os.environ.get("JAQSI_AMD_AUTOTUNE", "lookup")
+_MIN_GAIN = 0.01  # smallest relative gain that moves a knob off its current choice
 _lock = threading.Lock()
 _db: Optional[dict] = None
 _shipped: Dict[str, dict] = {}
@@ -186,12 +187,16 @@ def tune(low, dtype: int, batch: int, density: bool = False, jit: Optional[bool]
                   f"({tried[-1]['passes']} passes)", flush=True)
         return ms
 
+    def better(ms: float, best: float) -> bool:
+        # a candidate must win by more than the run-to-run spread
+        return ms < best * (1.0 - _MIN_GAIN)
+
     cands = _LOW_CANDIDATES[dtype]
     best_low, best_ms = cands[0], measure(cands[0], 0)
     default_ms = best_ms
     for lb in cands[1:]:
         ms = measure(lb, 0)
-        if ms < best_ms:
+        if better(ms, best_ms):
             best_low, best_ms = lb, ms
     # 14-bit tiles (complex64): one 1024-thread workgroup per 2^14 amplitudes
     # (plan-specialised kernels only; wide-pass and dense 3-wire blocks keep
@@ -202,14 +207,14 @@ def tune(low, dtype: int, batch: int, density: bool = False, jit: Optional[bool]
             ms = measure(best_low, 0, 14)
         except (NotImplementedError, ValueError, RuntimeError):
             ms = float("inf")
-        if ms < best_ms:
+        if better(ms, best_ms):
             best_tile, best_ms = 14, ms
     best_flags = 0
     # (14-bit plans always run the non-persistent, 32-amplitude backward)
     flags = _FLAG_CANDIDATES[dtype] if best_tile != 14 else (nat.JQ_TUNE_BWD16, nat.JQ_TUNE_NO_DMA_FWD)
     for fl in flags:
         ms = measure(best_low, best_flags | fl, best_tile)
-        if ms < best_ms:
+        if better(ms, best_ms):
             best_flags, best_ms = best_flags | fl, ms
     rec = {"low_bits": best_low, "tile_bits": best_tile, "variant_flags": best_flags, "ms": round(best_ms, 4),
            "default_ms": round(default_ms, 4), "batch": batch, "jit": use_jit, "tried": tried}
