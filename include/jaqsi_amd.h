@@ -117,7 +117,8 @@ typedef struct jq_plan_desc {
   int32_t n_params;       /* length of the angle vector of one circuit */
   int32_t n_matrix_elems; /* complex elements of the explicit-matrix pool of one circuit */
   int32_t dtype;          /* 0 = complex64, 1 = complex128 */
-  int32_t tile_bits;      /* 0 = choose; else tile size 2^tile_bits (<= n_wires) */
+  int32_t tile_bits;      /* 0 = choose; else tile size 2^tile_bits (<= n_wires); complex64 also takes 14
+                           * (plan-specialised kernels only, no 4x4 or dense 3-wire blocks) */
   int32_t checkpoints;    /* 1 = keep every pass output for the backward sweep */
   /* Tuning knobs, 0 = the built-in choice (jaqsi_amd.autotune times the
    * candidates per circuit structure and batch and caches the winner):
