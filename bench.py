@@ -91,7 +91,10 @@ def record_hea(theta, n, features=None):
             Gates.CX(wires=[q, (q + 1) % n])
 
 
-def build_plan(n, layers, features=False, tile_bits=0):
+def build_plan(n, layers, features=False, tile_bits=0, fold=True):
+    """Plan of the ansatz, as the Script path builds it for Z expectation
+    values: the final CX ring is measured through the observables
+    (fold_tail), unless fold is False."""
     from jaqsi_amd import autotune, lowering
     from jaqsi_amd import _native as nat
     from jaqsi_amd.tape import recording
@@ -100,15 +103,16 @@ def build_plan(n, layers, features=False, tile_bits=0):
     with recording() as tape:
         record_hea(theta, n, np.zeros(n) if features else None)
     low = lowering.lower_pure(tape, n)
-    knobs = autotune.config_for(low, 0, 1, jit=USE_JIT)
+    knobs = autotune.config_for(low, 0, 1, jit=USE_JIT, fold=fold)
     plan = nat.Plan(n, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, 0,
-                    tile_bits or knobs.get("tile_bits", 0), 1, knobs["low_bits"], knobs["variant_flags"])
+                    tile_bits or knobs.get("tile_bits", 0), 1, knobs["low_bits"], knobs["variant_flags"],
+                    fold_tail=fold)
     mats = lowering.matrix_tensor(low.matrix_values, low.n_matrix_elems, None)
-    plan.tuning = autotune.report(low, 0, 1)
+    plan.tuning = autotune.report(low, 0, 1, fold=fold)
     return plan, mats
 
 
-def bench_c2(steps, warmup, n=24, layers=8):
+def bench_c2(steps, warmup, n=24, layers=8, fold=True):
     import torch
 
     from jaqsi_amd import _native as nat
@@ -117,7 +121,7 @@ def bench_c2(steps, warmup, n=24, layers=8):
     dev = torch.device("cuda", 0)
     rng = np.random.default_rng(1002)
     theta = rng.uniform(0, 2 * np.pi, (layers, n, 3))
-    plan, mats = build_plan(n, layers)
+    plan, mats = build_plan(n, layers, fold=fold)
     compile_s = specialise(plan)
     mats = mats.reshape(1, -1).to(dev) if mats is not None else None
     info = plan.info()
@@ -165,6 +169,7 @@ def bench_c2(steps, warmup, n=24, layers=8):
         "n": n,
         "compile_s": compile_s,
         "tuning": plan.tuning,
+        "folded": plan.fold()[0],
     }
 
 
@@ -1033,6 +1038,7 @@ def main():
             "params": 576,
             "passes": res["info"]["n_passes"],
             "tile_bits": res["info"]["tile_bits"],
+            "folded_tail_gates": res["folded"],
             "pass_kernels": "plan-specialised" if USE_JIT else "interpreter",
             "specialise_compile_s": res["compile_s"],
             "autotune": res["tuning"],
@@ -1040,6 +1046,17 @@ def main():
         "roofline": roofline_c2(res),
     }
     print(f"c2 leg done: {value:.3f} gradient evaluations/s", file=sys.stderr, flush=True)
+    # the same step with every gate in the passes (the final CX ring
+    # simulated instead of measured through the observables)
+    uf = bench_c2(10, 2, fold=False)
+    line["unfolded"] = {
+        "value": round(10 / uf["seconds"], 3),
+        "ms_per_step": round(uf["seconds"] / 10 * 1e3, 4),
+        "passes": uf["info"]["n_passes"],
+        "max_abs_diff_expvals": float(np.max(np.abs(uf["expvals"] - res["expvals"]))),
+        "max_abs_diff_grads": float(np.max(np.abs(uf["grads"] - res["grads"]))),
+        "note": "the plan with fold_tail off: the final CX ring runs in the passes (10 timed steps)",
+    }
     sp_ms = script_path_c2(res["theta"])
     line["script_path"] = {
         "ms_per_step": round(sp_ms, 4),

@@ -62,6 +62,8 @@ extern "C" {
 #define JQ_MF_CONJ 1    /* use the complex conjugate of the factor          */
 #define JQ_MF_SUPEROP 2 /* use kron(m, conj(m)) on (ket wires, bra wires)   */
 #define JQ_MF_GRAD 4    /* an explicit matrix whose cotangent is requested  */
+#define JQ_MF_UNITARY 8 /* an explicit matrix known to be unitary (a library
+                           gate's own matrix): lets fold_tail drop it when diagonal */
 
 /* op structure flags (GateStructure, operations.py:176-185) */
 #define JQ_S_DIAGONAL 1
@@ -128,6 +130,19 @@ typedef struct jq_plan_desc {
    *   variant_flags  JQ_TUNE_* bits selecting kernel variants */
   int32_t low_bits;
   int32_t variant_flags;
+  /* 1 = the plan is only measured through statevector observables
+   * (JQ_OBS_SV_DIAG / JQ_OBS_SV_PARITY expectation values and their
+   * gradients): trailing parameter-free blocks that permute basis states
+   * affinely (X, CX, SWAP and their products) or are diagonal (CZ, S, T,
+   * fixed phases), with no later op on their wires, are not simulated; the
+   * observables are measured through them instead (a Pauli-Z word after the
+   * tail is a signed Pauli-Z word before it).  Expectation values and
+   * gradients are those of the full circuit; `state` outputs hold the vector
+   * before the folded tail, and a final-vector cotangent (lam_seed) or
+   * density-matrix observables are refused (JQ_ERR_UNSUPPORTED).
+   * jq_plan_tables(6) reports the fold.  No reference counterpart: the
+   * reference simulates every gate (simulation.py:400-409). */
+  int32_t fold_tail;
 } jq_plan_desc;
 
 #define JQ_TUNE_NO_PERSIST_BWD 1 /* backward: one tile per workgroup, no persistent loop with prefetch */
@@ -156,7 +171,9 @@ int jq_plan_program(const jq_plan* plan, int32_t* out, int64_t capacity, int64_t
 
 /* Host copies of the plan's tables: which = 0 block table, 1 member list,
  * 2 member table, 3 cotangent slot map, 4 backward program of the prebuilt
- * kernels, 5 backward program of the plan-specialised kernels.  No
+ * kernels, 5 backward program of the plan-specialised kernels, 6 the folded
+ * tail (fold_tail): [ops folded, flip low/high 32 bits, then per wire w the
+ * low/high 32 bits of the wire mask whose parity is wire w's final bit].  No
  * reference counterpart: these let tests execute the scheduled program on
  * the CPU. */
 int jq_plan_tables(const jq_plan* plan, int32_t which, int32_t* out, int64_t capacity, int64_t* size);

@@ -10,7 +10,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import threading
-from typing import Optional
+from typing import Optional, Tuple
 
 import numpy as np
 
@@ -25,7 +25,7 @@ JQ_ERR_DEVICE = 4
 JQ_ERR_NONFINITE = 5
 
 JQ_M_ROT, JQ_M_PHASE, JQ_M_MATRIX = 0, 1, 2
-JQ_MF_CONJ, JQ_MF_SUPEROP, JQ_MF_GRAD = 1, 2, 4
+JQ_MF_CONJ, JQ_MF_SUPEROP, JQ_MF_GRAD, JQ_MF_UNITARY = 1, 2, 4, 8
 JQ_S_DIAGONAL, JQ_S_PERMUTATION, JQ_S_NONUNITARY = 1, 2, 4
 JQ_OBS_NONE, JQ_OBS_SV_DIAG, JQ_OBS_DM_DIAG, JQ_OBS_SV_PARITY = 0, 1, 2, 3
 MAX_WIRES = 6
@@ -76,6 +76,7 @@ class PlanDesc(C.Structure):
         ("checkpoints", C.c_int32),
         ("low_bits", C.c_int32),
         ("variant_flags", C.c_int32),
+        ("fold_tail", C.c_int32),
     ]
 
 
@@ -233,14 +234,15 @@ class Plan:
     """Owner of a native jq_plan (one fused, scheduled circuit structure)."""
 
     def __init__(self, n_wires, ops, members, n_params, n_matrix_elems, dtype, tile_bits=0, checkpoints=1,
-                 low_bits=0, variant_flags=0):
+                 low_bits=0, variant_flags=0, fold_tail=0):
         L = lib()
         self._ops = (Op * max(1, len(ops)))(*ops)
         self._members = (Member * max(1, len(members)))(*members)
         d = PlanDesc(n_wires, len(ops), self._ops, len(members), self._members, n_params, n_matrix_elems, dtype,
-                     tile_bits, checkpoints, low_bits, variant_flags)
+                     tile_bits, checkpoints, low_bits, variant_flags, int(fold_tail))
         self.low_bits = low_bits
         self.variant_flags = variant_flags
+        self.fold_tail = int(fold_tail)
         h = C.c_void_p()
         check(L.jq_plan_create(C.byref(d), C.byref(h)))
         self.handle = h
@@ -274,6 +276,15 @@ class Plan:
         out = np.zeros(max(1, n.value), dtype=np.int32)
         check(lib().jq_plan_program(self.handle, out.ctypes.data_as(C.POINTER(C.c_int32)), out.size, C.byref(n)))
         return out[: n.value]
+
+    def fold(self) -> Tuple[int, np.ndarray, int]:
+        """(ops folded, rows, flip) of the tail measured through the
+        observables: the final bit of wire w is the parity of the wires in
+        rows[w] (bit v = wire v) on the vector the passes compute, XOR bit w
+        of flip."""
+        t = self.table(6).astype(np.int64) & 0xFFFFFFFF
+        rows = t[3::2] | (t[4::2] << 32)
+        return int(t[0]), rows.astype(np.uint64), int(t[1] | (t[2] << 32))
 
     def table(self, which: int) -> np.ndarray:
         n = C.c_int64()

@@ -73,8 +73,11 @@ def batch_bucket(batch: int) -> int:
     return 1 << max(0, int(batch) - 1).bit_length()
 
 
-def entry_key(arch_name: str, low_key: tuple, dtype: int, batch: int) -> str:
-    return f"{arch_name}|c{128 if dtype else 64}|b{batch_bucket(batch)}|{structure_hash(low_key)}"
+def entry_key(arch_name: str, low_key: tuple, dtype: int, batch: int, fold: bool = False) -> str:
+    """Cache key; ``fold`` marks plans whose trailing permutation / diagonal
+    gates are measured through the observables (fewer passes)."""
+    k = f"{arch_name}|c{128 if dtype else 64}|b{batch_bucket(batch)}|{structure_hash(low_key)}"
+    return k + "|fold" if fold else k
 
 
 def db_path() -> str:
@@ -104,19 +107,24 @@ def _shipped_table(arch_name: str) -> dict:
     return _shipped[arch_name]
 
 
-def lookup(low_key: tuple, dtype: int, batch: int) -> Optional[dict]:
-    """The cached choice (user database first, then the shipped table)."""
+def lookup(low_key: tuple, dtype: int, batch: int, fold: bool = False) -> Optional[dict]:
+    """The cached choice (user database first, then the shipped table); a
+    folded plan without an entry of its own takes the unfolded one's."""
     a = arch()
-    k = entry_key(a, low_key, dtype, batch)
-    with _lock:
-        rec = _load_db().get(k)
-    if rec is None:
-        rec = _shipped_table(a).get(k)
+    keys = [entry_key(a, low_key, dtype, batch, True)] if fold else []
+    keys.append(entry_key(a, low_key, dtype, batch))
+    for k in keys:
+        with _lock:
+            rec = _load_db().get(k)
+        if rec is None:
+            rec = _shipped_table(a).get(k)
+            if rec is not None:
+                rec = dict(rec, source="table")
+        elif "source" not in rec:
+            rec = dict(rec, source="database")
         if rec is not None:
-            rec = dict(rec, source="table")
-    elif "source" not in rec:
-        rec = dict(rec, source="database")
-    return rec
+            return rec
+    return None
 
 
 def _store(k: str, rec: dict) -> None:
@@ -162,7 +170,7 @@ def _time_step(plan, batch: int, n_params: int, mats, density: bool, reps: int, 
 
 
 def tune(low, dtype: int, batch: int, density: bool = False, jit: Optional[bool] = None, reps: int = 3,
-         rounds: int = 3, verbose: bool = False) -> dict:
+         rounds: int = 3, verbose: bool = False, fold: bool = False) -> dict:
     """Time the candidates for a lowered circuit and cache the fastest.
 
     ``jit`` selects plan-specialised kernels for the timing (default: for
@@ -176,7 +184,7 @@ def tune(low, dtype: int, batch: int, density: bool = False, jit: Optional[bool]
 
     def measure(low_bits: int, flags: int, tile_bits: int = 0) -> float:
         plan = nat.Plan(low.n_wires, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, dtype,
-                        tile_bits, 1, low_bits, flags)
+                        tile_bits, 1, low_bits, flags, fold_tail=fold and not density)
         if use_jit or tile_bits == 14:
             plan.jit(nat.JQ_JIT_SYNC)
         ms = _time_step(plan, batch, len(low.angle_values), mats, density, reps, rounds)
@@ -218,23 +226,26 @@ def tune(low, dtype: int, batch: int, density: bool = False, jit: Optional[bool]
             best_flags, best_ms = best_flags | fl, ms
     rec = {"low_bits": best_low, "tile_bits": best_tile, "variant_flags": best_flags, "ms": round(best_ms, 4),
            "default_ms": round(default_ms, 4), "batch": batch, "jit": use_jit, "tried": tried}
-    _store(entry_key(arch(), low.key, dtype, batch), rec)
+    _store(entry_key(arch(), low.key, dtype, batch, fold and not density), rec)
     _memo.clear()
     return dict(rec, source="tuned")
 
 
-def config_for(low, dtype: int, batch: int, density: bool = False, jit: Optional[bool] = None) -> dict:
+def config_for(low, dtype: int, batch: int, density: bool = False, jit: Optional[bool] = None,
+               fold: bool = False) -> dict:
     """The knobs plan creation uses for this structure (see the module
     docstring for the modes)."""
     if _mode == "off" or not torch.cuda.is_available():
         return dict(DEFAULT)
-    mk = (low.key, dtype, batch_bucket(batch), density, _mode)
+    fold = bool(fold) and not density
+    mk = (low.key, dtype, batch_bucket(batch), density, _mode, fold)
     hit = _memo.get(mk)
     if hit is not None:
         return dict(hit)
-    rec = lookup(low.key, dtype, batch)
+    rec = lookup(low.key, dtype, batch, fold)
     if rec is None and _mode == "tune":
-        rec = tune(low, dtype, batch, density, jit=jit, verbose=bool(os.environ.get("JAQSI_AMD_AUTOTUNE_VERBOSE")))
+        rec = tune(low, dtype, batch, density, jit=jit, verbose=bool(os.environ.get("JAQSI_AMD_AUTOTUNE_VERBOSE")),
+                   fold=fold)
     out = dict(DEFAULT) if rec is None else {"low_bits": int(rec["low_bits"]),
                                                "variant_flags": int(rec["variant_flags"]),
                                                "tile_bits": int(rec.get("tile_bits", 0))}
@@ -265,12 +276,12 @@ def _hea_low(n: int, layers: int, features: bool = False):
     return lowering.lower_pure(tape, n)
 
 
-def report(low, dtype: int, batch: int) -> dict:
+def report(low, dtype: int, batch: int, fold: bool = False) -> dict:
     """The knobs plan creation uses for this structure and where they come
     from ("table", "database", "tuned" or "built-in"), for logs and bench
     records."""
-    knobs = config_for(low, dtype, batch)
-    rec = lookup(low.key, dtype, batch) if _mode != "off" and torch.cuda.is_available() else None
+    knobs = config_for(low, dtype, batch, fold=fold)
+    rec = lookup(low.key, dtype, batch, fold) if _mode != "off" and torch.cuda.is_available() else None
     out = dict(knobs, source=rec.get("source", "database") if rec else "built-in", mode=_mode)
     if rec and "default_ms" in rec:
         out.update(tuned_ms=rec["ms"], built_in_ms=rec["default_ms"])
@@ -285,15 +296,17 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     picked = {}
     for w in args.workloads.split(","):
+        # (the 24-qubit gradient is measured through Z observables: its plan
+        # folds the final CX ring)
         if w == "hea24":
-            low, batch = _hea_low(24, 8), 1
+            low, batch, fold = _hea_low(24, 8), 1, True
         elif w == "hea14":
-            low, batch = _hea_low(14, 12, features=True), 256
+            low, batch, fold = _hea_low(14, 12, features=True), 256, False
         else:
             raise SystemExit(f"unknown workload {w!r}")
         print(f"tuning {w}", flush=True)
-        rec = tune(low, 0, batch, jit=True, verbose=True)
-        picked[entry_key(arch(), low.key, 0, batch)] = {k: v for k, v in rec.items() if k != "source"}
+        rec = tune(low, 0, batch, jit=True, verbose=True, fold=fold)
+        picked[entry_key(arch(), low.key, 0, batch, fold)] = {k: v for k, v in rec.items() if k != "source"}
         print(f"{w}: low_bits={rec['low_bits']} variant_flags={rec['variant_flags']} {rec['ms']} ms "
               f"(built-in {rec['default_ms']} ms)", flush=True)
     if args.out:

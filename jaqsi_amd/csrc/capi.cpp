@@ -204,6 +204,8 @@ struct jq_plan {
   std::vector<int32_t> words_bj;  // backward program of the plan-specialised kernels
   DevBuf prog_bj;
   int regions_b = 2, regions_bj = 2;  // LDS state regions of the interpreter's / specialised backward
+  jq::TailFold fold;                  // trailing blocks measured through the observables (fold_tail)
+  std::vector<int32_t> fold_words;    // jq_plan_tables(6)
   bool uploaded = false;
   int class_mask = 0;  // block classes of the prep / chain kernels (jq::block_class)
   std::vector<int32_t> wg_blocks;  // blocks of the workgroup prep / chain classes
@@ -514,7 +516,29 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
     if (const char* e = std::getenv("JQ_T14_PASS_BLOCKS")) max_blocks = std::max(1, std::atoi(e));
     max_blocks = std::min(max_blocks, kJitMaxPassGates);
   }
+  if (desc->fold_tail != 0 && desc->fold_tail != 1) return fail(JQ_ERR_ARG, "fold_tail must be 0 or 1");
   jq_plan* p = new jq_plan();
+  // the passes run the ops before the folded tail
+  std::vector<jq_op> kept_ops;
+  jq_plan_desc dd = *desc;
+  if (desc->fold_tail) {
+    jq::fold_tail(*desc, kept_ops, p->fold);
+    if (p->fold.n_folded > 0) {
+      dd.ops = kept_ops.data();
+      dd.n_ops = (int32_t)kept_ops.size();
+    }
+  }
+  {
+    std::vector<int32_t>& fw = p->fold_words;
+    fw.push_back(p->fold.n_folded);
+    fw.push_back((int32_t)(uint32_t)p->fold.flip);
+    fw.push_back((int32_t)(uint32_t)(p->fold.flip >> 32));
+    for (uint64_t r : p->fold.rows) {
+      fw.push_back((int32_t)(uint32_t)r);
+      fw.push_back((int32_t)(uint32_t)(r >> 32));
+    }
+  }
+  desc = &dd;
   try {
     jq::build_program(*desc, T, RB, RBb, p->hp, slot_budget, max_blocks);
   } catch (const jq::PlanError& e) {
@@ -727,6 +751,7 @@ int jq_plan_tables(const jq_plan* p, int32_t which, int32_t* out, int64_t capaci
     case 3: v = &p->slot_words; break;
     case 4: v = &p->hp.words_b; break;
     case 5: v = &p->words_bj; break;
+    case 6: v = &p->fold_words; break;
     default: return fail(JQ_ERR_ARG, "unknown table");
   }
   if (size) *size = (int64_t)v->size();
@@ -768,13 +793,36 @@ int fill_obs(jq_plan* p, jq::SvArgs& a, int n_wires, int obs_kind, int n_obs, co
     return fail(JQ_ERR_ARG, "unknown observable kind");
   if (n_obs < 0 || n_obs > 64) return fail(JQ_ERR_ARG, "between 0 and 64 observables are supported");
   if (n_obs > 0 && (!obs_wires || !obs_d)) return fail(JQ_ERR_ARG, "null observable arrays");
+  const bool folded = p->fold.n_folded > 0;
+  if (folded && obs_kind == JQ_OBS_DM_DIAG)
+    return fail(JQ_ERR_UNSUPPORTED, "a plan with a folded tail measures statevector observables only");
   const int span = obs_kind == JQ_OBS_DM_DIAG ? n_wires / 2 : n_wires;
   jq::ObsTable t;
   std::memset(&t, 0, sizeof(t));
   for (int o = 0; o < n_obs; ++o) {
-    if (obs_kind == JQ_OBS_SV_PARITY) {
-      const uint64_t wm = (uint64_t)(uint32_t)obs_wires[2 * o] | ((uint64_t)(uint32_t)obs_wires[2 * o + 1] << 32);
-      if (span < 64 && (wm >> span) != 0) return fail(JQ_ERR_ARG, "observable wire out of range");
+    double d0 = obs_d[2 * o], d1 = obs_d[2 * o + 1];
+    if (obs_kind == JQ_OBS_SV_PARITY || folded) {
+      uint64_t wm;
+      if (obs_kind == JQ_OBS_SV_PARITY) {
+        wm = (uint64_t)(uint32_t)obs_wires[2 * o] | ((uint64_t)(uint32_t)obs_wires[2 * o + 1] << 32);
+        if (span < 64 && (wm >> span) != 0) return fail(JQ_ERR_ARG, "observable wire out of range");
+      } else {
+        if (obs_wires[o] < 0 || obs_wires[o] >= span) return fail(JQ_ERR_ARG, "observable wire out of range");
+        wm = 1ull << obs_wires[o];
+      }
+      if (folded) {
+        // the parity of the final bits in wm is the parity of the pre-tail
+        // bits in the XOR of their rows, flipped by their flip bits
+        uint64_t m2 = 0;
+        int fl = 0;
+        for (int w = 0; w < span; ++w)
+          if ((wm >> w) & 1ull) {
+            m2 ^= p->fold.rows[w];
+            fl ^= (int)((p->fold.flip >> w) & 1ull);
+          }
+        wm = m2;
+        if (fl) std::swap(d0, d1);
+      }
       uint64_t gm = 0;
       for (int w = 0; w < span; ++w)
         if ((wm >> w) & 1ull) gm |= 1ull << (n_wires - 1 - w);
@@ -783,9 +831,10 @@ int fill_obs(jq_plan* p, jq::SvArgs& a, int n_wires, int obs_kind, int n_obs, co
       if (obs_wires[o] < 0 || obs_wires[o] >= span) return fail(JQ_ERR_ARG, "observable wire out of range");
       t.gbit[o] = n_wires - 1 - obs_wires[o];
     }
-    t.d[2 * o] = obs_d[2 * o];
-    t.d[2 * o + 1] = obs_d[2 * o + 1];
+    t.d[2 * o] = d0;
+    t.d[2 * o + 1] = d1;
   }
+  if (folded) a.obs_kind = obs_kind = JQ_OBS_SV_PARITY;
   if (!table) {
     HIP_TRY(WS(p, st).obs.ensure(sizeof(jq::ObsTable), st));
     table = WS(p, st).obs.p;
@@ -946,6 +995,8 @@ int run_backward(jq_plan* p, int batch, const double* angles, const void* matric
                  const double* obs_d, const double* ct, void* lam_seed, double* grad_angles, void* grad_matrices,
                  void* grad_psi0, bool ck, hipStream_t st) {
   jq::SvArgs a = base_args(p, st);
+  if (lam_seed && p->fold.n_folded > 0)
+    return fail(JQ_ERR_UNSUPPORTED, "a plan with a folded tail takes observables, not a final-vector cotangent");
   if (lam_seed) {
     a.lam_given = 1;
   } else {
@@ -1154,9 +1205,13 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
   // (groups that run their own specialised adjoints may differ in tile size;
   // the shared launches below need one program geometry)
   if (!(all_jit || any_wide))
-    for (int g = 0; g < n_groups; ++g)
+    for (int g = 0; g < n_groups; ++g) {
       if (groups[g].plan->hp.T != p0->hp.T || groups[g].plan->wide_bwd != p0->wide_bwd)
         return fail(JQ_ERR_ARG, "groups on the shared launches must share tile size and backward register tile");
+      // (one observable table serves every group there)
+      if (!(groups[g].plan->fold == p0->fold))
+        return fail(JQ_ERR_UNSUPPORTED, "groups on the shared launches must fold the same tail");
+    }
   // (wide-block passes run their own kernels: such groups take this path too)
   if (all_jit || any_wide) {
     // Groups share nothing (each plan owns its workspaces), so they run on

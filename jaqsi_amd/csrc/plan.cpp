@@ -628,4 +628,114 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
   out.ntiles = ntiles;
 }
 
+// Local affine map of a permutation op: basis state |c> goes to |pi(c)>
+// with pi = perm^-1 (perm maps output rows to input columns).  Returns false
+// unless pi(c) = A c ^ b for all c; cols[j] = A e_j.
+static bool affine_perm(const jq_op& op, std::vector<uint32_t>& cols, uint32_t& b) {
+  const int k = op.nwires, full = 1 << k;
+  std::vector<int> inv(full, -1);
+  for (int r = 0; r < full; ++r) {
+    const int c = op.perm[r];
+    if (c < 0 || c >= full || inv[c] >= 0) return false;
+    inv[c] = r;
+  }
+  b = (uint32_t)inv[0];
+  cols.assign(k, 0);
+  for (int j = 0; j < k; ++j) cols[j] = (uint32_t)inv[1 << j] ^ b;
+  for (int c = 0; c < full; ++c) {
+    uint32_t y = b;
+    for (int j = 0; j < k; ++j)
+      if ((c >> j) & 1) y ^= cols[j];
+    if (y != (uint32_t)inv[c]) return false;
+  }
+  return true;
+}
+
+void fold_tail(const jq_plan_desc& d, std::vector<jq_op>& kept, TailFold& f) {
+  const int n = d.n_wires;
+  f = TailFold();
+  f.rows.resize(n);
+  for (int w = 0; w < n; ++w) f.rows[w] = 1ull << w;
+  kept.assign(d.ops, d.ops + d.n_ops);
+  if (n > 64) return;
+  auto wire_mask = [&](const jq_op& op) {
+    uint64_t m = 0;
+    for (int i = 0; i < op.nwires; ++i)
+      if (op.wires[i] >= 0 && op.wires[i] < n) m |= 1ull << op.wires[i];
+    return m;
+  };
+  auto foldable = [&](const jq_op& op) {
+    if (op.nwires < 1 || op.nwires > JQ_MAX_WIRES || (op.structure & JQ_S_NONUNITARY)) return false;
+    for (int i = 0; i < op.nwires; ++i)
+      if (op.wires[i] < 0 || op.wires[i] >= n) return false;
+    const bool perm = (op.structure & JQ_S_PERMUTATION) != 0;
+    const bool diag = (op.structure & JQ_S_DIAGONAL) != 0;
+    if (!perm && !diag) return false;
+    for (int m = op.member_begin; m < op.member_end; ++m) {
+      if (m < 0 || m >= d.n_members) return false;
+      const jq_member& mm = d.members[m];
+      if (mm.param >= 0 || (mm.flags & (JQ_MF_GRAD | JQ_MF_SUPEROP))) return false;
+      // a diagonal op is unitary when it is built from rotations, phases and
+      // matrices the recording vouches for
+      if (!perm && mm.kind == JQ_M_MATRIX && !(mm.flags & JQ_MF_UNITARY)) return false;
+    }
+    if (perm) {
+      std::vector<uint32_t> cols;
+      uint32_t b;
+      return affine_perm(op, cols, b);
+    }
+    return true;
+  };
+  // walk back from the end: an op folds when no kept op comes after it on
+  // its wires (ops on disjoint wires commute)
+  std::vector<char> folded(d.n_ops, 0);
+  uint64_t blocked = 0;
+  int nf = 0;
+  for (int i = d.n_ops - 1; i >= 0; --i) {
+    const jq_op& op = d.ops[i];
+    const uint64_t wm = wire_mask(op);
+    if (!(wm & blocked) && foldable(op)) {
+      folded[i] = 1;
+      ++nf;
+    } else {
+      blocked |= wm;
+    }
+  }
+  if (nf == 0 || nf == d.n_ops) return;
+  // compose the folded permutations in circuit order
+  for (int i = 0; i < d.n_ops; ++i) {
+    if (!folded[i] || !(d.ops[i].structure & JQ_S_PERMUTATION)) continue;
+    const jq_op& op = d.ops[i];
+    const int k = op.nwires;
+    std::vector<uint32_t> cols;
+    uint32_t b;
+    affine_perm(op, cols, b);
+    // local bit j is wire op.wires[k - 1 - j] (the first wire is the most
+    // significant bit of the op's index)
+    std::vector<uint64_t> nr(k);
+    std::vector<int> nflip(k);
+    for (int li = 0; li < k; ++li) {
+      uint64_t r = 0;
+      int fl = (b >> li) & 1;
+      for (int j = 0; j < k; ++j)
+        if ((cols[j] >> li) & 1) {
+          const int w = op.wires[k - 1 - j];
+          r ^= f.rows[w];
+          fl ^= (int)((f.flip >> w) & 1);
+        }
+      nr[li] = r;
+      nflip[li] = fl;
+    }
+    for (int li = 0; li < k; ++li) {
+      const int w = op.wires[k - 1 - li];
+      f.rows[w] = nr[li];
+      f.flip = (f.flip & ~(1ull << w)) | ((uint64_t)nflip[li] << w);
+    }
+  }
+  kept.clear();
+  for (int i = 0; i < d.n_ops; ++i)
+    if (!folded[i]) kept.push_back(d.ops[i]);
+  f.n_folded = nf;
+}
+
 }  // namespace jq

@@ -1,5 +1,6 @@
 // Planner output shared between plan.cpp and the C-ABI layer.
 #pragma once
+#include <cstdint>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -42,5 +43,29 @@ struct HostProgram {
 // specialised code grows with its gate count).
 void build_program(const jq_plan_desc& d, int T, int RB, int RBb, HostProgram& out, int slot_budget,
                    int max_blocks = 0);
+
+// Trailing blocks folded into the observables.  A unitary block that
+// permutes basis states by an affine map over GF(2) (x -> A x ^ b: X, CX,
+// SWAP and their products) or multiplies them by phases (diagonal blocks),
+// and that nothing later in the circuit touches, does not change the
+// expectation value of a diagonal observable measured after it except by
+// moving which index bits it reads: the final bit of wire w is the parity of
+// the pre-tail bits in rows[w], XOR bit w of flip.  Those blocks are then
+// left out of the passes, and a Pauli-Z word on the final vector becomes a
+// Pauli-Z word (with the sign of `flip`) on the vector the passes compute.
+struct TailFold {
+  int n_folded = 0;            // ops left out of the passes
+  std::vector<uint64_t> rows;  // per wire: wire mask (bit v = wire v)
+  uint64_t flip = 0;           // per wire
+  bool operator==(const TailFold& o) const {
+    return n_folded == o.n_folded && (n_folded == 0 || (rows == o.rows && flip == o.flip));
+  }
+};
+
+// Splits d.ops into the ops the passes run (`kept`, in order) and the
+// folded tail.  Only parameter-free ops fold (their angle gradients would be
+// zero), never non-unitary ones or explicit matrices, and at least one op
+// always stays.
+void fold_tail(const jq_plan_desc& d, std::vector<jq_op>& kept, TailFold& f);
 
 }  // namespace jq

@@ -107,6 +107,12 @@ class Lowered:
         return self._members
 
 
+def _library_unitary(op: Operation) -> bool:
+    """A gate of the library's gate set with its own (unitary) matrix."""
+    return (type(op).__module__ == "jaqsi_amd.gateset" and bool(getattr(op, "is_unitary", False))
+            and not getattr(op, "_explicit", False))
+
+
 def _structure_flags(op: Operation) -> Tuple[int, int, tuple]:
     s = op._structure
     flags = 0
@@ -133,7 +139,9 @@ class _Builder:
         self.grad_mats: list = []
         self.key: list = []
 
-    def add_member(self, m: Member, extra_flags: int = 0) -> None:
+    def add_member(self, m: Member, extra_flags: int = 0, unitary: bool = False) -> None:
+        """``unitary``: the member's matrix is a library gate's own (unitary)
+        matrix, which the planner may then measure through observables."""
         flags = extra_flags
         param, mat = -1, -1
         if m.kind in ("rot", "phase"):
@@ -151,12 +159,15 @@ class _Builder:
             if _requires_grad(val):
                 flags |= nat.JQ_MF_GRAD
                 self.grad_mats.append(len(self.mats))
+            elif unitary:
+                flags |= nat.JQ_MF_UNITARY
             self.mats.append((mat, val))
             self.key.append((kind, shape, m.nctrl, flags))
         self.members.append((kind, flags, m.nctrl, word, param, mat, 0.0))
 
     def add_op(self, wires: Sequence[int], members: Sequence[Member], structure: int, controls: int, perm: tuple,
-               extra_flags: int = 0, superop_matrix: Optional[np.ndarray] = None, cond: float = 1.0) -> None:
+               extra_flags: int = 0, superop_matrix: Optional[np.ndarray] = None, cond: float = 1.0,
+               unitary: bool = False) -> None:
         if len(wires) > nat.MAX_WIRES:
             raise NotImplementedError(f"gates on more than {nat.MAX_WIRES} wires are not supported")
         mb = len(self.members)
@@ -164,7 +175,7 @@ class _Builder:
             self.add_member(Member("matrix", matrix=superop_matrix))
         else:
             for m in members:
-                self.add_member(m, extra_flags)
+                self.add_member(m, extra_flags, unitary)
         self.ops.append((tuple(wires), structure, controls, perm, mb, len(self.members), cond))
         # the planner routes ill-conditioned blocks (product of their channels'
         # condition numbers above JQ_COND_MAX) to checkpointed passes, so a
@@ -199,7 +210,7 @@ def _fragment(op: Operation):
     if frag is None:
         b = _Builder(0)
         flags, controls, perm = _structure_flags(op)
-        b.add_op(op.wires, op.lower(), flags, controls, perm)
+        b.add_op(op.wires, op.lower(), flags, controls, perm, unitary=bool(op.is_unitary))
         if b.grad_mats:
             return None
         ids = {id(v): i for i, v in enumerate(b.angles)}
@@ -256,7 +267,7 @@ def lower_pure(tape: Sequence[Operation], n_qubits: int, fast: bool = True) -> L
         if frag is None:
             cacheable = False
             flags, controls, perm = _structure_flags(op)
-            b.add_op(op.wires, op.lower(), flags, controls, perm)
+            b.add_op(op.wires, op.lower(), flags, controls, perm, unitary=_library_unitary(op))
             continue
         if op.__class__ not in classes:
             cacheable = False

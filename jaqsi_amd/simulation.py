@@ -82,21 +82,25 @@ def _cdtype_torch(code: int) -> torch.dtype:
 
 
 def get_plan(low: lowering.Lowered, dtype: int, tile_bits: int = 0, checkpoints: int = 1, batch: int = 1,
-             density: bool = False, knobs: Optional[dict] = None) -> nat.Plan:
+             density: bool = False, knobs: Optional[dict] = None, fold: bool = False) -> nat.Plan:
     """The cached plan of a lowered structure; its tuning knobs come from the
-    autotuner's cache for (structure, dtype, batch) unless given."""
+    autotuner's cache for (structure, dtype, batch) unless given.  ``fold``:
+    the plan is only measured through diagonal statevector observables, so
+    its trailing permutation / diagonal gates are measured through them
+    instead of simulated (``fold_tail`` of the C-ABI)."""
     if knobs is None:
         from jaqsi_amd import autotune
 
-        knobs = autotune.config_for(low, dtype, batch, density)
+        knobs = autotune.config_for(low, dtype, batch, density, fold=fold)
     lb, vf = int(knobs.get("low_bits", 0)), int(knobs.get("variant_flags", 0))
     tile_bits = int(knobs.get("tile_bits", tile_bits))
-    key = (low.key, dtype, tile_bits, checkpoints, lb, vf)
+    fold = bool(fold) and not density
+    key = (low.key, dtype, tile_bits, checkpoints, lb, vf, fold)
     with _plans_lock:
         p = _plans.get(key)
         if p is None:
             p = nat.Plan(low.n_wires, low.ops, low.members, len(low.angle_values), low.n_matrix_elems, dtype,
-                         tile_bits, checkpoints, lb, vf)
+                         tile_bits, checkpoints, lb, vf, fold_tail=fold)
             if _jit_mode is not None:
                 p.jit(_jit_mode)
             if len(_plans) >= _PLAN_CACHE_MAX:
@@ -456,7 +460,7 @@ class _DmExpvalMatrix(torch.autograd.Function):
 # ---------------------------------------------------------------------------
 # the seam
 # ---------------------------------------------------------------------------
-def _prepare(tape, n_qubits, use_density, batch, initial_state, dtype, values=None):
+def _prepare(tape, n_qubits, use_density, batch, initial_state, dtype, values=None, fold=False):
     low = lowering.lower_mixed(tape, n_qubits) if use_density else lowering.lower_pure(tape, n_qubits)
     dev = device()
     if values is None:
@@ -484,7 +488,7 @@ def _prepare(tape, n_qubits, use_density, batch, initial_state, dtype, values=No
         if psi0.dim() == 2 and psi0.shape[0] > 1:
             psi0_batched = True
         psi0 = psi0.reshape(psi0.shape[0] if psi0_batched else 1, -1).resolve_conj().contiguous()
-    plan = get_plan(low, dtype, batch=angles.shape[0], density=use_density)
+    plan = get_plan(low, dtype, batch=angles.shape[0], density=use_density, fold=fold)
     return low, plan, angles, mats, psi0, psi0_batched
 
 
@@ -519,8 +523,14 @@ def simulate_and_measure(
     if any(isinstance(op, PendingEvolution) for op in tape):
         resolve_pending(tape)
     dtype = _dtype_code()
-    low, plan, angles, mats, psi0, psi0_batched = _prepare(tape, n_qubits, use_density, batch, initial_state, dtype, values)
     runs = getattr(_capture, "runs", None)
+    # expectation values of diagonal single-wire observables measure the
+    # circuit's trailing permutation / diagonal gates through the
+    # observables (a captured run may be replayed for its final vector: no fold)
+    diags = diag_observables(obs) if type == "expval" and shots is None else None
+    fold = bool(diags) and not use_density and runs is None
+    low, plan, angles, mats, psi0, psi0_batched = _prepare(tape, n_qubits, use_density, batch, initial_state, dtype,
+                                                           values, fold)
     if runs is not None:
         runs.append(CapturedRun(plan, n_qubits, use_density, batch, dtype, angles, mats, psi0, psi0_batched))
     B = angles.shape[0]
@@ -531,7 +541,6 @@ def simulate_and_measure(
 
     needs_grad = torch.is_grad_enabled() and any(t is not None and t.requires_grad for t in (angles, mats, psi0))
     if type == "expval" and shots is None:
-        diags = diag_observables(obs)
         if diags is not None and len(diags) > 0:
             runner = _Runner(plan, B, use_density, diags, n_qubits, psi0_batched, dtype)
             ev = _AdjointExpval.apply(angles, mats, psi0, runner)
