@@ -170,6 +170,7 @@ def bench_c2(steps, warmup, n=24, layers=8, fold=True):
         "compile_s": compile_s,
         "tuning": plan.tuning,
         "folded": plan.fold()[0],
+        "bwd_fma": bwd_fma([plan]),
     }
 
 
@@ -186,6 +187,19 @@ def stored_traffic(leg, kernel):
     return rec["total"] if rec else None
 
 
+def bwd_fma(plans):
+    """Backward FMA per amplitude and fused 2x2 block, averaged over the
+    blocks of `plans`: 24 (psi <- U^-1 psi, lambda <- U^T lambda, the 4
+    complex cotangent entries), 22 for Pauli-rotation blocks (block table
+    word 10), whose cotangent the specialised backward keeps as 3 reals."""
+    quat = total = 0
+    for p in plans:
+        bt = p.table(0).reshape(-1, 12)
+        total += len(bt)
+        quat += int((bt[:, 10] != 0).sum())
+    return (22 * quat + 24 * (total - quat)) / max(total, 1)
+
+
 def roofline_c2(res):
     n = res["n"]
     sweep = 2 * 8 * (1 << n)  # read + write of one complex64 vector
@@ -197,8 +211,8 @@ def roofline_c2(res):
     traffic = stored_traffic("c2", dom)
     others = {k: {"avg_us": round(v[0] / max(v[1], 1) * 1e3, 2), "launches": v[1]} for k, v in res["kernels"].items()}
     # The same launches against the FP32 VALU: per amplitude and fused 2x2
-    # block 8 FMA forward, 24 backward, over the plan's blocks per pass
-    fma = dict(zip(kernel_names(), (8, 24)))
+    # block 8 FMA forward, 22 / 24 backward (bwd_fma), over the plan's blocks per pass
+    fma = dict(zip(kernel_names(), (8, res["bwd_fma"])))
     blocks_per_pass = res["info"]["n_blocks"] / res["info"]["n_passes"]
     valu = {}
     for k, (t, c) in res["kernels"].items():
@@ -217,9 +231,9 @@ def roofline_c2(res):
         "algorithmic_bytes_per_launch": bytes_per[dom],
         "kernels": others,
         "valu_fp32": dict(valu, peak_tflops=VALU_F32_PEAK_TFLOPS,
-                          note="fused-block FMAs (8 forward / 24 backward per amplitude and block, averaged over the "
-                          "plan's passes) over the same launch times, against FP32 VALU issue without packed "
-                          "instructions; the backward's FMA floor (~94 us per pass) lies above its HBM floor (50 us)"),
+                          note="fused-block FMAs (8 forward / 22 backward per amplitude and Pauli-rotation block, "
+                          "averaged over the plan's passes) over the same launch times, against FP32 VALU issue "
+                          "without packed instructions; the backward's FMA floor lies above its HBM floor"),
     }
 
 
@@ -247,8 +261,8 @@ def bench_c3(steps, warmup, B=4096, n=14, seed=1003):
     from jaqsi_amd import autotune, lowering
 
     rep = lowering.lower_pure(tapes[-1], n)
-    knobs = autotune.config_for(rep, 0, len(members[uniq[-1]]), jit=USE_JIT)
-    tuning = autotune.report(rep, 0, len(members[uniq[-1]]))
+    knobs = autotune.config_for(rep, 0, len(members[uniq[-1]]), jit=USE_JIT, fold=True)
+    tuning = autotune.report(rep, 0, len(members[uniq[-1]]), fold=True)
     rb = RaggedBatch(tapes, [len(members[d]) for d in uniq], n, [(q, 1.0, -1.0) for q in range(n)], dtype=0,
                      knobs=knobs)
     t_c = time.perf_counter()
@@ -308,6 +322,8 @@ def bench_c3(steps, warmup, B=4096, n=14, seed=1003):
         "kernels": kt,
         "circuit_passes": circuit_passes,
         "circuit_blocks": circuit_blocks,
+        "folded_tail_gates": sorted(set(p.fold()[0] for p in rb.plans)),
+        "bwd_fma": bwd_fma(rb.plans),
         "plans": rb.plans,
         "groups": len(uniq),
         "tuning": tuning,
@@ -333,7 +349,7 @@ def roofline_c3(res, steps):
     beside it; launches of concurrent groups overlap, so the per-kernel
     entries are summed event times of the eager steps."""
     n = res["n"]
-    flops = 2 * (8 + 24) * (1 << n) * res["circuit_blocks"] * steps
+    flops = 2 * (8 + res["bwd_fma"]) * (1 << n) * res["circuit_blocks"] * steps
     ach = flops / res["seconds"] / 1e12
     sweep = 2 * 8 * (1 << n)
     per = dict(zip(kernel_names(), (sweep, sweep * 3 // 2)))
@@ -352,8 +368,8 @@ def roofline_c3(res, steps):
         "frac": round(ach / VALU_F32_PEAK_TFLOPS, 4),
         "peak_packed_fp32": 157.3,
         "traffic": None,
-        "flop_note": "per circuit: fused 2x2 blocks x 2^14 amplitudes x (8 + 24) FMA x 2 FLOP, summed over the "
-        "4096 circuits of each timed step",
+        "flop_note": "per circuit: fused 2x2 blocks x 2^14 amplitudes x (8 + 22) FMA x 2 FLOP (22: the backward "
+        "of a Pauli-rotation block), summed over the 4096 circuits of each timed step",
         "hbm": {"achieved": round(hbm, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm / HBM_PEAK_GBS, 4),
                 "traffic_per_launch": {k: stored_traffic("c3", k) for k in kernel_names()},
                 "bytes_note": "per circuit pass 2*8*2^14 B forward, 3*8*2^14 B backward"},
@@ -1090,6 +1106,7 @@ def main():
                 "groups": r3["groups"],
                 "autotune": r3["tuning"],
                 "circuit_passes": r3["circuit_passes"],
+                "folded_tail_gates": r3["folded_tail_gates"],
                 "circuit_blocks": r3["circuit_blocks"],
                 "tile_bits": sorted({p.info()["tile_bits"] for p in r3["plans"]}),
                 "pass_kernels": "plan-specialised (one launch sequence per group)" if USE_JIT

@@ -900,8 +900,10 @@ int max_slots(jq_plan* p) {
 
 
 // per-tile cotangent partials -> block cotangents -> angle / matrix gradients
+// quat_t: the partials come from the plan-specialised backward, whose
+// rotation blocks flush their cotangent in su(2) form (ChainArgs::quat_t).
 int finish_grads(jq_plan* p, int batch, const double* angles, const void* matrices, double* grad_part,
-                 int64_t part_stride, double* grad_angles, void* grad_matrices, hipStream_t st) {
+                 int64_t part_stride, double* grad_angles, void* grad_matrices, bool quat_t, hipStream_t st) {
   const int ntiles = p->hp.ntiles;
   const int nreals = p->hp.bars * 2;
   if (grad_angles && p->n_params > 0)
@@ -936,6 +938,7 @@ int finish_grads(jq_plan* p, int batch, const double* angles, const void* matric
     c.grad_angles = grad_angles;
     c.grad_matrices = grad_matrices;
     c.class_mask = p->class_mask;
+    c.quat_t = quat_t ? 1 : 0;
     c.wg_blocks = (const int*)p->wg_list.p;
     c.n_wg = (int)p->wg_blocks.size();
     if (grad_angles || grad_matrices) HIP_TRY(jq::launch_chain(c, st));
@@ -1046,7 +1049,7 @@ int run_backward(jq_plan* p, int batch, const double* angles, const void* matric
     HIP_TRY(timed_launch(1, p->pass_full[q], p->dtype, p->hp.T, a, ntiles, batch, sm, st, p->wide_bwd,
                          jit_fn(p, jit, 1, q), q == P - 1, q == 0));
   }
-  return finish_grads(p, batch, angles, matrices, a.grad_part, a.part_stride, grad_angles, grad_matrices, st);
+  return finish_grads(p, batch, angles, matrices, a.grad_part, a.part_stride, grad_angles, grad_matrices, jit, st);
 }
 }  // namespace
 
@@ -1338,7 +1341,7 @@ int jq_sv_adjoint_groups(int32_t n_groups, const jq_group* groups, int32_t obs_k
     const jq_group& G = groups[order[k]];
     jq_plan* p = G.plan;
     rc = finish_grads(p, G.batch, G.angles, G.matrices, (double*)WS(p, st).partials.p, p->hp.partial_reals, G.grad_angles,
-                      nullptr, st);
+                      nullptr, false, st);
     if (rc) return rc;
   }
   return JQ_OK;

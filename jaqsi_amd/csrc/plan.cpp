@@ -273,6 +273,7 @@ void build_program(const jq_plan_desc& d, int T_req, int RB, int RBb, HostProgra
     rec[B_BAR] = b.bar;
     rec[B_DIAGF] = b.diagf;
     rec[B_OFFS] = b.offs;
+    rec[B_QUAT] = b.kind == AK_MC1 && b.quat ? 1 : 0;
     out.block_table.insert(out.block_table.end(), rec, rec + BW);
     if (!b.wide && b.D > 8 && !b.diagf) throw PlanError(JQ_ERR_ARG, "internal: dense tile block wider than 3 wires");
     if (!b.wide && b.D > 32) throw PlanError(JQ_ERR_ARG, "internal: diagonal tile block wider than 5 wires");

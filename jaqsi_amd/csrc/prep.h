@@ -58,6 +58,8 @@ struct ChainArgs {
   double* grad_angles;      // [batch][n_params], pre-zeroed
   void* grad_matrices;      // [batch][n_matrix_elems] complex128, pre-zeroed, may be null
   int class_mask;           // block classes present (block_class)
+  int quat_t;               // 1: B_QUAT blocks hold (t_x, t_y, t_z) = Im Tr(bar^T U sigma_k) in
+                            // (bar00.re, bar00.im, bar10.re) (the plan-specialised backward's form)
   const int* wg_blocks;     // blocks of the workgroup classes (block_class >= 3)
   int n_wg;
 };

@@ -451,6 +451,32 @@ __device__ __forceinline__ void chain_block(const ChainArgs& a, int circ, const 
   cd Rp[N], Lm[N], E[N], W[N], B[N];
 #pragma unroll
   for (int e = 0; e < N; ++e) B[e] = e < ne ? bar[e] : cd{0.0, 0.0};
+  if constexpr (!DIAG && DM == 2) {
+    if (a.quat_t && bt[B_QUAT]) {
+      // The block U = [[al, -conj(be)], [be, conj(al)]] is in SU(2) and every
+      // angle derivative is U X with X in su(2): only t_k = Im Tr(bar^T U
+      // sigma_k) enters the gradients.  The effective cotangent with zero
+      // bar01 / bar11 giving the same Re Tr(bar^T U X) for every X is
+      //   bar00 = -be (i t_x + t_y) + i t_z conj(al),
+      //   bar10 =  al (i t_x + t_y) + i t_z conj(be).
+      identity_t<DIAG, DM>(Lm);
+      for (int j = mb; j < me; ++j) {
+        const int m = a.member_list[j];
+        const int* mt = a.member_table + m * MW;
+        member_eff_t<DIAG, DM>(mt, member_theta(mt, a.member_values[m], angles), xm, D, offs, false, E);
+        left_mul_t<DIAG, DM>(E, Lm, D);
+      }
+      const cd al = Lm[0], be = Lm[2];
+      const double tx = B[0].x, ty = B[0].y, tz = B[2].x;
+      const cd w = {ty, tx};  // i t_x + t_y
+      const cd izt = {0.0, tz};
+      const cd mbe = {-be.x, -be.y};
+      B[0] = cadd(cmul(mbe, w), cmul(izt, cd{al.x, -al.y}));
+      B[2] = cadd(cmul(al, w), cmul(izt, cd{be.x, -be.y}));
+      B[1] = cd{0.0, 0.0};
+      B[3] = cd{0.0, 0.0};
+    }
+  }
   identity_t<DIAG, DM>(Rp);
   for (int j = mb; j < me; ++j) {
     const int m = a.member_list[j];

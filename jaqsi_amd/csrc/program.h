@@ -86,6 +86,7 @@ constexpr int B_INV = 6;      // pool offset of its inverse, -1
 constexpr int B_BAR = 7;      // offset (complex elements) of the cotangent in bars, -1
 constexpr int B_DIAGF = 8;    // 1 if diagonal storage
 constexpr int B_OFFS = 9;     // active-block offset inside the full 2^k matrix
+constexpr int B_QUAT = 10;    // 1: MC1 block of Pauli rotations of its target (G_QUAT)
 
 // member table (int32 records of MW words + the double angle)
 constexpr int MW = 8;

@@ -358,6 +358,42 @@ __device__ __forceinline__ void bw_dense(R* pr, R* pi, R* lr, R* li, const Mat<R
   }
 }
 
+// The backward step of a 2x2 block in SU(2) (a product of Pauli rotations
+// of its target): psi <- U^{-1} psi, lambda <- U^T lambda, and instead of
+// the 4 complex entries of bar = lambda_out (x) psi_in only
+//   t_k = Im(lambda_in^T sigma_k psi_in) = Im Tr(bar^T U sigma_k),
+// (12 FMA per pair instead of 16), which carry every angle derivative
+// (d U / d theta = U X with X in su(2)).  acc = (t_x, t_y, t_z).
+template <typename R, int NR, typename G, bool CTRL>
+__device__ __forceinline__ void bw_su2(R* pr, R* pi, R* lr, R* li, const Mat<R, 2>& u, const Mat<R, 2>& ui,
+                                       uint32_t creg, R* acc) {
+#pragma unroll
+  for (int k = 0; k < G::n; ++k) {
+    if (CTRL && ((uint32_t)G::base(k) & creg) != creg) continue;
+    R xr[2], xi[2], yr[2], yi[2], l0r[2], l0i[2], mr[2], mi[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      xr[q] = pr[G::at(k, q)];
+      xi[q] = pi[G::at(k, q)];
+      l0r[q] = lr[G::at(k, q)];
+      l0i[q] = li[G::at(k, q)];
+    }
+    mv<R, 2>(ui, xr, xi, yr, yi);
+    mvt<R, 2>(u, l0r, l0i, mr, mi);
+    // t_x: Im(m0 y1 + m1 y0); t_y: Re(m1 y0 - m0 y1); t_z: Im(m0 y0 - m1 y1)
+    acc[0] = fma(mr[0], yi[1], fma(mi[0], yr[1], fma(mr[1], yi[0], fma(mi[1], yr[0], acc[0]))));
+    acc[1] = fma(mr[1], yr[0], fma(-mi[1], yi[0], fma(-mr[0], yr[1], fma(mi[0], yi[1], acc[1]))));
+    acc[2] = fma(mr[0], yi[0], fma(mi[0], yr[0], fma(-mr[1], yi[1], fma(-mi[1], yr[1], acc[2]))));
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      pr[G::at(k, q)] = yr[q];
+      pi[G::at(k, q)] = yi[q];
+      lr[G::at(k, q)] = mr[q];
+      li[G::at(k, q)] = mi[q];
+    }
+  }
+}
+
 // The same step in three light sweeps (fewer live registers for 4x4 blocks):
 template <typename R, int NR, int D, typename G>
 __device__ __forceinline__ void sweep_apply(R* re, R* im, const Mat<R, D>& m, uint32_t creg, bool transpose) {
@@ -760,7 +796,7 @@ __device__ __forceinline__ void flush8(const R* v, int nv, double* slots, Dst ds
 // quad_perm xor 1, xor 2, row_half_mirror).  Lane 16k holds the wave total of
 // v[k] and adds it to slots[dst(k)].
 template <typename R, typename Dst>
-__device__ __forceinline__ void flush4(const R* v, double* slots, Dst dst) {
+__device__ __forceinline__ void flush4(const R* v, double* slots, Dst dst, int nv = 4) {
   const R w0 = swap_add32(v[0], v[2]);
   const R w1 = swap_add32(v[1], v[3]);
   R r = swap_add16(w0, w1);
@@ -769,7 +805,7 @@ __device__ __forceinline__ void flush4(const R* v, double* slots, Dst dst) {
   r += dpp_f<0x4E>(r);
   r += dpp_f<0x141>(r);
   const int lane = threadIdx.x & 63;
-  if ((lane & 15) == 0) lds_add_lane(&slots[dst(lane >> 4)], (double)r);
+  if ((lane & 15) == 0 && (lane >> 4) < nv) lds_add_lane(&slots[dst(lane >> 4)], (double)r);
 }
 
 // Reduce per-thread accumulators acc[0 .. nv) (nv <= 8) over the workgroup
@@ -1583,10 +1619,9 @@ __device__ __forceinline__ void jf_diag(R* re, R* im, cmat<R>* d, const Layout& 
 }
 
 // backward gates: psi <- U^{-1} psi, bar += lambda (x) psi, lambda <- U^T lambda
-// QUAT (blocks of Pauli rotations, whose angle derivatives have the form
-// [[a, -b*], [b, a*]]): only S = bar00 + conj(bar11) and D = bar10 - conj(bar01)
-// enter the angle gradients, so S goes to the bar00 slot reals, D to the
-// bar10 ones, and the bar01 / bar11 reals stay zero.
+// QUAT (blocks of Pauli rotations, in SU(2)): the cotangent is flushed as
+// t_x, t_y (bar00 slot reals) and t_z (first bar10 slot real) of bw_su2;
+// the gradient chain turns them back into an effective bar for U.
 template <typename R, int RB, int A, uint32_t CREG, bool INV, bool QUAT = false>
 __device__ __forceinline__ void jb_mc1(R* pr, R* pi, R* lr, R* li, cmat<R>* mp, cmat<R>* ip, bool ok, double* slots,
                                        int slot) {
@@ -1597,15 +1632,17 @@ __device__ __forceinline__ void jb_mc1(R* pr, R* pi, R* lr, R* li, cmat<R>* mp, 
     load2<R>(ip, ui);
   else
     adjoint_of<R, 2>(u, ui);
-  R acc[8];
-#pragma unroll
-  for (int e = 0; e < 8; ++e) acc[e] = 0;
-  if (ok) bw_dense<R, NR, 2, G1<NR, A>, (CREG != 0)>(pr, pi, lr, li, u, ui, CREG, acc);
-  if (slot < 0) return;
   if constexpr (QUAT) {
-    const R v[4] = {acc[0] + acc[6], acc[1] - acc[7], acc[4] - acc[2], acc[5] + acc[3]};
-    flush4<R>(v, slots, [slot](int k) { return slot + (k < 2 ? k : k + 2); });
+    R acc[4] = {0, 0, 0, 0};
+    if (ok) bw_su2<R, NR, G1<NR, A>, (CREG != 0)>(pr, pi, lr, li, u, ui, CREG, acc);
+    if (slot < 0) return;
+    flush4<R>(acc, slots, [slot](int k) { return slot + (k < 2 ? k : k + 2); }, 3);
   } else {
+    R acc[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) acc[e] = 0;
+    if (ok) bw_dense<R, NR, 2, G1<NR, A>, (CREG != 0)>(pr, pi, lr, li, u, ui, CREG, acc);
+    if (slot < 0) return;
     flush_vals<R>(acc, 8, slots, slot);
   }
 }

@@ -24,18 +24,31 @@ from jaqsi_amd.tape import recording
 class RaggedBatch:
     """Groups of circuits; group g has ``batch[g]`` circuits of structure
     ``tapes[g]`` (a recorded tape) on ``n_qubits`` wires.  Observables are
-    single-wire diagonals given as (wire, d0, d1) (PauliZ: (q, 1, -1))."""
+    single-wire diagonals given as (wire, d0, d1) (PauliZ: (q, 1, -1)).
+
+    Only expectation values and gradients leave the batch, so by default the
+    circuits' trailing permutation / diagonal gates are measured through the
+    observables (``fold_tail``) when every group ends in the same such tail
+    (the groups share one observable table)."""
 
     def __init__(self, tapes: Sequence[list], batch: Sequence[int], n_qubits: int,
                  diags: Sequence[Tuple[int, float, float]], dtype: Optional[int] = None,
-                 knobs: Optional[dict] = None):
+                 knobs: Optional[dict] = None, fold: bool = True):
         self.n_qubits = n_qubits
         self.dtype = _dtype_code() if dtype is None else dtype
         self.batch = list(batch)
         self.lowered = [lowering.lower_pure(t, n_qubits) for t in tapes]
         # one set of tuning knobs for every group (given, or the autotuner's
         # cached choice for each group's own structure)
-        self.plans = [get_plan(low, self.dtype, batch=B, knobs=knobs) for low, B in zip(self.lowered, self.batch)]
+        self.plans = [get_plan(low, self.dtype, batch=B, knobs=knobs, fold=fold)
+                      for low, B in zip(self.lowered, self.batch)]
+        if fold:
+            folds = [p.fold() for p in self.plans]
+            same = all(f[0] == folds[0][0] and (f[0] == 0 or (np.array_equal(f[1], folds[0][1]) and f[2] == folds[0][2]))
+                       for f in folds)
+            if not same:
+                self.plans = [get_plan(low, self.dtype, batch=B, knobs=knobs)
+                              for low, B in zip(self.lowered, self.batch)]
         self.n_params = [len(low.angle_values) for low in self.lowered]
         self.diags = list(diags)
         self.wires = np.array([d[0] for d in diags], dtype=np.int32)

@@ -17,6 +17,7 @@ import numpy as np
 HDR = 16
 GW, BW, MW = 16, 12, 8
 AK_NOP, AK_MC1, AK_MC2, AK_MCX, AK_MCSWAP, AK_DIAG, AK_DENSEK, AK_WIDE = range(8)
+SIGMA = [np.array([[0, 1], [1, 0]], dtype=complex), np.array([[0, -1j], [1j, 0]]), np.diag([1.0, -1.0]).astype(complex)]
 PF_WIDE = 4
 PAULI = [
     np.eye(2, dtype=complex),
@@ -66,9 +67,12 @@ def member_full(mt, theta, xm, deriv=False):
 
 class Program:
     def __init__(self, plan, quat=False, backward_table=4):
-        # quat: reduce the cotangent of MC1 gates flagged in word 15 to
-        # S = bar00 + conj(bar11) (bar00 slot) and D = bar10 - conj(bar01)
-        # (bar10 slot), as the plan-specialised backward does
+        # quat: the cotangent of MC1 gates flagged in word 15 is kept as
+        # t_k = Im Tr(bar^T U sigma_k) (t_x + i t_y in the bar00 slot, t_z in
+        # the bar10 slot's real part), and the chain turns it into
+        # bar00 = -be (i t_x + t_y) + i t_z conj(al), bar10 = al (i t_x + t_y)
+        # + i t_z conj(be) for U = [[al, .], [be, .]], as the plan-specialised
+        # backward and the chain kernel do
         self.quat = quat
         self.w = plan.program()
         # backward program: 4 the prebuilt kernels', 5 the plan-specialised kernels'
@@ -336,7 +340,7 @@ class Program:
                     for g in reversed(gates):
                         x = self._gate(x, g, reg, base, pool, 1, tbits)
                         if g[10] >= 0:
-                            self._grad(x, l, g, reg, base, tbits, partial, p, t)
+                            self._grad(x, l, g, reg, base, tbits, partial, p, t, pool)
                         l = self._gate(l, g, reg, base, pool, 2, tbits)
                 newlam[gl[valid]] = l[valid]
             lam = newlam
@@ -356,7 +360,7 @@ class Program:
         grads = self.chain(angles, xm, bars)
         return final, bars, grads, lam
 
-    def _grad(self, psi_in, lam_out, g, reg, base, tbits, partial, p, tile):
+    def _grad(self, psi_in, lam_out, g, reg, base, tbits, partial, p, tile, pool=None):
         kind = int(g[0])
         T = self.T
         t = np.arange(1 << T, dtype=np.int64)
@@ -397,7 +401,9 @@ class Program:
                     vals[o, c] = np.sum(lam_out[idx[o]] * psi_in[idx[c]])
             vals = vals.reshape(-1)
             if self.quat and kind == AK_MC1 and int(g[15]):
-                vals = np.array([vals[0] + np.conj(vals[3]), 0, vals[2] - np.conj(vals[1]), 0])
+                U = pool[int(g[1])][0]
+                tk = [np.imag(np.trace(vals.reshape(2, 2).T @ U @ s)) for s in SIGMA]
+                vals = np.array([tk[0] + 1j * tk[1], 0, tk[2], 0])
         key = (p, int(g[10]))
         flat = np.stack([vals.real, vals.imag], axis=-1).reshape(-1)
         partial[key] = partial.get(key, 0) + flat
@@ -414,6 +420,12 @@ class Program:
             bar = barc[rec[7] : rec[7] + ne]
             if not diagf:
                 bar = bar.reshape(D, D)
+            if self.quat and rec[10]:
+                U = self.block_matrix(b, angles, xm)
+                al, be = U[0, 0], U[1, 0]
+                tx, ty, tz = bar[0, 0].real, bar[0, 0].imag, bar[1, 0].real
+                w = 1j * tx + ty
+                bar = np.array([[-be * w + 1j * tz * np.conj(al), 0], [al * w + 1j * tz * np.conj(be), 0]])
             mem = self.members(b)
             Es = [self.eff(m, angles, xm, b) for m in mem]
             for j, m in enumerate(mem):

@@ -60,9 +60,10 @@ def test_gate_zoo_program_matches_oracle(dtype):
 
 @pytest.mark.parametrize("n,layers", [(3, 2), (9, 2)])
 def test_rotation_blocks_reduced_cotangent_matches_oracle(n, layers):
-    """The specialised backward keeps only bar00 + conj(bar11) and
-    bar10 - conj(bar01) of blocks the planner flags as Pauli-rotation
-    products; the angle gradients through the chain rule stay exact."""
+    """The specialised backward keeps only t_k = Im Tr(bar^T U sigma_k)
+    (3 reals) of blocks the planner flags as Pauli-rotation products, and the
+    chain turns them into an effective cotangent of U; the angle gradients
+    through the chain rule stay exact."""
     from circuits import record_hea
     from plan_interp import Program
 
@@ -79,9 +80,9 @@ def test_rotation_blocks_reduced_cotangent_matches_oracle(n, layers):
     _, _, g_full, _ = Program(plan).adjoint(angles, lambda f: seed, xm)
     _, _, g_red, _ = prog.adjoint(angles, lambda f: seed, xm)
     np.testing.assert_allclose(g_red, g_full, atol=1e-12)
-    # every HEA rotation block is flagged: half of its bar reals are zero
+    # every HEA rotation block is flagged: 3 of its 8 bar reals are kept
     _, full, _, _ = Program(plan).adjoint(angles, lambda f: 2 * zs * np.conj(f), xm)
-    assert np.count_nonzero(bars) * 2 == np.count_nonzero(full)
+    assert np.count_nonzero(bars) * 8 == np.count_nonzero(full) * 3
 
 
 def test_rotation_block_flags_on_the_gate_zoo():
