@@ -329,8 +329,9 @@ void jit_plan(const JitSpec& spec, JitPlan& out) {
         id = (int)out.kernels.size();
         JitPlan::Kernel k{dir, body, e.first, body0, e.last};
         k.full = full;
-        // the forward reads through LDS already (sv_forward_dma); only its store changes
-        k.load = dir == 1 ? access_mode(pv, e.first_sub, lowrun) : 0;
+        // (the forward reads directly under its first layout when it can,
+        // else through LDS: sv_forward_dma)
+        k.load = access_mode(pv, e.first_sub, lowrun);
         k.store = access_mode(pv, e.last_sub, lowrun);
         out.kernels.push_back(k);
       }
@@ -409,6 +410,7 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
       o << "struct JF" << id << " {\n"
         << "  static constexpr bool kFull = " << (k.full ? "true" : "false") << ";\n"
         << "  static constexpr int kStore = " << (dma ? k.store : 0) << ";\n"
+        << "  static constexpr int kLoad = " << (dma ? k.load : 0) << ";\n"
         << "  static constexpr bool kSplit = " << (dma ? "true" : "false") << ";\n"
         << "  template <int RB, int LNT>\n"
         << "  static __device__ __forceinline__ void last_layout(cint* tb, Layout& L) {\n"
@@ -421,7 +423,10 @@ std::string jit_source(const JitSpec& spec, const JitPlan& plan, const std::vect
         << "    (void)lds; (void)base; (void)pool;\n  }\n"
         << "  template <int RB, int LNT>\n"
         << "  static __device__ __forceinline__ void first_layout(cint* tb, Layout& L) {\n"
-        << "    layout_lit<RB, LNT" << (k.first.empty() ? canonical_args(pv) : k.first) << ">(tb, L);\n  }\n};\n";
+        << "    layout_lit<RB, LNT" << (k.first.empty() ? canonical_args(pv) : k.first) << ">(tb, L);\n  }\n"
+        << "  template <int RB, int LNT>\n"
+        << "  static __device__ __forceinline__ void first_layout_g(cint* tb, Layout& L) {\n"
+        << "    layout_lit_g<RB, LNT" << (k.first.empty() ? canonical_args(pv) : k.first) << ">(tb, L);\n  }\n};\n";
       if (dma)
         o << "extern \"C\" __global__ __launch_bounds__(" << nt << ", " << occ << ") void " << kname
           << "(SvArgs a) {\n  sv_forward_dma<" << rt << ", " << pv.RB << ", " << nt << ", JF" << id

@@ -2074,12 +2074,22 @@ __device__ __forceinline__ void sv_forward_dma(const SvArgs& a) {
     }
   };
   const uint64_t base = tile_base(tile, tbits, T, a.n_wires);
-  if (dma) prefetch(base);
+  // Body::kLoad: the first substep's lanes run over the lowest global bits,
+  // so the tile is read straight into registers under that layout (no LDS
+  // round trip)
+  const bool direct = Body::kLoad != 0 && dma;
+  if (dma && !direct) prefetch(base);
   {
     R re[NR], im[NR];
     Layout L;
     bool started = false;  // registers already hold the first substep's layout
-    if (dma) {
+    if (direct) {
+      if constexpr (Body::kLoad != 0) {
+        Body::template first_layout_g<RB, LNT>(tbits, L);
+        load_direct<R, RB, Body::kLoad == 1>(re, im, src, L, base);
+      }
+      started = true;
+    } else if (dma) {
       __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's LDS loads have landed
       __syncthreads();                     // ... and every other wave's
       Body::template first_layout<RB, LNT>(tbits, L);
