@@ -208,6 +208,7 @@ struct jq_plan {
   std::vector<int32_t> fold_words;    // jq_plan_tables(6)
   bool uploaded = false;
   int class_mask = 0;  // block classes of the prep / chain kernels (jq::block_class)
+  int max_members2 = 0;  // most members of a 2x2 (class 2) block: the chain's threads per block
   std::vector<int32_t> wg_blocks;  // blocks of the workgroup prep / chain classes
   DevBuf wg_list;
   std::vector<char> pass_wide;               // pass q is one wide block (program.h PF_WIDE)
@@ -668,6 +669,8 @@ int jq_plan_create(const jq_plan_desc* desc, jq_plan** out) {
   p->slot_words = sm;
   for (size_t r = 0; r + jq::BW <= p->hp.block_table.size(); r += jq::BW) {
     const int cls = jq::block_class(p->hp.block_table[r + jq::B_DIAGF] != 0, p->hp.block_table[r + jq::B_D]);
+    if (cls == 2)
+      p->max_members2 = std::max(p->max_members2, p->hp.block_table[r + jq::B_MEND] - p->hp.block_table[r + jq::B_MBEG]);
     if (cls >= jq::WG_FIRST_CLASS)
       p->wg_blocks.push_back((int32_t)(r / jq::BW));
     else
@@ -938,6 +941,7 @@ int finish_grads(jq_plan* p, int batch, const double* angles, const void* matric
     c.grad_angles = grad_angles;
     c.grad_matrices = grad_matrices;
     c.class_mask = p->class_mask;
+    c.max_members2 = p->max_members2;
     c.quat_t = quat_t ? 1 : 0;
     c.wg_blocks = (const int*)p->wg_list.p;
     c.n_wg = (int)p->wg_blocks.size();

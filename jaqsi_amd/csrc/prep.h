@@ -62,6 +62,7 @@ struct ChainArgs {
                             // (bar00.re, bar00.im, bar10.re) (the plan-specialised backward's form)
   const int* wg_blocks;     // blocks of the workgroup classes (block_class >= 3)
   int n_wg;
+  int max_members2;         // most members of a 2x2 (class 2) block
 };
 
 struct ExpvalArgs {

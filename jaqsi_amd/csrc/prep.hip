@@ -262,8 +262,11 @@ __device__ __forceinline__ void prep_block(const PrepArgs& a, int circ, const in
 // One instance per block class (launch_prep starts those the plan holds):
 // a kernel compiled for one matrix size keeps it in registers, and the
 // scratch of the large classes is not reserved for the common small ones.
+// (all three kernel families launch 64 threads per workgroup: the bound
+// lets a thread keep its block's matrices in up to 512 VGPRs instead of
+// scratch)
 template <typename R, int CLS>
-__global__ void prep_kernel(PrepArgs a) {
+__global__ __launch_bounds__(64) void prep_kernel(PrepArgs a) {
   const int nb = a.n_blocks;
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (int64_t)nb * a.batch) return;
@@ -548,8 +551,116 @@ __device__ __forceinline__ void chain_block(const ChainArgs& a, int circ, const 
   }
 }
 
+// The 2x2 class with one thread per (circuit, block, member j): the thread
+// rebuilds only its own prefix R_j = E_{j-1} .. E_0 and suffix L_j = E_last
+// .. E_{j+1} (j and m-1-j factors) instead of one thread walking every
+// member of the block and every suffix in turn, so the chain's serial depth
+// is one member product instead of the block's whole quadratic walk.
+// Rotation blocks in su(2) form (quat_t) take U = L_j E_j R_j.
+__global__ __launch_bounds__(64) void chain2_kernel(ChainArgs a) {
+  constexpr int DM = 2, N = 4;
+  const int mm = a.max_members2;
+  const int nb = a.n_blocks;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (int64_t)nb * a.batch * mm) return;
+  const int jo = (int)(gid % mm);
+  const int64_t bc = gid / mm;
+  const int circ = (int)(bc / nb);
+  const int b = (int)(bc % nb);
+  const int* bt = a.block_table + b * BW;
+  if (bt[B_BAR] < 0 || block_class(bt[B_DIAGF] != 0, bt[B_D]) != 2) return;
+  const int mb = bt[B_MBEG], me = bt[B_MEND];
+  const int j = mb + jo;
+  if (j >= me) return;
+  const int D = bt[B_D], offs = bt[B_OFFS];
+  const double* angles = a.angles + (int64_t)circ * a.n_params;
+  const cd* xm = reinterpret_cast<const cd*>(a.matrices) + (a.matrices ? (int64_t)circ * a.n_matrix_elems : 0);
+  double* gang = a.grad_angles ? a.grad_angles + (int64_t)circ * a.n_params : nullptr;
+  cd* gmat = a.grad_matrices ? reinterpret_cast<cd*>(a.grad_matrices) + (int64_t)circ * a.n_matrix_elems : nullptr;
+  const int m = a.member_list[j];
+  const int* mt = a.member_table + m * MW;
+  const bool want_theta = mt[M_PARAM] >= 0 && gang != nullptr;
+  const bool want_mat = mt[M_KIND] == JQ_M_MATRIX && (mt[M_FLAGS] & JQ_MF_GRAD) && gmat != nullptr;
+  if (!want_theta && !want_mat) return;
+  const cd* bar = reinterpret_cast<const cd*>(a.bars) + (int64_t)circ * a.bars_stride + bt[B_BAR];
+  cd Rp[N], Lm[N], E[N], W[N], B[N];
+#pragma unroll
+  for (int e = 0; e < N; ++e) B[e] = bar[e];
+  identity_t<false, DM>(Rp);
+  for (int q = mb; q < j; ++q) {
+    const int* mtq = a.member_table + a.member_list[q] * MW;
+    member_eff_t<false, DM>(mtq, member_theta(mtq, a.member_values[a.member_list[q]], angles), xm, D, offs, false, E);
+    left_mul_t<false, DM>(E, Rp, D);
+  }
+  identity_t<false, DM>(Lm);
+  for (int q = j + 1; q < me; ++q) {
+    const int* mtq = a.member_table + a.member_list[q] * MW;
+    member_eff_t<false, DM>(mtq, member_theta(mtq, a.member_values[a.member_list[q]], angles), xm, D, offs, false, E);
+    left_mul_t<false, DM>(E, Lm, D);
+  }
+  const double th = member_theta(mt, a.member_values[m], angles);
+  if (a.quat_t && bt[B_QUAT]) {
+    // effective cotangent of U from (t_x, t_y, t_z), as in chain_block
+    cd U[N];
+#pragma unroll
+    for (int e = 0; e < N; ++e) U[e] = Rp[e];
+    member_eff_t<false, DM>(mt, th, xm, D, offs, false, E);
+    left_mul_t<false, DM>(E, U, D);
+    left_mul_t<false, DM>(Lm, U, D);
+    const cd al = U[0], be = U[2];
+    const double tx = B[0].x, ty = B[0].y, tz = B[2].x;
+    const cd w = {ty, tx};
+    const cd izt = {0.0, tz};
+    const cd mbe = {-be.x, -be.y};
+    B[0] = cadd(cmul(mbe, w), cmul(izt, cd{al.x, -al.y}));
+    B[2] = cadd(cmul(al, w), cmul(izt, cd{be.x, -be.y}));
+    B[1] = cd{0.0, 0.0};
+    B[3] = cd{0.0, 0.0};
+  }
+  // W = L^T bar R^T
+  {
+    cd T[N];
+#pragma unroll
+    for (int r = 0; r < DM; ++r)
+#pragma unroll
+      for (int c = 0; c < DM; ++c) {
+        cd acc = {0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < DM; ++k) acc = cadd(acc, cmul(Lm[k * DM + r], B[k * DM + c]));
+        T[r * DM + c] = acc;
+      }
+#pragma unroll
+    for (int r = 0; r < DM; ++r)
+#pragma unroll
+      for (int c = 0; c < DM; ++c) {
+        cd acc = {0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < DM; ++k) acc = cadd(acc, cmul(T[r * DM + k], Rp[c * DM + k]));
+        W[r * DM + c] = acc;
+      }
+  }
+  if (want_theta) {
+    member_eff_t<false, DM>(mt, th, xm, D, offs, true, E);
+    double g = 0;
+#pragma unroll
+    for (int e = 0; e < N; ++e) g += W[e].x * E[e].x - W[e].y * E[e].y;
+    atomicAdd(&gang[mt[M_PARAM]], g);
+  }
+  if (want_mat) {
+    const int dim = 1 << mt[M_KB];
+#pragma unroll
+    for (int r = 0; r < DM; ++r)
+#pragma unroll
+      for (int c = 0; c < DM; ++c) {
+        cd* dst = &gmat[mt[M_MATRIX] + r * dim + c];
+        atomicAdd(&dst->x, W[r * DM + c].x);
+        atomicAdd(&dst->y, W[r * DM + c].y);
+      }
+  }
+}
+
 template <int CLS>
-__global__ void chain_kernel(ChainArgs a) {
+__global__ __launch_bounds__(64) void chain_kernel(ChainArgs a) {
   const int nb = a.n_blocks;
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (int64_t)nb * a.batch) return;
@@ -818,7 +929,14 @@ hipError_t launch_chain(const ChainArgs& a, hipStream_t st) {
   const dim3 grid((unsigned)((n + 63) / 64));
   if (a.class_mask & 1) hipLaunchKernelGGL(chain_kernel<0>, grid, dim3(64), 0, st, a);
   if (a.class_mask & 2) hipLaunchKernelGGL(chain_kernel<1>, grid, dim3(64), 0, st, a);
-  if (a.class_mask & 4) hipLaunchKernelGGL(chain_kernel<2>, grid, dim3(64), 0, st, a);
+  if (a.class_mask & 4) {
+    if (a.max_members2 > 0) {
+      const int64_t n2 = n * a.max_members2;
+      hipLaunchKernelGGL(chain2_kernel, dim3((unsigned)((n2 + 63) / 64)), dim3(64), 0, st, a);
+    } else {
+      hipLaunchKernelGGL(chain_kernel<2>, grid, dim3(64), 0, st, a);
+    }
+  }
   if (a.n_wg > 0) hipLaunchKernelGGL(chain_wg_kernel, dim3((unsigned)a.n_wg, (unsigned)a.batch), dim3(WG_NT), 0, st, a);
   return hipGetLastError();
 }
