@@ -1309,6 +1309,20 @@ __device__ __forceinline__ uint32_t parity_reg_mask(const Layout& L, uint64_t m)
   return r;
 }
 
+// In-register Walsh-Hadamard transform: v[k] <- sum_i v[i] (-1)^popc(i & k).
+template <typename T, int N>
+__device__ __forceinline__ void wht(T* v) {
+#pragma unroll
+  for (int h = 1; h < N; h <<= 1)
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      if (!(i & h)) {
+        const T x = v[i], y = v[i | h];
+        v[i] = x + y;
+        v[i | h] = x - y;
+      }
+}
+
 // Register bit holding global bit gb, or -1 (wave-uniform).
 template <int RB>
 __device__ __forceinline__ int reg_bit_of(const Layout& L, int gb) {
@@ -1358,24 +1372,31 @@ __device__ __forceinline__ void expval_partials(const R* re, const R* im, const 
     return;
   }
   if (a.obs_kind == JQ_OBS_SV_PARITY) {
-    R p[NR], ptot = 0;
+    // sum_i p_i d(parity) = (d0 + d1)/2 P + (d0 - d1)/2 (-1)^pt W[mreg] with
+    // W the Walsh-Hadamard transform of the thread's probabilities over the
+    // register index (W[0] = P): one transform serves every Z word, whose
+    // register part is then one entry (mreg is wave-uniform) and whose thread
+    // and outside bits give the sign pt
+    R p[NR];
 #pragma unroll
-    for (int i = 0; i < NR; ++i) {
-      p[i] = fma(re[i], re[i], im[i] * im[i]);
-      ptot += p[i];
-    }
+    for (int i = 0; i < NR; ++i) p[i] = fma(re[i], re[i], im[i] * im[i]);
+    wht<R, NR>(p);
     const uint64_t tg = base | L.gthr;  // register bit positions are zero here
-    for (int o = 0; o < a.n_obs; ++o) {
-      const uint64_t m = t->mask[o];
-      const uint32_t mreg = parity_reg_mask<RB>(L, m);
-      const int pt = __popcll(tg & m) & 1;
-      R se = 0;
+    for (int o0 = 0; o0 < a.n_obs; o0 += 8) {
+      const int nv = a.n_obs - o0 < 8 ? a.n_obs - o0 : 8;
+      R v[8];
 #pragma unroll
-      for (int i = 0; i < NR; ++i) se += (__popc((uint32_t)i & mreg) & 1) ? R(0) : p[i];
-      const R so = ptot - se;
-      const R d0 = (R)t->d[2 * o], d1 = (R)t->d[2 * o + 1];
-      const R v = pt ? fma(d1, se, d0 * so) : fma(d0, se, d1 * so);
-      lds_add(&ev_lds[o], (double)wave_sum<R>(v));
+      for (int k = 0; k < 8; ++k) {
+        v[k] = 0;
+        if (k < nv) {
+          const uint64_t m = t->mask[o0 + k];
+          const uint32_t mreg = __builtin_amdgcn_readfirstlane(parity_reg_mask<RB>(L, m));
+          const R d0 = (R)t->d[2 * (o0 + k)], d1 = (R)t->d[2 * (o0 + k) + 1];
+          const R h = (__popcll(tg & m) & 1) ? (d1 - d0) : (d0 - d1);
+          v[k] = R(0.5) * fma(h, p[mreg], (d0 + d1) * p[0]);
+        }
+      }
+      flush8<R>(v, nv, ev_lds, [o0](int k) { return o0 + k; });
     }
     return;
   }
@@ -1900,21 +1921,27 @@ __device__ __forceinline__ void seed_lambda(const R* pr, const R* pi, R* lr, R* 
     return;
   }
   if (a.obs_kind == JQ_OBS_SV_PARITY) {
+    // w_i = sum_o ct_o d_o[parity] = C + sum_o h_o (-1)^(pt_o + popc(i & mreg_o)):
+    // each Z word adds its half-sum to C and its half-difference h_o, signed
+    // by its thread / outside bits (pt_o), to every register's weight with
+    // the wave-uniform sign of its register mask: one FMA per register
     const uint64_t tg = base | L.gthr;
     double w[NR];
 #pragma unroll
     for (int i = 0; i < NR; ++i) w[i] = 0;
+    double C = 0;
     for (int o = 0; o < a.n_obs; ++o) {
       const uint64_t m = t->mask[o];
-      const uint32_t mreg = parity_reg_mask<RB>(L, m);
-      const int pt = __popcll(tg & m) & 1;
+      const uint32_t mreg = __builtin_amdgcn_readfirstlane(parity_reg_mask<RB>(L, m));
       const double c0 = ct[o] * t->d[2 * o], c1 = ct[o] * t->d[2 * o + 1];
+      C += 0.5 * (c0 + c1);
+      const double h = (__popcll(tg & m) & 1) ? 0.5 * (c1 - c0) : 0.5 * (c0 - c1);
 #pragma unroll
-      for (int i = 0; i < NR; ++i) w[i] += ((pt ^ (__popc((uint32_t)i & mreg) & 1)) ? c1 : c0);
+      for (int i = 0; i < NR; ++i) w[i] = fma((__popc((uint32_t)i & mreg) & 1) ? -1.0 : 1.0, h, w[i]);
     }
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
-      const R w2 = (R)(2.0 * w[i]);
+      const R w2 = (R)(2.0 * (C + w[i]));
       lr[i] = w2 * pr[i];
       li[i] = -w2 * pi[i];
     }
