@@ -34,8 +34,11 @@ class _GateHandler:
     def __init__(self, gates, gate_name: str):
         self._gates = gates
         self.__name__ = gate_name
-        # the unitary gate of this name, called directly when only wires are given
-        self._plain = None if gate_name == "Barrier" else getattr(UnitaryGates, gate_name, None)
+        # the unitary gate of this name, called directly when only wires are
+        # given (for rotations and fixed gates that is the gate class itself:
+        # their noise-free wrapper only forwards to it)
+        plain = None if gate_name == "Barrier" else getattr(UnitaryGates, gate_name, None)
+        self._plain = getattr(plain, "gate_class", plain)
 
     def __call__(self, *args, **kwargs):
         if self._plain is not None and kwargs.keys() <= _PLAIN_KWARGS:

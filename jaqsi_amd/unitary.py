@@ -113,6 +113,7 @@ def _rotation(cls_name: str):
         UnitaryGates.Noise(wires, noise_params)
 
     gate.__name__ = cls_name
+    gate.gate_class = cls  # without noise the gate is just its class
     return staticmethod(gate)
 
 
@@ -125,6 +126,7 @@ def _fixed(cls_name: str):
             UnitaryGates.Noise(wires, noise_params)
 
     gate.__name__ = cls_name
+    gate.gate_class = cls  # without noise the gate is just its class
     return staticmethod(gate)
 
 
