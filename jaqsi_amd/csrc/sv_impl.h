@@ -1279,9 +1279,11 @@ __device__ __forceinline__ uint64_t tile_base(int tile, cint* tbits, int T, int 
 // Observable table, staged in LDS for the per-amplitude reductions.
 typedef ObsTable ObsLds;
 
-__device__ __forceinline__ void load_obs(const SvArgs& a, ObsLds* t) {
+// (need = false: only the barrier, for passes that read no observable; the
+// copy's global loads would otherwise hold the tile loads behind them)
+__device__ __forceinline__ void load_obs(const SvArgs& a, ObsLds* t, bool need = true) {
   const ObsTable* src = reinterpret_cast<const ObsTable*>(a.obs);
-  if (src != nullptr) {
+  if (need && src != nullptr) {
     for (int o = threadIdx.x; o < 64; o += blockDim.x) t->gbit[o] = src->gbit[o];
     for (int o = threadIdx.x; o < 128; o += blockDim.x) t->d[o] = src->d[o];
     for (int o = threadIdx.x; o < 64; o += blockDim.x) t->mask[o] = src->mask[o];
@@ -2186,7 +2188,7 @@ __device__ __forceinline__ void sv_backward_loop(const SvArgs& a) {
   int64_t w = blockIdx.x;
   if (w >= total) return;
   for (int e = threadIdx.x; e < 2 * nslots; e += NT) slot_buf[e] = 0;
-  load_obs(a, obs);  // (ends with a barrier)
+  load_obs(a, obs, seed);  // (ends with a barrier)
   const bool full = Body::kFull;  // n_wires >= T, fixed when the kernel is generated
   const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src);
   C2<R>* lam = reinterpret_cast<C2<R>*>(a.lam);
@@ -2301,7 +2303,7 @@ __device__ __forceinline__ void sv_backward_direct(const SvArgs& a0) {
   const int slot_base = pass[3];
   const int tile = blockIdx.x;
   for (int e = threadIdx.x; e < nslots; e += NT) slots[e] = 0;
-  load_obs(a, obs);  // (ends with a barrier)
+  load_obs(a, obs, seed);  // (ends with a barrier)
   const bool full = Body::kFull;
   const uint64_t base = tile_base(tile, tbits, T, a.n_wires);
   const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + (int64_t)circ * a.src_stride;
