@@ -1325,6 +1325,17 @@ __device__ __forceinline__ void wht(T* v) {
       }
 }
 
+// The 64-bit value v of lane l (wave-uniform l), as a wave-uniform value.
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ double lane_f64(double v, int l) {
+  return __longlong_as_double((long long)lane_u64((uint64_t)__double_as_longlong(v), l));
+}
+
 // Register bit holding global bit gb, or -1 (wave-uniform).
 template <int RB>
 __device__ __forceinline__ int reg_bit_of(const Layout& L, int gb) {
@@ -1890,16 +1901,24 @@ __device__ __forceinline__ void seed_lambda(const R* pr, const R* pi, R* lr, R* 
                                             int circ, const Layout& L, uint64_t base) {
   constexpr int NR = 1 << RB;
   const double* ct = a.ct + (int64_t)circ * a.n_obs;
+  // The statevector kinds stage observable o's cotangent and (d0, d1) in
+  // lane o (at most 64 observables, the table's size) with one load each,
+  // and read them in the loop over observables with readlane: the loop then
+  // waits on no memory.
+  const int lane = threadIdx.x & 63;
+  const bool has = lane < a.n_obs;
   if (a.obs_kind == JQ_OBS_SV_DIAG) {
     // w(i) = c + sum_b A_b bit_b(i): observables on thread / outside bits
     // fold into c, those on register bit b into A_b.
     const uint64_t tg = base | L.gthr;
+    const double ctl = has ? ct[lane] : 0.0, d0l = has ? t->d[2 * lane] : 0.0, d1l = has ? t->d[2 * lane + 1] : 0.0;
+    const int gbl = has ? t->gbit[lane] : 0;
     double c = 0, A[RB];
 #pragma unroll
     for (int b = 0; b < RB; ++b) A[b] = 0;
     for (int o = 0; o < a.n_obs; ++o) {
-      const int gb = t->gbit[o];
-      const double d0 = t->d[2 * o], d1 = t->d[2 * o + 1], w = ct[o];
+      const int gb = __builtin_amdgcn_readlane(gbl, o);
+      const double d0 = lane_f64(d0l, o), d1 = lane_f64(d1l, o), w = lane_f64(ctl, o);
       const int rb = reg_bit_of<RB>(L, gb);
       if (rb >= 0) {
         c += w * d0;
@@ -1926,21 +1945,39 @@ __device__ __forceinline__ void seed_lambda(const R* pr, const R* pi, R* lr, R* 
     // w_i = sum_o ct_o d_o[parity] = C + sum_o h_o (-1)^(pt_o + popc(i & mreg_o)):
     // each Z word adds its half-sum to C and its half-difference h_o, signed
     // by its thread / outside bits (pt_o), to every register's weight with
-    // the wave-uniform sign of its register mask: one FMA per register
+    // the wave-uniform sign of its register mask: one FMA per register.  A
+    // word with no register bit (mreg == 0, a uniform branch) adds h to a
+    // per-thread constant instead.  The sign bits (-1)^popc(i & mreg) are
+    // built by doubling over i, one scalar XOR per register.
     const uint64_t tg = base | L.gthr;
+    const double ctl = has ? ct[lane] : 0.0;
+    const double c0l = has ? ctl * t->d[2 * lane] : 0.0, c1l = has ? ctl * t->d[2 * lane + 1] : 0.0;
+    const double hsl = 0.5 * (c0l + c1l), hdl = 0.5 * (c0l - c1l);
+    const uint64_t ml = has ? t->mask[lane] : 0;
+    const uint32_t mregl = parity_reg_mask<RB>(L, ml);
     double w[NR];
 #pragma unroll
     for (int i = 0; i < NR; ++i) w[i] = 0;
-    double C = 0;
+    double C = 0, Ct = 0;
     for (int o = 0; o < a.n_obs; ++o) {
-      const uint64_t m = t->mask[o];
-      const uint32_t mreg = __builtin_amdgcn_readfirstlane(parity_reg_mask<RB>(L, m));
-      const double c0 = ct[o] * t->d[2 * o], c1 = ct[o] * t->d[2 * o + 1];
-      C += 0.5 * (c0 + c1);
-      const double h = (__popcll(tg & m) & 1) ? 0.5 * (c1 - c0) : 0.5 * (c0 - c1);
+      const uint64_t m = lane_u64(ml, o);
+      const uint32_t mreg = (uint32_t)__builtin_amdgcn_readlane((int)mregl, o);
+      C += lane_f64(hsl, o);
+      const double hd = lane_f64(hdl, o);
+      const double h = (__popcll(tg & m) & 1) ? -hd : hd;
+      if (mreg == 0) {
+        Ct += h;
+        continue;
+      }
+      uint32_t sg[NR];
+      sg[0] = 0;
 #pragma unroll
-      for (int i = 0; i < NR; ++i) w[i] = fma((__popc((uint32_t)i & mreg) & 1) ? -1.0 : 1.0, h, w[i]);
+      for (int i = 1; i < NR; ++i) sg[i] = sg[i & (i - 1)] ^ (((mreg >> __builtin_ctz(i)) & 1u) << 31);
+#pragma unroll
+      for (int i = 0; i < NR; ++i)
+        w[i] = fma(__longlong_as_double((long long)(0x3FF0000000000000ull | ((uint64_t)sg[i] << 32))), h, w[i]);
     }
+    C += Ct;
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
       const R w2 = (R)(2.0 * (C + w[i]));
